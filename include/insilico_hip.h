@@ -1,0 +1,204 @@
+/*
+ * insilico_hip.h -- C ABI of the MI355X finite-element hot path
+ * (element assembly -> CSR -> Jacobi-PCG), the drop-in boundary between the
+ * reference's template API and hand-written gfx950 HIP kernels.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 = INS_OK, < 0 = an INS_ERR_*
+ *    code.  Nothing aborts, throws or exits across this boundary (the
+ *    reference aborts through VERIFY_MSG, base/verify.hpp:63-65,139-149);
+ *    ins_last_error() gives the message of the last failure of a context.
+ *  - Host arrays are owned by the caller and copied in or out; device memory
+ *    is owned by the context.  No torch or HIP types appear here.
+ *  - A context is used by one host thread at a time.  Separate contexts are
+ *    independent.  All work of a context runs on its own HIP stream.
+ *  - Index types: node ids, element connectivity, DoF ids and CSR arrays are
+ *    int32 (the reference's Eigen::SparseMatrix<double> uses int indices,
+ *    base/solver/Eigen3.hpp:341).  Sizes are int64.
+ *
+ * Reference seams replaced (file:line in thrueberg/inSilico):
+ *  SOLVER concept      base/solver/Eigen3.hpp:64-342
+ *  pattern             base/solver/TripletContainer.hpp:159-301
+ *  stiffness assembly  base/asmb/StiffnessMatrix.hpp:52-73,159-225
+ *  residual forces     base/asmb/ForceIntegrator.hpp:40-56,126-160
+ *  body / Neumann      base/asmb/BodyForce.hpp:67-82, NeumannForce.hpp:36-63
+ *  DoF distribution    base/dof/Distribute.hpp:36-80,141-215
+ */
+#ifndef INSILICO_HIP_H
+#define INSILICO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define INS_OK                 0
+#define INS_ERR_ARG           -1  /* bad argument (null pointer, bad size)      */
+#define INS_ERR_HIP           -2  /* a HIP runtime call failed                  */
+#define INS_ERR_OOM           -3  /* device allocation failed / over budget     */
+#define INS_ERR_RANGE         -4  /* DoF or node id out of range, ids not a
+                                     permutation of 0..n_active-1
+                                     (Eigen3.hpp:95-98 "index out of bound")   */
+#define INS_ERR_PATTERN       -5  /* entry outside the registered pattern
+                                     (TripletContainer.hpp:342-346)             */
+#define INS_ERR_BREAKDOWN     -6  /* CG breakdown: p.Ap <= 0 or non-finite      */
+#define INS_ERR_NOCONV        -7  /* iteration limit reached                    */
+#define INS_ERR_STATE         -8  /* call out of order (e.g. solve before
+                                     assembly)                                  */
+#define INS_ERR_UNSUPPORTED   -9  /* element/kernel combination not provided    */
+
+/* ---- enumerations ------------------------------------------------------ */
+/* element kernels (KERNEL concept implementations) */
+#define INS_LAPLACE   1  /* heat::Laplace (heat/Laplace.hpp:113-126)           */
+#define INS_STVENANT  2  /* solid::HyperElastic<mat::hypel::StVenant>          */
+#define INS_NEOHOOKE  3  /* solid::HyperElastic<mat::hypel::NeoHookeanCompr.>  */
+
+/* arithmetic of element integration */
+#define INS_FP64 0
+#define INS_FP32 1
+
+/* ins_assemble flags */
+#define INS_ASM_MATRIX       1  /* tangent / stiffness values into the CSR    */
+#define INS_ASM_RHS          2  /* lifting + body + Neumann (+ residual)      */
+#define INS_ASM_RESIDUAL     4  /* add -F_int(u) (ForceIntegrator factor -1)  */
+#define INS_ASM_INCREMENTAL  8  /* lifting with g - u (StiffnessMatrix.hpp:56) */
+
+/* body force kinds (f evaluated at x(xi), BodyForce.hpp:172-201) */
+#define INS_BF_NONE     0
+#define INS_BF_CONST    1  /* params[c] = f_c                                */
+#define INS_BF_SINPROD  2  /* f = params[0] * prod_d sin(params[1+d] * x_d)  */
+
+/* solve modes */
+#define INS_SOLVE_FP64   0  /* FP64 Jacobi-PCG on the exported CSR           */
+#define INS_SOLVE_MIXED  1  /* FP32 PCG sweeps, FP64 residual/update/dots    */
+
+/* matrix storage used by the solver */
+#define INS_FMT_CSR   0  /* plain CSR, int32 column indices                  */
+#define INS_FMT_SELL  1  /* 64-row slices, column-major values, per-row column
+                            offset pattern ids (indices not re-read per nnz) */
+
+/* ins_distribute operations (base/dof/Distribute.hpp:36-50) */
+#define INS_DIST_SET 0
+#define INS_DIST_ADD 1
+
+typedef struct ins_ctx ins_ctx;
+
+typedef struct ins_stats {
+    double setup_ms;        /* pattern + maps (last ins_build_pattern)       */
+    double assemble_ms;     /* last ins_assemble                             */
+    double solve_ms;        /* last ins_solve                                */
+    int    iterations;      /* last ins_solve: inner PCG iterations          */
+    int    refinements;     /* last ins_solve: FP64 residual refreshes       */
+    double relres;          /* last ins_solve: |b - A x| / |b| in FP64       */
+    int64_t n_active;
+    int64_t nnz;
+    int64_t device_bytes;   /* device memory held by the context             */
+    double spmv_ms;         /* profiled: total time of the dominant SpMV     */
+    int64_t spmv_calls;     /* profiled: launches timed                      */
+} ins_stats;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (the reference has no device); hbm_budget_bytes
+ * caps the context's device memory (0 = no cap). */
+int ins_create(ins_ctx** out, int device, uint64_t hbm_budget_bytes);
+int ins_destroy(ins_ctx* ctx);
+const char* ins_strerror(int status);
+const char* ins_last_error(const ins_ctx* ctx);
+int ins_get_stats(const ins_ctx* ctx, ins_stats* out);
+/* time every dominant-kernel launch with HIP events (adds small overhead) */
+int ins_set_profiling(ins_ctx* ctx, int on);
+int ins_sync(ins_ctx* ctx);
+
+/* ---- discretisation (host setup of the reference, flattened) ----------- */
+/* Mesh: base::Unstructured nodes and elements (base/mesh).  conn lists the
+ * element vertices in the reference's hierarchic order
+ * (base/mesh/HierarchicOrder.hpp:201-272,345-392); npe = 4 (QUAD4) or
+ * 8 (HEX8). */
+int ins_set_mesh(ins_ctx* ctx, int dim, int64_t n_nodes, const double* xyz,
+                 int64_t n_elem, int npe, const int32_t* conn);
+
+/* DoFs: dof_id[node*dpn+c] = active row index (base/dof/numbering.hpp:45-66)
+ * or -1 for a CONSTRAINED component (base/dof/DegreeOfFreedom.hpp:232-242).
+ * Ids must be a permutation of 0..n_active-1. */
+int ins_set_dofs(ins_ctx* ctx, int dofs_per_node, const int32_t* dof_id,
+                 int64_t n_active);
+/* prescribed values g per (node, comp) of constrained components
+ * (Constraint values); NULL = all zero */
+int ins_set_prescribed(ins_ctx* ctx, const double* g);
+/* base::dof::scaleConstraints (base/dof/Distribute.hpp:76-80) */
+int ins_scale_prescribed(ins_ctx* ctx, double factor);
+/* current DoF values u per (node, comp) (DegreeOfFreedom values_[0]);
+ * NULL = zero */
+int ins_set_dof_values(ins_ctx* ctx, const double* u);
+int ins_get_dof_values(ins_ctx* ctx, double* u);
+
+/* ---- pattern ------------------------------------------------------------- */
+/* Replaces Eigen3::registerFields -> TripletContainer::registerFields
+ * (base/solver/Eigen3.hpp:331-335, TripletContainer.hpp:159-301): the CSR
+ * pattern of all (active row, active col) pairs of every element, explicit
+ * zeros kept, columns ascending.  Also builds the node->element map and the
+ * row structure used by the device assembly. */
+int ins_build_pattern(ins_ctx* ctx, int64_t* nnz_out);
+int ins_get_pattern(ins_ctx* ctx, int32_t* row_ptr, int32_t* col_idx);
+
+/* ---- kernels and loads ----------------------------------------------------- */
+/* KERNEL object + quadrature: kernel = INS_LAPLACE (ncoef = 1: kappa_e, the
+ * conductivity function of heat/Laplace.hpp:94-97 evaluated per element) or
+ * INS_STVENANT / INS_NEOHOOKE (ncoef = 2: lambda_e, mu_e, mat/Lame.hpp:28-40);
+ * nq = Gauss points per direction of base::Quadrature
+ * (base/quad/GaussLegendre.hpp:33-36: nq = (DEG+2)/2). */
+int ins_set_material(ins_ctx* ctx, int kernel, int nq, int ncoef,
+                     const double* elem_coef);
+/* bodyForceComputation (base/asmb/BodyForce.hpp:67-82) with a built-in f */
+int ins_set_body_force(ins_ctx* ctx, int kind, const double* params);
+/* neumannForceComputation (base/asmb/NeumannForce.hpp:36-63) with a
+ * constant traction over boundary faces given by their nodes (2 per edge in
+ * 2-D, 4 per quadrilateral in 3-D, face-local Q1 order) */
+int ins_set_traction(ins_ctx* ctx, int64_t n_faces, int nodes_per_face,
+                     const int32_t* faces, const double* traction);
+
+/* ---- assembly ------------------------------------------------------------- */
+/* stiffnessMatrixComputation + computeResidualForces + bodyForceComputation
+ * + neumannForceComputation into the device CSR values and RHS
+ * (base/asmb/StiffnessMatrix.hpp:52-73, ForceIntegrator.hpp:40-56,
+ * BodyForce.hpp:67-82, NeumannForce.hpp:36-63), deterministic: each CSR
+ * entry and RHS entry is summed over elements in ascending element order.
+ * precision = INS_FP64 or INS_FP32 (element integration arithmetic). */
+int ins_assemble(ins_ctx* ctx, int precision, int flags);
+/* Eigen3::debugLHS/debugRHS equivalents (base/solver/Eigen3.hpp:311-323) */
+int ins_get_csr_values(ins_ctx* ctx, double* val);
+int ins_get_rhs(ins_ctx* ctx, double* b);
+int ins_set_rhs(ins_ctx* ctx, const double* b);
+/* insertToRHS of a full-length vector (base/solver/Eigen3.hpp:112-124) */
+int ins_add_rhs(ins_ctx* ctx, const double* b);
+/* Eigen3::norm(): |b|_2 / N (base/solver/Eigen3.hpp:128-138) */
+int ins_rhs_norm(ins_ctx* ctx, double* norm);
+/* Upload a caller-assembled CSR (the SOLVER-concept insertToLHS path,
+ * base/solver/Eigen3.hpp:82-106 + finishAssembly :142-153). */
+int ins_set_csr(ins_ctx* ctx, int64_t n, const int32_t* row_ptr,
+                const int32_t* col_idx, const double* val);
+
+/* ---- solve ----------------------------------------------------------------- */
+/* Eigen3::cgSolve (base/solver/Eigen3.hpp:265-275): Jacobi-preconditioned
+ * CG from x0 = 0 until |r|/|b| <= rtol.  mode INS_SOLVE_FP64 or
+ * INS_SOLVE_MIXED; format INS_FMT_CSR or INS_FMT_SELL.  Returns
+ * INS_ERR_NOCONV if max_it iterations did not reach rtol (the solution of the
+ * last iterate is still available). */
+int ins_solve(ins_ctx* ctx, int mode, int format, double rtol, int max_it,
+              int* iters, double* relres);
+/* Eigen3::getValue for all rows (base/solver/Eigen3.hpp:293-296) */
+int ins_get_solution(ins_ctx* ctx, double* x);
+/* |solution|_2 / N as Eigen3::norm() reads it after cgSolve */
+int ins_solution_norm(ins_ctx* ctx, double* norm);
+/* setDoFsFromSolver / addToDoFsFromSolver (base/dof/Distribute.hpp:36-50,
+ * 141-215): active components get (or add) the solution, constrained
+ * components get their prescribed value. */
+int ins_distribute(ins_ctx* ctx, int op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

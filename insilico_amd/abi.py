@@ -1,0 +1,281 @@
+"""ctypes binding of the C ABI (include/insilico_hip.h) and a thin Python
+mirror of the reference's SOLVER-concept workflow.
+
+The library is loaded from ``insilico_amd/libinsilico_hip.so`` (built in
+tree).  If it is missing, loading raises: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libinsilico_hip.so")
+
+INS_OK = 0
+ERRORS = {-1: "ARG", -2: "HIP", -3: "OOM", -4: "RANGE", -5: "PATTERN",
+          -6: "BREAKDOWN", -7: "NOCONV", -8: "STATE", -9: "UNSUPPORTED"}
+LAPLACE, STVENANT, NEOHOOKE = 1, 2, 3
+FP64, FP32 = 0, 1
+ASM_MATRIX, ASM_RHS, ASM_RESIDUAL, ASM_INCREMENTAL = 1, 2, 4, 8
+BF_NONE, BF_CONST, BF_SINPROD = 0, 1, 2
+SOLVE_FP64, SOLVE_MIXED = 0, 1
+FMT_CSR, FMT_SELL = 0, 1
+DIST_SET, DIST_ADD = 0, 1
+
+# every symbol include/insilico_hip.h declares
+SYMBOLS = [
+    "ins_create", "ins_destroy", "ins_strerror", "ins_last_error", "ins_get_stats",
+    "ins_set_profiling", "ins_sync", "ins_set_mesh", "ins_set_dofs", "ins_set_prescribed",
+    "ins_scale_prescribed", "ins_set_dof_values", "ins_get_dof_values", "ins_build_pattern",
+    "ins_get_pattern", "ins_set_material", "ins_set_body_force", "ins_set_traction",
+    "ins_assemble", "ins_get_csr_values", "ins_get_rhs", "ins_set_rhs", "ins_add_rhs",
+    "ins_rhs_norm", "ins_set_csr", "ins_solve", "ins_get_solution", "ins_solution_norm",
+    "ins_distribute",
+]
+
+
+class Stats(C.Structure):
+    _fields_ = [("setup_ms", C.c_double), ("assemble_ms", C.c_double),
+                ("solve_ms", C.c_double), ("iterations", C.c_int),
+                ("refinements", C.c_int), ("relres", C.c_double),
+                ("n_active", C.c_int64), ("nnz", C.c_int64),
+                ("device_bytes", C.c_int64), ("spmv_ms", C.c_double),
+                ("spmv_calls", C.c_int64)]
+
+
+class InsError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(status, "?"), status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libinsilico_hip.so not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    sig = {
+        "ins_create": [C.POINTER(vp), i32, C.c_uint64],
+        "ins_destroy": [vp],
+        "ins_strerror": [i32],
+        "ins_last_error": [vp],
+        "ins_get_stats": [vp, C.POINTER(Stats)],
+        "ins_set_profiling": [vp, i32],
+        "ins_sync": [vp],
+        "ins_set_mesh": [vp, i32, i64, vp, i64, i32, vp],
+        "ins_set_dofs": [vp, i32, vp, i64],
+        "ins_set_prescribed": [vp, vp],
+        "ins_scale_prescribed": [vp, dbl],
+        "ins_set_dof_values": [vp, vp],
+        "ins_get_dof_values": [vp, vp],
+        "ins_build_pattern": [vp, C.POINTER(i64)],
+        "ins_get_pattern": [vp, vp, vp],
+        "ins_set_material": [vp, i32, i32, i32, vp],
+        "ins_set_body_force": [vp, i32, vp],
+        "ins_set_traction": [vp, i64, i32, vp, vp],
+        "ins_assemble": [vp, i32, i32],
+        "ins_get_csr_values": [vp, vp],
+        "ins_get_rhs": [vp, vp],
+        "ins_set_rhs": [vp, vp],
+        "ins_add_rhs": [vp, vp],
+        "ins_rhs_norm": [vp, C.POINTER(dbl)],
+        "ins_set_csr": [vp, i64, vp, vp, vp],
+        "ins_solve": [vp, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
+        "ins_get_solution": [vp, vp],
+        "ins_solution_norm": [vp, C.POINTER(dbl)],
+        "ins_distribute": [vp, i32],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_char_p if name in ("ins_strerror", "ins_last_error") else C.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One device context (owns all device state of one problem)."""
+
+    def __init__(self, device=0, budget_bytes=0):
+        self.L = load()
+        h = C.c_void_p()
+        s = self.L.ins_create(C.byref(h), device, budget_bytes)
+        if s != INS_OK:
+            raise InsError(s, self.L.ins_strerror(s).decode())
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            self.L.ins_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, s):
+        if s != INS_OK:
+            raise InsError(s, self.L.ins_last_error(self.h).decode())
+        return s
+
+    # -- discretisation -----------------------------------------------------
+    def set_mesh(self, xyz, conn):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        conn = np.ascontiguousarray(conn, np.int32)
+        self._chk(self.L.ins_set_mesh(self.h, xyz.shape[1], xyz.shape[0], _p(xyz),
+                                      conn.shape[0], conn.shape[1], _p(conn)))
+        self.n_nodes, self.dim = xyz.shape
+
+    def set_dofs(self, dpn, dof_id, n_active):
+        d = np.ascontiguousarray(dof_id, np.int32)
+        self._chk(self.L.ins_set_dofs(self.h, dpn, _p(d), n_active))
+        self.dpn, self.n_active = dpn, n_active
+
+    def set_prescribed(self, g):
+        g = None if g is None else np.ascontiguousarray(g, np.float64)
+        self._chk(self.L.ins_set_prescribed(self.h, _p(g)))
+
+    def scale_prescribed(self, f):
+        self._chk(self.L.ins_scale_prescribed(self.h, f))
+
+    def set_dof_values(self, u):
+        u = None if u is None else np.ascontiguousarray(u, np.float64)
+        self._chk(self.L.ins_set_dof_values(self.h, _p(u)))
+
+    def get_dof_values(self):
+        u = np.zeros(self.n_nodes * self.dpn)
+        self._chk(self.L.ins_get_dof_values(self.h, _p(u)))
+        return u
+
+    def build_pattern(self):
+        nnz = C.c_int64()
+        self._chk(self.L.ins_build_pattern(self.h, C.byref(nnz)))
+        self.nnz = nnz.value
+        return nnz.value
+
+    def get_pattern(self):
+        rp = np.zeros(self.n_active + 1, np.int32)
+        ci = np.zeros(self.nnz, np.int32)
+        self._chk(self.L.ins_get_pattern(self.h, _p(rp), _p(ci)))
+        return rp, ci
+
+    def set_material(self, kernel, nq, coef):
+        coef = np.ascontiguousarray(coef, np.float64)
+        if coef.ndim == 1:
+            coef = coef[:, None]
+        self._chk(self.L.ins_set_material(self.h, kernel, nq, coef.shape[1], _p(coef)))
+
+    def set_body_force(self, kind, params=(0.0, 0.0, 0.0, 0.0)):
+        p = np.zeros(4)
+        p[:len(params)] = params
+        self._chk(self.L.ins_set_body_force(self.h, kind, _p(p)))
+
+    def set_traction(self, faces, traction):
+        if faces is None or len(faces) == 0:
+            self._chk(self.L.ins_set_traction(self.h, 0, 0, None, None))
+            return
+        f = np.ascontiguousarray(faces, np.int32)
+        t = np.zeros(3)
+        t[:len(traction)] = traction
+        self._chk(self.L.ins_set_traction(self.h, f.shape[0], f.shape[1], _p(f), _p(t)))
+
+    # -- assembly -------------------------------------------------------------
+    def assemble(self, precision=FP64, flags=ASM_MATRIX | ASM_RHS):
+        self._chk(self.L.ins_assemble(self.h, precision, flags))
+
+    def get_csr_values(self):
+        v = np.zeros(self.nnz)
+        self._chk(self.L.ins_get_csr_values(self.h, _p(v)))
+        return v
+
+    def get_rhs(self):
+        b = np.zeros(self.n_active)
+        self._chk(self.L.ins_get_rhs(self.h, _p(b)))
+        return b
+
+    def set_rhs(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        self._chk(self.L.ins_set_rhs(self.h, _p(b)))
+
+    def add_rhs(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        self._chk(self.L.ins_add_rhs(self.h, _p(b)))
+
+    def rhs_norm(self):
+        v = C.c_double()
+        self._chk(self.L.ins_rhs_norm(self.h, C.byref(v)))
+        return v.value
+
+    def set_csr(self, row_ptr, col_idx, val):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        ci = np.ascontiguousarray(col_idx, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        self._chk(self.L.ins_set_csr(self.h, rp.shape[0] - 1, _p(rp), _p(ci), _p(v)))
+        self.n_active, self.nnz = rp.shape[0] - 1, ci.shape[0]
+
+    # -- solve ------------------------------------------------------------------
+    def solve(self, mode=SOLVE_FP64, fmt=FMT_CSR, rtol=1e-8, max_it=None, check=True):
+        it = C.c_int()
+        rr = C.c_double()
+        s = self.L.ins_solve(self.h, mode, fmt, rtol,
+                             max_it if max_it is not None else max(2 * self.n_active, 100),
+                             C.byref(it), C.byref(rr))
+        if check:
+            self._chk(s)
+        return it.value, rr.value, s
+
+    def get_solution(self):
+        x = np.zeros(self.n_active)
+        self._chk(self.L.ins_get_solution(self.h, _p(x)))
+        return x
+
+    def solution_norm(self):
+        v = C.c_double()
+        self._chk(self.L.ins_solution_norm(self.h, C.byref(v)))
+        return v.value
+
+    def distribute(self, op=DIST_SET):
+        self._chk(self.L.ins_distribute(self.h, op))
+
+    def stats(self):
+        st = Stats()
+        self._chk(self.L.ins_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def set_profiling(self, on):
+        self._chk(self.L.ins_set_profiling(self.h, int(on)))
+
+    def sync(self):
+        self._chk(self.L.ins_sync(self.h))
+
+    # -- convenience --------------------------------------------------------------
+    def load_problem(self, prob):
+        """Mesh, DoFs, prescribed values, kernel and loads of a Problem."""
+        self.set_mesh(prob.xyz, prob.conn)
+        self.set_dofs(prob.dpn, prob.dof_id, prob.n_active)
+        self.set_prescribed(prob.g)
+        self.set_material(prob.kernel, prob.nq, prob.coef)
+        self.set_body_force(prob.bf_kind, prob.bf)
+        self.set_traction(prob.faces, prob.traction)
+        return self.build_pattern()

@@ -1,0 +1,526 @@
+// C-ABI entry points (include/insilico_hip.h).  Host-side validation,
+// uploads and orchestration; all compute is in the device kernels of
+// setup.hip, assemble.hip and solve.hip.  No entry point aborts or throws:
+// failures return an INS_ERR_* code and leave a message in the context.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "common.hpp"
+
+namespace ins {
+
+int fail(Ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int dalloc_bytes(Ctx* c, void** p, size_t* cap_bytes, size_t bytes)
+{
+    if (c->budget && (uint64_t)(c->held + (int64_t)bytes) > c->budget)
+        return fail(c, INS_ERR_OOM, "device memory budget exceeded");
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, INS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    *p = q;
+    *cap_bytes = bytes;
+    c->held += (int64_t)bytes;
+    return INS_OK;
+}
+
+void dfree_bytes(Ctx* c, void** p, size_t* cap_bytes)
+{
+    if (*p) {
+        hipFree(*p);
+        c->held -= (int64_t)*cap_bytes;
+    }
+    *p = nullptr;
+    *cap_bytes = 0;
+}
+
+template <typename T>
+static int upload(Ctx* c, DArr<T>& a, const T* h, size_t n)
+{
+    INS_TRY(ensure(c, a, n));
+    if (n) INS_HIP_CHECK(c, hipMemcpyAsync(a.p, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return INS_OK;
+}
+
+template <typename T>
+static int download(Ctx* c, T* h, const DArr<T>& a, size_t n)
+{
+    if (n) INS_HIP_CHECK(c, hipMemcpyAsync(h, a.p, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return INS_OK;
+}
+
+static void release_all(Ctx* c)
+{
+    release(c, c->xyz); release(c, c->conn); release(c, c->dof_id); release(c, c->rowmap);
+    release(c, c->g); release(c, c->u); release(c, c->adj_ptr); release(c, c->adj);
+    release(c, c->nb_ptr); release(c, c->nb); release(c, c->nboff); release(c, c->row_ptr);
+    release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
+    release(c, c->faces); release(c, c->fadj_ptr); release(c, c->fadj); release(c, c->val64);
+    release(c, c->val32); release(c, c->b); release(c, c->dinv64); release(c, c->dinv32);
+    release(c, c->stage_ke); release(c, c->stage_fe); release(c, c->stage_ff);
+    release(c, c->sell.slice_ptr); release(c, c->sell.slice_len); release(c, c->sell.row_pat);
+    release(c, c->sell.pat_ptr); release(c, c->sell.pat_delta); release(c, c->sell.pat_len);
+    release(c, c->sell.v64); release(c, c->sell.v32);
+    release(c, c->x); release(c, c->r); release(c, c->p); release(c, c->q); release(c, c->t64);
+    release(c, c->x32); release(c, c->r32); release(c, c->p32); release(c, c->q32);
+    release(c, c->partial); release(c, c->counter); release(c, c->scal); release(c, c->dflag);
+}
+
+static void invalidate_pattern(Ctx* c)
+{
+    c->have_pattern = false;
+    c->have_matrix = false;
+    c->have_rhs = false;
+    c->have_solution = false;
+    c->sell.built = false;
+}
+
+}  // namespace ins
+
+using namespace ins;
+
+#define CTX_CHECK(ctx)                                            \
+    do {                                                          \
+        if (!(ctx)) return INS_ERR_ARG;                           \
+        hipError_t de_ = hipSetDevice((ctx)->device);             \
+        if (de_ != hipSuccess)                                    \
+            return fail((ctx), INS_ERR_HIP, "hipSetDevice failed"); \
+    } while (0)
+
+extern "C" {
+
+int ins_create(ins_ctx** out, int device, uint64_t hbm_budget_bytes)
+{
+    if (!out) return INS_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return INS_ERR_HIP;
+    if (device < 0 || device >= ndev) return INS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return INS_ERR_HIP;
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) return INS_ERR_OOM;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evp0) != hipSuccess || hipEventCreate(&c->evp1) != hipSuccess) {
+        delete c;
+        return INS_ERR_HIP;
+    }
+    *out = reinterpret_cast<ins_ctx*>(c);
+    return INS_OK;
+}
+
+int ins_destroy(ins_ctx* h)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    if (!c) return INS_ERR_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    release_all(c);
+    if (c->h_scal) hipHostFree(c->h_scal);
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipEventDestroy(c->evp0);
+    hipEventDestroy(c->evp1);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return INS_OK;
+}
+
+const char* ins_strerror(int s)
+{
+    switch (s) {
+    case INS_OK: return "ok";
+    case INS_ERR_ARG: return "invalid argument";
+    case INS_ERR_HIP: return "HIP runtime error";
+    case INS_ERR_OOM: return "out of device memory or over budget";
+    case INS_ERR_RANGE: return "index out of range";
+    case INS_ERR_PATTERN: return "entry outside the sparsity pattern";
+    case INS_ERR_BREAKDOWN: return "CG breakdown (matrix not SPD?)";
+    case INS_ERR_NOCONV: return "no convergence within the iteration limit";
+    case INS_ERR_STATE: return "call out of order";
+    case INS_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown status";
+    }
+}
+
+const char* ins_last_error(const ins_ctx* h)
+{
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    return c ? c->err.c_str() : "null context";
+}
+
+int ins_get_stats(const ins_ctx* h, ins_stats* out)
+{
+    const Ctx* c = reinterpret_cast<const Ctx*>(h);
+    if (!c || !out) return INS_ERR_ARG;
+    *out = c->stats;
+    out->n_active = c->n_active;
+    out->nnz = c->nnz;
+    out->device_bytes = c->held;
+    return INS_OK;
+}
+
+int ins_set_profiling(ins_ctx* h, int on)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    if (!c) return INS_ERR_ARG;
+    c->profiling = on != 0;
+    return INS_OK;
+}
+
+int ins_sync(ins_ctx* h)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return INS_OK;
+}
+
+int ins_set_mesh(ins_ctx* h, int dim, int64_t n_nodes, const double* xyz, int64_t n_elem,
+                 int npe, const int32_t* conn)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if ((dim != 2 && dim != 3) || n_nodes <= 0 || n_elem <= 0 || !xyz || !conn)
+        return fail(c, INS_ERR_ARG, "ins_set_mesh: bad arguments");
+    if (npe != (1 << dim))
+        return fail(c, INS_ERR_UNSUPPORTED, "only Q1 QUAD4/HEX8 elements are provided");
+    if (n_nodes > INT32_MAX || n_elem * npe > INT32_MAX)
+        return fail(c, INS_ERR_UNSUPPORTED, "mesh exceeds int32 indexing");
+    for (int64_t i = 0; i < n_elem * npe; i++)
+        if (conn[i] < 0 || conn[i] >= n_nodes)
+            return fail(c, INS_ERR_RANGE, "connectivity refers to a node out of range");
+    c->dim = dim;
+    c->npe = npe;
+    c->lg_npe = (npe == 8) ? 3 : 2;
+    c->n_nodes = n_nodes;
+    c->n_elem = n_elem;
+    INS_TRY(upload(c, c->xyz, xyz, (size_t)n_nodes * dim));
+    INS_TRY(upload(c, c->conn, conn, (size_t)n_elem * npe));
+    invalidate_pattern(c);
+    c->have_dofs = false;
+    c->n_faces = 0;
+    return INS_OK;
+}
+
+int ins_set_dofs(ins_ctx* h, int dpn, const int32_t* dof_id, int64_t n_active)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (c->n_nodes == 0) return fail(c, INS_ERR_STATE, "ins_set_dofs before ins_set_mesh");
+    if (dpn < 1 || dpn > 3 || !dof_id || n_active < 0)
+        return fail(c, INS_ERR_ARG, "ins_set_dofs: bad arguments");
+    const int64_t nd = c->n_nodes * dpn;
+    // ids must be a permutation of 0..n_active-1 (base/dof/numbering.hpp:45-66)
+    std::vector<uint8_t> seen((size_t)n_active, 0);
+    bool mono = true;
+    int32_t last = -1;
+    for (int64_t k = 0; k < nd; k++) {
+        const int32_t id = dof_id[k];
+        if (id < -1 || id >= n_active)
+            return fail(c, INS_ERR_RANGE, "DoF id out of range: " + std::to_string(id));
+        if (id < 0) continue;
+        if (seen[id]) return fail(c, INS_ERR_RANGE, "DoF id used twice: " + std::to_string(id));
+        seen[id] = 1;
+        if (id < last) mono = false;
+        last = id;
+    }
+    for (int64_t i = 0; i < n_active; i++)
+        if (!seen[i]) return fail(c, INS_ERR_RANGE, "DoF id missing: " + std::to_string(i));
+    c->dpn = dpn;
+    c->n_active = n_active;
+    c->monotone = mono;
+    INS_TRY(upload(c, c->dof_id, dof_id, (size_t)nd));
+    INS_TRY(ensure(c, c->g, (size_t)nd));
+    INS_TRY(ensure(c, c->u, (size_t)nd));
+    INS_HIP_CHECK(c, hipMemsetAsync(c->g.p, 0, nd * sizeof(double), c->stream));
+    INS_HIP_CHECK(c, hipMemsetAsync(c->u.p, 0, nd * sizeof(double), c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    invalidate_pattern(c);
+    c->have_dofs = true;
+    return INS_OK;
+}
+
+int ins_set_prescribed(ins_ctx* h, const double* g)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_set_prescribed before ins_set_dofs");
+    const int64_t nd = c->n_nodes * c->dpn;
+    if (!g) {
+        INS_HIP_CHECK(c, hipMemsetAsync(c->g.p, 0, nd * sizeof(double), c->stream));
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return INS_OK;
+    }
+    return upload(c, c->g, g, (size_t)nd);
+}
+
+int ins_scale_prescribed(ins_ctx* h, double factor)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_scale_prescribed before ins_set_dofs");
+    const int64_t nd = c->n_nodes * c->dpn;
+    std::vector<double> g((size_t)nd);
+    INS_TRY(download(c, g.data(), c->g, (size_t)nd));
+    for (auto& v : g) v *= factor;
+    return upload(c, c->g, g.data(), (size_t)nd);
+}
+
+int ins_set_dof_values(ins_ctx* h, const double* u)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_set_dof_values before ins_set_dofs");
+    const int64_t nd = c->n_nodes * c->dpn;
+    if (!u) {
+        INS_HIP_CHECK(c, hipMemsetAsync(c->u.p, 0, nd * sizeof(double), c->stream));
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return INS_OK;
+    }
+    return upload(c, c->u, u, (size_t)nd);
+}
+
+int ins_get_dof_values(ins_ctx* h, double* u)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_dofs || !u) return fail(c, INS_ERR_STATE, "no DoF values");
+    return download(c, u, c->u, (size_t)(c->n_nodes * c->dpn));
+}
+
+int ins_build_pattern(ins_ctx* h, int64_t* nnz_out)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_build_pattern before ins_set_dofs");
+    INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    INS_TRY(build_pattern(c));
+    INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    INS_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    INS_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.setup_ms = ms;
+    c->have_matrix = false;
+    c->sell.built = false;
+    if (nnz_out) *nnz_out = c->nnz;
+    return INS_OK;
+}
+
+int ins_get_pattern(ins_ctx* h, int32_t* row_ptr, int32_t* col_idx)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_pattern) return fail(c, INS_ERR_STATE, "no pattern");
+    if (row_ptr) INS_TRY(download(c, row_ptr, c->row_ptr, (size_t)c->n_active + 1));
+    if (col_idx) INS_TRY(download(c, col_idx, c->col, (size_t)c->nnz));
+    return INS_OK;
+}
+
+int ins_set_material(ins_ctx* h, int kernel, int nq, int ncoef, const double* coef)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (c->n_elem == 0) return fail(c, INS_ERR_STATE, "ins_set_material before ins_set_mesh");
+    if (kernel != INS_LAPLACE && kernel != INS_STVENANT && kernel != INS_NEOHOOKE)
+        return fail(c, INS_ERR_UNSUPPORTED, "unknown kernel");
+    if (nq < 1 || nq > 3) return fail(c, INS_ERR_UNSUPPORTED, "1..3 Gauss points per direction");
+    const int need = (kernel == INS_LAPLACE) ? 1 : 2;
+    if (ncoef < need || !coef) return fail(c, INS_ERR_ARG, "too few element coefficients");
+    c->kernel = kernel;
+    c->nq = nq;
+    c->ncoef = ncoef;
+    INS_TRY(upload(c, c->coef, coef, (size_t)c->n_elem * ncoef));
+    INS_TRY(make_tables(c));
+    c->have_matrix = false;
+    return INS_OK;
+}
+
+int ins_set_body_force(ins_ctx* h, int kind, const double* params)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (kind != INS_BF_NONE && kind != INS_BF_CONST && kind != INS_BF_SINPROD)
+        return fail(c, INS_ERR_ARG, "unknown body force kind");
+    if (kind != INS_BF_NONE && !params) return fail(c, INS_ERR_ARG, "body force parameters");
+    c->bf_kind = kind;
+    for (int i = 0; i < 4; i++) c->bf[i] = (kind != INS_BF_NONE) ? params[i] : 0.;
+    return INS_OK;
+}
+
+int ins_set_traction(ins_ctx* h, int64_t n_faces, int nfn, const int32_t* faces,
+                     const double* traction)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (c->n_nodes == 0) return fail(c, INS_ERR_STATE, "ins_set_traction before ins_set_mesh");
+    if (n_faces == 0) {
+        c->n_faces = 0;
+        return INS_OK;
+    }
+    if (n_faces < 0 || !faces || !traction || nfn != (1 << (c->dim - 1)))
+        return fail(c, INS_ERR_ARG, "ins_set_traction: bad arguments");
+    for (int64_t i = 0; i < n_faces * nfn; i++)
+        if (faces[i] < 0 || faces[i] >= c->n_nodes)
+            return fail(c, INS_ERR_RANGE, "face refers to a node out of range");
+    c->n_faces = n_faces;
+    c->nfn = nfn;
+    for (int i = 0; i < 3; i++) c->traction[i] = i < c->dim ? traction[i] : 0.;
+    INS_TRY(upload(c, c->faces, faces, (size_t)n_faces * nfn));
+    INS_TRY(build_adjacency(c, c->faces.p, n_faces, nfn, c->n_nodes, c->fadj_ptr, c->fadj,
+                            nullptr));
+    return INS_OK;
+}
+
+int ins_assemble(ins_ctx* h, int precision, int flags)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (precision != INS_FP64 && precision != INS_FP32)
+        return fail(c, INS_ERR_ARG, "precision must be INS_FP64 or INS_FP32");
+    if (!(flags & (INS_ASM_MATRIX | INS_ASM_RHS)))
+        return fail(c, INS_ERR_ARG, "nothing to assemble");
+    if (c->kernel == INS_LAPLACE && c->dpn > 1 && c->bf_kind != INS_BF_NONE && (flags & INS_ASM_RHS))
+        return fail(c, INS_ERR_UNSUPPORTED, "vector Laplace with a body force");
+    return assemble(c, precision, flags);
+}
+
+int ins_get_csr_values(ins_ctx* h, double* val)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_matrix || !val) return fail(c, INS_ERR_STATE, "no assembled matrix");
+    if (c->matrix_precision == INS_FP64) return download(c, val, c->val64, (size_t)c->nnz);
+    std::vector<float> tmp((size_t)c->nnz);
+    INS_TRY(download(c, tmp.data(), c->val32, (size_t)c->nnz));
+    for (int64_t i = 0; i < c->nnz; i++) val[i] = tmp[i];
+    return INS_OK;
+}
+
+int ins_get_rhs(ins_ctx* h, double* b)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_rhs || !b) return fail(c, INS_ERR_STATE, "no assembled rhs");
+    return download(c, b, c->b, (size_t)c->n_active);
+}
+
+int ins_set_rhs(ins_ctx* h, const double* b)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (c->n_active <= 0 || !b) return fail(c, INS_ERR_STATE, "ins_set_rhs before ins_set_dofs");
+    INS_TRY(upload(c, c->b, b, (size_t)c->n_active));
+    c->have_rhs = true;
+    c->have_solution = false;
+    return INS_OK;
+}
+
+int ins_add_rhs(ins_ctx* h, const double* b)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_rhs || !b) return fail(c, INS_ERR_STATE, "ins_add_rhs before the rhs exists");
+    std::vector<double> cur((size_t)c->n_active);
+    INS_TRY(download(c, cur.data(), c->b, (size_t)c->n_active));
+    for (int64_t i = 0; i < c->n_active; i++) cur[i] += b[i];
+    return upload(c, c->b, cur.data(), (size_t)c->n_active);
+}
+
+int ins_rhs_norm(ins_ctx* h, double* norm)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_rhs || !norm) return fail(c, INS_ERR_STATE, "no rhs");
+    double v = 0.;
+    INS_TRY(vec_norm2(c, c->b.p, c->n_active, &v));
+    *norm = c->n_active ? v / (double)c->n_active : 0.;
+    return INS_OK;
+}
+
+int ins_set_csr(ins_ctx* h, int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                const double* val)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (n <= 0 || !row_ptr || !col_idx || !val) return fail(c, INS_ERR_ARG, "ins_set_csr: bad arguments");
+    if (row_ptr[0] != 0) return fail(c, INS_ERR_ARG, "row_ptr[0] must be 0");
+    for (int64_t i = 0; i < n; i++)
+        if (row_ptr[i + 1] < row_ptr[i]) return fail(c, INS_ERR_ARG, "row_ptr not ascending");
+    const int64_t nnz = row_ptr[n];
+    for (int64_t i = 0; i < n; i++)
+        for (int32_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+            if (col_idx[k] < 0 || col_idx[k] >= n)
+                return fail(c, INS_ERR_RANGE, "column index out of bound");
+            if (k > row_ptr[i] && col_idx[k] <= col_idx[k - 1])
+                return fail(c, INS_ERR_ARG, "columns must be strictly ascending per row");
+        }
+    c->n_active = n;
+    c->nnz = nnz;
+    INS_TRY(upload(c, c->row_ptr, row_ptr, (size_t)n + 1));
+    INS_TRY(upload(c, c->col, col_idx, (size_t)nnz));
+    INS_TRY(upload(c, c->val64, val, (size_t)nnz));
+    std::vector<float> v32(val, val + nnz);
+    INS_TRY(upload(c, c->val32, v32.data(), (size_t)nnz));
+    c->matrix_precision = INS_FP64;
+    c->have_pattern = true;
+    c->have_matrix = true;
+    c->have_solution = false;
+    c->sell.built = false;
+    INS_TRY(compute_dinv(c));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return INS_OK;
+}
+
+int ins_solve(ins_ctx* h, int mode, int format, double rtol, int max_it, int* iters,
+              double* relres)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (mode != INS_SOLVE_FP64 && mode != INS_SOLVE_MIXED) return fail(c, INS_ERR_ARG, "solve mode");
+    if (format != INS_FMT_CSR && format != INS_FMT_SELL) return fail(c, INS_ERR_ARG, "format");
+    if (!(rtol > 0.) || max_it < 0) return fail(c, INS_ERR_ARG, "rtol/max_it");
+    return solve(c, mode, format, rtol, max_it, iters, relres);
+}
+
+int ins_get_solution(ins_ctx* h, double* x)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_solution || !x) return fail(c, INS_ERR_STATE, "no solution");
+    return download(c, x, c->x, (size_t)c->n_active);
+}
+
+int ins_solution_norm(ins_ctx* h, double* norm)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!c->have_solution || !norm) return fail(c, INS_ERR_STATE, "no solution");
+    double v = 0.;
+    INS_TRY(vec_norm2(c, c->x.p, c->n_active, &v));
+    *norm = c->n_active ? v / (double)c->n_active : 0.;
+    return INS_OK;
+}
+
+int ins_distribute(ins_ctx* h, int op)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (op != INS_DIST_SET && op != INS_DIST_ADD) return fail(c, INS_ERR_ARG, "distribute op");
+    return distribute(c, op);
+}
+
+}  // extern "C"

@@ -1,0 +1,1131 @@
+// Element integration and deterministic assembly on gfx950.
+//
+//  1. element kernels write every element matrix (and force vector) once
+//     into a staging buffer:
+//       - heat::Laplace: one element per lane, the Q1 local-gradient table of
+//         all quadrature points in LDS, 36 (3-D) / 10 (2-D) unique entries of
+//         the symmetric element matrix kept in registers, written in 64-element
+//         slabs ([e/64][entry][e%64]) so the stores are coalesced;
+//       - solid::HyperElastic: one wavefront per element; lanes < nqp build
+//         geometry, F, S and the Voigt C of their quadrature point, all 64
+//         lanes then form the effective elasticity A_iJkL of every point, and
+//         lane (M,N) accumulates the 3x3 block K_MN over the points.
+//  2. a gather kernel sums, for every CSR entry, the staged element
+//     contributions of the elements shared by the row node and the column
+//     node in ascending element order: the summation order of the
+//     reference's serial element loop (base/auxi/parallel.hpp:26-60 with
+//     base/solver/TripletContainer.hpp:305-352), with no float atomics.
+//  3. an RHS kernel does the same per active row for the Dirichlet lifting
+//     (base/asmb/assembleMatrix.hpp:91-95), element forces and face tractions.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace ins {
+
+namespace {
+
+constexpr int BLK = 256;
+
+struct Bf {
+    int kind;
+    double p[4];
+};
+
+template <int B, int E>
+struct IPow {
+    static constexpr int value = B * IPow<B, E - 1>::value;
+};
+template <int B>
+struct IPow<B, 0> {
+    static constexpr int value = 1;
+};
+
+// table layout: dphi[nq][npe][dim], phi[nq][npe], w[nq]
+template <int DIM, int NQ>
+struct Tab {
+    static constexpr int NPE = 1 << DIM;
+    static constexpr int DPHI = 0;
+    static constexpr int PHI = NQ * NPE * DIM;
+    static constexpr int W = PHI + NQ * NPE;
+    static constexpr int SIZE = W + NQ;
+};
+
+template <typename T>
+__device__ inline T body_force(const Bf& bf, const T* x, int dim, int comp)
+{
+    if (bf.kind == INS_BF_CONST) return (T)bf.p[comp];
+    if (bf.kind == INS_BF_SINPROD) {
+        T f = (T)bf.p[0];
+        for (int d = 0; d < dim; d++) f *= sin((T)bf.p[1 + d] * x[d]);
+        return f;
+    }
+    return (T)0;
+}
+
+// inverse of m (= J^T) by cofactors and det (Eigen computeInverseAndDetWithCheck)
+template <int DIM, typename T>
+__device__ inline T inverse_det(const T (&m)[DIM][DIM], T (&inv)[DIM][DIM])
+{
+    if constexpr (DIM == 2) {
+        const T det = m[0][0] * m[1][1] - m[1][0] * m[0][1];
+        const T id = (T)1 / det;
+        inv[0][0] = m[1][1] * id;
+        inv[1][0] = -m[1][0] * id;
+        inv[0][1] = -m[0][1] * id;
+        inv[1][1] = m[0][0] * id;
+        return det;
+    } else {
+        T cof[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+                const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                cof[i][j] = m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+            }
+        const T det = cof[0][0] * m[0][0] + cof[1][0] * m[1][0] + cof[2][0] * m[2][0];
+        const T id = (T)1 / det;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) inv[r][c] = cof[c][r] * id;
+        return det;
+    }
+}
+
+// physical gradients at one quadrature point; returns detJ
+// (base/geometry.hpp:143-177,420-445; base/LagrangeShapeFun.hpp:182-200)
+template <int DIM, int NPE, typename T>
+__device__ inline T phys_grad(const T (&X)[NPE][DIM], const T* dphi, T (&gx)[NPE][DIM])
+{
+    T J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int a = 0; a < DIM; a++) J[i][a] = (T)0;
+#pragma unroll
+    for (int n = 0; n < NPE; n++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int a = 0; a < DIM; a++) J[i][a] += X[n][i] * dphi[n * DIM + a];
+    T m[DIM][DIM], inv[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int a = 0; a < DIM; a++) m[i][a] = J[a][i];
+    const T det = inverse_det<DIM, T>(m, inv);
+#pragma unroll
+    for (int n = 0; n < NPE; n++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            T s = (T)0;
+#pragma unroll
+            for (int a = 0; a < DIM; a++) s += inv[i][a] * dphi[n * DIM + a];
+            gx[n][i] = s;
+        }
+    return det;
+}
+
+// ---------------------------------------------------------------------------
+// heat::Laplace element kernel (heat/Laplace.hpp:113-126 ->
+// base/kernel/Laplace.hpp:100-151): one element per lane.
+template <int DIM, int NQ1, typename T>
+__global__ __launch_bounds__(BLK) void k_elem_laplace(
+    int64_t E, const int32_t* __restrict__ conn, const double* __restrict__ xyz,
+    const double* __restrict__ coef, int ncoef, const T* __restrict__ tab, Bf bf,
+    T* __restrict__ ke, double* __restrict__ fe)
+{
+    constexpr int NPE = 1 << DIM;
+    constexpr int NQ = IPow<NQ1, DIM>::value;
+    constexpr int NK = NPE * (NPE + 1) / 2;
+    using TB = Tab<DIM, NQ>;
+    __shared__ T s_tab[TB::SIZE];
+    for (int i = threadIdx.x; i < TB::SIZE; i += blockDim.x) s_tab[i] = tab[i];
+    __syncthreads();
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        T X[NPE][DIM];
+#pragma unroll
+        for (int n = 0; n < NPE; n++) {
+            const int64_t node = conn[e * NPE + n];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) X[n][d] = (T)xyz[node * DIM + d];
+        }
+        const T kappa = (T)coef[e * ncoef];
+        T K[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) K[k] = (T)0;
+        T fb[NPE];
+#pragma unroll
+        for (int n = 0; n < NPE; n++) fb[n] = (T)0;
+        for (int q = 0; q < NQ; q++) {
+            const T* dphi = s_tab + TB::DPHI + q * NPE * DIM;
+            T gx[NPE][DIM];
+            const T det = phys_grad<DIM, NPE, T>(X, dphi, gx);
+            const T w = s_tab[TB::W + q];
+            const T scal = kappa * det * w;
+            int k = 0;
+#pragma unroll
+            for (int M = 0; M < NPE; M++)
+#pragma unroll
+                for (int N = M; N < NPE; N++) {
+                    T entry = (T)0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) entry += gx[M][d] * gx[N][d];
+                    entry *= scal;
+                    K[k++] += entry;
+                }
+            if (fe && bf.kind != INS_BF_NONE) {
+                const T* phi = s_tab + TB::PHI + q * NPE;
+                T x[3] = {0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < NPE; n++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) x[d] += X[n][d] * phi[n];
+                const T f = body_force<T>(bf, x, DIM, 0);
+#pragma unroll
+                for (int n = 0; n < NPE; n++) fb[n] += f * phi[n] * w * det;
+            }
+        }
+        T* dst = ke + (e >> 6) * (int64_t)NK * 64 + (e & 63);
+#pragma unroll
+        for (int k = 0; k < NK; k++) dst[k * 64] = K[k];
+        if (fe) {
+#pragma unroll
+            for (int n = 0; n < NPE; n++) fe[e * NPE + n] = (double)fb[n];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Material laws at one quadrature point: S and the Voigt C.
+// mat/hypel/StVenant.hpp:77-114, mat/hypel/NeoHookeanCompressible.hpp:77-172,
+// mat/TensorAlgebra.hpp:55-60 (det), :141-164 (Voigt).
+__device__ inline int voigt_idx(int i, int j)
+{
+    // map[(i+1)*(j+1)-1] with map = {0,3,4,1,-,5,-,-,2}
+    const int d = (i + 1) * (j + 1);
+    return d == 1 ? 0 : d == 2 ? 3 : d == 3 ? 4 : d == 4 ? 1 : d == 6 ? 5 : 2;
+}
+
+template <typename T>
+__device__ inline void inv3(const T m[3][3], T inv[3][3])
+{
+    T cof[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            cof[i][j] = m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+        }
+    const T det = cof[0][0] * m[0][0] + cof[1][0] * m[1][0] + cof[2][0] * m[2][0];
+    const T id = (T)1 / det;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) inv[r][c] = cof[c][r] * id;
+}
+
+template <int MAT, typename T>
+__device__ inline void material(T lam, T mu, const T F[3][3], T S[3][3], T C[6][6],
+                                bool wantC)
+{
+    T CG[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            T s = (T)0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) s += F[k][i] * F[k][j];
+            CG[i][j] = s;
+        }
+    if constexpr (MAT == INS_STVENANT) {
+        T E[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) E[i][j] = (T)0.5 * (CG[i][j] - (i == j ? (T)1 : (T)0));
+        const T trE = E[0][0] + E[1][1] + E[2][2];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                S[i][j] = lam * trE * (i == j ? (T)1 : (T)0) + (T)2 * mu * E[i][j];
+        if (wantC) {
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) C[i][j] = (T)0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) C[i][j] = lam;
+#pragma unroll
+            for (int i = 0; i < 3; i++) C[i][i] += (T)2 * mu;
+#pragma unroll
+            for (int i = 3; i < 6; i++) C[i][i] += mu;
+        }
+    } else {
+        T Ci[3][3];
+        inv3<T>(CG, Ci);
+        const T J = (F[0][0] * F[1][1] * F[2][2] + F[0][1] * F[1][2] * F[2][0] +
+                     F[0][2] * F[1][0] * F[2][1] - F[0][0] * F[1][2] * F[2][1] -
+                     F[0][1] * F[1][0] * F[2][2] - F[0][2] * F[1][1] * F[2][0]);
+        const T logJ = log(J);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                S[i][j] = (lam * logJ - mu) * Ci[i][j] + mu * (i == j ? (T)1 : (T)0);
+        if (wantC) {
+            const T fac2 = mu - lam * logJ;
+#pragma unroll
+            for (int A = 0; A < 3; A++)
+#pragma unroll
+                for (int B = A; B < 3; B++) {
+                    const int AB = voigt_idx(A, B);
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+#pragma unroll
+                        for (int D = c; D < 3; D++)
+                            C[AB][voigt_idx(c, D)] =
+                                lam * Ci[A][B] * Ci[c][D] +
+                                fac2 * (Ci[A][c] * Ci[B][D] + Ci[A][D] * Ci[B][c]);
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solid::HyperElastic element kernel (solid/HyperElastic.hpp:110-179 tangent,
+// :210-257 residual, :282-310 effective elasticity): one wavefront per
+// element, EPB elements per block.
+template <int DIM, int NQ1>
+struct ElLds {
+    static constexpr int NPE = 1 << DIM;
+    static constexpr int NQ = IPow<NQ1, DIM>::value;
+    static constexpr int S = DIM;
+    static constexpr int NEE = NPE * S;
+    static constexpr int NA = S * S * S * S;
+    static constexpr int X = 0;                 // [NPE][3]
+    static constexpr int U = X + NPE * 3;       // [NPE][3]
+    static constexpr int GX = U + NPE * 3;      // [NQ][NPE][DIM]
+    static constexpr int DW = GX + NQ * NPE * DIM;
+    static constexpr int F = DW + NQ;           // [NQ][9]
+    static constexpr int SS = F + NQ * 9;       // [NQ][9]
+    static constexpr int C = SS + NQ * 9;       // [NQ][36]
+    static constexpr int A = C + NQ * 36;       // [NQ][NA]
+    static constexpr int FR = A + NQ * NA;      // [NQ][NEE]
+    static constexpr int FB = FR + NQ * NEE;    // [NQ][NEE]
+    static constexpr int SIZE = FB + NQ * NEE;
+};
+
+template <int DIM, int NQ1, typename T, int MAT, bool WANT_K>
+__global__ __launch_bounds__(256) void k_elem_elastic(
+    int64_t E, const int32_t* __restrict__ conn, const double* __restrict__ xyz,
+    const double* __restrict__ coef, const double* __restrict__ u, int ncoef,
+    const T* __restrict__ tab, Bf bf, int want_resid, T* __restrict__ ke,
+    double* __restrict__ fe)
+{
+    using L = ElLds<DIM, NQ1>;
+    constexpr int NPE = L::NPE, NQ = L::NQ, S = L::S, NEE = L::NEE;
+    constexpr int EPB = (NQ <= 8) ? 4 : 1;
+    using TB = Tab<DIM, NQ>;
+    __shared__ T s_tab[TB::SIZE];
+    __shared__ T s_all[EPB * L::SIZE];
+    for (int i = threadIdx.x; i < TB::SIZE; i += blockDim.x) s_tab[i] = tab[i];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    T* sm = s_all + wave * L::SIZE;
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * EPB; base < E; base += (int64_t)gridDim.x * EPB) {
+        const int64_t e = base + wave;
+        const bool valid = (wave < EPB) && (e < E);
+        if (valid && lane < NPE * 3) {
+            const int n = lane / 3, d = lane % 3;
+            const int64_t node = conn[e * NPE + n];
+            sm[L::X + lane] = d < DIM ? (T)xyz[node * DIM + d] : (T)0;
+            sm[L::U + lane] = (d < S && u) ? (T)u[node * S + d] : (T)0;
+        }
+        __syncthreads();
+        // phase 1: one quadrature point per lane
+        if (valid && lane < NQ) {
+            const int q = lane;
+            T X[NPE][DIM];
+#pragma unroll
+            for (int n = 0; n < NPE; n++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) X[n][d] = sm[L::X + n * 3 + d];
+            T gx[NPE][DIM];
+            const T det = phys_grad<DIM, NPE, T>(X, s_tab + TB::DPHI + q * NPE * DIM, gx);
+            const T w = s_tab[TB::W + q];
+            const T dw = det * w;
+#pragma unroll
+            for (int n = 0; n < NPE; n++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) sm[L::GX + (q * NPE + n) * DIM + d] = gx[n][d];
+            sm[L::DW + q] = dw;
+            // F = I + (sum_f gx_f u_f^T)^T (solid/Deformation.hpp:26-46)
+            T GU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int f = 0; f < NPE; f++)
+#pragma unroll
+                for (int J = 0; J < DIM; J++)
+#pragma unroll
+                    for (int i = 0; i < S; i++) GU[J][i] += gx[f][J] * sm[L::U + f * 3 + i];
+            T F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+            for (int J = 0; J < DIM; J++)
+#pragma unroll
+                for (int i = 0; i < S; i++) F[i][J] += GU[J][i];
+            const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
+            T Sm[3][3], Cm[6][6];
+            material<MAT, T>(lam, mu, F, Sm, Cm, WANT_K);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    sm[L::F + q * 9 + i * 3 + j] = F[i][j];
+                    sm[L::SS + q * 9 + i * 3 + j] = Sm[i][j];
+                }
+            if (WANT_K) {
+#pragma unroll
+                for (int i = 0; i < 6; i++)
+#pragma unroll
+                    for (int j = 0; j < 6; j++) sm[L::C + q * 36 + i * 6 + j] = Cm[i][j];
+            }
+            // residual: sum_J P_iJ gx_M[J] * (detJ w), P = F S
+            if (want_resid) {
+                T P[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int J = 0; J < 3; J++) {
+                        T t = (T)0;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) t += F[i][k] * Sm[k][J];
+                        P[i][J] = t;
+                    }
+#pragma unroll
+                for (int M = 0; M < NPE; M++)
+#pragma unroll
+                    for (int i = 0; i < S; i++) {
+                        T s = (T)0;
+#pragma unroll
+                        for (int J = 0; J < S; J++) s += P[i][J] * gx[M][J];
+                        s *= dw;
+                        sm[L::FR + q * NEE + M * S + i] = s;
+                    }
+            }
+            if (bf.kind != INS_BF_NONE) {
+                const T* phi = s_tab + TB::PHI + q * NPE;
+                T x[3] = {0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < NPE; n++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) x[d] += X[n][d] * phi[n];
+#pragma unroll
+                for (int n = 0; n < NPE; n++)
+#pragma unroll
+                    for (int c = 0; c < S; c++)
+                        sm[L::FB + q * NEE + n * S + c] = body_force<T>(bf, x, DIM, c) * phi[n] * w * det;
+            }
+        }
+        __syncthreads();
+        if (WANT_K) {
+            // phase 2: A_iJkL = delta_ik S_JL + sum_AB F_iA C_{V(A,J)V(B,L)} F_kB
+            if (valid) {
+                for (int idx = lane; idx < NQ * L::NA; idx += 64) {
+                    const int q = idx / L::NA;
+                    int r = idx % L::NA;
+                    const int L_ = r % S; r /= S;
+                    const int k = r % S; r /= S;
+                    const int J = r % S; r /= S;
+                    const int i = r;
+                    const T* F = sm + L::F + q * 9;
+                    const T* Cq = sm + L::C + q * 36;
+                    T res = (i == k) ? sm[L::SS + q * 9 + J * 3 + L_] : (T)0;
+#pragma unroll
+                    for (int A = 0; A < S; A++) {
+                        const int v1 = voigt_idx(A, J);
+#pragma unroll
+                        for (int B = 0; B < S; B++)
+                            res += F[i * 3 + A] * Cq[v1 * 6 + voigt_idx(B, L_)] * F[k * 3 + B];
+                    }
+                    sm[L::A + idx] = res;
+                }
+            }
+            __syncthreads();
+            // phase 3: lane (M,N) accumulates the S x S block K_MN
+            if (valid && lane < NPE * NPE) {
+                const int M = lane / NPE, N = lane % NPE;
+                T K[S][S];
+#pragma unroll
+                for (int i = 0; i < S; i++)
+#pragma unroll
+                    for (int k = 0; k < S; k++) K[i][k] = (T)0;
+                for (int q = 0; q < NQ; q++) {
+                    T gM[S], gN[S];
+#pragma unroll
+                    for (int d = 0; d < S; d++) {
+                        gM[d] = sm[L::GX + (q * NPE + M) * DIM + d];
+                        gN[d] = sm[L::GX + (q * NPE + N) * DIM + d];
+                    }
+                    const T dw = sm[L::DW + q];
+                    const T* A = sm + L::A + q * L::NA;
+#pragma unroll
+                    for (int i = 0; i < S; i++)
+#pragma unroll
+                        for (int k = 0; k < S; k++) {
+                            T s = (T)0;
+#pragma unroll
+                            for (int J = 0; J < S; J++)
+#pragma unroll
+                                for (int L_ = 0; L_ < S; L_++)
+                                    s += gM[J] * A[((i * S + J) * S + k) * S + L_] * gN[L_];
+                            s *= dw;
+                            K[i][k] += s;
+                        }
+                }
+                T* dst = ke + e * (int64_t)(NEE * NEE) + (M * NPE + N) * S * S;
+#pragma unroll
+                for (int i = 0; i < S; i++)
+#pragma unroll
+                    for (int k = 0; k < S; k++) dst[i * S + k] = K[i][k];
+            }
+        }
+        // element force vector: body force minus internal force, each summed
+        // over the quadrature points in order
+        if (valid && fe && lane < NEE) {
+            T sb = (T)0, sr = (T)0;
+            if (bf.kind != INS_BF_NONE)
+                for (int q = 0; q < NQ; q++) sb += sm[L::FB + q * NEE + lane];
+            if (want_resid)
+                for (int q = 0; q < NQ; q++) sr += sm[L::FR + q * NEE + lane];
+            fe[e * NEE + lane] = (double)sb - (double)sr;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// staged element-matrix accessors
+template <typename T, int NPE>
+struct KeLaplace {
+    const T* p;
+    __device__ inline double get(int64_t e, int a, int cr, int b, int cc) const
+    {
+        if (cr != cc) return 0.;
+        const int i = a < b ? a : b, j = a < b ? b : a;
+        const int k = i * NPE - (i * (i - 1)) / 2 + (j - i);
+        constexpr int NK = NPE * (NPE + 1) / 2;
+        return (double)p[((e >> 6) * NK + k) * 64 + (e & 63)];
+    }
+};
+
+template <typename T, int NPE, int S>
+struct KeElastic {
+    const T* p;
+    __device__ inline double get(int64_t e, int a, int cr, int b, int cc) const
+    {
+        return (double)p[e * (int64_t)(NPE * S * NPE * S) + ((a * NPE + b) * S + cr) * S + cc];
+    }
+};
+
+// Gather: G lanes per node, lane j handles neighbour j of the node and the
+// S x S block of CSR entries coupling the two nodes.
+template <int NPE, int S, class KE, typename TA>
+__global__ __launch_bounds__(BLK) void k_gather(
+    int64_t n_nodes, int lgG, int lg_npe, const int32_t* __restrict__ adj_ptr,
+    const int32_t* __restrict__ adj, const int32_t* __restrict__ nb_ptr,
+    const int32_t* __restrict__ nb, const int32_t* __restrict__ nboff,
+    const int32_t* __restrict__ dof_id, const int32_t* __restrict__ row_ptr,
+    const int32_t* __restrict__ col, int monotone, KE ke, double* __restrict__ v64,
+    float* __restrict__ v32, int* __restrict__ err)
+{
+    const int G = 1 << lgG;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = gt >> lgG;
+    const int j0 = (int)(gt & (G - 1));
+    if (n >= n_nodes) return;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < S; c++) any |= dof_id[n * S + c] >= 0;
+    if (!any) return;
+    const int32_t nbs = nb_ptr[n], nbe = nb_ptr[n + 1];
+    const int32_t as = adj_ptr[n], ae = adj_ptr[n + 1];
+    const int mask = (1 << lg_npe) - 1;
+    for (int j = j0; j < nbe - nbs; j += G) {
+        const int64_t m = nb[nbs + j];
+        TA v[S][S];
+#pragma unroll
+        for (int a = 0; a < S; a++)
+#pragma unroll
+            for (int b = 0; b < S; b++) v[a][b] = (TA)0;
+        int32_t ia = as, ib = adj_ptr[m];
+        const int32_t eb = adj_ptr[m + 1];
+        while (ia < ae && ib < eb) {
+            const int32_t xa = adj[ia], xb = adj[ib];
+            const int32_t ea = xa >> lg_npe, ebb = xb >> lg_npe;
+            if (ea == ebb) {
+                const int la = xa & mask, lb = xb & mask;
+#pragma unroll
+                for (int cr = 0; cr < S; cr++)
+#pragma unroll
+                    for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(ea, la, cr, lb, cc);
+                ia++;
+                ib++;
+            } else if (ea < ebb) {
+                ia++;
+            } else {
+                ib++;
+            }
+        }
+        const int32_t off = nboff[nbs + j];
+#pragma unroll
+        for (int cr = 0; cr < S; cr++) {
+            const int32_t r = dof_id[n * S + cr];
+            if (r < 0) continue;
+            const int32_t rb = row_ptr[r], re = row_ptr[r + 1];
+            int k = 0;
+#pragma unroll
+            for (int cc = 0; cc < S; cc++) {
+                const int32_t cid = dof_id[m * S + cc];
+                if (cid < 0) continue;
+                int32_t pos;
+                if (monotone) {
+                    pos = rb + off + k;
+                } else {
+                    int32_t lo = rb, hi = re;
+                    while (lo < hi) {
+                        const int32_t mid = (lo + hi) >> 1;
+                        if (col[mid] < cid) lo = mid + 1; else hi = mid;
+                    }
+                    pos = lo;
+                    if (pos >= re || col[pos] != cid) {
+                        atomicOr(err, 1);
+                        continue;
+                    }
+                }
+                k++;
+                if (v64) v64[pos] = (double)v[cr][cc];
+                if (v32) v32[pos] = (float)v[cr][cc];
+            }
+        }
+    }
+}
+
+// RHS per node: lifting (per element: t = -sum_c g_c K_rc, then b += t),
+// element forces, face forces; each in ascending element / face order.
+template <int NPE, int S, class KE>
+__global__ __launch_bounds__(BLK) void k_rhs(
+    int64_t n_nodes, int lg_npe, const int32_t* __restrict__ adj_ptr,
+    const int32_t* __restrict__ adj, const int32_t* __restrict__ conn,
+    const int32_t* __restrict__ dof_id, const double* __restrict__ g,
+    const double* __restrict__ u, int incremental, int lift, KE ke,
+    const double* __restrict__ fe, int lg_nfn, const int32_t* __restrict__ fadj_ptr,
+    const int32_t* __restrict__ fadj, const double* __restrict__ ff, int nfn,
+    double* __restrict__ b)
+{
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t as = adj_ptr[n], ae = adj_ptr[n + 1];
+        const int mask = (1 << lg_npe) - 1;
+        for (int cr = 0; cr < S; cr++) {
+            const int32_t r = dof_id[n * S + cr];
+            if (r < 0) continue;
+            double acc = 0.;
+            if (lift) {
+                for (int32_t k = as; k < ae; k++) {
+                    const int64_t e = adj[k] >> lg_npe;
+                    const int a = adj[k] & mask;
+                    double t = 0.;
+                    for (int bnode = 0; bnode < NPE; bnode++) {
+                        const int64_t mnode = conn[e * NPE + bnode];
+#pragma unroll
+                        for (int cc = 0; cc < S; cc++) {
+                            const int64_t idx = mnode * S + cc;
+                            if (dof_id[idx] >= 0) continue;
+                            const double gv = incremental ? g[idx] - u[idx] : g[idx];
+                            t -= gv * ke.get(e, a, cr, bnode, cc);
+                        }
+                    }
+                    acc += t;
+                }
+            }
+            if (fe) {
+                for (int32_t k = as; k < ae; k++) {
+                    const int64_t e = adj[k] >> lg_npe;
+                    const int a = adj[k] & mask;
+                    acc += fe[e * (NPE * S) + a * S + cr];
+                }
+            }
+            if (ff) {
+                const int fmask = (1 << lg_nfn) - 1;
+                for (int32_t k = fadj_ptr[n]; k < fadj_ptr[n + 1]; k++) {
+                    const int64_t f = fadj[k] >> lg_nfn;
+                    const int a = fadj[k] & fmask;
+                    acc += ff[f * (nfn * S) + a * S + cr];
+                }
+            }
+            b[r] = acc;
+        }
+    }
+}
+
+// Neumann traction on faces (base/asmb/NeumannForce.hpp:132-175 with the
+// surface metric of base/geometry.hpp:314-346): one face per lane.
+template <int DIM>
+__global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restrict__ faces,
+                                              const double* __restrict__ xyz,
+                                              const double* __restrict__ ftab, int nqf,
+                                              double t0, double t1, double t2,
+                                              double* __restrict__ ff)
+{
+    constexpr int NFN = (DIM == 3) ? 4 : 2;
+    constexpr int LD = DIM - 1;
+    const double tr[3] = {t0, t1, t2};
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        double X[NFN][DIM];
+        for (int n = 0; n < NFN; n++) {
+            const int64_t node = faces[f * NFN + n];
+            for (int d = 0; d < DIM; d++) X[n][d] = xyz[node * DIM + d];
+        }
+        double out[NFN * DIM];
+        for (int i = 0; i < NFN * DIM; i++) out[i] = 0.;
+        for (int q = 0; q < nqf; q++) {
+            const double* dphi = ftab + q * NFN * LD;
+            const double* phi = ftab + nqf * NFN * LD + q * NFN;
+            const double w = ftab[nqf * NFN * (LD + 1) + q];
+            double detG;
+            if constexpr (DIM == 3) {
+                double J[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+                for (int n = 0; n < 4; n++)
+                    for (int i = 0; i < 3; i++)
+                        for (int a = 0; a < 2; a++) J[i][a] += X[n][i] * dphi[n * 2 + a];
+                const double cx = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+                const double cy = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                const double cz = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+                detG = sqrt(cx * cx + cy * cy + cz * cz);
+            } else {
+                const double tx = X[1][0] - X[0][0], ty = X[1][1] - X[0][1];
+                detG = sqrt(tx * tx + ty * ty);
+            }
+            for (int n = 0; n < NFN; n++)
+                for (int c = 0; c < DIM; c++) out[n * DIM + c] += tr[c] * phi[n] * w * detG;
+        }
+        for (int i = 0; i < NFN * DIM; i++) ff[f * NFN * DIM + i] = out[i];
+    }
+}
+
+// Jacobi: inverse diagonal of each row (Eigen DiagonalPreconditioner:
+// 1/a_ii, or 1 where a_ii = 0)
+__global__ void k_dinv(int64_t n, const int32_t* __restrict__ row_ptr,
+                       const int32_t* __restrict__ col, const double* __restrict__ v64,
+                       const float* __restrict__ v32, double* __restrict__ d64,
+                       float* __restrict__ d32)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = row_ptr[r], hi = row_ptr[r + 1];
+        const int32_t end = hi;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (col[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        double d = 0.;
+        if (lo < end && col[lo] == r) d = v64 ? v64[lo] : (double)v32[lo];
+        const double id = d != 0. ? 1. / d : 1.;
+        d64[r] = id;
+        d32[r] = (float)id;
+    }
+}
+
+// ---- host-side restatement of the rules for the tables -----------------------
+// base/quad/GaussLegendre.hpp:114-134 constants and order;
+// base/quad/TensorProduct.hpp:123-175 tensor index (first coordinate fastest).
+int gauss1d(int np, double* w, double* x)
+{
+    if (np == 1) { w[0] = 1.0; x[0] = 0.5; return 0; }
+    if (np == 2) {
+        w[0] = 0.5; x[0] = 0.788675134594813;
+        w[1] = 0.5; x[1] = 0.211324865405187;
+        return 0;
+    }
+    if (np == 3) {
+        w[0] = 0.277777777777777; x[0] = 0.887298334620741;
+        w[1] = 0.444444444444444; x[1] = 0.5;
+        w[2] = 0.277777777777777; x[2] = 0.112701665379259;
+        return 0;
+    }
+    return -1;
+}
+
+int tensor_rule(int dim, int np, std::vector<double>& w, std::vector<double>& xi)
+{
+    double w1[4], x1[4];
+    if (gauss1d(np, w1, x1)) return -1;
+    int nq = 1;
+    for (int d = 0; d < dim; d++) nq *= np;
+    w.assign(nq, 0.);
+    xi.assign((size_t)nq * dim, 0.);
+    for (int q = 0; q < nq; q++) {
+        int r = q;
+        const int i0 = r % np;
+        r /= np;
+        double ww = w1[i0];
+        xi[q * dim] = x1[i0];
+        for (int d = 1; d < dim; d++) {
+            const int id = r % np;
+            r /= np;
+            ww *= w1[id];
+            xi[q * dim + d] = x1[id];
+        }
+        w[q] = ww;
+    }
+    return nq;
+}
+
+// Q1 shape functions in hierarchic order: base/sfun/Lagrange1D.ipp:46-58,
+// base/sfun/TensorProduct.hpp:40-57,225-320, HierarchicOrder.hpp:201-272.
+void q1_shape(int dim, const double* xi, double* phi, double* dphi)
+{
+    static const int H2[4] = {0, 1, 3, 2};
+    static const int H3[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    double f[3][2], g[3][2];
+    for (int d = 0; d < dim; d++) {
+        f[d][0] = 1. - xi[d];
+        f[d][1] = xi[d];
+        g[d][0] = -1.;
+        g[d][1] = 1.;
+    }
+    double f2[4], g2[4][2];
+    int c = 0;
+    for (int o = 0; o < 2; o++)
+        for (int i = 0; i < 2; i++) {
+            f2[c] = f[0][i] * f[1][o];
+            g2[c][0] = f[1][o] * g[0][i];
+            g2[c][1] = g[1][o] * f[0][i];
+            c++;
+        }
+    if (dim == 1) {
+        phi[0] = f[0][0]; phi[1] = f[0][1];
+        dphi[0] = g[0][0]; dphi[1] = g[0][1];
+        return;
+    }
+    if (dim == 2) {
+        for (int n = 0; n < 4; n++) {
+            phi[H2[n]] = f2[n];
+            dphi[H2[n] * 2] = g2[n][0];
+            dphi[H2[n] * 2 + 1] = g2[n][1];
+        }
+        return;
+    }
+    c = 0;
+    double lf[8], lg[8][3];
+    for (int o = 0; o < 2; o++)
+        for (int i = 0; i < 4; i++) {
+            lf[c] = f2[i] * f[2][o];
+            lg[c][0] = f[2][o] * g2[i][0];
+            lg[c][1] = f[2][o] * g2[i][1];
+            lg[c][2] = g[2][o] * f2[i];
+            c++;
+        }
+    for (int n = 0; n < 8; n++) {
+        phi[H3[n]] = lf[n];
+        for (int d = 0; d < 3; d++) dphi[H3[n] * 3 + d] = lg[n][d];
+    }
+}
+
+// volume table followed by the face table (dphi, phi, w on faces)
+void fill_tables(int dim, int nq1, std::vector<double>& t, int* face_off, int* nqf)
+{
+    std::vector<double> w, xi;
+    const int nq = tensor_rule(dim, nq1, w, xi);
+    const int npe = 1 << dim;
+    t.assign((size_t)nq * npe * (dim + 1) + nq, 0.);
+    for (int q = 0; q < nq; q++) {
+        double phi[8], dphi[24];
+        q1_shape(dim, &xi[q * dim], phi, dphi);
+        for (int n = 0; n < npe; n++) {
+            for (int d = 0; d < dim; d++) t[(q * npe + n) * dim + d] = dphi[n * dim + d];
+            t[(size_t)nq * npe * dim + q * npe + n] = phi[n];
+        }
+        t[(size_t)nq * npe * (dim + 1) + q] = w[q];
+    }
+    *face_off = (int)t.size();
+    std::vector<double> wf, xf;
+    const int ld = dim - 1;
+    const int nqfv = tensor_rule(ld, nq1, wf, xf);
+    const int nfn = 1 << ld;
+    const size_t base = t.size();
+    t.resize(base + (size_t)nqfv * nfn * (ld + 1) + nqfv, 0.);
+    for (int q = 0; q < nqfv; q++) {
+        double phi[4], dphi[8];
+        q1_shape(ld, &xf[q * ld], phi, dphi);
+        for (int n = 0; n < nfn; n++) {
+            for (int d = 0; d < ld; d++) t[base + (q * nfn + n) * ld + d] = dphi[n * ld + d];
+            t[base + (size_t)nqfv * nfn * ld + q * nfn + n] = phi[n];
+        }
+        t[base + (size_t)nqfv * nfn * (ld + 1) + q] = wf[q];
+    }
+    *nqf = nqfv;
+}
+
+template <int DIM, typename T>
+int launch_laplace(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
+{
+    const unsigned g = grid_for(c->n_elem, BLK);
+    switch (c->nq) {
+    case 1: k_elem_laplace<DIM, 1, T><<<g, BLK, 0, c->stream>>>(c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 2: k_elem_laplace<DIM, 2, T><<<g, BLK, 0, c->stream>>>(c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 3: k_elem_laplace<DIM, 3, T><<<g, BLK, 0, c->stream>>>(c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
+    }
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
+
+template <int DIM, int NQ1, typename T, int MAT>
+int launch_elastic_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+{
+    constexpr int NQ = IPow<NQ1, DIM>::value;
+    constexpr int EPB = (NQ <= 8) ? 4 : 1;
+    const unsigned g = grid_for((c->n_elem + EPB - 1) / EPB, 1, 1u << 22);
+    if (want_k)
+        k_elem_elastic<DIM, NQ1, T, MAT, true><<<g, 64 * EPB, 0, c->stream>>>(
+            c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf, want_r, ke, fe);
+    else
+        k_elem_elastic<DIM, NQ1, T, MAT, false><<<g, 64 * EPB, 0, c->stream>>>(
+            c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf, want_r, ke, fe);
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
+
+template <int DIM, typename T, int MAT>
+int launch_elastic(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+{
+    switch (c->nq) {
+    case 1: return launch_elastic_nq<DIM, 1, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 2: return launch_elastic_nq<DIM, 2, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 3: return launch_elastic_nq<DIM, 3, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
+    }
+}
+
+template <typename T, int DIM>
+int launch_elastic_mat(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+{
+    if (c->kernel == INS_STVENANT)
+        return launch_elastic<DIM, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
+    return launch_elastic<DIM, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
+}
+
+int lg2(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+template <int NPE, int S, class KE>
+int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int incremental,
+                      const double* fe, const double* ff, int precision, int* derr)
+{
+    if (do_matrix) {
+        int G = 1;
+        while (G < c->max_nb && G < 64) G <<= 1;
+        const int lgG = lg2(G);
+        const int64_t threads = c->n_nodes << lgG;
+        const unsigned g = (unsigned)((threads + BLK - 1) / BLK);
+        double* v64 = (precision == INS_FP64) ? c->val64.p : nullptr;
+        float* v32 = c->val32.p;
+        if (precision == INS_FP64)
+            k_gather<NPE, S, KE, double><<<g, BLK, 0, c->stream>>>(
+                c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->monotone ? 1 : 0, ke, v64,
+                v32, derr);
+        else
+            k_gather<NPE, S, KE, float><<<g, BLK, 0, c->stream>>>(
+                c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->monotone ? 1 : 0, ke, nullptr,
+                v32, derr);
+        INS_HIP_CHECK(c, hipGetLastError());
+    }
+    if (do_rhs) {
+        k_rhs<NPE, S, KE><<<grid_for(c->n_nodes, BLK), BLK, 0, c->stream>>>(
+            c->n_nodes, c->lg_npe, c->adj_ptr.p, c->adj.p, c->conn.p, c->dof_id.p, c->g.p, c->u.p,
+            incremental, lift, ke, fe, lg2(c->nfn > 0 ? c->nfn : 1), c->fadj_ptr.p, c->fadj.p, ff,
+            c->nfn, c->b.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+    }
+    return INS_OK;
+}
+
+}  // namespace
+
+int make_tables(Ctx* c)
+{
+    std::vector<double> t;
+    fill_tables(c->dim, c->nq, t, &c->tab_face_off, &c->tab_nqf);
+    std::vector<float> tf(t.begin(), t.end());
+    INS_TRY(ensure(c, c->tab64, t.size()));
+    INS_TRY(ensure(c, c->tab32, t.size()));
+    INS_HIP_CHECK(c, hipMemcpyAsync(c->tab64.p, t.data(), t.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, c->stream));
+    INS_HIP_CHECK(c, hipMemcpyAsync(c->tab32.p, tf.data(), tf.size() * sizeof(float),
+                                    hipMemcpyHostToDevice, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return INS_OK;
+}
+
+int compute_dinv(Ctx* c)
+{
+    INS_TRY(ensure(c, c->dinv64, c->n_active));
+    INS_TRY(ensure(c, c->dinv32, c->n_active));
+    const double* v64 = (c->matrix_precision == INS_FP64) ? c->val64.p : nullptr;
+    k_dinv<<<grid_for(c->n_active, BLK), BLK, 0, c->stream>>>(
+        c->n_active, c->row_ptr.p, c->col.p, v64, c->val32.p, c->dinv64.p, c->dinv32.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
+
+int assemble(Ctx* c, int precision, int flags)
+{
+    if (!c->have_pattern) return fail(c, INS_ERR_STATE, "ins_assemble before ins_build_pattern");
+    if (c->kernel == 0) return fail(c, INS_ERR_STATE, "ins_assemble before ins_set_material");
+    const bool elastic = (c->kernel == INS_STVENANT || c->kernel == INS_NEOHOOKE);
+    if (elastic && c->dpn != c->dim)
+        return fail(c, INS_ERR_UNSUPPORTED, "elasticity needs dofs_per_node == dim");
+    if (!elastic && c->dpn != 1 && c->dpn != c->dim)
+        return fail(c, INS_ERR_UNSUPPORTED, "Laplace supports 1 or dim components");
+    const int do_matrix = flags & INS_ASM_MATRIX;
+    const int do_rhs = flags & INS_ASM_RHS;
+    const int want_r = (flags & INS_ASM_RESIDUAL) && elastic;
+    const int incremental = (flags & INS_ASM_INCREMENTAL) ? 1 : 0;
+    const int npe = c->npe, s = c->dpn;
+    const int64_t E = c->n_elem;
+    Bf bf;
+    bf.kind = c->bf_kind;
+    for (int i = 0; i < 4; i++) bf.p[i] = c->bf[i];
+
+    INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    int* derr = nullptr;
+    INS_TRY(ensure(c, c->dflag, 4));
+    derr = c->dflag.p;
+    INS_HIP_CHECK(c, hipMemsetAsync(derr, 0, sizeof(int), c->stream));
+
+    // staging
+    const size_t esize = (precision == INS_FP64) ? sizeof(double) : sizeof(float);
+    size_t ke_elems;
+    if (!elastic)
+        ke_elems = (size_t)((E + 63) / 64) * 64 * (npe * (npe + 1) / 2);
+    else
+        ke_elems = (size_t)E * (npe * s) * (npe * s);
+    if (do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
+    const bool need_fe = do_rhs && (c->bf_kind != INS_BF_NONE || want_r);
+    if (need_fe) INS_TRY(ensure(c, c->stage_fe, (size_t)E * npe * s));
+    if (do_matrix) {
+        if (precision == INS_FP64) INS_TRY(ensure(c, c->val64, c->nnz));
+        INS_TRY(ensure(c, c->val32, c->nnz));
+    }
+    if (do_rhs) {
+        INS_TRY(ensure(c, c->b, c->n_active));
+        INS_TRY(ensure(c, c->g, (size_t)c->n_nodes * s));
+        INS_TRY(ensure(c, c->u, (size_t)c->n_nodes * s));
+    }
+    double* fe = need_fe ? c->stage_fe.p : nullptr;
+
+    // 1. element kernels
+    if (!elastic) {
+        if (do_matrix || need_fe) {
+            if (!do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
+            if (precision == INS_FP64) {
+                if (c->dim == 3) INS_TRY((launch_laplace<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
+                else INS_TRY((launch_laplace<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
+            } else {
+                if (c->dim == 3) INS_TRY((launch_laplace<3, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
+                else INS_TRY((launch_laplace<2, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
+            }
+        }
+    } else {
+        if (do_matrix) {
+            if (precision == INS_FP64) {
+                if (c->dim == 3) INS_TRY((launch_elastic_mat<double, 3>(c, c->tab64.p, bf, 1, want_r && need_fe, (double*)c->stage_ke.p, fe)));
+                else INS_TRY((launch_elastic_mat<double, 2>(c, c->tab64.p, bf, 1, want_r && need_fe, (double*)c->stage_ke.p, fe)));
+            } else {
+                if (c->dim == 3) INS_TRY((launch_elastic_mat<float, 3>(c, c->tab32.p, bf, 1, 0, (float*)c->stage_ke.p, nullptr)));
+                else INS_TRY((launch_elastic_mat<float, 2>(c, c->tab32.p, bf, 1, 0, (float*)c->stage_ke.p, nullptr)));
+            }
+        }
+        // forces always in FP64 (the Newton residual stays FP64); a
+        // residual-only pass when the tangent was FP32 or not requested
+        if (need_fe && (precision != INS_FP64 || !do_matrix)) {
+            if (c->dim == 3) INS_TRY((launch_elastic_mat<double, 3>(c, c->tab64.p, bf, 0, want_r, nullptr, fe)));
+            else INS_TRY((launch_elastic_mat<double, 2>(c, c->tab64.p, bf, 0, want_r, nullptr, fe)));
+        }
+    }
+    // faces
+    double* ff = nullptr;
+    if (do_rhs && c->n_faces > 0) {
+        INS_TRY(ensure(c, c->stage_ff, (size_t)c->n_faces * c->nfn * s));
+        ff = c->stage_ff.p;
+        const unsigned g = grid_for(c->n_faces, BLK);
+        if (c->dim == 3)
+            k_face<3><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, c->xyz.p, c->tab64.p + c->tab_face_off, c->tab_nqf, c->traction[0], c->traction[1], c->traction[2], ff);
+        else
+            k_face<2><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, c->xyz.p, c->tab64.p + c->tab_face_off, c->tab_nqf, c->traction[0], c->traction[1], c->traction[2], ff);
+        INS_HIP_CHECK(c, hipGetLastError());
+    }
+
+    // 2./3. gather into CSR and RHS
+    const int lift = do_matrix;
+    if (!elastic) {
+        if (precision == INS_FP64) {
+            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
+        } else {
+            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
+        }
+    } else {
+        if (precision == INS_FP64) {
+            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<double, 8, 3>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<double, 4, 2>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+        } else {
+            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<float, 8, 3>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<float, 4, 2>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+        }
+    }
+    if (do_matrix) {
+        c->matrix_precision = precision;
+        INS_TRY(compute_dinv(c));
+        c->sell.built = false;
+    }
+    INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    int herr = 0;
+    INS_HIP_CHECK(c, hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    INS_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.assemble_ms = ms;
+    if (herr) return fail(c, INS_ERR_PATTERN, "element entry outside the pattern");
+    if (do_matrix) c->have_matrix = true;
+    if (do_rhs) c->have_rhs = true;
+    c->have_solution = false;
+    return INS_OK;
+}
+
+}  // namespace ins

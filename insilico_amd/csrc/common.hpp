@@ -1,0 +1,212 @@
+// Internal declarations of the gfx950 finite-element library: the context
+// that owns all device state, device buffers with budget accounting, and the
+// launchers each translation unit provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "insilico_hip.h"
+
+namespace ins {
+
+// -------------------------------------------------------------------------
+// status helpers
+struct Ctx;
+int fail(Ctx* c, int code, const std::string& msg);
+
+#define INS_HIP_CHECK(ctx, call)                                              \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::ins::fail((ctx), INS_ERR_HIP,                            \
+                               std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define INS_TRY(expr)                                                         \
+    do {                                                                      \
+        int s_ = (expr);                                                      \
+        if (s_ != INS_OK) return s_;                                          \
+    } while (0)
+
+// -------------------------------------------------------------------------
+// device array owned by a context; grows on demand, counts against budget
+template <typename T>
+struct DArr {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+};
+
+// device-side scalars of one PCG solve
+struct Scal {
+    double rho;      // r.z of the current direction
+    double pq;       // p.Ap
+    double alpha;
+    double beta;
+    double rr;       // |r|^2 of the recursive residual
+    double bb;       // |b|^2
+    double thr;      // rtol^2 |b|^2
+    double tt;       // |b - A x|^2 (true residual)
+    double rz_new;
+    int done;        // 1 = stop (converged, breakdown or empty rhs)
+    int breakdown;
+    int iters;       // SpMVs performed inside the loop
+    int pad;
+};
+
+// SELL-64 matrix: 64-row slices, values column-major inside a slice, per row
+// an index into a dictionary of column-offset lists (col = row + delta).
+struct Sell {
+    int64_t n = 0;
+    int64_t nslices = 0;
+    int64_t nvals = 0;       // padded value count
+    int npat = 0;
+    DArr<int64_t> slice_ptr; // nslices+1 value offsets (multiples of 64)
+    DArr<int32_t> slice_len; // max row length per slice
+    DArr<int32_t> row_pat;   // n: pattern id
+    DArr<int32_t> pat_ptr;   // npat+1
+    DArr<int32_t> pat_delta; // concatenated column offsets
+    DArr<int32_t> pat_len;   // npat
+    DArr<double> v64;
+    DArr<float> v32;
+    bool ok = false;         // dictionary small enough to be worth it
+    bool built = false;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    int64_t held = 0;
+    std::string err;
+    bool profiling = false;
+    ins_stats stats{};
+
+    // mesh
+    int dim = 0, npe = 0, lg_npe = 0;
+    int64_t n_nodes = 0, n_elem = 0;
+    DArr<double> xyz;
+    DArr<int32_t> conn;
+
+    // dofs
+    int dpn = 0;
+    int64_t n_active = 0;
+    bool monotone = true;
+    DArr<int32_t> dof_id;   // n_nodes*dpn
+    DArr<int32_t> rowmap;   // n_active: node*dpn + comp of each row
+    DArr<double> g;         // prescribed
+    DArr<double> u;         // current values
+    bool have_dofs = false;
+
+    // pattern
+    bool have_pattern = false;
+    int64_t nnz = 0;
+    int max_nb = 0;         // max neighbour nodes of a node
+    DArr<int32_t> adj_ptr, adj;   // node -> element*npe + local, by element
+    DArr<int32_t> nb_ptr, nb, nboff;
+    DArr<int32_t> row_ptr, col;
+
+    // material / loads
+    int kernel = 0, nq = 2, ncoef = 0;
+    DArr<double> coef;
+    DArr<double> tab64;     // quadrature tables (dphi, phi, w)
+    DArr<float> tab32;
+    int tab_face_off = 0, tab_nqf = 0;
+    int bf_kind = INS_BF_NONE;
+    double bf[4] = {0, 0, 0, 0};
+    int64_t n_faces = 0;
+    int nfn = 0;
+    double traction[3] = {0, 0, 0};
+    DArr<int32_t> faces, fadj_ptr, fadj;
+
+    // assembly products
+    bool have_matrix = false, have_rhs = false;
+    int matrix_precision = INS_FP64;
+    DArr<double> val64;
+    DArr<float> val32;
+    DArr<double> b;
+    DArr<double> dinv64;
+    DArr<float> dinv32;
+    DArr<uint8_t> stage_ke;  // element matrices (type depends on precision)
+    DArr<double> stage_fe;   // element force vectors (FP64)
+    DArr<double> stage_ff;   // face force vectors
+    Sell sell;
+
+    // solver state
+    DArr<double> x, r, p, q, t64;
+    DArr<float> x32, r32, p32, q32;
+    DArr<double> partial;
+    DArr<unsigned> counter;
+    DArr<Scal> scal;
+    Scal* h_scal = nullptr;  // pinned
+    DArr<int> dflag;
+    bool have_solution = false;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+};
+
+// allocation with budget accounting (count in elements)
+int dalloc_bytes(Ctx* c, void** p, size_t* cap_bytes, size_t bytes);
+void dfree_bytes(Ctx* c, void** p, size_t* cap_bytes);
+
+template <typename T>
+int ensure(Ctx* c, DArr<T>& a, size_t n)
+{
+    if (n <= a.cap && a.p) return INS_OK;
+    size_t capb = a.cap * sizeof(T);
+    void* p = a.p;
+    dfree_bytes(c, &p, &capb);
+    a.p = nullptr;
+    a.cap = 0;
+    size_t want = (n ? n : 1) * sizeof(T);
+    int s = dalloc_bytes(c, &p, &capb, want);
+    if (s != INS_OK) return s;
+    a.p = static_cast<T*>(p);
+    a.cap = capb / sizeof(T);
+    return INS_OK;
+}
+
+template <typename T>
+void release(Ctx* c, DArr<T>& a)
+{
+    size_t capb = a.cap * sizeof(T);
+    void* p = a.p;
+    dfree_bytes(c, &p, &capb);
+    a.p = nullptr;
+    a.cap = 0;
+}
+
+// grid helper
+inline unsigned grid_for(int64_t work, int block, unsigned cap = 65535u * 8)
+{
+    int64_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (int64_t)cap) g = cap;
+    return (unsigned)g;
+}
+
+// ---- launchers provided by the translation units ---------------------------
+// scan.hip
+int exclusive_scan_i32(Ctx* c, const int32_t* in, int32_t* out, int64_t n,
+                       int64_t* total);
+// setup.hip
+int build_adjacency(Ctx* c, const int32_t* items, int64_t n_items, int per_item,
+                    int64_t n_nodes, DArr<int32_t>& ptr, DArr<int32_t>& list,
+                    int* max_deg);
+int build_pattern(Ctx* c);
+// assemble.hip
+int make_tables(Ctx* c);
+int assemble(Ctx* c, int precision, int flags);
+int compute_dinv(Ctx* c);
+// solve.hip
+int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
+          double* relres);
+int build_sell(Ctx* c);
+int distribute(Ctx* c, int op);
+int vec_norm2(Ctx* c, const double* v, int64_t n, double* out);
+
+}  // namespace ins

@@ -1,0 +1,279 @@
+// Setup kernels: node->item adjacency (items are elements or Neumann faces),
+// node neighbour lists and the CSR pattern of the reduced system.
+//
+// The pattern is the one base/solver/TripletContainer.hpp:159-301 builds:
+// every (active row, active col) pair of every element, explicit zeros kept,
+// columns ascending.  With iso-parametric Q1 DoFs (DoF id = node id,
+// base/dof/copyConnectivity.hpp:34-60) two components couple exactly when
+// their nodes share an element, so the row of (node n, comp c) is the list of
+// active components of the sorted neighbour nodes of n.
+#include "common.hpp"
+
+namespace ins {
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr int NB_BLK = 64;    // threads per block of the neighbour kernels
+constexpr int MAXNB = 96;     // neighbour nodes per node kept in LDS
+
+__global__ void k_adj_count(const int32_t* items, int64_t nent, int64_t n_nodes,
+                            int32_t* cnt, int* err)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nent;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = items[t];
+        if (v < 0 || v >= n_nodes) {
+            atomicOr(err, 1);
+            continue;
+        }
+        atomicAdd(&cnt[v], 1);
+    }
+}
+
+__global__ void k_adj_fill(const int32_t* items, int64_t nent, int64_t n_nodes,
+                           const int32_t* ptr, int32_t* fillpos, int32_t* list)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nent;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = items[t];
+        if (v < 0 || v >= n_nodes) continue;
+        const int32_t slot = atomicAdd(&fillpos[v], 1);
+        list[ptr[v] + slot] = (int32_t)t;  // t = item*per_item + local
+    }
+}
+
+// sort every node's list ascending (entries are unique); lists are short
+__global__ void k_adj_sort(int64_t n_nodes, const int32_t* ptr, int32_t* list,
+                           int* maxdeg)
+{
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = ptr[n], e = ptr[n + 1];
+        for (int32_t i = b + 1; i < e; i++) {
+            const int32_t key = list[i];
+            int32_t j = i - 1;
+            while (j >= b && list[j] > key) {
+                list[j + 1] = list[j];
+                j--;
+            }
+            list[j + 1] = key;
+        }
+        atomicMax(maxdeg, e - b);
+    }
+}
+
+// Collect the sorted unique neighbour nodes of node n into buf (LDS).
+template <int NPE>
+__device__ inline int collect_neighbours(int64_t n, const int32_t* adj_ptr,
+                                         const int32_t* adj, const int32_t* conn,
+                                         int32_t* buf, int lg_npe, int* err)
+{
+    int cnt = 0;
+    const int32_t b = adj_ptr[n], e = adj_ptr[n + 1];
+    for (int32_t k = b; k < e; k++) {
+        const int32_t el = adj[k] >> lg_npe;
+        for (int v = 0; v < NPE; v++) {
+            const int32_t m = conn[(int64_t)el * NPE + v];
+            // insert m keeping buf sorted and unique
+            int pos = cnt;
+            while (pos > 0 && buf[pos - 1] > m) pos--;
+            if (pos > 0 && buf[pos - 1] == m) continue;
+            if (cnt >= MAXNB) {
+                atomicOr(err, 2);
+                return cnt;
+            }
+            for (int i = cnt; i > pos; i--) buf[i] = buf[i - 1];
+            buf[pos] = m;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+__device__ inline int n_active_comps(const int32_t* dof_id, int64_t node, int dpn)
+{
+    int a = 0;
+    for (int c = 0; c < dpn; c++) a += dof_id[node * dpn + c] >= 0;
+    return a;
+}
+
+// pass 1: neighbour count and row length of every node
+template <int NPE>
+__global__ __launch_bounds__(NB_BLK) void k_nb_count(
+    int64_t n_nodes, const int32_t* adj_ptr, const int32_t* adj, const int32_t* conn,
+    const int32_t* dof_id, int dpn, int lg_npe, int32_t* nb_cnt, int32_t* row_len,
+    int* err, int* maxnb)
+{
+    __shared__ int32_t sbuf[NB_BLK * MAXNB];
+    int32_t* buf = sbuf + threadIdx.x * MAXNB;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int cnt = collect_neighbours<NPE>(n, adj_ptr, adj, conn, buf, lg_npe, err);
+        int len = 0;
+        for (int j = 0; j < cnt; j++) len += n_active_comps(dof_id, buf[j], dpn);
+        nb_cnt[n] = cnt;
+        atomicMax(maxnb, cnt);
+        for (int c = 0; c < dpn; c++) {
+            const int32_t r = dof_id[n * dpn + c];
+            if (r >= 0) row_len[r] = len;
+        }
+    }
+}
+
+// pass 2: neighbour lists, per-neighbour column offsets, column indices
+template <int NPE>
+__global__ __launch_bounds__(NB_BLK) void k_nb_fill(
+    int64_t n_nodes, const int32_t* adj_ptr, const int32_t* adj, const int32_t* conn,
+    const int32_t* dof_id, int dpn, int lg_npe, const int32_t* nb_ptr, int32_t* nb,
+    int32_t* nboff, const int32_t* row_ptr, int32_t* col, int* err)
+{
+    __shared__ int32_t sbuf[NB_BLK * MAXNB];
+    int32_t* buf = sbuf + threadIdx.x * MAXNB;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int cnt = collect_neighbours<NPE>(n, adj_ptr, adj, conn, buf, lg_npe, err);
+        const int32_t nbs = nb_ptr[n];
+        int off = 0;
+        for (int j = 0; j < cnt; j++) {
+            nb[nbs + j] = buf[j];
+            nboff[nbs + j] = off;
+            off += n_active_comps(dof_id, buf[j], dpn);
+        }
+        for (int c = 0; c < dpn; c++) {
+            const int32_t r = dof_id[n * dpn + c];
+            if (r < 0) continue;
+            int32_t pos = row_ptr[r];
+            for (int j = 0; j < cnt; j++) {
+                const int64_t m = buf[j];
+                for (int d = 0; d < dpn; d++) {
+                    const int32_t cid = dof_id[m * dpn + d];
+                    if (cid >= 0) col[pos++] = cid;
+                }
+            }
+        }
+    }
+}
+
+// rows whose column ids are not ascending (non-monotone numbering): sort
+__global__ void k_sort_rows(int64_t n, const int32_t* row_ptr, int32_t* col)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = row_ptr[r], e = row_ptr[r + 1];
+        for (int32_t i = b + 1; i < e; i++) {
+            const int32_t key = col[i];
+            int32_t j = i - 1;
+            while (j >= b && col[j] > key) {
+                col[j + 1] = col[j];
+                j--;
+            }
+            col[j + 1] = key;
+        }
+    }
+}
+
+}  // namespace
+
+int build_adjacency(Ctx* c, const int32_t* items, int64_t n_items, int per_item,
+                    int64_t n_nodes, DArr<int32_t>& ptr, DArr<int32_t>& list,
+                    int* max_deg)
+{
+    const int64_t nent = n_items * per_item;
+    INS_TRY(ensure(c, ptr, n_nodes + 1));
+    INS_TRY(ensure(c, list, nent));
+    int32_t* cnt = nullptr;
+    int* flags = nullptr;
+    INS_HIP_CHECK(c, hipMalloc((void**)&cnt, (n_nodes + 1) * sizeof(int32_t)));
+    INS_HIP_CHECK(c, hipMalloc((void**)&flags, 2 * sizeof(int)));
+    INS_HIP_CHECK(c, hipMemsetAsync(cnt, 0, (n_nodes + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMemsetAsync(flags, 0, 2 * sizeof(int), c->stream));
+    k_adj_count<<<grid_for(nent, BLK), BLK, 0, c->stream>>>(items, nent, n_nodes, cnt, flags);
+    INS_HIP_CHECK(c, hipGetLastError());
+    int64_t total = 0;
+    int s = exclusive_scan_i32(c, cnt, ptr.p, n_nodes, &total);
+    if (s != INS_OK) return s;
+    INS_HIP_CHECK(c, hipMemsetAsync(cnt, 0, (n_nodes + 1) * sizeof(int32_t), c->stream));
+    k_adj_fill<<<grid_for(nent, BLK), BLK, 0, c->stream>>>(items, nent, n_nodes, ptr.p, cnt,
+                                                          list.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    k_adj_sort<<<grid_for(n_nodes, BLK), BLK, 0, c->stream>>>(n_nodes, ptr.p, list.p,
+                                                             flags + 1);
+    INS_HIP_CHECK(c, hipGetLastError());
+    int hflags[2];
+    INS_HIP_CHECK(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost,
+                                    c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    hipFree(cnt);
+    hipFree(flags);
+    if (hflags[0]) return fail(c, INS_ERR_RANGE, "node id out of range in connectivity");
+    if (max_deg) *max_deg = hflags[1];
+    return INS_OK;
+}
+
+int build_pattern(Ctx* c)
+{
+    const int64_t nn = c->n_nodes;
+    int maxadj = 0;
+    INS_TRY(build_adjacency(c, c->conn.p, c->n_elem, c->npe, nn, c->adj_ptr, c->adj, &maxadj));
+    if (maxadj * c->npe > 4 * MAXNB)
+        return fail(c, INS_ERR_UNSUPPORTED, "node belongs to too many elements");
+
+    int32_t* nb_cnt = nullptr;
+    int32_t* row_len = nullptr;
+    int* flags = nullptr;
+    INS_HIP_CHECK(c, hipMalloc((void**)&nb_cnt, (nn + 1) * sizeof(int32_t)));
+    INS_HIP_CHECK(c, hipMalloc((void**)&row_len, (c->n_active + 1) * sizeof(int32_t)));
+    INS_HIP_CHECK(c, hipMalloc((void**)&flags, 2 * sizeof(int)));
+    INS_HIP_CHECK(c, hipMemsetAsync(flags, 0, 2 * sizeof(int), c->stream));
+    INS_HIP_CHECK(c, hipMemsetAsync(row_len, 0, (c->n_active + 1) * sizeof(int32_t), c->stream));
+    const unsigned g = grid_for(nn, NB_BLK, 1u << 20);
+    if (c->npe == 8)
+        k_nb_count<8><<<g, NB_BLK, 0, c->stream>>>(nn, c->adj_ptr.p, c->adj.p, c->conn.p,
+                                                    c->dof_id.p, c->dpn, c->lg_npe, nb_cnt,
+                                                    row_len, flags, flags + 1);
+    else
+        k_nb_count<4><<<g, NB_BLK, 0, c->stream>>>(nn, c->adj_ptr.p, c->adj.p, c->conn.p,
+                                                    c->dof_id.p, c->dpn, c->lg_npe, nb_cnt,
+                                                    row_len, flags, flags + 1);
+    INS_HIP_CHECK(c, hipGetLastError());
+    int64_t nbtot = 0, nnz = 0;
+    INS_TRY(ensure(c, c->nb_ptr, nn + 1));
+    INS_TRY(exclusive_scan_i32(c, nb_cnt, c->nb_ptr.p, nn, &nbtot));
+    INS_TRY(ensure(c, c->row_ptr, c->n_active + 1));
+    INS_TRY(exclusive_scan_i32(c, row_len, c->row_ptr.p, c->n_active, &nnz));
+    INS_TRY(ensure(c, c->nb, nbtot));
+    INS_TRY(ensure(c, c->nboff, nbtot));
+    INS_TRY(ensure(c, c->col, nnz));
+    if (c->npe == 8)
+        k_nb_fill<8><<<g, NB_BLK, 0, c->stream>>>(nn, c->adj_ptr.p, c->adj.p, c->conn.p,
+                                                   c->dof_id.p, c->dpn, c->lg_npe, c->nb_ptr.p,
+                                                   c->nb.p, c->nboff.p, c->row_ptr.p, c->col.p,
+                                                   flags);
+    else
+        k_nb_fill<4><<<g, NB_BLK, 0, c->stream>>>(nn, c->adj_ptr.p, c->adj.p, c->conn.p,
+                                                   c->dof_id.p, c->dpn, c->lg_npe, c->nb_ptr.p,
+                                                   c->nb.p, c->nboff.p, c->row_ptr.p, c->col.p,
+                                                   flags);
+    INS_HIP_CHECK(c, hipGetLastError());
+    if (!c->monotone) {
+        k_sort_rows<<<grid_for(c->n_active, BLK), BLK, 0, c->stream>>>(c->n_active,
+                                                                       c->row_ptr.p, c->col.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+    }
+    int hflags[2];
+    INS_HIP_CHECK(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost,
+                                    c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    hipFree(nb_cnt);
+    hipFree(row_len);
+    hipFree(flags);
+    if (hflags[0] & 2)
+        return fail(c, INS_ERR_UNSUPPORTED, "more neighbour nodes than supported (96)");
+    c->max_nb = hflags[1];
+    c->nnz = nnz;
+    c->have_pattern = true;
+    return INS_OK;
+}
+
+}  // namespace ins

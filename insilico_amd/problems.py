@@ -1,0 +1,288 @@
+"""Host-side problem setup: the reference's mesh, DoF and boundary-condition
+conventions, flattened into the arrays the C ABI takes.
+
+This restates the setup half of the reference that defines the index space of
+the hot path (it runs once per problem, on the host):
+
+* ``unit_cube``: tools/meshGeneration/unitCube/unitCube.hpp:85-106 (nodes,
+  first index fastest, ``x = h*i`` with ``h = 1/e``) and :217-257 (elements,
+  first index fastest, lexicographic corners reordered to the hierarchic
+  vertex order of base/mesh/HierarchicOrder.hpp:201-272,345-392).
+* ``boundary_faces``: faces of the element face table
+  (base/mesh/ElementFaces.hpp:284-286 QUAD, :422-434 HEX) that belong to
+  exactly one element (base/mesh/MeshBoundary.hpp:57-62,
+  base/mesh/createBoundaryFromUnstructured.hpp:51-109).
+* ``number_dofs``: iso-parametric DoF = node (base/dof/copyConnectivity.hpp
+  :34-60), components interleaved, CONSTRAINED components skipped
+  (base/dof/numbering.hpp:45-66).
+* random element fields: SplitMix64 + Box-Muller, drawn in element order.
+"""
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+LAPLACE, STVENANT, NEOHOOKE = 1, 2, 3
+BF_NONE, BF_CONST, BF_SINPROD = 0, 1, 2
+
+# lexicographic corner -> hierarchic position (HierarchicOrder.hpp, K = 1)
+_H2 = np.array([0, 1, 3, 2])
+_H3 = np.array([0, 1, 3, 2, 4, 5, 7, 6])
+# element faces in hierarchic vertex numbers (ElementFaces.hpp)
+HEX_FACES = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4],
+                      [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]])
+QUAD_EDGES = np.array([[0, 1], [1, 2], [2, 3], [3, 0]])
+
+
+@dataclass
+class Problem:
+    """Flat description of one boundary-value problem."""
+    dim: int
+    xyz: np.ndarray            # (n_nodes, dim) float64
+    conn: np.ndarray           # (n_elem, 2**dim) int32, hierarchic order
+    dpn: int                   # dofs per node
+    dof_id: np.ndarray         # (n_nodes*dpn,) int32, -1 = constrained
+    n_active: int
+    g: np.ndarray              # (n_nodes*dpn,) prescribed values
+    kernel: int
+    coef: np.ndarray           # (n_elem, ncoef)
+    nq: int = 2
+    bf_kind: int = BF_NONE
+    bf: tuple = (0.0, 0.0, 0.0, 0.0)
+    faces: np.ndarray = None   # (n_faces, 2**(dim-1)) int32
+    traction: tuple = (0.0, 0.0, 0.0)
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_nodes(self):
+        return self.xyz.shape[0]
+
+    @property
+    def n_elem(self):
+        return self.conn.shape[0]
+
+
+# ---------------------------------------------------------------------------
+# mesh
+def unit_cube(dim, n1, n2=None, n3=None):
+    """Structured Q1 mesh of the unit square/cube, reference conventions."""
+    n2 = n1 if n2 is None else n2
+    n3 = n1 if n3 is None else n3
+    h = [1.0 / float(n1), 1.0 / float(n2), 1.0 / float(n3)]
+    cnt = [n1 + 1, n2 + 1, (n3 + 1) if dim == 3 else 1]
+    axes = [h[d] * np.arange(cnt[d], dtype=np.float64) for d in range(dim)]
+    if dim == 2:
+        Y, X = np.meshgrid(axes[1], axes[0], indexing="ij")
+        xyz = np.stack([X.ravel(), Y.ravel()], axis=1)
+    else:
+        Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    N1, N2 = cnt[0], cnt[1]
+    if dim == 2:
+        i2, i1 = np.meshgrid(np.arange(n2), np.arange(n1), indexing="ij")
+        base = (i1 + i2 * N1).ravel()
+        lex = [base + d2 * N1 + d1 for d2 in (0, 1) for d1 in (0, 1)]
+        H = _H2
+    else:
+        i3, i2, i1 = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+        base = (i1 + i2 * N1 + i3 * N1 * N2).ravel()
+        lex = [base + d3 * N1 * N2 + d2 * N1 + d1
+               for d3 in (0, 1) for d2 in (0, 1) for d1 in (0, 1)]
+        H = _H3
+    conn = np.empty((base.shape[0], 2 ** dim), np.int32)
+    for v, col in enumerate(lex):
+        conn[:, H[v]] = col
+    return np.ascontiguousarray(xyz), conn
+
+
+def round_smf(xyz):
+    """Coordinates as the reference's SMF writer prints them: default ostream
+    precision, 6 significant digits (tools/meshGeneration/meshGeneration.hpp
+    :32-36)."""
+    f = np.vectorize(lambda v: float("%g" % v))
+    return f(xyz).astype(np.float64)
+
+
+def read_smf(path, dim):
+    """Minimal reader of the SMF mesh format (base/io/smf/Reader.hpp:97-312):
+    header lines, ``nNodes nElements``, node coordinates, connectivity."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh if ln.strip() and not ln.startswith("#")]
+    k = 0
+    while lines[k].startswith("!"):
+        k += 1
+    nn, ne = (int(v) for v in lines[k].split())
+    k += 1
+    xyz = np.array([[float(v) for v in lines[k + i].split()[:dim]] for i in range(nn)])
+    k += nn
+    conn = np.array([[int(v) for v in lines[k + i].split()] for i in range(ne)], np.int32)
+    return xyz, conn
+
+
+def boundary_faces(conn, dim):
+    """(element, face) pairs on the boundary and their face node lists."""
+    table = HEX_FACES if dim == 3 else QUAD_EDGES
+    nf = table.shape[0]
+    fn = conn[:, table].reshape(-1, table.shape[1])          # (E*nf, nfn)
+    key = np.sort(fn, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    on = cnt[inv.ravel()] == 1
+    idx = np.nonzero(on)[0]
+    return idx // nf, idx % nf, fn[idx].astype(np.int32)
+
+
+def number_dofs(constrained):
+    """numberDoFsConsecutively: constrained (n_nodes, dpn) bool -> dof ids."""
+    c = np.asarray(constrained, bool).ravel()
+    ids = np.full(c.shape[0], -1, np.int64)
+    act = ~c
+    ids[act] = np.arange(int(act.sum()))
+    return ids.astype(np.int32), int(act.sum())
+
+
+# ---------------------------------------------------------------------------
+# random element fields
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, n, offset=0):
+    """Outputs offset..offset+n-1 of SplitMix64 seeded with `seed`."""
+    with np.errstate(over="ignore"):
+        k = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, n, offset=0):
+    return (splitmix64(seed, n, offset) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal(seed, n):
+    """Box-Muller over consecutive uniform pairs, one N(0,1) per element."""
+    u = uniform01(seed, 2 * n)
+    u1, u2 = u[0::2], u[1::2]
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def lame(E, nu):
+    """mat/Lame.hpp:28-40"""
+    lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu)
+    mu = E / 2.0 / (1.0 + nu)
+    return lam, mu
+
+
+# ---------------------------------------------------------------------------
+# benchmark configurations
+def poisson2d(n=64):
+    """-Lap u = 2 pi^2 sin(pi x) sin(pi y) on the unit square, u = 0 on the
+    boundary, Q1, 2x2 Gauss (exact solution sin(pi x) sin(pi y))."""
+    xyz, conn = unit_cube(2, n)
+    _, _, bf = boundary_faces(conn, 2)
+    con = np.zeros(xyz.shape[0], bool)
+    con[np.unique(bf)] = True
+    dof, na = number_dofs(con[:, None])
+    pi = math.pi
+    return Problem(2, xyz, conn, 1, dof, na, np.zeros(xyz.shape[0]), LAPLACE,
+                   np.ones((conn.shape[0], 1)), 2, BF_SINPROD, (2 * pi * pi, pi, pi, 0.0),
+                   name="poisson2d-%d" % n)
+
+
+def diffusion3d(n=128, seed=12345):
+    """-div(k grad u) = 1 on the unit cube, k_e = exp(g_e), g_e ~ N(0,1) per
+    element, u = 0 on x = 0, natural elsewhere; Q1, 2x2x2 Gauss."""
+    xyz, conn = unit_cube(3, n)
+    _, _, bf = boundary_faces(conn, 3)
+    bn = np.unique(bf)
+    con = np.zeros(xyz.shape[0], bool)
+    con[bn[np.abs(xyz[bn, 0]) < 1e-12]] = True
+    dof, na = number_dofs(con[:, None])
+    k = np.exp(normal(seed, conn.shape[0]))[:, None]
+    return Problem(3, xyz, conn, 1, dof, na, np.zeros(xyz.shape[0]), LAPLACE, k, 2,
+                   BF_CONST, (1.0, 0.0, 0.0, 0.0), name="diffusion3d-%d" % n,
+                   meta={"seed": seed})
+
+
+def elasticity3d(n=96, seed=12345, nu=0.3):
+    """Linear (StVenant at u = 0) elasticity on the unit cube, E_e ~ U[0.5, 2]
+    per element, clamped at z = 0, traction (0, 0, -0.01) on z = 1."""
+    xyz, conn = unit_cube(3, n)
+    _, _, bf = boundary_faces(conn, 3)
+    bn = np.unique(bf)
+    con = np.zeros((xyz.shape[0], 3), bool)
+    con[bn[np.abs(xyz[bn, 2]) < 1e-12], :] = True
+    dof, na = number_dofs(con)
+    E = 0.5 + 1.5 * uniform01(seed, conn.shape[0])
+    lam, mu = lame(E, nu)
+    top = bf[np.all(np.abs(xyz[bf, 2] - 1.0) < 1e-12, axis=1)]
+    return Problem(3, xyz, conn, 3, dof, na, np.zeros(xyz.shape[0] * 3), STVENANT,
+                   np.stack([lam, mu], axis=1), 2, faces=top, traction=(0.0, 0.0, -0.01),
+                   name="elasticity3d-%d" % n, meta={"seed": seed})
+
+
+def neohooke3d(n=64, mu=1.0, lam=10.0):
+    """Compressible neo-Hookean block: clamped at x = 0, u_x prescribed on
+    x = 1 (reference/06-elastic/PulledSheet.hpp: left face all components,
+    right face component 0); the prescribed value is set per load step."""
+    xyz, conn = unit_cube(3, n)
+    _, _, bf = boundary_faces(conn, 3)
+    bn = np.unique(bf)
+    con = np.zeros((xyz.shape[0], 3), bool)
+    left = bn[np.abs(xyz[bn, 0]) < 1e-5]
+    right = bn[np.abs(xyz[bn, 0] - 1.0) < 1e-5]
+    con[left, :] = True
+    con[right, 0] = True
+    dof, na = number_dofs(con)
+    coef = np.tile(np.array([[lam, mu]]), (conn.shape[0], 1))
+    g = np.zeros(xyz.shape[0] * 3)
+    p = Problem(3, xyz, conn, 3, dof, na, g, NEOHOOKE, coef, 2, name="neohooke3d-%d" % n)
+    p.meta["right_x"] = right
+    return p
+
+
+def fundsol_elasto(dim, lam, mu, y, x, direc):
+    """Kelvin solution U(y, x) dir (base/auxi/FundamentalSolution.hpp:160-200),
+    evaluated at the points y (m, dim) for the source x."""
+    G = mu
+    nu = lam / 2.0 / (lam + mu)
+    surf = 2.0 * (dim - 1.0) * math.pi
+    fac1 = 1.0 / (4.0 * surf * G * (1.0 - nu))
+    fac2 = 3.0 - 4.0 * nu
+    R = np.asarray(y, np.float64) - np.asarray(x, np.float64)[None, :]
+    dist = np.sqrt(np.sum(R * R, axis=1))
+    lk = np.log(1.0 / dist) if dim == 2 else 1.0 / dist
+    pw = dist ** 2 if dim == 2 else dist ** 3
+    d = np.asarray(direc, np.float64)
+    out = np.zeros_like(R)
+    for i in range(dim):
+        s = np.zeros(R.shape[0])
+        for j in range(dim):
+            Uij = fac1 * ((fac2 * lk if i == j else 0.0) + R[:, i] * R[:, j] / pw)
+            s = s + Uij * d[j]
+        out[:, i] = s
+    return out
+
+
+def linear_elastic_fundsol(xyz, conn, dim, E=1000.0, nu=0.25, nq=2):
+    """reference/06-elastic/linearElastic.cpp:80-216: Q1 StVenant, the whole
+    boundary constrained to the elastostatic fundamental solution with source
+    (-0.1, ...) and direction e_1."""
+    lam, mu = lame(E, nu)
+    _, _, bf = boundary_faces(conn, dim)
+    bn = np.unique(bf)
+    con = np.zeros((xyz.shape[0], dim), bool)
+    con[bn, :] = True
+    dof, na = number_dofs(con)
+    src = np.full(dim, -0.1)
+    direc = np.zeros(dim)
+    direc[1] = 1.0
+    g = np.zeros((xyz.shape[0], dim))
+    g[bn] = fundsol_elasto(dim, lam, mu, xyz[bn], src, direc)
+    coef = np.tile(np.array([[lam, mu]]), (conn.shape[0], 1))
+    p = Problem(dim, np.ascontiguousarray(xyz, np.float64), np.ascontiguousarray(conn, np.int32),
+                dim, dof, na, g.ravel(), STVENANT, coef, nq, name="linearElastic%dD" % dim)
+    p.meta.update(lam=lam, mu=mu, src=src, dir=direc)
+    return p

@@ -1,0 +1,88 @@
+/*
+ * CPU oracle for the finite-element hot path (assembly -> CSR -> Jacobi-CG).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
+ * path is compared against.  Only the test-suite, the smoke check and the CPU
+ * baseline leg of the benchmark may load it; the product library never links,
+ * loads or calls it.
+ *
+ * Every function restates the reference algorithm of thrueberg/inSilico and
+ * cites the reference file:line it follows.  Arithmetic order follows the
+ * reference wherever it is visible in the reference sources (quadrature order,
+ * product order of tensor-product shape functions, per-quadrature-point
+ * accumulation, element order of the global sums).  Eigen-internal arithmetic
+ * (3x3 inverse, CG loop) is restated from the published algorithms.
+ */
+#ifndef INSILICO_ORACLE_H
+#define INSILICO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kernel ids (same numbering as the product C-ABI) */
+enum { ORC_LAPLACE = 1, ORC_STVENANT = 2, ORC_NEOHOOKE = 3 };
+/* body force kinds */
+enum { ORC_BF_NONE = 0, ORC_BF_CONST = 1, ORC_BF_SINPROD = 2 };
+
+typedef struct orc_problem {
+    int dim;                 /* 2 or 3 */
+    int64_t n_nodes;
+    const double* xyz;       /* n_nodes*dim */
+    int64_t n_elem;
+    int npe;                 /* 4 (QUAD4) or 8 (HEX8), hierarchic vertex order */
+    const int32_t* conn;     /* n_elem*npe node ids */
+    int dpn;                 /* dofs per node (1 scalar, dim for elasticity) */
+    const int32_t* dof_id;   /* n_nodes*dpn: >=0 active row, -1 constrained */
+    int64_t n_active;
+    const double* g;         /* n_nodes*dpn prescribed values (constrained comps) */
+    const double* u;         /* n_nodes*dpn current values, NULL = zero */
+    int incremental;         /* lifting uses g-u instead of g */
+    int kernel;              /* ORC_LAPLACE / ORC_STVENANT / ORC_NEOHOOKE */
+    int ncoef;               /* coefficients per element: 1 (kappa) or 2 (lambda,mu) */
+    const double* coef;      /* n_elem*ncoef */
+    int nq;                  /* Gauss points per direction */
+    int bf_kind;
+    double bf[4];            /* CONST: f per component; SINPROD: amp, k_x, k_y, k_z */
+    int64_t n_faces;         /* Neumann faces with constant traction */
+    int nfn;                 /* nodes per face (2 in 2D, 4 in 3D) */
+    const int32_t* faces;    /* n_faces*nfn node ids */
+    double traction[3];
+    int residual;            /* add -F_int(u) (Newton residual) to the RHS */
+    int nthreads;            /* 1 = reference default (serial) */
+} orc_problem;
+
+int  orc_gauss1d(int npts, double* w, double* x);
+int  orc_tensor_rule(int dim, int npts, double* w, double* xi);
+void orc_q1_shape(int dim, const double* xi, double* phi, double* dphi);
+double orc_contravariant(int dim, const double* X, const double* dphi,
+                         double* gradx);
+int  orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe);
+int  orc_unit_cube(int dim, int e1, int e2, int e3, double* xyz, int32_t* conn);
+
+/* pattern as the reference's TripletContainer::registerFields builds it */
+int  orc_pattern(const orc_problem* p, int64_t* nnz_out,
+                 int32_t* row_ptr /* n_active+1 or NULL */,
+                 int32_t* col_idx /* nnz or NULL */);
+/* assembly; row_ptr/col_idx must hold the pattern; t[0]=assembly seconds */
+int  orc_assemble(const orc_problem* p, const int32_t* row_ptr,
+                  const int32_t* col_idx, double* val, double* b, double* t);
+/* Jacobi-preconditioned CG as Eigen::ConjugateGradient with x0=0 */
+int  orc_pcg(int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
+             const double* val, const double* b, double* x, double tol,
+             int maxit, int nthreads, int* iters, double* relres);
+void orc_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
+              const double* val, const double* x, double* y, int nthreads);
+/* L2 error of a Q1 field against the elastostatic fundamental solution */
+double orc_l2_error_fundsol(const orc_problem* p, const double* uh,
+                            double lambda, double mu, const double* src,
+                            const double* dir);
+void orc_fundsol_elasto(int dim, double lambda, double mu, const double* y,
+                        const double* x, const double* dir, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
