@@ -1,0 +1,212 @@
+"""Benchmark: solved DOF/s of the assembly + Jacobi-PCG hot path.
+
+Workload (one GPU): 3-D scalar diffusion -div(k grad u) = 1 on the unit cube,
+128^3 Q1 hex mesh (2,130,048 active DoF, 56,621,950 CSR entries), 2x2x2 Gauss,
+k_e = exp(g_e), g_e ~ N(0,1) per element (seed 12345), u = 0 on x = 0,
+relative residual 1e-8.
+
+A step = one full pass of the hot path on inputs resident in HBM: element
+assembly into the CSR values and RHS, then Jacobi-PCG to |b - A x|/|b| <= 1e-8.
+The mesh, DoF numbering and CSR pattern are set up once before the timed
+region (reported as setup_ms).  value = mixed-precision path (FP32 PCG sweeps,
+FP64 residual replacement, SELL-64 storage); fp64_value = the plain FP64 path
+(FP64 assembly + FP64 PCG on the exported CSR).
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "solved DOF/s at 1e-8 rel. residual (mixed vs FP64 PCG); p50/p99 solve latency"
+
+
+def spmv_bytes(n, nnz, nslice_vals, fmt, vbytes, xbytes):
+    """Algorithmic bytes of one SpMV launch of the given storage."""
+    if fmt == "sell":
+        # values (padded slices) + pattern id per row + x read + q write
+        return nslice_vals * vbytes + n * 4 + 2 * n * xbytes
+    # values + column indices + row pointers + x read + q write
+    return nnz * (vbytes + 4) + n * 4 + 2 * n * xbytes
+
+
+def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup):
+    from insilico_amd import abi
+    lat, its, rels, asm, sol = [], [], [], [], []
+    for s in range(warmup + steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.assemble(precision, abi.ASM_MATRIX | abi.ASM_RHS)
+        it, rel, _ = ctx.solve(mode, fmt, rtol)
+        ctx.sync()
+        t1 = time.perf_counter()
+        if s >= warmup:
+            st = ctx.stats()
+            lat.append((t1 - t0) * 1e3)
+            its.append(it)
+            rels.append(rel)
+            asm.append(st["assemble_ms"])
+            sol.append(st["solve_ms"])
+    return lat, its, rels, asm, sol
+
+
+def profile_spmv(ctx, mode, fmt, precision, rtol):
+    """HIP-event timing of every SpMV launch of one solve (dominant kernel)."""
+    from insilico_amd import abi
+    ctx.assemble(precision, abi.ASM_MATRIX | abi.ASM_RHS)
+    ctx.set_profiling(True)
+    ctx.solve(mode, fmt, rtol)
+    ctx.set_profiling(False)
+    st = ctx.stats()
+    return st["spmv_ms"] / max(st["spmv_calls"], 1), st["spmv_calls"]
+
+
+def cpu_baseline(prob, ctx, iters_fp64, budget_s=20.0):
+    """The oracle (reference algorithm restated in C, -O3 -fopenmp) on the host
+    cores: assembly of a bounded element sample and a bounded number of
+    Jacobi-CG iterations on the full matrix, extrapolated to the full job."""
+    from oracle import pyoracle as O
+    from insilico_amd import problems as P
+    ncores = len(os.sched_getaffinity(0))
+    # assembly sample: first E_s elements (contiguous z-slabs)
+    E_s = min(prob.n_elem, 131072)
+    sub = P.Problem(prob.dim, prob.xyz, prob.conn[:E_s], prob.dpn, prob.dof_id, prob.n_active,
+                    prob.g, prob.kernel, prob.coef[:E_s], prob.nq, prob.bf_kind, prob.bf)
+    rp_s, ci_s = O.pattern(sub)
+    _, _, _, _, t_asm_s = O.assemble(sub, rp_s, ci_s, nthreads=ncores)
+    t_elem = t_asm_s / E_s
+    # CG sample on the full system (values equal the oracle's to 1e-12)
+    rp, ci = ctx.get_pattern()
+    val = ctx.get_csr_values()
+    b = ctx.get_rhs()
+    k = 20
+    t0 = time.perf_counter()
+    O.pcg(rp, ci, val, b, tol=1e-30, maxit=k, nthreads=ncores)
+    t_it = (time.perf_counter() - t0) / (k + 1)
+    t_total = prob.n_elem * t_elem + iters_fp64 * t_it
+    return {
+        "value": prob.n_active / t_total, "unit": "DOF/s", "cores": ncores, "kind": "port",
+        "sample": "oracle (reference algorithm in C, -O3 -fopenmp, %d threads): assembly of "
+                  "%d of %d elements (%.2f s) and %d Jacobi-CG iterations on the full "
+                  "matrix (%.3f s/iter), extrapolated to %d elements + %d iterations"
+                  % (ncores, E_s, prob.n_elem, t_asm_s, k, t_it, prob.n_elem, iters_fp64),
+        "t_assemble_s": prob.n_elem * t_elem, "t_iter_s": t_it,
+    }
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "spmv_counters_diffusion128.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch[%s]" % kernel_tag):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fmt", default="sell", choices=["sell", "csr"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from insilico_amd import abi, problems as P
+
+    t0 = time.perf_counter()
+    prob = P.diffusion3d(args.n, seed=args.seed)
+    t_gen = time.perf_counter() - t0
+    log("generated %s: %d elements, %d dof (%.1f s)" % (prob.name, prob.n_elem, prob.n_active, t_gen))
+    ctx = abi.Context(0)
+    nnz = ctx.load_problem(prob)
+    log("pattern: nnz=%d" % nnz)
+    setup_ms = ctx.stats()["setup_ms"]
+    fmt = abi.FMT_SELL if args.fmt == "sell" else abi.FMT_CSR
+
+    # plain FP64 path
+    lat64, its64, rel64, asm64, sol64 = run_path(ctx, prob, abi.SOLVE_FP64, abi.FMT_CSR, abi.FP64,
+                                                 args.rtol, args.steps, args.warmup)
+    log("fp64 path: %.1f ms/step, %d iterations" % (statistics.mean(lat64), its64[0]))
+    # mixed-precision path (headline)
+    latm, itsm, relm, asmm, solm = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
+                                            args.rtol, args.steps, args.warmup)
+    log("mixed path: %.1f ms/step, %d iterations" % (statistics.mean(latm), itsm[0]))
+    st = ctx.stats()
+    ms_step = statistics.mean(latm)
+    value = prob.n_active / (ms_step / 1e3)
+    value64 = prob.n_active / (statistics.mean(lat64) / 1e3)
+
+    # dominant kernel: the SpMV of the mixed path
+    spmv_ms, spmv_calls = profile_spmv(ctx, abi.SOLVE_MIXED, fmt, abi.FP64, args.rtol)
+    n = prob.n_active
+    sell_vals = ctx.stats()["sell_values"] or nnz
+    fmt_name = "sell" if fmt == abi.FMT_SELL else "csr"
+    alg = spmv_bytes(n, nnz, sell_vals, fmt_name, 4, 4)
+    achieved = alg / (spmv_ms / 1e3) / 1e9
+    traffic = read_traffic("mixed_" + fmt_name)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_spmv_pq_%s<float,float,float> (q = A p fused with p.q)" % fmt_name,
+            "bytes_per_launch": alg, "avg_launch_ms": round(spmv_ms, 5),
+            "launches_timed": spmv_calls}
+
+    log("spmv: %.4f ms/launch over %d launches" % (spmv_ms, spmv_calls))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (bounded sample) ...")
+        cpu = cpu_baseline(prob, ctx, int(statistics.median(its64)))
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "DOF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 sweeps / f64 residual", "data": "synthetic",
+        "config": {"workload": "3-D diffusion 128^3 Q1 hex, k=exp(N(0,1)) per element, "
+                               "u=0 on x=0, assemble + Jacobi-PCG to 1e-8",
+                   "n_elements": prob.n_elem, "n_dof": n, "nnz": nnz, "rtol": args.rtol,
+                   "format": fmt_name},
+        "p50_ms": round(statistics.median(latm), 3),
+        "p99_ms": round(float(np.percentile(latm, 99)), 3),
+        "iterations": int(statistics.median(itsm)), "relres": max(relm),
+        "assemble_ms": round(statistics.mean(asmm), 3), "solve_ms": round(statistics.mean(solm), 3),
+        "refinements": st["refinements"],
+        "fp64_value": round(value64, 1), "fp64_ms_per_step": round(statistics.mean(lat64), 3),
+        "fp64_iterations": int(statistics.median(its64)), "fp64_relres": max(rel64),
+        "fp64_assemble_ms": round(statistics.mean(asm64), 3),
+        "mixed_over_fp64": round(value / value64, 3),
+        "setup_ms": round(setup_ms, 3), "host_generate_s": round(t_gen, 3),
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

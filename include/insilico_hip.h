@@ -98,6 +98,9 @@ typedef struct ins_stats {
     int64_t device_bytes;   /* device memory held by the context             */
     double spmv_ms;         /* profiled: total time of the dominant SpMV     */
     int64_t spmv_calls;     /* profiled: launches timed                      */
+    int64_t sell_values;    /* padded value count of the SELL storage (0 = none) */
+    int     sell_patterns;  /* distinct column-offset lists of the SELL storage  */
+    int     reserved;
 } ins_stats;
 
 /* ---- context ------------------------------------------------------------ */

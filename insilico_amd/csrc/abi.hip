@@ -69,12 +69,12 @@ static void release_all(Ctx* c)
     release(c, c->faces); release(c, c->fadj_ptr); release(c, c->fadj); release(c, c->val64);
     release(c, c->val32); release(c, c->b); release(c, c->dinv64); release(c, c->dinv32);
     release(c, c->stage_ke); release(c, c->stage_fe); release(c, c->stage_ff);
-    release(c, c->sell.slice_ptr); release(c, c->sell.slice_len); release(c, c->sell.row_pat);
+    release(c, c->sell.slice_ptr); release(c, c->sell.slice_pat);
     release(c, c->sell.pat_ptr); release(c, c->sell.pat_delta); release(c, c->sell.pat_len);
     release(c, c->sell.v64); release(c, c->sell.v32);
     release(c, c->x); release(c, c->r); release(c, c->p); release(c, c->q); release(c, c->t64);
     release(c, c->x32); release(c, c->r32); release(c, c->p32); release(c, c->q32);
-    release(c, c->partial); release(c, c->counter); release(c, c->scal); release(c, c->dflag);
+    release(c, c->partial); release(c, c->scal); release(c, c->dflag);
 }
 
 static void invalidate_pattern(Ctx* c)
@@ -170,6 +170,8 @@ int ins_get_stats(const ins_ctx* h, ins_stats* out)
     out->n_active = c->n_active;
     out->nnz = c->nnz;
     out->device_bytes = c->held;
+    out->sell_values = (c->sell.built && c->sell.ok) ? c->sell.nvals : 0;
+    out->sell_patterns = (c->sell.built && c->sell.ok) ? c->sell.npat : 0;
     return INS_OK;
 }
 
@@ -480,6 +482,7 @@ int ins_set_csr(ins_ctx* h, int64_t n, const int32_t* row_ptr, const int32_t* co
     c->have_matrix = true;
     c->have_solution = false;
     c->sell.built = false;
+    c->sell.stale = true;
     INS_TRY(compute_dinv(c));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return INS_OK;

@@ -1112,7 +1112,7 @@ int assemble(Ctx* c, int precision, int flags)
     if (do_matrix) {
         c->matrix_precision = precision;
         INS_TRY(compute_dinv(c));
-        c->sell.built = false;
+        c->sell.stale = true;
     }
     INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     int herr = 0;

@@ -43,7 +43,7 @@ struct DArr {
 
 // device-side scalars of one PCG solve
 struct Scal {
-    double rho;      // r.z of the current direction
+    double rho[2];   // r.z of the current direction, by iteration parity
     double pq;       // p.Ap
     double alpha;
     double beta;
@@ -51,23 +51,21 @@ struct Scal {
     double bb;       // |b|^2
     double thr;      // rtol^2 |b|^2
     double tt;       // |b - A x|^2 (true residual)
-    double rz_new;
     int done;        // 1 = stop (converged, breakdown or empty rhs)
     int breakdown;
     int iters;       // SpMVs performed inside the loop
     int pad;
 };
 
-// SELL-64 matrix: 64-row slices, values column-major inside a slice, per row
-// an index into a dictionary of column-offset lists (col = row + delta).
+// SELL-64 matrix: 64-row slices, values column-major inside a slice; every
+// slice uses one column-offset list (col = row + delta) from a dictionary.
 struct Sell {
     int64_t n = 0;
     int64_t nslices = 0;
     int64_t nvals = 0;       // padded value count
     int npat = 0;
     DArr<int64_t> slice_ptr; // nslices+1 value offsets (multiples of 64)
-    DArr<int32_t> slice_len; // max row length per slice
-    DArr<int32_t> row_pat;   // n: pattern id
+    DArr<int32_t> slice_pat; // nslices: id of the slice's column-offset list
     DArr<int32_t> pat_ptr;   // npat+1
     DArr<int32_t> pat_delta; // concatenated column offsets
     DArr<int32_t> pat_len;   // npat
@@ -75,6 +73,7 @@ struct Sell {
     DArr<float> v32;
     bool ok = false;         // dictionary small enough to be worth it
     bool built = false;
+    bool stale = true;       // values older than the last assembly
 };
 
 struct Ctx {
@@ -140,7 +139,6 @@ struct Ctx {
     DArr<double> x, r, p, q, t64;
     DArr<float> x32, r32, p32, q32;
     DArr<double> partial;
-    DArr<unsigned> counter;
     DArr<Scal> scal;
     Scal* h_scal = nullptr;  // pinned
     DArr<int> dflag;

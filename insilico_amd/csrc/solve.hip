@@ -3,13 +3,16 @@
 // |r|^2 < rtol^2 |b|^2 on the recursive residual).
 //
 // Every iteration is three kernels that stream HBM once each:
-//   K1  q = A p fused with the p.q reduction            -> alpha
-//   K2  x += alpha p, r -= alpha q fused with r.r, r.D^-1 r -> beta, stop test
-//   K3  p = D^-1 r + beta p
-// Reductions are deterministic: each block reduces its rows in a fixed tree,
-// writes one partial, and the last block to finish (integer ticket) sums the
-// partials in block order and updates the device-side scalars.  No float
-// atomics anywhere.  The host only synchronises once per chunk of iterations.
+//   K1  q = A p, per-block partials of p.q
+//   K2  alpha = rho / sum(p.q); x += alpha p; r -= alpha q; partials of r.r, r.D^-1 r
+//   K3  stop test on sum(r.r); beta = sum(r.z) / rho; p = D^-1 r + beta p
+// Reductions are deterministic and atomic-free: a producer kernel writes one
+// partial per block (fixed tree inside the block); every block of the
+// consumer kernel sums the same partials in the same order in its prologue,
+// so all blocks agree on the scalar without a grid barrier, a float atomic or
+// an extra launch.  Block 0 records the scalars for the host.  rho is kept in
+// two slots (iteration parity) so that no block reads a slot being written.
+// The host synchronises once per chunk of iterations.
 //
 // Mixed precision (INS_SOLVE_MIXED): FP32 matrix values and FP32 vectors in
 // K1-K3 with FP32 accumulation in the SpMV and FP64 accumulation of all dot
@@ -26,9 +29,12 @@ namespace ins {
 
 namespace {
 
-constexpr int RB = 256;       // threads per block of vector kernels
-constexpr int MAXG = 2048;    // max blocks of a reducing kernel
+constexpr int VB = 512;       // threads per block of the vector kernels
+constexpr int SB = 256;       // threads per block of the SpMV kernels
+constexpr int MAXG = 2048;    // max blocks of a partial-producing kernel
 constexpr int CHUNK = 40;     // iterations between host checks
+// partial slots
+constexpr int P_PQ = 0, P_RR = 1, P_RZ = 2, P_TT = 3, P_BB = 4, NSLOT = 5;
 
 __device__ inline double wave_sum(double v)
 {
@@ -37,64 +43,100 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// sums NV values over the block; result valid in thread 0
+// sum of v over the block, returned to every thread (fixed order)
 template <int NV>
-__device__ inline void block_sum(double (&v)[NV])
+__device__ inline void block_allsum(double (&v)[NV])
 {
-    __shared__ double sh[NV][RB / 64];
+    __shared__ double sh[NV][16];
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
-    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) sh[k][wave] = v[k];
+        for (int k = 0; k < NV; k++) sh[k][threadIdx.x >> 6] = v[k];
     __syncthreads();
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double s = 0.;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[k][w];
-            v[k] = s;
-        }
+    for (int k = 0; k < NV; k++) {
+        double s = 0.;
+        for (int w = 0; w < nw; w++) s += sh[k][w];
+        v[k] = s;
     }
 }
 
-// Deterministic grid reduction: returns true in thread 0 of the last block,
-// with tot[] = sum over blocks in block order.
+// one partial per block
 template <int NV>
-__device__ inline bool grid_sum(double (&v)[NV], double* partial, unsigned* counter,
-                                double (&tot)[NV])
+__device__ inline void block_partials(double (&v)[NV], double* part, const int (&slot)[NV])
 {
-    block_sum<NV>(v);
-    __shared__ int amlast;
-    if (threadIdx.x == 0) {
+    block_allsum<NV>(v);
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) partial[k * MAXG + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        amlast = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amlast) return false;
-    __threadfence();
-    double s[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) s[k] = 0.;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++) s[k] += partial[k * MAXG + i];
-    block_sum<NV>(s);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) tot[k] = s[k];
-        *counter = 0u;
-    }
-    return threadIdx.x == 0;
+        for (int k = 0; k < NV; k++) part[slot[k] * MAXG + blockIdx.x] = v[k];
 }
+
+// every block: the same total of cnt partials of the given slots
+template <int NV>
+__device__ inline void totals(const double* part, int cnt, const int (&slot)[NV], double (&t)[NV])
+{
+#pragma unroll
+    for (int k = 0; k < NV; k++) t[k] = 0.;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++) t[k] += part[slot[k] * MAXG + i];
+    block_allsum<NV>(t);
+}
+
+__device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
 // ---------------------------------------------------------------------------
-// CSR SpMV, LPR lanes per row; optional fused p.q reduction
+// SpMV rows
+struct SellView {
+    const int64_t* sptr;   // value offset of every slice
+    const int32_t* spat;   // column-offset list (pattern) of every slice
+    const int32_t* pptr;
+    const int32_t* plen;
+    const int32_t* pdel;
+};
+
+constexpr int SU = 9;  // SELL unroll (27-point rows = 3 chunks, 81 = 9)
+
+// one lane per row of a 64-row slice; all rows of a slice share one
+// column-offset list (the union of their own lists, values zero-padded), so
+// the offsets are wave-uniform and read with scalar loads.  Padded entries
+// may point outside 0..n-1: their index is clamped (their value is 0).
+template <typename TV, typename TX, typename TA>
+__device__ inline TA sell_row(int64_t row, int64_t n, int lane, int64_t slice, const SellView& S,
+                              const TV* __restrict__ v, const TX* __restrict__ x)
+{
+    TA s = (TA)0;
+    const bool live = row < n;
+    const int64_t base = S.sptr[slice] + lane;
+    const int pid = S.spat[slice];
+    const int32_t* d = S.pdel + S.pptr[pid];
+    const int len = S.plen[pid];
+    const int64_t last = n - 1;
+    for (int k0 = 0; k0 < len; k0 += SU) {
+        TV vv[SU];
+        TX xx[SU];
+#pragma unroll
+        for (int j = 0; j < SU; j++) {
+            const int k = k0 + j;
+            vv[j] = (TV)0;
+            xx[j] = (TX)0;
+            if (k < len && live) {
+                int64_t idx = row + d[k];
+                idx = idx < 0 ? 0 : (idx > last ? last : idx);
+                vv[j] = v[base + (int64_t)k * 64];
+                xx[j] = x[idx];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SU; j++) s += (TA)vv[j] * (TA)xx[j];
+    }
+    return s;
+}
+
+// CSR row with LPR lanes; each lane keeps 4 entries in flight
 template <int LPR, typename TV, typename TX, typename TA>
 __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ rp,
                              const int32_t* __restrict__ col, const TV* __restrict__ val,
@@ -102,62 +144,64 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
 {
     TA s = (TA)0;
     const int32_t e = rp[row + 1];
-    for (int32_t k = rp[row] + sub; k < e; k += LPR) s += (TA)val[k] * (TA)x[col[k]];
+    for (int32_t k0 = rp[row] + sub; k0 < e; k0 += 4 * LPR) {
+        int32_t cc[4];
+        TV vv[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int32_t k = k0 + j * LPR;
+            cc[j] = (k < e) ? col[k] : -1;
+            vv[j] = (k < e) ? val[k] : (TV)0;
+        }
+        TX xx[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) xx[j] = (cc[j] >= 0) ? x[cc[j]] : (TX)0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) s += (TA)vv[j] * (TA)xx[j];
+    }
 #pragma unroll
     for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
     return s;
 }
 
-// SELL-64 row: one lane per row, 64 rows per wavefront
+// ---------------------------------------------------------------------------
+// K1: q = A p with per-block partials of p.q
 template <typename TV, typename TX, typename TA>
-__device__ inline TA sell_row(int64_t row, int64_t n, int lane, int64_t slice,
-                              const int64_t* __restrict__ sptr, const int32_t* __restrict__ slen,
-                              const int32_t* __restrict__ rpat, const int32_t* __restrict__ pptr,
-                              const int32_t* __restrict__ plen, const int32_t* __restrict__ pdel,
-                              const TV* __restrict__ v, const TX* __restrict__ x)
+__global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
+                                                  const TX* __restrict__ p, TX* __restrict__ q,
+                                                  double* part, const Scal* sc)
 {
-    TA s = (TA)0;
-    const int64_t base = sptr[slice] + lane;
-    const int L = slen[slice];
-    const int pid = row < n ? rpat[row] : 0;
-    const int uni = __builtin_amdgcn_readfirstlane(pid);
-    if (__all(row >= n || pid == uni)) {
-        // uniform slice: the offset list is wave-uniform
-        const int32_t* d = pdel + pptr[uni];
-        const int len = plen[uni];
-        if (row < n)
-            for (int k = 0; k < len; k++) s += (TA)v[base + (int64_t)k * 64] * (TA)x[row + d[k]];
-    } else if (row < n) {
-        const int32_t* d = pdel + pptr[pid];
-        const int len = plen[pid];
-        for (int k = 0; k < len; k++) s += (TA)v[base + (int64_t)k * 64] * (TA)x[row + d[k]];
+    if (sc->done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t nsl = (n + 63) / 64;
+    double pq = 0.;
+    for (int64_t slice = (int64_t)blockIdx.x * (SB / 64) + (threadIdx.x >> 6); slice < nsl;
+         slice += (int64_t)gridDim.x * (SB / 64)) {
+        const int64_t row = slice * 64 + lane;
+        const TA s = sell_row<TV, TX, TA>(row, n, lane, slice, S, val, p);
+        if (row < n) {
+            const TX qs = (TX)s;
+            q[row] = qs;
+            pq += (double)p[row] * (double)qs;
+        }
     }
-    (void)L;
-    return s;
+    double v[1] = {pq};
+    const int sl[1] = {P_PQ};
+    block_partials<1>(v, part, sl);
 }
 
-struct SellView {
-    const int64_t* sptr;
-    const int32_t* slen;
-    const int32_t* rpat;
-    const int32_t* pptr;
-    const int32_t* plen;
-    const int32_t* pdel;
-};
-
-// K1: q = A p ; pq -> alpha
 template <int LPR, typename TV, typename TX, typename TA>
-__global__ __launch_bounds__(RB) void k_spmv_pq_csr(int64_t n, const int32_t* __restrict__ rp,
-                                                    const int32_t* __restrict__ col,
-                                                    const TV* __restrict__ val,
-                                                    const TX* __restrict__ p, TX* __restrict__ q,
-                                                    double* partial, unsigned* counter, Scal* sc)
+__global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ col,
+                                                 const TV* __restrict__ val,
+                                                 const TX* __restrict__ p, TX* __restrict__ q,
+                                                 double* part, const Scal* sc)
 {
     if (sc->done) return;
     const int sub = threadIdx.x & (LPR - 1);
-    const int64_t stride = (int64_t)gridDim.x * (RB / LPR);
+    const int64_t stride = (int64_t)gridDim.x * (SB / LPR);
     double pq = 0.;
-    for (int64_t row = ((int64_t)blockIdx.x * RB + threadIdx.x) / LPR; row < n; row += stride) {
+    for (int64_t row = ((int64_t)blockIdx.x * SB + threadIdx.x) / LPR; row < n; row += stride) {
         const TA s = csr_row<LPR, TV, TX, TA>(row, sub, rp, col, val, p);
         if (sub == 0) {
             const TX qs = (TX)s;
@@ -165,65 +209,42 @@ __global__ __launch_bounds__(RB) void k_spmv_pq_csr(int64_t n, const int32_t* __
             pq += (double)p[row] * (double)qs;
         }
     }
-    double v[1] = {pq}, tot[1];
-    if (grid_sum<1>(v, partial, counter, tot)) {
-        sc->pq = tot[0];
-        if (!(tot[0] > 0.) || !isfinite(tot[0])) {
-            sc->breakdown = 1;
-            sc->done = 1;
-        } else {
-            sc->alpha = sc->rho / tot[0];
-        }
-    }
+    double v[1] = {pq};
+    const int sl[1] = {P_PQ};
+    block_partials<1>(v, part, sl);
 }
 
-template <typename TV, typename TX, typename TA>
-__global__ __launch_bounds__(RB) void k_spmv_pq_sell(int64_t n, SellView S,
-                                                     const TV* __restrict__ val,
-                                                     const TX* __restrict__ p, TX* __restrict__ q,
-                                                     double* partial, unsigned* counter, Scal* sc)
-{
-    if (sc->done) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t nsl = (n + 63) / 64;
-    const int64_t wstride = (int64_t)gridDim.x * (RB / 64);
-    double pq = 0.;
-    for (int64_t slice = ((int64_t)blockIdx.x * RB + threadIdx.x) / 64; slice < nsl;
-         slice += wstride) {
-        const int64_t row = slice * 64 + lane;
-        const TA s = sell_row<TV, TX, TA>(row, n, lane, slice, S.sptr, S.slen, S.rpat, S.pptr,
-                                          S.plen, S.pdel, val, p);
-        if (row < n) {
-            const TX qs = (TX)s;
-            q[row] = qs;
-            pq += (double)p[row] * (double)qs;
-        }
-    }
-    double v[1] = {pq}, tot[1];
-    if (grid_sum<1>(v, partial, counter, tot)) {
-        sc->pq = tot[0];
-        if (!(tot[0] > 0.) || !isfinite(tot[0])) {
-            sc->breakdown = 1;
-            sc->done = 1;
-        } else {
-            sc->alpha = sc->rho / tot[0];
-        }
-    }
-}
-
-// K2: x += alpha p ; r -= alpha q ; rr, rz -> stop test, beta
+// K2: alpha from the p.q partials; x += alpha p; r -= alpha q; r.r, r.z partials
 template <typename TX, typename TD>
-__global__ __launch_bounds__(RB) void k_update_xr(int64_t n, TX* __restrict__ x, TX* __restrict__ r,
+__global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x, TX* __restrict__ r,
                                                   const TX* __restrict__ p,
                                                   const TX* __restrict__ q,
-                                                  const TD* __restrict__ dinv, double* partial,
-                                                  unsigned* counter, Scal* sc)
+                                                  const TD* __restrict__ dinv, double* part,
+                                                  int g_pq, Scal* sc, int par)
 {
     if (sc->done) return;
-    const TX alpha = (TX)sc->alpha;
+    double t[1];
+    const int slq[1] = {P_PQ};
+    totals<1>(part, g_pq, slq, t);
+    const double pq = t[0];
+    if (!(pq > 0.) || !isfinite(pq)) {
+        if (leader()) {
+            sc->pq = pq;
+            sc->breakdown = 1;
+            sc->done = 1;
+        }
+        return;
+    }
+    const double ad = sc->rho[par] / pq;
+    if (leader()) {
+        sc->pq = pq;
+        sc->alpha = ad;
+        sc->iters += 1;
+    }
+    const TX alpha = (TX)ad;
     double rr = 0., rz = 0.;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB) {
+    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * VB) {
         x[i] += alpha * p[i];
         const TX ri = r[i] - alpha * q[i];
         r[i] = ri;
@@ -231,98 +252,145 @@ __global__ __launch_bounds__(RB) void k_update_xr(int64_t n, TX* __restrict__ x,
         rr += (double)ri * (double)ri;
         rz += (double)ri * (double)zi;
     }
-    double v[2] = {rr, rz}, tot[2];
-    if (grid_sum<2>(v, partial, counter, tot)) {
-        sc->rr = tot[0];
-        sc->iters += 1;
-        if (!isfinite(tot[0])) {
-            sc->breakdown = 1;
-            sc->done = 1;
-        } else if (tot[0] < sc->thr) {
-            sc->done = 1;
-        } else {
-            sc->beta = tot[1] / sc->rho;
-            sc->rho = tot[1];
-        }
-    }
+    double v[2] = {rr, rz};
+    const int sl[2] = {P_RR, P_RZ};
+    block_partials<2>(v, part, sl);
 }
 
-// K3: p = D^-1 r + beta p
+// K3: stop test; beta; p = D^-1 r + beta p
 template <typename TX, typename TD>
-__global__ __launch_bounds__(RB) void k_update_p(int64_t n, TX* __restrict__ p,
+__global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
                                                  const TX* __restrict__ r,
-                                                 const TD* __restrict__ dinv, const Scal* sc)
+                                                 const TD* __restrict__ dinv, const double* part,
+                                                 int g, Scal* sc, int par)
 {
     if (sc->done) return;
-    const TX beta = (TX)sc->beta;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB)
+    double t[2];
+    const int sl[2] = {P_RR, P_RZ};
+    totals<2>(part, g, sl, t);
+    const double rr = t[0], rz = t[1];
+    const bool bad = !isfinite(rr);
+    const bool conv = rr < sc->thr;
+    const double rho = sc->rho[par];
+    if (leader()) {
+        sc->rr = rr;
+        if (bad) {
+            sc->breakdown = 1;
+            sc->done = 1;
+        } else if (conv) {
+            sc->done = 1;
+        } else {
+            sc->rho[par ^ 1] = rz;
+            sc->beta = rz / rho;
+        }
+    }
+    if (bad || conv) return;
+    const TX beta = (TX)(rz / rho);
+    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * VB)
         p[i] = (TX)dinv[i] * r[i] + beta * p[i];
 }
 
-// start: x = 0, r = b, p = D^-1 b, rho = r.p, rr, bb
+// start: x = 0, r = b, p = D^-1 b; partials of b.b and r.z
 template <typename TX, typename TD>
-__global__ __launch_bounds__(RB) void k_init(int64_t n, const double* __restrict__ b,
+__global__ __launch_bounds__(VB) void k_init(int64_t n, const double* __restrict__ b,
                                              TX* __restrict__ x, TX* __restrict__ r,
                                              TX* __restrict__ p, const TD* __restrict__ dinv,
-                                             double rtol2, double* partial, unsigned* counter,
-                                             Scal* sc)
+                                             double* part)
 {
-    double rz = 0., rr = 0.;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB) {
+    double bb = 0., rz = 0.;
+    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * VB) {
         const TX bi = (TX)b[i];
         x[i] = (TX)0;
         r[i] = bi;
         const TX zi = (TX)dinv[i] * bi;
         p[i] = zi;
-        rr += (double)b[i] * (double)b[i];
+        bb += b[i] * b[i];
         rz += (double)bi * (double)zi;
     }
-    double v[2] = {rr, rz}, tot[2];
-    if (grid_sum<2>(v, partial, counter, tot)) {
-        sc->bb = tot[0];
-        sc->rr = tot[0];
-        sc->rho = tot[1];
-        sc->thr = rtol2 * tot[0];
+    double v[2] = {bb, rz};
+    const int sl[2] = {P_BB, P_RZ};
+    block_partials<2>(v, part, sl);
+}
+
+__global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, double rtol2, Scal* sc)
+{
+    double t[2];
+    const int sl[2] = {P_BB, P_RZ};
+    totals<2>(part, g, sl, t);
+    if (threadIdx.x == 0) {
+        sc->bb = t[0];
+        sc->rr = t[0];
+        sc->tt = t[0];
+        sc->thr = rtol2 * t[0];
+        sc->rho[0] = t[1];
+        sc->rho[1] = t[1];
         sc->iters = 0;
         sc->breakdown = 0;
-        sc->done = (tot[0] == 0. || tot[0] < sc->thr) ? 1 : 0;
+        sc->done = (t[0] == 0.) ? 1 : 0;
     }
 }
 
 // ---- mixed precision: residual replacement ------------------------------------
-// x64 += x32, x32 = 0
-__global__ __launch_bounds__(RB) void k_fold(int64_t n, double* __restrict__ x64,
+__global__ __launch_bounds__(VB) void k_fold(int64_t n, double* __restrict__ x64,
                                              float* __restrict__ x32)
 {
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB) {
+    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * VB) {
         x64[i] += (double)x32[i];
         x32[i] = 0.f;
     }
 }
 
-// t = b - A x (FP64), tt = |t|^2; optionally r32 = (float)t and rz
+// t = b - A x (FP64); partials of t.t and, when replacing, r32 = (float)t and r.z
+template <typename TV>
+__global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const TV* __restrict__ val,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ b,
+                                                   float* __restrict__ r32,
+                                                   const float* __restrict__ dinv32,
+                                                   double* part, int replace)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nsl = (n + 63) / 64;
+    double tt = 0., rz = 0.;
+    for (int64_t slice = (int64_t)blockIdx.x * (SB / 64) + (threadIdx.x >> 6); slice < nsl;
+         slice += (int64_t)gridDim.x * (SB / 64)) {
+        const int64_t row = slice * 64 + lane;
+        const double s = sell_row<TV, double, double>(row, n, lane, slice, S, val, x);
+        if (row < n) {
+            const double ti = b[row] - s;
+            tt += ti * ti;
+            if (replace) {
+                const float rf = (float)ti;
+                r32[row] = rf;
+                rz += (double)rf * (double)(dinv32[row] * rf);
+            }
+        }
+    }
+    double v[2] = {tt, rz};
+    const int sl[2] = {P_TT, P_RZ};
+    block_partials<2>(v, part, sl);
+}
+
 template <int LPR, typename TV>
-__global__ __launch_bounds__(RB) void k_resid_csr(int64_t n, const int32_t* __restrict__ rp,
+__global__ __launch_bounds__(SB) void k_resid_csr(int64_t n, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ col,
                                                   const TV* __restrict__ val,
                                                   const double* __restrict__ x,
                                                   const double* __restrict__ b,
-                                                  double* __restrict__ t, float* __restrict__ r32,
+                                                  float* __restrict__ r32,
                                                   const float* __restrict__ dinv32,
-                                                  double* partial, unsigned* counter, Scal* sc,
-                                                  int replace)
+                                                  double* part, int replace)
 {
     const int sub = threadIdx.x & (LPR - 1);
-    const int64_t stride = (int64_t)gridDim.x * (RB / LPR);
+    const int64_t stride = (int64_t)gridDim.x * (SB / LPR);
     double tt = 0., rz = 0.;
-    for (int64_t row = ((int64_t)blockIdx.x * RB + threadIdx.x) / LPR; row < n; row += stride) {
+    for (int64_t row = ((int64_t)blockIdx.x * SB + threadIdx.x) / LPR; row < n; row += stride) {
         const double s = csr_row<LPR, TV, double, double>(row, sub, rp, col, val, x);
         if (sub == 0) {
             const double ti = b[row] - s;
-            if (t) t[row] = ti;
             tt += ti * ti;
             if (replace) {
                 const float rf = (float)ti;
@@ -331,64 +399,97 @@ __global__ __launch_bounds__(RB) void k_resid_csr(int64_t n, const int32_t* __re
             }
         }
     }
-    double v[2] = {tt, rz}, tot[2];
-    if (grid_sum<2>(v, partial, counter, tot)) {
-        sc->tt = tot[0];
+    double v[2] = {tt, rz};
+    const int sl[2] = {P_TT, P_RZ};
+    block_partials<2>(v, part, sl);
+}
+
+// after a residual: record |b - A x|^2; when replacing, restart the
+// recursion from the replaced residual (rho of the next iteration's slot)
+__global__ __launch_bounds__(VB) void k_resid_fin(const double* part, int g, Scal* sc, int replace,
+                                                  int par_next)
+{
+    double t[2];
+    const int sl[2] = {P_TT, P_RZ};
+    totals<2>(part, g, sl, t);
+    if (threadIdx.x == 0) {
+        sc->tt = t[0];
         if (replace) {
-            if (tot[0] < sc->thr) {
+            if (!isfinite(t[0])) {
+                sc->breakdown = 1;
+                sc->done = 1;
+            } else if (t[0] < sc->thr) {
                 sc->done = 1;
             } else {
                 sc->done = 0;
-                sc->rho = tot[1];
+                sc->rho[par_next] = t[1];
             }
         }
     }
 }
 
-template <typename TV>
-__global__ __launch_bounds__(RB) void k_resid_sell(int64_t n, SellView S, const TV* __restrict__ val,
-                                                   const double* __restrict__ x,
-                                                   const double* __restrict__ b,
-                                                   double* __restrict__ t, float* __restrict__ r32,
-                                                   const float* __restrict__ dinv32,
-                                                   double* partial, unsigned* counter, Scal* sc,
-                                                   int replace)
+__global__ __launch_bounds__(VB) void k_norm2(int64_t n, const double* __restrict__ v, double* part)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t nsl = (n + 63) / 64;
-    const int64_t wstride = (int64_t)gridDim.x * (RB / 64);
-    double tt = 0., rz = 0.;
-    for (int64_t slice = ((int64_t)blockIdx.x * RB + threadIdx.x) / 64; slice < nsl;
-         slice += wstride) {
-        const int64_t row = slice * 64 + lane;
-        const double s = sell_row<TV, double, double>(row, n, lane, slice, S.sptr, S.slen, S.rpat,
-                                                      S.pptr, S.plen, S.pdel, val, x);
-        if (row < n) {
-            const double ti = b[row] - s;
-            if (t) t[row] = ti;
-            tt += ti * ti;
-            if (replace) {
-                const float rf = (float)ti;
-                r32[row] = rf;
-                rz += (double)rf * (double)(dinv32[row] * rf);
-            }
-        }
-    }
-    double v[2] = {tt, rz}, tot[2];
-    if (grid_sum<2>(v, partial, counter, tot)) {
-        sc->tt = tot[0];
-        if (replace) {
-            if (tot[0] < sc->thr) {
-                sc->done = 1;
-            } else {
-                sc->done = 0;
-                sc->rho = tot[1];
-            }
-        }
-    }
+    double s = 0.;
+    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * VB)
+        s += v[i] * v[i];
+    double a[1] = {s};
+    const int sl[1] = {P_TT};
+    block_partials<1>(a, part, sl);
+}
+
+__global__ __launch_bounds__(VB) void k_norm2_fin(const double* part, int g, Scal* sc)
+{
+    double t[1];
+    const int sl[1] = {P_TT};
+    totals<1>(part, g, sl, t);
+    if (threadIdx.x == 0) sc->tt = t[0];
 }
 
 // ---- SELL construction --------------------------------------------------------
+constexpr int MAXU = 160;  // longest per-slice offset list kept
+
+__device__ inline int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// union of the column-offset lists (col - row) of the 64 rows of a slice,
+// ascending: one wavefront per slice, repeated wave minimum of list heads
+__global__ __launch_bounds__(256) void k_slice_union(int64_t n, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ col,
+                                                     int32_t* __restrict__ udel,
+                                                     int32_t* __restrict__ ulen, int* err)
+{
+    const int64_t nsl = (n + 63) / 64;
+    const int lane = threadIdx.x & 63;
+    for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
+         slice += (int64_t)gridDim.x * 4) {
+        const int64_t row = slice * 64 + lane;
+        int32_t k = 0, e = 0;
+        if (row < n) {
+            k = rp[row];
+            e = rp[row + 1];
+        }
+        int cnt = 0;
+        while (true) {
+            const int head = (k < e) ? col[k] - (int32_t)row : INT32_MAX;
+            const int m = wave_min_i(head);
+            if (m == INT32_MAX) break;
+            if (lane == 0 && cnt < MAXU) udel[slice * MAXU + cnt] = m;
+            cnt++;
+            if (head == m) k++;
+        }
+        if (lane == 0) {
+            ulen[slice] = cnt;
+            if (cnt > MAXU) atomicOr(err, 4);
+        }
+    }
+}
+
 __device__ inline uint64_t mix64(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
@@ -396,27 +497,35 @@ __device__ inline uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-__global__ void k_row_hash(int64_t n, const int32_t* __restrict__ rp,
-                           const int32_t* __restrict__ col, uint64_t* __restrict__ h)
+__global__ void k_list_hash(int64_t m, const int32_t* __restrict__ udel,
+                            const int32_t* __restrict__ ulen, uint64_t* __restrict__ h)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t z = mix64(0x9e3779b97f4a7c15ULL + (uint64_t)(rp[r + 1] - rp[r]));
-        for (int32_t k = rp[r]; k < rp[r + 1]; k++)
-            z = mix64(z ^ (uint64_t)(uint32_t)(col[k] - (int32_t)r) * 0x100000001b3ULL);
-        h[r] = z | 1ULL;  // never 0 (0 marks an empty slot)
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < m;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int len = min(ulen[s], MAXU);
+        uint64_t z = mix64(0x9e3779b97f4a7c15ULL + (uint64_t)len);
+        for (int k = 0; k < len; k++)
+            z = mix64(z ^ (uint64_t)(uint32_t)udel[s * MAXU + k] * 0x100000001b3ULL);
+        h[s] = z | 1ULL;  // never 0 (0 marks an empty slot)
     }
 }
 
-// open-addressing table of distinct row hashes; value = a representative row
-__global__ void k_hash_insert(int64_t n, const uint64_t* __restrict__ h, uint64_t* keys,
+// open-addressing table of distinct list hashes; value = a representative
+__global__ void k_hash_insert(int64_t m, const uint64_t* __restrict__ h, uint64_t* keys,
                               int32_t* reps, uint64_t mask, int* err)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = h[r];
         uint64_t slot = key & mask;
         for (uint64_t probe = 0; probe <= mask; probe++) {
+            // most lists repeat: read first, CAS only empty slots
+            const uint64_t cur = keys[slot];
+            if (cur == key) break;
+            if (cur != 0ULL) {
+                slot = (slot + 1) & mask;
+                continue;
+            }
             const unsigned long long prev =
                 atomicCAS((unsigned long long*)&keys[slot], 0ULL, (unsigned long long)key);
             if (prev == 0ULL) {
@@ -430,11 +539,11 @@ __global__ void k_hash_insert(int64_t n, const uint64_t* __restrict__ h, uint64_
     }
 }
 
-__global__ void k_hash_find(int64_t n, const uint64_t* __restrict__ h,
+__global__ void k_hash_find(int64_t m, const uint64_t* __restrict__ h,
                             const uint64_t* __restrict__ keys, uint64_t mask,
                             int32_t* __restrict__ slot_of)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = h[r];
         uint64_t slot = key & mask;
@@ -451,25 +560,26 @@ __global__ void k_mark_slots(uint64_t cap, const uint64_t* __restrict__ keys,
         used[s] = keys[s] != 0ULL;
 }
 
-// pattern id of every row and check that the row equals its representative
-__global__ void k_row_pattern(int64_t n, const int32_t* __restrict__ rp,
-                              const int32_t* __restrict__ col, const int32_t* __restrict__ slot_of,
-                              const int32_t* __restrict__ slot_id,
-                              const int32_t* __restrict__ reps, int32_t* __restrict__ row_pat,
-                              int* err)
+// pattern id of every slice; check that the list equals its representative
+__global__ void k_list_pattern(int64_t m, const int32_t* __restrict__ udel,
+                               const int32_t* __restrict__ ulen,
+                               const int32_t* __restrict__ slot_of,
+                               const int32_t* __restrict__ slot_id,
+                               const int32_t* __restrict__ reps, int32_t* __restrict__ pat,
+                               int* err)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * blockDim.x) {
         const int32_t s = slot_of[r];
         const int64_t rep = reps[s];
-        row_pat[r] = slot_id[s];
-        const int32_t len = rp[r + 1] - rp[r];
-        if (rp[rep + 1] - rp[rep] != len) {
+        pat[r] = slot_id[s];
+        const int len = ulen[r];
+        if (ulen[rep] != len) {
             atomicOr(err, 2);
             continue;
         }
-        for (int32_t k = 0; k < len; k++)
-            if (col[rp[r] + k] - (int32_t)r != col[rp[rep] + k] - (int32_t)rep) {
+        for (int k = 0; k < min(len, MAXU); k++)
+            if (udel[r * MAXU + k] != udel[rep * MAXU + k]) {
                 atomicOr(err, 2);
                 break;
             }
@@ -478,19 +588,16 @@ __global__ void k_row_pattern(int64_t n, const int32_t* __restrict__ rp,
 
 __global__ void k_pat_len(uint64_t cap, const uint64_t* __restrict__ keys,
                           const int32_t* __restrict__ slot_id, const int32_t* __restrict__ reps,
-                          const int32_t* __restrict__ rp, int32_t* __restrict__ plen)
+                          const int32_t* __restrict__ ulen, int32_t* __restrict__ plen)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < (int64_t)cap;
          s += (int64_t)gridDim.x * blockDim.x)
-        if (keys[s]) {
-            const int64_t rep = reps[s];
-            plen[slot_id[s]] = rp[rep + 1] - rp[rep];
-        }
+        if (keys[s]) plen[slot_id[s]] = min(ulen[reps[s]], MAXU);
 }
 
 __global__ void k_pat_fill(uint64_t cap, const uint64_t* __restrict__ keys,
                            const int32_t* __restrict__ slot_id, const int32_t* __restrict__ reps,
-                           const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                           const int32_t* __restrict__ udel, const int32_t* __restrict__ ulen,
                            const int32_t* __restrict__ pptr, int32_t* __restrict__ pdel)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < (int64_t)cap;
@@ -498,56 +605,58 @@ __global__ void k_pat_fill(uint64_t cap, const uint64_t* __restrict__ keys,
         if (keys[s]) {
             const int64_t rep = reps[s];
             const int32_t o = pptr[slot_id[s]];
-            for (int32_t k = rp[rep]; k < rp[rep + 1]; k++) pdel[o + (k - rp[rep])] = col[k] - (int32_t)rep;
+            const int len = min(ulen[rep], MAXU);
+            for (int k = 0; k < len; k++) pdel[o + k] = udel[rep * MAXU + k];
         }
 }
 
-__global__ void k_slice_len(int64_t n, const int32_t* __restrict__ rp, int32_t* __restrict__ slen)
+__global__ void k_slice_vals(int64_t nsl, const int32_t* __restrict__ ulen, int32_t* __restrict__ sv)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nsl;
+         s += (int64_t)gridDim.x * blockDim.x)
+        sv[s] = ulen[s] * 64;
+}
+
+__global__ void k_widen(int64_t n, const int32_t* __restrict__ a, int64_t* __restrict__ b)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+         s += (int64_t)gridDim.x * blockDim.x)
+        b[s] = a[s];
+}
+
+// scatter CSR values into the slices: lane = row, walk the slice's offset
+// list and the row's columns together; absent entries are 0
+__global__ __launch_bounds__(256) void k_sell_fill(int64_t n, SellView S,
+                                                   const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col,
+                                                   const double* __restrict__ v64,
+                                                   const float* __restrict__ v32,
+                                                   double* __restrict__ s64, float* __restrict__ s32)
 {
     const int64_t nsl = (n + 63) / 64;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nsl;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        int32_t m = 0;
-        for (int64_t r = s * 64; r < n && r < s * 64 + 64; r++) m = max(m, rp[r + 1] - rp[r]);
-        slen[s] = m;
-    }
-}
-
-__global__ void k_slice_ptr(int64_t nsl, const int32_t* __restrict__ slen, int64_t* __restrict__ sptr)
-{
-    // single-thread scan over slices (nsl <= n/64 is small)
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        int64_t acc = 0;
-        for (int64_t s = 0; s < nsl; s++) {
-            sptr[s] = acc;
-            acc += (int64_t)slen[s] * 64;
+    const int lane = threadIdx.x & 63;
+    for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
+         slice += (int64_t)gridDim.x * 4) {
+        const int64_t row = slice * 64 + lane;
+        const int pid = S.spat[slice];
+        const int32_t* d = S.pdel + S.pptr[pid];
+        const int len = S.plen[pid];
+        const int64_t base = S.sptr[slice] + lane;
+        int32_t k = 0, e = 0;
+        if (row < n) {
+            k = rp[row];
+            e = rp[row + 1];
         }
-        sptr[nsl] = acc;
-    }
-}
-
-// scatter CSR values into slices (zero padding)
-__global__ void k_sell_fill(int64_t n, const int32_t* __restrict__ rp,
-                            const double* __restrict__ v64, const float* __restrict__ v32,
-                            const int64_t* __restrict__ sptr, const int32_t* __restrict__ slen,
-                            double* __restrict__ s64, float* __restrict__ s32)
-{
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t slice = r / 64;
-        const int lane = (int)(r % 64);
-        const int64_t base = sptr[slice] + lane;
-        const int32_t len = rp[r + 1] - rp[r];
-        const int32_t L = slen[slice];
-        for (int32_t k = 0; k < L; k++) {
-            const int64_t o = base + (int64_t)k * 64;
-            if (k < len) {
-                if (s64) s64[o] = v64 ? v64[rp[r] + k] : (double)v32[rp[r] + k];
-                if (s32) s32[o] = v32[rp[r] + k];
-            } else {
-                if (s64) s64[o] = 0.;
-                if (s32) s32[o] = 0.f;
+        for (int j = 0; j < len; j++) {
+            double a = 0.;
+            float f = 0.f;
+            if (k < e && col[k] - (int32_t)row == d[j]) {
+                if (v64) a = v64[k];
+                f = v32[k];
+                k++;
             }
+            if (s64) s64[base + (int64_t)j * 64] = a;
+            if (s32) s32[base + (int64_t)j * 64] = f;
         }
     }
 }
@@ -565,43 +674,40 @@ __global__ void k_distribute(int64_t nd, const int32_t* __restrict__ dof_id,
     }
 }
 
-__global__ __launch_bounds__(RB) void k_norm2(int64_t n, const double* __restrict__ v,
-                                              double* partial, unsigned* counter, Scal* sc)
+unsigned vec_grid(int64_t n)
 {
-    double s = 0.;
-    for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * RB)
-        s += v[i] * v[i];
-    double a[1] = {s}, tot[1];
-    if (grid_sum<1>(a, partial, counter, tot)) sc->tt = tot[0];
+    int64_t g = (n + VB - 1) / VB;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 512));
 }
 
-unsigned red_grid(int64_t n, int rows_per_block)
+unsigned sell_grid(int64_t n)
 {
-    int64_t g = (n + rows_per_block - 1) / rows_per_block;
-    if (g < 1) g = 1;
-    if (g > MAXG) g = MAXG;
-    return (unsigned)g;
+    const int64_t nsl = (n + 63) / 64;
+    int64_t g = (nsl + SB / 64 - 1) / (SB / 64);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, MAXG));
+}
+
+unsigned csr_grid(int64_t n, int lpr)
+{
+    int64_t g = (n * lpr + SB - 1) / SB;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, MAXG));
 }
 
 int ensure_solver_state(Ctx* c, int mode)
 {
     const int64_t n = c->n_active;
     INS_TRY(ensure(c, c->x, n));
-    INS_TRY(ensure(c, c->r, n));
-    INS_TRY(ensure(c, c->p, n));
-    INS_TRY(ensure(c, c->q, n));
-    if (mode == INS_SOLVE_MIXED) {
+    if (mode == INS_SOLVE_FP64) {
+        INS_TRY(ensure(c, c->r, n));
+        INS_TRY(ensure(c, c->p, n));
+        INS_TRY(ensure(c, c->q, n));
+    } else {
         INS_TRY(ensure(c, c->x32, n));
         INS_TRY(ensure(c, c->r32, n));
         INS_TRY(ensure(c, c->p32, n));
         INS_TRY(ensure(c, c->q32, n));
     }
-    if (!c->partial.p) INS_TRY(ensure(c, c->partial, (size_t)4 * MAXG));
-    if (!c->counter.p) {
-        INS_TRY(ensure(c, c->counter, 4));
-        INS_HIP_CHECK(c, hipMemsetAsync(c->counter.p, 0, 4 * sizeof(unsigned), c->stream));
-    }
+    if (!c->partial.p) INS_TRY(ensure(c, c->partial, (size_t)NSLOT * MAXG));
     if (!c->scal.p) INS_TRY(ensure(c, c->scal, 1));
     if (!c->h_scal) INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_scal, sizeof(Scal), 0));
     return INS_OK;
@@ -611,8 +717,7 @@ SellView sell_view(Ctx* c)
 {
     SellView v;
     v.sptr = c->sell.slice_ptr.p;
-    v.slen = c->sell.slice_len.p;
-    v.rpat = c->sell.row_pat.p;
+    v.spat = c->sell.slice_pat.p;
     v.pptr = c->sell.pat_ptr.p;
     v.plen = c->sell.pat_len.p;
     v.pdel = c->sell.pat_delta.p;
@@ -628,54 +733,59 @@ int pick_lpr(Ctx* c)
     return 4;
 }
 
+// K1; returns the number of partials written
 template <typename TV, typename TX, typename TA>
-int launch_spmv_pq(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q)
+int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
+                int* g_out)
 {
     const int64_t n = c->n_active;
     if (format == INS_FMT_SELL) {
-        const unsigned g = red_grid((n + 63) / 64, RB / 64);
-        k_spmv_pq_sell<TV, TX, TA><<<g, RB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q,
-                                                           c->partial.p, c->counter.p, c->scal.p);
+        const unsigned g = sell_grid(n);
+        k_spmv_sell<TV, TX, TA><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q,
+                                                         c->partial.p, c->scal.p);
+        *g_out = (int)g;
     } else {
         const int lpr = pick_lpr(c);
-        const unsigned g = red_grid(n, RB / lpr);
+        const unsigned g = csr_grid(n, lpr);
         if (lpr == 16)
-            k_spmv_pq_csr<16, TV, TX, TA><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->counter.p, c->scal.p);
+            k_spmv_csr<16, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
         else if (lpr == 8)
-            k_spmv_pq_csr<8, TV, TX, TA><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->counter.p, c->scal.p);
+            k_spmv_csr<8, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
         else
-            k_spmv_pq_csr<4, TV, TX, TA><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->counter.p, c->scal.p);
+            k_spmv_csr<4, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
+        *g_out = (int)g;
     }
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }
 
-int launch_resid(Ctx* c, int format, int replace, double* t)
+// |b - A x64| (and the residual replacement); then the finalising kernel
+int launch_resid(Ctx* c, int format, int replace, int par_next)
 {
     const int64_t n = c->n_active;
+    unsigned g;
+    const bool v64 = c->matrix_precision == INS_FP64;
     if (format == INS_FMT_SELL) {
-        const unsigned g = red_grid((n + 63) / 64, RB / 64);
-        if (c->matrix_precision == INS_FP64)
-            k_resid_sell<double><<<g, RB, 0, c->stream>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, t,
-                                                          c->r32.p, c->dinv32.p, c->partial.p,
-                                                          c->counter.p, c->scal.p, replace);
+        g = sell_grid(n);
+        if (v64)
+            k_resid_sell<double><<<g, SB, 0, c->stream>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
         else
-            k_resid_sell<float><<<g, RB, 0, c->stream>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, t,
-                                                         c->r32.p, c->dinv32.p, c->partial.p,
-                                                         c->counter.p, c->scal.p, replace);
+            k_resid_sell<float><<<g, SB, 0, c->stream>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
     } else {
         const int lpr = pick_lpr(c);
-        const unsigned g = red_grid(n, RB / lpr);
-        if (c->matrix_precision == INS_FP64) {
-            if (lpr == 16) k_resid_csr<16, double><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, t, c->r32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p, replace);
-            else if (lpr == 8) k_resid_csr<8, double><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, t, c->r32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p, replace);
-            else k_resid_csr<4, double><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, t, c->r32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p, replace);
+        g = csr_grid(n, lpr);
+        if (v64) {
+            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else k_resid_csr<4, double><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
         } else {
-            if (lpr == 16) k_resid_csr<16, float><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, t, c->r32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p, replace);
-            else if (lpr == 8) k_resid_csr<8, float><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, t, c->r32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p, replace);
-            else k_resid_csr<4, float><<<g, RB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, t, c->r32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p, replace);
+            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else k_resid_csr<4, float><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
         }
     }
+    INS_HIP_CHECK(c, hipGetLastError());
+    k_resid_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)g, c->scal.p, replace, par_next);
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }
@@ -688,86 +798,128 @@ int read_scal(Ctx* c)
     return INS_OK;
 }
 
-// profiling of the dominant kernel (the SpMV of K1)
+// profiling of the dominant kernel (the SpMV of K1) with HIP events on the
+// context stream
 struct ProfScope {
     Ctx* c;
     bool on;
     ProfScope(Ctx* c_) : c(c_), on(c_->profiling)
     {
-        if (on) hipEventRecord(c->evp0, c->stream);
+        if (on) (void)hipEventRecord(c->evp0, c->stream);
     }
     ~ProfScope()
     {
         if (on) {
-            hipEventRecord(c->evp1, c->stream);
-            hipEventSynchronize(c->evp1);
+            (void)hipEventRecord(c->evp1, c->stream);
+            (void)hipEventSynchronize(c->evp1);
             float ms = 0.f;
-            hipEventElapsedTime(&ms, c->evp0, c->evp1);
+            (void)hipEventElapsedTime(&ms, c->evp0, c->evp1);
             c->stats.spmv_ms += ms;
             c->stats.spmv_calls += 1;
         }
     }
 };
 
-}  // namespace
+// one chunk of PCG iterations (K1, K2, K3 per iteration)
+template <typename TV, typename TX>
+int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, TX* r, TX* p,
+              TX* q, const TX* dinv, int k, int64_t* it_launched)
+{
+    const int64_t n = c->n_active;
+    const unsigned gv = vec_grid(n);
+    for (int i = 0; i < k; i++) {
+        const int par = (int)(*it_launched & 1);
+        int g1 = 0;
+        {
+            ProfScope ps(c);
+            INS_TRY((launch_spmv<TV, TX, TX>(c, format, csr_val, sell_val, p, q, &g1)));
+        }
+        k_update_xr<TX, TX><<<gv, VB, 0, c->stream>>>(n, x, r, p, q, dinv, c->partial.p, g1,
+                                                      c->scal.p, par);
+        k_update_p<TX, TX><<<gv, VB, 0, c->stream>>>(n, p, r, dinv, c->partial.p, (int)gv,
+                                                     c->scal.p, par);
+        (*it_launched)++;
+    }
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
 
+}  // namespace
 int build_sell(Ctx* c)
 {
     Sell& S = c->sell;
     const int64_t n = c->n_active;
+    const int64_t nsl = (n + 63) / 64;
     S.n = n;
+    S.nslices = nsl;
     uint64_t* h = nullptr;
     uint64_t* keys = nullptr;
-    int32_t *reps = nullptr, *slot_of = nullptr, *used = nullptr, *slot_id = nullptr;
+    int32_t *udel = nullptr, *ulen = nullptr, *reps = nullptr, *slot_of = nullptr;
+    int32_t *used = nullptr, *slot_id = nullptr;
     int* err = nullptr;
     uint64_t cap = 1024;
-    while (cap < (uint64_t)(2 * n + 2)) cap <<= 1;
-    INS_HIP_CHECK(c, hipMalloc((void**)&h, n * sizeof(uint64_t) + 8));
+    while (cap < (uint64_t)(2 * nsl + 2)) cap <<= 1;
+    INS_HIP_CHECK(c, hipMalloc((void**)&udel, (size_t)nsl * MAXU * sizeof(int32_t) + 8));
+    INS_HIP_CHECK(c, hipMalloc((void**)&ulen, (nsl + 1) * sizeof(int32_t)));
+    INS_HIP_CHECK(c, hipMalloc((void**)&h, nsl * sizeof(uint64_t) + 8));
     INS_HIP_CHECK(c, hipMalloc((void**)&keys, cap * sizeof(uint64_t)));
     INS_HIP_CHECK(c, hipMalloc((void**)&reps, cap * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&slot_of, n * sizeof(int32_t) + 8));
+    INS_HIP_CHECK(c, hipMalloc((void**)&slot_of, nsl * sizeof(int32_t) + 8));
     INS_HIP_CHECK(c, hipMalloc((void**)&used, (cap + 1) * sizeof(int32_t)));
     INS_HIP_CHECK(c, hipMalloc((void**)&slot_id, (cap + 1) * sizeof(int32_t)));
     INS_HIP_CHECK(c, hipMalloc((void**)&err, sizeof(int)));
     INS_HIP_CHECK(c, hipMemsetAsync(keys, 0, cap * sizeof(uint64_t), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
-    const unsigned g = grid_for(n, 256);
-    k_row_hash<<<g, 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, h);
-    k_hash_insert<<<g, 256, 0, c->stream>>>(n, h, keys, reps, cap - 1, err);
-    k_hash_find<<<g, 256, 0, c->stream>>>(n, h, keys, cap - 1, slot_of);
+    k_slice_union<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p,
+                                                                      udel, ulen, err);
+    const unsigned g = grid_for(nsl, 256);
+    k_list_hash<<<g, 256, 0, c->stream>>>(nsl, udel, ulen, h);
+    k_hash_insert<<<g, 256, 0, c->stream>>>(nsl, h, keys, reps, cap - 1, err);
+    k_hash_find<<<g, 256, 0, c->stream>>>(nsl, h, keys, cap - 1, slot_of);
     k_mark_slots<<<grid_for(cap, 256), 256, 0, c->stream>>>(cap, keys, used);
     INS_HIP_CHECK(c, hipGetLastError());
     int64_t npat = 0;
     int s = exclusive_scan_i32(c, used, slot_id, (int64_t)cap, &npat);
     if (s != INS_OK) return s;
     S.npat = (int)npat;
-    INS_TRY(ensure(c, S.row_pat, n));
-    k_row_pattern<<<g, 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, slot_of, slot_id, reps,
-                                            S.row_pat.p, err);
+    INS_TRY(ensure(c, S.slice_pat, nsl + 1));
+    k_list_pattern<<<g, 256, 0, c->stream>>>(nsl, udel, ulen, slot_of, slot_id, reps,
+                                             S.slice_pat.p, err);
     INS_TRY(ensure(c, S.pat_len, npat + 1));
     INS_TRY(ensure(c, S.pat_ptr, npat + 1));
-    k_pat_len<<<grid_for(cap, 256), 256, 0, c->stream>>>(cap, keys, slot_id, reps, c->row_ptr.p,
+    k_pat_len<<<grid_for(cap, 256), 256, 0, c->stream>>>(cap, keys, slot_id, reps, ulen,
                                                          S.pat_len.p);
     INS_HIP_CHECK(c, hipGetLastError());
     int64_t ndel = 0;
     s = exclusive_scan_i32(c, S.pat_len.p, S.pat_ptr.p, npat, &ndel);
     if (s != INS_OK) return s;
-    INS_TRY(ensure(c, S.pat_delta, ndel));
-    k_pat_fill<<<grid_for(cap, 256), 256, 0, c->stream>>>(cap, keys, slot_id, reps, c->row_ptr.p,
-                                                          c->col.p, S.pat_ptr.p, S.pat_delta.p);
-    const int64_t nsl = (n + 63) / 64;
-    S.nslices = nsl;
-    INS_TRY(ensure(c, S.slice_len, nsl + 1));
+    INS_TRY(ensure(c, S.pat_delta, ndel + 1));
+    k_pat_fill<<<grid_for(cap, 256), 256, 0, c->stream>>>(cap, keys, slot_id, reps, udel, ulen,
+                                                          S.pat_ptr.p, S.pat_delta.p);
     INS_TRY(ensure(c, S.slice_ptr, nsl + 1));
-    k_slice_len<<<grid_for(nsl, 256), 256, 0, c->stream>>>(n, c->row_ptr.p, S.slice_len.p);
-    k_slice_ptr<<<1, 1, 0, c->stream>>>(nsl, S.slice_len.p, S.slice_ptr.p);
-    INS_HIP_CHECK(c, hipGetLastError());
+    {
+        int32_t* sv = nullptr;
+        int32_t* sp32 = nullptr;
+        INS_HIP_CHECK(c, hipMalloc((void**)&sv, (nsl + 1) * sizeof(int32_t)));
+        INS_HIP_CHECK(c, hipMalloc((void**)&sp32, (nsl + 1) * sizeof(int32_t)));
+        k_slice_vals<<<grid_for(nsl, 256), 256, 0, c->stream>>>(nsl, ulen, sv);
+        int64_t tv = 0;
+        s = exclusive_scan_i32(c, sv, sp32, nsl, &tv);
+        if (s != INS_OK) return s;
+        k_widen<<<grid_for(nsl + 1, 256), 256, 0, c->stream>>>(nsl + 1, sp32, S.slice_ptr.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        hipFree(sv);
+        hipFree(sp32);
+    }
     int herr = 0;
     int64_t nvals = 0;
     INS_HIP_CHECK(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     INS_HIP_CHECK(c, hipMemcpyAsync(&nvals, S.slice_ptr.p + nsl, sizeof(int64_t),
                                     hipMemcpyDeviceToHost, c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    hipFree(udel);
+    hipFree(ulen);
     hipFree(h);
     hipFree(keys);
     hipFree(reps);
@@ -776,22 +928,26 @@ int build_sell(Ctx* c)
     hipFree(slot_id);
     hipFree(err);
     S.nvals = nvals;
-    // worth it only if the offset dictionary is small and padding modest
-    S.ok = (herr == 0) && (ndel * 8 <= c->nnz + 64) && (nvals <= c->nnz + c->nnz / 4 + 64 * 64);
-    if (!S.ok) {
-        S.built = true;
-        return INS_OK;
-    }
-    if (c->matrix_precision == INS_FP64) INS_TRY(ensure(c, S.v64, nvals));
-    else release(c, S.v64);
-    INS_TRY(ensure(c, S.v32, nvals));
-    k_sell_fill<<<grid_for(n, 256), 256, 0, c->stream>>>(
-        n, c->row_ptr.p, (c->matrix_precision == INS_FP64) ? c->val64.p : nullptr, c->val32.p,
-        S.slice_ptr.p, S.slice_len.p, (c->matrix_precision == INS_FP64) ? S.v64.p : nullptr,
-        S.v32.p);
-    INS_HIP_CHECK(c, hipGetLastError());
-    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    // worth it only when the shared lists stay close to the rows' own lists
+    S.ok = (herr == 0) && (nvals <= c->nnz + c->nnz / 8 + 64 * 64);
     S.built = true;
+    S.stale = true;
+    return INS_OK;
+}
+
+// copy the current CSR values into the slices (after every assembly)
+int fill_sell(Ctx* c)
+{
+    Sell& S = c->sell;
+    const int64_t n = c->n_active;
+    if (c->matrix_precision == INS_FP64) INS_TRY(ensure(c, S.v64, S.nvals));
+    INS_TRY(ensure(c, S.v32, S.nvals));
+    k_sell_fill<<<grid_for((n + 63) / 64, 4, 1u << 16), 256, 0, c->stream>>>(
+        n, sell_view(c), c->row_ptr.p, c->col.p,
+        (c->matrix_precision == INS_FP64) ? c->val64.p : nullptr, c->val32.p,
+        (c->matrix_precision == INS_FP64) ? S.v64.p : nullptr, S.v32.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    S.stale = false;
     return INS_OK;
 }
 
@@ -803,61 +959,46 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
         return fail(c, INS_ERR_STATE, "FP64 solve needs an FP64 assembly");
     if (format == INS_FMT_SELL && !c->sell.built) INS_TRY(build_sell(c));
     if (format == INS_FMT_SELL && !c->sell.ok) format = INS_FMT_CSR;
+    if (format == INS_FMT_SELL && c->sell.stale) INS_TRY(fill_sell(c));
     INS_TRY(ensure_solver_state(c, mode));
     const int64_t n = c->n_active;
-    const unsigned gv = red_grid(n, RB);
+    const unsigned gv = vec_grid(n);
     c->stats.spmv_ms = 0.;
     c->stats.spmv_calls = 0;
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     int refinements = 0;
-    int status = INS_OK;
+    int64_t launched = 0;
     if (mode == INS_SOLVE_FP64) {
         const double* sv = (format == INS_FMT_SELL) ? c->sell.v64.p : nullptr;
-        k_init<double, double><<<gv, RB, 0, c->stream>>>(n, c->b.p, c->x.p, c->r.p, c->p.p,
-                                                         c->dinv64.p, rtol * rtol, c->partial.p,
-                                                         c->counter.p, c->scal.p);
+        k_init<double, double><<<gv, VB, 0, c->stream>>>(n, c->b.p, c->x.p, c->r.p, c->p.p,
+                                                         c->dinv64.p, c->partial.p);
+        k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, c->scal.p);
         INS_HIP_CHECK(c, hipGetLastError());
         INS_TRY(read_scal(c));
         while (!c->h_scal->done && c->h_scal->iters < max_it) {
             const int k = std::min(CHUNK, max_it - c->h_scal->iters);
-            for (int it = 0; it < k; it++) {
-                {
-                    ProfScope ps(c);
-                    INS_TRY((launch_spmv_pq<double, double, double>(c, format, c->val64.p, sv, c->p.p, c->q.p)));
-                }
-                k_update_xr<double, double><<<gv, RB, 0, c->stream>>>(n, c->x.p, c->r.p, c->p.p, c->q.p, c->dinv64.p, c->partial.p, c->counter.p, c->scal.p);
-                k_update_p<double, double><<<gv, RB, 0, c->stream>>>(n, c->p.p, c->r.p, c->dinv64.p, c->scal.p);
-            }
-            INS_HIP_CHECK(c, hipGetLastError());
+            INS_TRY((run_chunk<double, double>(c, format, c->val64.p, sv, c->x.p, c->r.p, c->p.p,
+                                               c->q.p, c->dinv64.p, k, &launched)));
             INS_TRY(read_scal(c));
         }
-        // true residual |b - A x|
-        INS_TRY(launch_resid(c, format, 0, nullptr));
+        INS_TRY(launch_resid(c, format, 0, 0));
     } else {
-        // mixed: FP32 sweeps, FP64 residual replacement at every chunk end
+        // FP32 sweeps with an FP64 residual replacement at every chunk end
         const float* sv = (format == INS_FMT_SELL) ? c->sell.v32.p : nullptr;
         INS_HIP_CHECK(c, hipMemsetAsync(c->x.p, 0, n * sizeof(double), c->stream));
-        k_init<float, float><<<gv, RB, 0, c->stream>>>(n, c->b.p, c->x32.p, c->r32.p, c->p32.p,
-                                                       c->dinv32.p, rtol * rtol, c->partial.p,
-                                                       c->counter.p, c->scal.p);
+        k_init<float, float><<<gv, VB, 0, c->stream>>>(n, c->b.p, c->x32.p, c->r32.p, c->p32.p,
+                                                       c->dinv32.p, c->partial.p);
+        k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, c->scal.p);
         INS_HIP_CHECK(c, hipGetLastError());
         INS_TRY(read_scal(c));
         bool converged = c->h_scal->done && !c->h_scal->breakdown;
         while (!converged && c->h_scal->iters < max_it && !c->h_scal->breakdown) {
             const int k = std::min(CHUNK, max_it - c->h_scal->iters);
-            for (int it = 0; it < k; it++) {
-                {
-                    ProfScope ps(c);
-                    INS_TRY((launch_spmv_pq<float, float, float>(c, format, c->val32.p, sv, c->p32.p, c->q32.p)));
-                }
-                k_update_xr<float, float><<<gv, RB, 0, c->stream>>>(n, c->x32.p, c->r32.p, c->p32.p, c->q32.p, c->dinv32.p, c->partial.p, c->counter.p, c->scal.p);
-                k_update_p<float, float><<<gv, RB, 0, c->stream>>>(n, c->p32.p, c->r32.p, c->dinv32.p, c->scal.p);
-            }
+            INS_TRY((run_chunk<float, float>(c, format, c->val32.p, sv, c->x32.p, c->r32.p,
+                                             c->p32.p, c->q32.p, c->dinv32.p, k, &launched)));
+            k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
             INS_HIP_CHECK(c, hipGetLastError());
-            // residual replacement: x64 += x32; r = b - A x64 (FP64)
-            k_fold<<<gv, RB, 0, c->stream>>>(n, c->x.p, c->x32.p);
-            INS_HIP_CHECK(c, hipGetLastError());
-            INS_TRY(launch_resid(c, format, 1, nullptr));
+            INS_TRY(launch_resid(c, format, 1, (int)(launched & 1)));
             refinements++;
             INS_TRY(read_scal(c));
             converged = c->h_scal->done && !c->h_scal->breakdown;
@@ -877,6 +1018,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     if (iters) *iters = S.iters;
     if (relres) *relres = rel;
     c->have_solution = true;
+    int status = INS_OK;
     if (S.breakdown) status = fail(c, INS_ERR_BREAKDOWN, "CG breakdown (p.Ap <= 0 or non-finite)");
     else if (!(rel <= rtol)) status = fail(c, INS_ERR_NOCONV, "iteration limit reached before rtol");
     return status;
@@ -896,7 +1038,9 @@ int distribute(Ctx* c, int op)
 int vec_norm2(Ctx* c, const double* v, int64_t n, double* out)
 {
     INS_TRY(ensure_solver_state(c, INS_SOLVE_FP64));
-    k_norm2<<<red_grid(n, RB), RB, 0, c->stream>>>(n, v, c->partial.p, c->counter.p, c->scal.p);
+    const unsigned g = vec_grid(n);
+    k_norm2<<<g, VB, 0, c->stream>>>(n, v, c->partial.p);
+    k_norm2_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)g, c->scal.p);
     INS_HIP_CHECK(c, hipGetLastError());
     INS_TRY(read_scal(c));
     *out = std::sqrt(c->h_scal->tt);
