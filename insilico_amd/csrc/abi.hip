@@ -71,7 +71,7 @@ static void release_all(Ctx* c)
     release(c, c->stage_ke); release(c, c->stage_fe); release(c, c->stage_ff);
     release(c, c->sell.slice_ptr); release(c, c->sell.slice_pat);
     release(c, c->sell.pat_ptr); release(c, c->sell.pat_delta); release(c, c->sell.pat_len);
-    release(c, c->sell.v64); release(c, c->sell.v32);
+    release(c, c->sell.v64); release(c, c->sell.v32); release(c, c->sell.map);
     release(c, c->x); release(c, c->r); release(c, c->p); release(c, c->q); release(c, c->t64);
     release(c, c->x32); release(c, c->r32); release(c, c->p32); release(c, c->q32);
     release(c, c->partial); release(c, c->scal); release(c, c->dflag);
@@ -130,6 +130,9 @@ int ins_destroy(ins_ctx* h)
     hipStreamSynchronize(c->stream);
     release_all(c);
     if (c->h_scal) hipHostFree(c->h_scal);
+    if (c->h_snap) hipHostFree(c->h_snap);
+    for (int i = 0; i < 2; i++)
+        if (c->evs[i]) hipEventDestroy(c->evs[i]);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipEventDestroy(c->evp0);

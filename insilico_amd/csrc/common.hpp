@@ -54,7 +54,11 @@ struct Scal {
     int done;        // 1 = stop (converged, breakdown or empty rhs)
     int breakdown;
     int iters;       // SpMVs performed inside the loop
+    int restart;     // mixed: restart the direction after a replacement
+    int pause;       // mixed: the residual dropped 10x, replace it now
     int pad;
+    double rref;     // mixed: |b - A x|^2 at the last replacement
+    double delta2;   // mixed: replace when |r|^2 < delta2 * rref
 };
 
 // SELL-64 matrix: 64-row slices, values column-major inside a slice; every
@@ -69,6 +73,7 @@ struct Sell {
     DArr<int32_t> pat_ptr;   // npat+1
     DArr<int32_t> pat_delta; // concatenated column offsets
     DArr<int32_t> pat_len;   // npat
+    DArr<int32_t> map;       // nnz: slice position of every CSR entry
     DArr<double> v64;
     DArr<float> v32;
     bool ok = false;         // dictionary small enough to be worth it
@@ -141,6 +146,8 @@ struct Ctx {
     DArr<double> partial;
     DArr<Scal> scal;
     Scal* h_scal = nullptr;  // pinned
+    Scal* h_snap = nullptr;  // pinned, 2 snapshots for the pipelined host loop
+    hipEvent_t evs[2] = {nullptr, nullptr};
     DArr<int> dflag;
     bool have_solution = false;
 

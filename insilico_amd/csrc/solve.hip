@@ -22,6 +22,7 @@
 // residual.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "common.hpp"
 
@@ -32,7 +33,8 @@ namespace {
 constexpr int VB = 512;       // threads per block of the vector kernels
 constexpr int SB = 256;       // threads per block of the SpMV kernels
 constexpr int MAXG = 2048;    // max blocks of a partial-producing kernel
-constexpr int CHUNK = 40;     // iterations between host checks
+constexpr int CHUNK = 32;     // FP64 iterations between host checks
+constexpr int MCHUNK = 16;    // mixed: iterations between host checks
 // partial slots
 constexpr int P_PQ = 0, P_RR = 1, P_RZ = 2, P_TT = 3, P_BB = 4, NSLOT = 5;
 
@@ -88,6 +90,50 @@ __device__ inline void totals(const double* part, int cnt, const int (&slot)[NV]
 
 __device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
+// Workgroups are placed on the 8 XCDs round-robin by block id.  Each XCD
+// owns one contiguous eighth of the items (rows or slices) and its blocks
+// sweep that eighth together in grid-stride order: 8 sequential streams,
+// one per XCD, with the x gathers of the SpMV and the vectors reused by the
+// next kernel local to that XCD's L2.
+constexpr int NXCD = 8;
+
+struct Sweep {
+    int64_t first;   // first item of this block
+    int64_t end;     // end of the XCD's range
+    int64_t stride;  // items between two visits of this block
+};
+
+// items are visited as first + j*stride (+ per-block offsets < step)
+__device__ int g_xcd_mode = 0;  // 1: per-XCD ranges (set from INS_XCD)
+__device__ int g_nt = 0;        // 1: streaming (non-temporal) vector stores and value loads
+
+template <typename T>
+__device__ inline void st(T* p, T v, bool nt)
+{
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+__device__ inline Sweep xcd_sweep(int64_t total, int64_t step)
+{
+    const int g = gridDim.x;
+    Sweep w;
+    if ((g % NXCD) || !g_xcd_mode) {
+        w.first = (int64_t)blockIdx.x * step;
+        w.end = total;
+        w.stride = (int64_t)g * step;
+        return w;
+    }
+    const int xcd = blockIdx.x % NXCD, local = blockIdx.x / NXCD, nloc = g / NXCD;
+    int64_t per = (total + NXCD - 1) / NXCD;
+    per = (per + step - 1) / step * step;
+    const int64_t lo = (int64_t)xcd * per;
+    w.first = lo + (int64_t)local * step;
+    w.end = lo + per < total ? lo + per : total;
+    w.stride = (int64_t)nloc * step;
+    return w;
+}
+
 // ---------------------------------------------------------------------------
 // SpMV rows
 struct SellView {
@@ -98,7 +144,27 @@ struct SellView {
     const int32_t* pdel;
 };
 
-constexpr int SU = 9;  // SELL unroll (27-point rows = 3 chunks, 81 = 9)
+// Values of a slice are stored in groups of 4 consecutive entries per row:
+// position = slice_base + (k/4)*256 + lane*4 + k%4, so a lane reads 4 entries
+// of its row with one 16-byte load (32 bytes for FP64) and a wavefront reads
+// 1 KB contiguous.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <typename TV>
+__device__ inline void load4(const TV* p, TV (&o)[4])
+{
+    if constexpr (sizeof(TV) == 4) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    } else {
+        const f64x2 a = *reinterpret_cast<const f64x2*>(p);
+        const f64x2 b = *reinterpret_cast<const f64x2*>(p + 2);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+}
+
+constexpr int SG = 3;  // groups of 4 entries in flight per lane (27-point rows: 3 rounds)
 
 // one lane per row of a 64-row slice; all rows of a slice share one
 // column-offset list (the union of their own lists, values zero-padded), so
@@ -110,28 +176,37 @@ __device__ inline TA sell_row(int64_t row, int64_t n, int lane, int64_t slice, c
 {
     TA s = (TA)0;
     const bool live = row < n;
-    const int64_t base = S.sptr[slice] + lane;
+    const int64_t base = S.sptr[slice] + lane * 4;
     const int pid = S.spat[slice];
     const int32_t* d = S.pdel + S.pptr[pid];
     const int len = S.plen[pid];
     const int64_t last = n - 1;
-    for (int k0 = 0; k0 < len; k0 += SU) {
-        TV vv[SU];
-        TX xx[SU];
+    for (int k0 = 0; k0 < len; k0 += 4 * SG) {
+        TV vv[SG][4];
+        TX xx[SG][4];
 #pragma unroll
-        for (int j = 0; j < SU; j++) {
-            const int k = k0 + j;
-            vv[j] = (TV)0;
-            xx[j] = (TX)0;
-            if (k < len && live) {
-                int64_t idx = row + d[k];
-                idx = idx < 0 ? 0 : (idx > last ? last : idx);
-                vv[j] = v[base + (int64_t)k * 64];
-                xx[j] = x[idx];
+        for (int g = 0; g < SG; g++) {
+            const int kg = k0 + 4 * g;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                vv[g][j] = (TV)0;
+                xx[g][j] = (TX)0;
+            }
+            if (kg < len && live) {
+                load4<TV>(v + base + (int64_t)(kg >> 2) * 256, vv[g]);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (kg + j < len) {
+                        int64_t idx = row + d[kg + j];
+                        idx = idx < 0 ? 0 : (idx > last ? last : idx);
+                        xx[g][j] = x[idx];
+                    }
             }
         }
 #pragma unroll
-        for (int j = 0; j < SU; j++) s += (TA)vv[j] * (TA)xx[j];
+        for (int g = 0; g < SG; g++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) s += (TA)vv[g][j] * (TA)xx[g][j];
     }
     return s;
 }
@@ -171,17 +246,17 @@ __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const T
                                                   const TX* __restrict__ p, TX* __restrict__ q,
                                                   double* part, const Scal* sc)
 {
-    if (sc->done) return;
+    if (sc->done | sc->pause) return;
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
     double pq = 0.;
-    for (int64_t slice = (int64_t)blockIdx.x * (SB / 64) + (threadIdx.x >> 6); slice < nsl;
-         slice += (int64_t)gridDim.x * (SB / 64)) {
+    const Sweep w = xcd_sweep(nsl, SB / 64);
+    for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
         const int64_t row = slice * 64 + lane;
         const TA s = sell_row<TV, TX, TA>(row, n, lane, slice, S, val, p);
         if (row < n) {
             const TX qs = (TX)s;
-            q[row] = qs;
+            st(&q[row], qs, g_nt);
             pq += (double)p[row] * (double)qs;
         }
     }
@@ -197,15 +272,15 @@ __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __res
                                                  const TX* __restrict__ p, TX* __restrict__ q,
                                                  double* part, const Scal* sc)
 {
-    if (sc->done) return;
+    if (sc->done | sc->pause) return;
     const int sub = threadIdx.x & (LPR - 1);
-    const int64_t stride = (int64_t)gridDim.x * (SB / LPR);
     double pq = 0.;
-    for (int64_t row = ((int64_t)blockIdx.x * SB + threadIdx.x) / LPR; row < n; row += stride) {
+    const Sweep w = xcd_sweep(n, SB / LPR);
+    for (int64_t row = w.first + threadIdx.x / LPR; row < w.end; row += w.stride) {
         const TA s = csr_row<LPR, TV, TX, TA>(row, sub, rp, col, val, p);
         if (sub == 0) {
             const TX qs = (TX)s;
-            q[row] = qs;
+            st(&q[row], qs, g_nt);
             pq += (double)p[row] * (double)qs;
         }
     }
@@ -222,7 +297,7 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
                                                   const TD* __restrict__ dinv, double* part,
                                                   int g_pq, Scal* sc, int par)
 {
-    if (sc->done) return;
+    if (sc->done | sc->pause) return;
     double t[1];
     const int slq[1] = {P_PQ};
     totals<1>(part, g_pq, slq, t);
@@ -242,12 +317,13 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
         sc->iters += 1;
     }
     const TX alpha = (TX)ad;
+    const bool nt = g_nt;
     double rr = 0., rz = 0.;
-    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * VB) {
-        x[i] += alpha * p[i];
+    const Sweep w = xcd_sweep(n, VB);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
+        st(&x[i], x[i] + alpha * p[i], nt);
         const TX ri = r[i] - alpha * q[i];
-        r[i] = ri;
+        st(&r[i], ri, nt);
         const TX zi = (TX)dinv[i] * ri;
         rr += (double)ri * (double)ri;
         rz += (double)ri * (double)zi;
@@ -264,7 +340,7 @@ __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
                                                  const TD* __restrict__ dinv, const double* part,
                                                  int g, Scal* sc, int par)
 {
-    if (sc->done) return;
+    if (sc->done | sc->pause) return;
     double t[2];
     const int sl[2] = {P_RR, P_RZ};
     totals<2>(part, g, sl, t);
@@ -282,13 +358,16 @@ __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
         } else {
             sc->rho[par ^ 1] = rz;
             sc->beta = rz / rho;
+            // mixed precision: ask for a residual replacement once the
+            // recursive residual dropped by 1/delta since the last one
+            if (rr < sc->delta2 * sc->rref) sc->pause = 1;
         }
     }
     if (bad || conv) return;
     const TX beta = (TX)(rz / rho);
-    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * VB)
-        p[i] = (TX)dinv[i] * r[i] + beta * p[i];
+    const Sweep w = xcd_sweep(n, VB);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride)
+        st(&p[i], (TX)dinv[i] * r[i] + beta * p[i], g_nt);
 }
 
 // start: x = 0, r = b, p = D^-1 b; partials of b.b and r.z
@@ -299,8 +378,8 @@ __global__ __launch_bounds__(VB) void k_init(int64_t n, const double* __restrict
                                              double* part)
 {
     double bb = 0., rz = 0.;
-    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * VB) {
+    const Sweep w = xcd_sweep(n, VB);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
         const TX bi = (TX)b[i];
         x[i] = (TX)0;
         r[i] = bi;
@@ -314,7 +393,8 @@ __global__ __launch_bounds__(VB) void k_init(int64_t n, const double* __restrict
     block_partials<2>(v, part, sl);
 }
 
-__global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, double rtol2, Scal* sc)
+__global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, double rtol2,
+                                                 double delta2, Scal* sc)
 {
     double t[2];
     const int sl[2] = {P_BB, P_RZ};
@@ -328,7 +408,11 @@ __global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, doub
         sc->rho[1] = t[1];
         sc->iters = 0;
         sc->breakdown = 0;
+        sc->delta2 = delta2;
         sc->done = (t[0] == 0.) ? 1 : 0;
+        sc->restart = 0;
+        sc->pause = 0;
+        sc->rref = t[0];
     }
 }
 
@@ -336,8 +420,8 @@ __global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, doub
 __global__ __launch_bounds__(VB) void k_fold(int64_t n, double* __restrict__ x64,
                                              float* __restrict__ x32)
 {
-    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * VB) {
+    const Sweep w = xcd_sweep(n, VB);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
         x64[i] += (double)x32[i];
         x32[i] = 0.f;
     }
@@ -355,8 +439,8 @@ __global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const 
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
     double tt = 0., rz = 0.;
-    for (int64_t slice = (int64_t)blockIdx.x * (SB / 64) + (threadIdx.x >> 6); slice < nsl;
-         slice += (int64_t)gridDim.x * (SB / 64)) {
+    const Sweep w = xcd_sweep(nsl, SB / 64);
+    for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
         const int64_t row = slice * 64 + lane;
         const double s = sell_row<TV, double, double>(row, n, lane, slice, S, val, x);
         if (row < n) {
@@ -385,9 +469,9 @@ __global__ __launch_bounds__(SB) void k_resid_csr(int64_t n, const int32_t* __re
                                                   double* part, int replace)
 {
     const int sub = threadIdx.x & (LPR - 1);
-    const int64_t stride = (int64_t)gridDim.x * (SB / LPR);
     double tt = 0., rz = 0.;
-    for (int64_t row = ((int64_t)blockIdx.x * SB + threadIdx.x) / LPR; row < n; row += stride) {
+    const Sweep w = xcd_sweep(n, SB / LPR);
+    for (int64_t row = w.first + threadIdx.x / LPR; row < w.end; row += w.stride) {
         const double s = csr_row<LPR, TV, double, double>(row, sub, rp, col, val, x);
         if (sub == 0) {
             const double ti = b[row] - s;
@@ -421,18 +505,35 @@ __global__ __launch_bounds__(VB) void k_resid_fin(const double* part, int g, Sca
             } else if (t[0] < sc->thr) {
                 sc->done = 1;
             } else {
+                // keep the search direction only while the recursion still
+                // tracks the true residual; once the FP32 recursion has
+                // stopped or drifted, restart from p = D^-1 r
+                sc->restart = (sc->done || t[0] > 64. * sc->rr) ? 1 : 0;
                 sc->done = 0;
+                sc->pause = 0;
                 sc->rho[par_next] = t[1];
+                sc->rr = t[0];
+                sc->rref = t[0];
             }
         }
     }
 }
 
+// p = D^-1 r after a residual replacement that asked for a restart
+__global__ __launch_bounds__(VB) void k_restart(int64_t n, float* __restrict__ p,
+                                                const float* __restrict__ r,
+                                                const float* __restrict__ dinv, const Scal* sc)
+{
+    if (!sc->restart || sc->done) return;
+    const Sweep w = xcd_sweep(n, VB);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) p[i] = dinv[i] * r[i];
+}
+
 __global__ __launch_bounds__(VB) void k_norm2(int64_t n, const double* __restrict__ v, double* part)
 {
     double s = 0.;
-    for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * VB)
+    const Sweep w = xcd_sweep(n, VB);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride)
         s += v[i] * v[i];
     double a[1] = {s};
     const int sl[1] = {P_TT};
@@ -614,7 +715,7 @@ __global__ void k_slice_vals(int64_t nsl, const int32_t* __restrict__ ulen, int3
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nsl;
          s += (int64_t)gridDim.x * blockDim.x)
-        sv[s] = ulen[s] * 64;
+        sv[s] = ((ulen[s] + 3) & ~3) * 64;
 }
 
 __global__ void k_widen(int64_t n, const int32_t* __restrict__ a, int64_t* __restrict__ b)
@@ -624,40 +725,42 @@ __global__ void k_widen(int64_t n, const int32_t* __restrict__ a, int64_t* __res
         b[s] = a[s];
 }
 
-// scatter CSR values into the slices: lane = row, walk the slice's offset
-// list and the row's columns together; absent entries are 0
-__global__ __launch_bounds__(256) void k_sell_fill(int64_t n, SellView S,
-                                                   const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ col,
-                                                   const double* __restrict__ v64,
-                                                   const float* __restrict__ v32,
-                                                   double* __restrict__ s64, float* __restrict__ s32)
+// position of every CSR entry inside the slices: lane = row, walk the
+// slice's offset list and the row's columns together (setup, once)
+__global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
+                                                  const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  int32_t* __restrict__ map, int* err)
 {
     const int64_t nsl = (n + 63) / 64;
     const int lane = threadIdx.x & 63;
     for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
          slice += (int64_t)gridDim.x * 4) {
         const int64_t row = slice * 64 + lane;
+        if (row >= n) continue;
         const int pid = S.spat[slice];
         const int32_t* d = S.pdel + S.pptr[pid];
         const int len = S.plen[pid];
-        const int64_t base = S.sptr[slice] + lane;
-        int32_t k = 0, e = 0;
-        if (row < n) {
-            k = rp[row];
-            e = rp[row + 1];
-        }
-        for (int j = 0; j < len; j++) {
-            double a = 0.;
-            float f = 0.f;
-            if (k < e && col[k] - (int32_t)row == d[j]) {
-                if (v64) a = v64[k];
-                f = v32[k];
-                k++;
-            }
-            if (s64) s64[base + (int64_t)j * 64] = a;
-            if (s32) s32[base + (int64_t)j * 64] = f;
-        }
+        const int64_t base = S.sptr[slice] + lane * 4;
+        int32_t k = rp[row];
+        const int32_t e = rp[row + 1];
+        for (int j = 0; j < len && k < e; j++)
+            if (col[k] - (int32_t)row == d[j])
+                map[k++] = (int32_t)(base + (int64_t)(j >> 2) * 256 + (j & 3));
+        if (k != e) atomicOr(err, 8);
+    }
+}
+
+// copy CSR values to their slice positions (padding stays zero)
+__global__ void k_sell_scatter(int64_t nnz, const int32_t* __restrict__ map,
+                               const double* __restrict__ v64, const float* __restrict__ v32,
+                               double* __restrict__ s64, float* __restrict__ s32)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t o = map[k];
+        if (s64) s64[o] = v64[k];
+        s32[o] = v32[k];
     }
 }
 
@@ -674,23 +777,37 @@ __global__ void k_distribute(int64_t nd, const int32_t* __restrict__ dof_id,
     }
 }
 
+unsigned round8(int64_t g)
+{
+    if (g <= 8) return (unsigned)std::max<int64_t>(1, g);
+    return (unsigned)((g + 7) / 8 * 8);
+}
+
 unsigned vec_grid(int64_t n)
 {
     int64_t g = (n + VB - 1) / VB;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 512));
+    return round8(std::max<int64_t>(1, std::min<int64_t>(g, 512)));
 }
 
 unsigned sell_grid(int64_t n)
 {
     const int64_t nsl = (n + 63) / 64;
     int64_t g = (nsl + SB / 64 - 1) / (SB / 64);
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, MAXG));
+    return round8(std::max<int64_t>(1, std::min<int64_t>(g, MAXG)));
 }
 
 unsigned csr_grid(int64_t n, int lpr)
 {
     int64_t g = (n * lpr + SB - 1) / SB;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, MAXG));
+    return round8(std::max<int64_t>(1, std::min<int64_t>(g, MAXG)));
+}
+
+// tuning knobs (environment, read once): INS_CSR_LPR = 4/8/16/32 lanes per
+// CSR row, INS_MIXED_CHUNK, INS_XCD, INS_NT
+int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    return v ? atoi(v) : dflt;
 }
 
 int ensure_solver_state(Ctx* c, int mode)
@@ -710,6 +827,15 @@ int ensure_solver_state(Ctx* c, int mode)
     if (!c->partial.p) INS_TRY(ensure(c, c->partial, (size_t)NSLOT * MAXG));
     if (!c->scal.p) INS_TRY(ensure(c, c->scal, 1));
     if (!c->h_scal) INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_scal, sizeof(Scal), 0));
+    if (!c->h_snap) {
+        INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_snap, 2 * sizeof(Scal), 0));
+        const int mode = env_int("INS_XCD", 0);
+        INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_mode), &mode, sizeof(int)));
+        const int nt = env_int("INS_NT", 0);
+        INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_nt), &nt, sizeof(int)));
+    }
+    for (int i = 0; i < 2; i++)
+        if (!c->evs[i]) INS_HIP_CHECK(c, hipEventCreateWithFlags(&c->evs[i], hipEventDisableTiming));
     return INS_OK;
 }
 
@@ -727,6 +853,8 @@ SellView sell_view(Ctx* c)
 // lanes per row of the CSR kernels, from the mean row length
 int pick_lpr(Ctx* c)
 {
+    static const int forced = env_int("INS_CSR_LPR", 0);
+    if (forced == 4 || forced == 8 || forced == 16 || forced == 32) return forced;
     const double avg = c->n_active ? (double)c->nnz / (double)c->n_active : 1.;
     if (avg > 48) return 16;
     if (avg > 12) return 8;
@@ -747,7 +875,9 @@ int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const
     } else {
         const int lpr = pick_lpr(c);
         const unsigned g = csr_grid(n, lpr);
-        if (lpr == 16)
+        if (lpr == 32)
+            k_spmv_csr<32, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
+        else if (lpr == 16)
             k_spmv_csr<16, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
         else if (lpr == 8)
             k_spmv_csr<8, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
@@ -929,7 +1059,24 @@ int build_sell(Ctx* c)
     hipFree(err);
     S.nvals = nvals;
     // worth it only when the shared lists stay close to the rows' own lists
-    S.ok = (herr == 0) && (nvals <= c->nnz + c->nnz / 8 + 64 * 64);
+    S.ok = (herr == 0) && (nvals <= c->nnz + c->nnz / 8 + 64 * 64) && nvals < INT32_MAX;
+    if (S.ok) {
+        INS_TRY(ensure(c, S.map, c->nnz));
+        int* e2 = nullptr;
+        INS_HIP_CHECK(c, hipMalloc((void**)&e2, sizeof(int)));
+        INS_HIP_CHECK(c, hipMemsetAsync(e2, 0, sizeof(int), c->stream));
+        k_sell_map<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, sell_view(c), c->row_ptr.p,
+                                                                       c->col.p, S.map.p, e2);
+        INS_HIP_CHECK(c, hipGetLastError());
+        int he = 0;
+        INS_HIP_CHECK(c, hipMemcpyAsync(&he, e2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        hipFree(e2);
+        if (he) S.ok = false;
+        // values: zero once, the padding is never written afterwards
+        release(c, S.v64);
+        release(c, S.v32);
+    }
     S.built = true;
     S.stale = true;
     return INS_OK;
@@ -939,17 +1086,60 @@ int build_sell(Ctx* c)
 int fill_sell(Ctx* c)
 {
     Sell& S = c->sell;
-    const int64_t n = c->n_active;
-    if (c->matrix_precision == INS_FP64) INS_TRY(ensure(c, S.v64, S.nvals));
-    INS_TRY(ensure(c, S.v32, S.nvals));
-    k_sell_fill<<<grid_for((n + 63) / 64, 4, 1u << 16), 256, 0, c->stream>>>(
-        n, sell_view(c), c->row_ptr.p, c->col.p,
-        (c->matrix_precision == INS_FP64) ? c->val64.p : nullptr, c->val32.p,
-        (c->matrix_precision == INS_FP64) ? S.v64.p : nullptr, S.v32.p);
+    const bool f64 = c->matrix_precision == INS_FP64;
+    if (f64 && !S.v64.p) {
+        INS_TRY(ensure(c, S.v64, S.nvals));
+        INS_HIP_CHECK(c, hipMemsetAsync(S.v64.p, 0, S.nvals * sizeof(double), c->stream));
+    }
+    if (!S.v32.p) {
+        INS_TRY(ensure(c, S.v32, S.nvals));
+        INS_HIP_CHECK(c, hipMemsetAsync(S.v32.p, 0, S.nvals * sizeof(float), c->stream));
+    }
+    k_sell_scatter<<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(
+        c->nnz, S.map.p, f64 ? c->val64.p : nullptr, c->val32.p, f64 ? S.v64.p : nullptr,
+        S.v32.p);
     INS_HIP_CHECK(c, hipGetLastError());
     S.stale = false;
     return INS_OK;
 }
+
+// Host loop.  Chunks of iterations are enqueued two ahead, each followed by
+// an asynchronous copy of the device scalars into a pinned snapshot; the host
+// waits only for the older snapshot, so the GPU never idles while the host
+// decides.  Once a snapshot shows the stop (done) or, in mixed precision, the
+// replacement request (pause), the iterations already enqueued behind it exit
+// at once.
+struct Pipe {
+    Ctx* c;
+    int inflight = 0;
+    int head = 0;  // oldest in-flight snapshot slot
+    int enq = 0;   // iterations enqueued so far (skipped ones included)
+
+    int issue_snapshot()
+    {
+        const int slot = (head + inflight) & 1;
+        INS_HIP_CHECK(c, hipMemcpyAsync(&c->h_snap[slot], c->scal.p, sizeof(Scal),
+                                        hipMemcpyDeviceToHost, c->stream));
+        INS_HIP_CHECK(c, hipEventRecord(c->evs[slot], c->stream));
+        inflight++;
+        return INS_OK;
+    }
+    // waits for the oldest snapshot
+    int pop(Scal* out)
+    {
+        INS_HIP_CHECK(c, hipEventSynchronize(c->evs[head]));
+        *out = c->h_snap[head];
+        head ^= 1;
+        inflight--;
+        return INS_OK;
+    }
+    int drain()
+    {
+        Scal tmp;
+        while (inflight) INS_TRY(pop(&tmp));
+        return INS_OK;
+    }
+};
 
 int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, double* relres)
 {
@@ -963,46 +1153,75 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     INS_TRY(ensure_solver_state(c, mode));
     const int64_t n = c->n_active;
     const unsigned gv = vec_grid(n);
+    const bool mixed = (mode == INS_SOLVE_MIXED);
+    static const int mchunk = std::max(1, env_int("INS_MIXED_CHUNK", MCHUNK));
+    const int chunk = mixed ? mchunk : CHUNK;
     c->stats.spmv_ms = 0.;
     c->stats.spmv_calls = 0;
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     int refinements = 0;
     int64_t launched = 0;
-    if (mode == INS_SOLVE_FP64) {
-        const double* sv = (format == INS_FMT_SELL) ? c->sell.v64.p : nullptr;
+    // replace the residual after every 10x reduction (delta = 0.1)
+    const double delta2 = 0.01;
+    if (!mixed) {
         k_init<double, double><<<gv, VB, 0, c->stream>>>(n, c->b.p, c->x.p, c->r.p, c->p.p,
                                                          c->dinv64.p, c->partial.p);
-        k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, c->scal.p);
-        INS_HIP_CHECK(c, hipGetLastError());
-        INS_TRY(read_scal(c));
-        while (!c->h_scal->done && c->h_scal->iters < max_it) {
-            const int k = std::min(CHUNK, max_it - c->h_scal->iters);
-            INS_TRY((run_chunk<double, double>(c, format, c->val64.p, sv, c->x.p, c->r.p, c->p.p,
-                                               c->q.p, c->dinv64.p, k, &launched)));
-            INS_TRY(read_scal(c));
-        }
-        INS_TRY(launch_resid(c, format, 0, 0));
+        k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, 0., c->scal.p);
     } else {
-        // FP32 sweeps with an FP64 residual replacement at every chunk end
-        const float* sv = (format == INS_FMT_SELL) ? c->sell.v32.p : nullptr;
         INS_HIP_CHECK(c, hipMemsetAsync(c->x.p, 0, n * sizeof(double), c->stream));
         k_init<float, float><<<gv, VB, 0, c->stream>>>(n, c->b.p, c->x32.p, c->r32.p, c->p32.p,
                                                        c->dinv32.p, c->partial.p);
-        k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, c->scal.p);
-        INS_HIP_CHECK(c, hipGetLastError());
-        INS_TRY(read_scal(c));
-        bool converged = c->h_scal->done && !c->h_scal->breakdown;
-        while (!converged && c->h_scal->iters < max_it && !c->h_scal->breakdown) {
-            const int k = std::min(CHUNK, max_it - c->h_scal->iters);
-            INS_TRY((run_chunk<float, float>(c, format, c->val32.p, sv, c->x32.p, c->r32.p,
-                                             c->p32.p, c->q32.p, c->dinv32.p, k, &launched)));
-            k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
-            INS_HIP_CHECK(c, hipGetLastError());
-            INS_TRY(launch_resid(c, format, 1, (int)(launched & 1)));
-            refinements++;
-            INS_TRY(read_scal(c));
-            converged = c->h_scal->done && !c->h_scal->breakdown;
+        k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, delta2,
+                                            c->scal.p);
+    }
+    INS_HIP_CHECK(c, hipGetLastError());
+    INS_TRY(read_scal(c));
+    Pipe pipe{c};
+    bool stop = c->h_scal->done != 0;
+    while (!stop) {
+        while (pipe.inflight < 2 && pipe.enq < max_it) {
+            const int k = std::min(chunk, max_it - pipe.enq);
+            if (mixed)
+                INS_TRY((run_chunk<float, float>(c, format, c->val32.p,
+                                                 format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
+                                                 c->x32.p, c->r32.p, c->p32.p, c->q32.p,
+                                                 c->dinv32.p, k, &launched)));
+            else
+                INS_TRY((run_chunk<double, double>(c, format, c->val64.p,
+                                                   format == INS_FMT_SELL ? c->sell.v64.p : nullptr,
+                                                   c->x.p, c->r.p, c->p.p, c->q.p, c->dinv64.p, k,
+                                                   &launched)));
+            pipe.enq += k;
+            INS_TRY(pipe.issue_snapshot());
         }
+        if (pipe.inflight == 0) break;  // iteration limit
+        Scal S;
+        INS_TRY(pipe.pop(&S));
+        if (S.breakdown) break;
+        if (!(S.done || S.pause)) continue;
+        INS_TRY(pipe.drain());
+        if (!mixed) break;  // FP64: converged on the recursive residual
+        // residual replacement: x64 += x32; r = b - A x64 in FP64
+        k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+        INS_TRY(launch_resid(c, format, 1, (int)(launched & 1)));
+        k_restart<<<gv, VB, 0, c->stream>>>(n, c->p32.p, c->r32.p, c->dinv32.p, c->scal.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+        refinements++;
+        INS_TRY(read_scal(c));
+        pipe.enq = c->h_scal->iters;
+        if (c->h_scal->done || c->h_scal->breakdown) stop = true;
+    }
+    INS_TRY(pipe.drain());
+    if (mixed) {
+        INS_TRY(read_scal(c));
+        if (!(c->h_scal->done && !c->h_scal->breakdown)) {
+            // iteration limit: fold what was computed and measure it
+            k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
+            INS_TRY(launch_resid(c, format, 0, 0));
+        }
+    } else {
+        INS_TRY(launch_resid(c, format, 0, 0));
     }
     INS_TRY(read_scal(c));
     INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
@@ -1019,8 +1238,11 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     if (relres) *relres = rel;
     c->have_solution = true;
     int status = INS_OK;
-    if (S.breakdown) status = fail(c, INS_ERR_BREAKDOWN, "CG breakdown (p.Ap <= 0 or non-finite)");
-    else if (!(rel <= rtol)) status = fail(c, INS_ERR_NOCONV, "iteration limit reached before rtol");
+    if (S.breakdown)
+        status = fail(c, INS_ERR_BREAKDOWN, "CG breakdown (p.Ap <= 0 or non-finite) after " +
+                                                std::to_string(S.iters) + " iterations");
+    else if (!(rel <= rtol))
+        status = fail(c, INS_ERR_NOCONV, "iteration limit reached before rtol");
     return status;
 }
 
