@@ -148,6 +148,10 @@ struct Ctx {
     Scal* h_scal = nullptr;  // pinned
     Scal* h_snap = nullptr;  // pinned, 2 snapshots for the pipelined host loop
     hipEvent_t evs[2] = {nullptr, nullptr};
+    // captured chunk of PCG iterations (kernel launches only)
+    hipGraphExec_t chunk_graph = nullptr;
+    int chunk_key[4] = {0, 0, 0, 0};   // mode, format, chunk, n (low bits)
+    const void* chunk_ptrs[14] = {};
     DArr<int> dflag;
     bool have_solution = false;
 

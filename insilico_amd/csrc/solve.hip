@@ -164,13 +164,12 @@ __device__ inline void load4(const TV* p, TV (&o)[4])
     }
 }
 
-constexpr int SG = 3;  // groups of 4 entries in flight per lane (27-point rows: 3 rounds)
 
 // one lane per row of a 64-row slice; all rows of a slice share one
 // column-offset list (the union of their own lists, values zero-padded), so
 // the offsets are wave-uniform and read with scalar loads.  Padded entries
 // may point outside 0..n-1: their index is clamped (their value is 0).
-template <typename TV, typename TX, typename TA>
+template <typename TV, typename TX, typename TA, int SG = 3>
 __device__ inline TA sell_row(int64_t row, int64_t n, int lane, int64_t slice, const SellView& S,
                               const TV* __restrict__ v, const TX* __restrict__ x)
 {
@@ -241,7 +240,7 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
 
 // ---------------------------------------------------------------------------
 // K1: q = A p with per-block partials of p.q
-template <typename TV, typename TX, typename TA>
+template <typename TV, typename TX, typename TA, int SGK>
 __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
                                                   const TX* __restrict__ p, TX* __restrict__ q,
                                                   double* part, const Scal* sc)
@@ -253,7 +252,7 @@ __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const T
     const Sweep w = xcd_sweep(nsl, SB / 64);
     for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
         const int64_t row = slice * 64 + lane;
-        const TA s = sell_row<TV, TX, TA>(row, n, lane, slice, S, val, p);
+        const TA s = sell_row<TV, TX, TA, SGK>(row, n, lane, slice, S, val, p);
         if (row < n) {
             const TX qs = (TX)s;
             st(&q[row], qs, g_nt);
@@ -869,8 +868,15 @@ int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const
     const int64_t n = c->n_active;
     if (format == INS_FMT_SELL) {
         const unsigned g = sell_grid(n);
-        k_spmv_sell<TV, TX, TA><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q,
-                                                         c->partial.p, c->scal.p);
+        static const int sg = env_int("INS_SG", 4);
+        if (sg == 7)
+            k_spmv_sell<TV, TX, TA, 7><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+        else if (sg == 2)
+            k_spmv_sell<TV, TX, TA, 2><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+        else if (sg == 4)
+            k_spmv_sell<TV, TX, TA, 4><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+        else
+            k_spmv_sell<TV, TX, TA, 3><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
         *g_out = (int)g;
     } else {
         const int lpr = pick_lpr(c);
@@ -1103,6 +1109,47 @@ int fill_sell(Ctx* c)
     return INS_OK;
 }
 
+// A chunk of iterations captured once as a HIP graph (kernels only) and
+// replayed; re-captured when the solve configuration or buffers change.
+int launch_chunk_graph(Ctx* c, bool mixed, int format, int chunk, int64_t* launched)
+{
+    const int key[4] = {mixed ? 1 : 0, format, chunk, (int)(c->n_active & 0x7fffffff)};
+    const void* ptrs[14] = {c->p32.p, c->p.p, c->q32.p, c->q.p, c->r32.p, c->r.p, c->x32.p, c->x.p,
+                            c->dinv32.p, c->dinv64.p, c->partial.p, c->scal.p,
+                            format == INS_FMT_SELL ? (const void*)(mixed ? (const void*)c->sell.v32.p : (const void*)c->sell.v64.p)
+                                                   : (const void*)(mixed ? (const void*)c->val32.p : (const void*)c->val64.p),
+                            format == INS_FMT_SELL ? (const void*)c->sell.slice_ptr.p : (const void*)c->row_ptr.p};
+    bool same = c->chunk_graph != nullptr;
+    for (int i = 0; i < 4 && same; i++) same = key[i] == c->chunk_key[i];
+    for (int i = 0; i < 14 && same; i++) same = ptrs[i] == c->chunk_ptrs[i];
+    if (!same) {
+        if (c->chunk_graph) {
+            (void)hipGraphExecDestroy(c->chunk_graph);
+            c->chunk_graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        INS_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int64_t dummy = 0;
+        int s;
+        if (mixed)
+            s = run_chunk<float, float>(c, format, c->val32.p, format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
+                                        c->x32.p, c->r32.p, c->p32.p, c->q32.p, c->dinv32.p, chunk, &dummy);
+        else
+            s = run_chunk<double, double>(c, format, c->val64.p, format == INS_FMT_SELL ? c->sell.v64.p : nullptr,
+                                          c->x.p, c->r.p, c->p.p, c->q.p, c->dinv64.p, chunk, &dummy);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (s != INS_OK) return s;
+        if (e != hipSuccess) return fail(c, INS_ERR_HIP, "graph capture failed");
+        INS_HIP_CHECK(c, hipGraphInstantiate(&c->chunk_graph, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        for (int i = 0; i < 4; i++) c->chunk_key[i] = key[i];
+        for (int i = 0; i < 14; i++) c->chunk_ptrs[i] = ptrs[i];
+    }
+    INS_HIP_CHECK(c, hipGraphLaunch(c->chunk_graph, c->stream));
+    *launched += chunk;
+    return INS_OK;
+}
+
 // Host loop.  Chunks of iterations are enqueued two ahead, each followed by
 // an asynchronous copy of the device scalars into a pinned snapshot; the host
 // waits only for the older snapshot, so the GPU never idles while the host
@@ -1158,6 +1205,8 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     const int chunk = mixed ? mchunk : CHUNK;
     c->stats.spmv_ms = 0.;
     c->stats.spmv_calls = 0;
+    static const int graph_knob = env_int("INS_GRAPH", 0);
+    const bool use_graph = graph_knob && !c->profiling && (chunk % 2 == 0);
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     int refinements = 0;
     int64_t launched = 0;
@@ -1181,7 +1230,9 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     while (!stop) {
         while (pipe.inflight < 2 && pipe.enq < max_it) {
             const int k = std::min(chunk, max_it - pipe.enq);
-            if (mixed)
+            if (use_graph && k == chunk && (launched & 1) == 0) {
+                INS_TRY(launch_chunk_graph(c, mixed, format, chunk, &launched));
+            } else if (mixed)
                 INS_TRY((run_chunk<float, float>(c, format, c->val32.p,
                                                  format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
                                                  c->x32.p, c->r32.p, c->p32.p, c->q32.p,
