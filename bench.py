@@ -127,7 +127,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--config", default="diffusion", choices=["diffusion", "elasticity"])
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -141,7 +142,15 @@ def main():
     from insilico_amd import abi, problems as P
 
     t0 = time.perf_counter()
-    prob = P.diffusion3d(args.n, seed=args.seed)
+    if args.config == "diffusion":
+        prob = P.diffusion3d(args.n or 128, seed=args.seed)
+        workload = ("3-D diffusion %d^3 Q1 hex, k=exp(N(0,1)) per element, u=0 on x=0, "
+                    "assemble + Jacobi-PCG to %g" % (args.n or 128, args.rtol))
+    else:
+        prob = P.elasticity3d(args.n or 96, seed=args.seed)
+        workload = ("3-D linear elasticity (StVenant at u=0) %d^3 Q1 hex, E~U[0.5,2] per "
+                    "element, nu=0.3, clamped z=0, traction (0,0,-0.01) on z=1, assemble + "
+                    "Jacobi-PCG to %g" % (args.n or 96, args.rtol))
     t_gen = time.perf_counter() - t0
     log("generated %s: %d elements, %d dof (%.1f s)" % (prob.name, prob.n_elem, prob.n_active, t_gen))
     ctx = abi.Context(0)
@@ -188,8 +197,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 sweeps / f64 residual", "data": "synthetic",
-        "config": {"workload": "3-D diffusion 128^3 Q1 hex, k=exp(N(0,1)) per element, "
-                               "u=0 on x=0, assemble + Jacobi-PCG to 1e-8",
+        "config": {"workload": workload,
                    "n_elements": prob.n_elem, "n_dof": n, "nnz": nnz, "rtol": args.rtol,
                    "format": fmt_name},
         "p50_ms": round(statistics.median(latm), 3),

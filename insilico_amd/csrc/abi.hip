@@ -71,7 +71,7 @@ static void release_all(Ctx* c)
     release(c, c->stage_ke); release(c, c->stage_fe); release(c, c->stage_ff);
     release(c, c->sell.slice_ptr); release(c, c->sell.slice_pat);
     release(c, c->sell.pat_ptr); release(c, c->sell.pat_delta); release(c, c->sell.pat_len);
-    release(c, c->sell.v64); release(c, c->sell.v32); release(c, c->sell.map);
+    release(c, c->sell.v64); release(c, c->sell.v32); release(c, c->sell.map); release(c, c->sell.perm);
     release(c, c->x); release(c, c->r); release(c, c->p); release(c, c->q); release(c, c->t64);
     release(c, c->x32); release(c, c->r32); release(c, c->p32); release(c, c->q32);
     release(c, c->partial); release(c, c->scal); release(c, c->dflag);

@@ -74,6 +74,8 @@ struct Sell {
     DArr<int32_t> pat_delta; // concatenated column offsets
     DArr<int32_t> pat_len;   // npat
     DArr<int32_t> map;       // nnz: slice position of every CSR entry
+    DArr<int32_t> perm;      // slice row -> matrix row (vector problems)
+    bool perm_on = false;
     DArr<double> v64;
     DArr<float> v32;
     bool ok = false;         // dictionary small enough to be worth it

@@ -137,6 +137,7 @@ __device__ inline Sweep xcd_sweep(int64_t total, int64_t step)
 // ---------------------------------------------------------------------------
 // SpMV rows
 struct SellView {
+    const int32_t* perm;   // slice row -> matrix row (null: identity)
     const int64_t* sptr;   // value offset of every slice
     const int32_t* spat;   // column-offset list (pattern) of every slice
     const int32_t* pptr;
@@ -170,11 +171,11 @@ __device__ inline void load4(const TV* p, TV (&o)[4])
 // the offsets are wave-uniform and read with scalar loads.  Padded entries
 // may point outside 0..n-1: their index is clamped (their value is 0).
 template <typename TV, typename TX, typename TA, int SG = 3>
-__device__ inline TA sell_row(int64_t row, int64_t n, int lane, int64_t slice, const SellView& S,
-                              const TV* __restrict__ v, const TX* __restrict__ x)
+__device__ inline TA sell_row(int64_t row, bool live, int64_t n, int lane, int64_t slice,
+                              const SellView& S, const TV* __restrict__ v,
+                              const TX* __restrict__ x)
 {
     TA s = (TA)0;
-    const bool live = row < n;
     const int64_t base = S.sptr[slice] + lane * 4;
     const int pid = S.spat[slice];
     const int32_t* d = S.pdel + S.pptr[pid];
@@ -251,9 +252,11 @@ __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const T
     double pq = 0.;
     const Sweep w = xcd_sweep(nsl, SB / 64);
     for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
-        const int64_t row = slice * 64 + lane;
-        const TA s = sell_row<TV, TX, TA, SGK>(row, n, lane, slice, S, val, p);
-        if (row < n) {
+        const int64_t rho = slice * 64 + lane;
+        const bool live = rho < n;
+        const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
+        const TA s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
+        if (live) {
             const TX qs = (TX)s;
             st(&q[row], qs, g_nt);
             pq += (double)p[row] * (double)qs;
@@ -440,9 +443,11 @@ __global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const 
     double tt = 0., rz = 0.;
     const Sweep w = xcd_sweep(nsl, SB / 64);
     for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
-        const int64_t row = slice * 64 + lane;
-        const double s = sell_row<TV, double, double>(row, n, lane, slice, S, val, x);
-        if (row < n) {
+        const int64_t rho = slice * 64 + lane;
+        const bool live = rho < n;
+        const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
+        const double s = sell_row<TV, double, double, 4>(row, live, n, lane, slice, S, val, x);
+        if (live) {
             const double ti = b[row] - s;
             tt += ti * ti;
             if (replace) {
@@ -559,7 +564,8 @@ __device__ inline int wave_min_i(int v)
 
 // union of the column-offset lists (col - row) of the 64 rows of a slice,
 // ascending: one wavefront per slice, repeated wave minimum of list heads
-__global__ __launch_bounds__(256) void k_slice_union(int64_t n, const int32_t* __restrict__ rp,
+__global__ __launch_bounds__(256) void k_slice_union(int64_t n, const int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ rp,
                                                      const int32_t* __restrict__ col,
                                                      int32_t* __restrict__ udel,
                                                      int32_t* __restrict__ ulen, int* err)
@@ -568,9 +574,11 @@ __global__ __launch_bounds__(256) void k_slice_union(int64_t n, const int32_t* _
     const int lane = threadIdx.x & 63;
     for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
          slice += (int64_t)gridDim.x * 4) {
-        const int64_t row = slice * 64 + lane;
+        const int64_t rho = slice * 64 + lane;
+        int64_t row = 0;
         int32_t k = 0, e = 0;
-        if (row < n) {
+        if (rho < n) {
+            row = perm ? (int64_t)perm[rho] : rho;
             k = rp[row];
             e = rp[row + 1];
         }
@@ -588,6 +596,35 @@ __global__ __launch_bounds__(256) void k_slice_union(int64_t n, const int32_t* _
             if (cnt > MAXU) atomicOr(err, 4);
         }
     }
+}
+
+// component of every row (vector problems)
+__global__ void k_row_comp(int64_t nd, const int32_t* __restrict__ dof_id, int dpn,
+                           int32_t* __restrict__ comp)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nd;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t id = dof_id[k];
+        if (id >= 0) comp[id] = (int32_t)(k % dpn);
+    }
+}
+
+__global__ void k_comp_flag(int64_t n, const int32_t* __restrict__ comp, int c,
+                            int32_t* __restrict__ flag)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        flag[r] = comp[r] == c;
+}
+
+// rows of component c, ascending, after `off` earlier rows
+__global__ void k_comp_place(int64_t n, const int32_t* __restrict__ comp, int c,
+                             const int32_t* __restrict__ pos, int64_t off,
+                             int32_t* __restrict__ perm)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        if (comp[r] == c) perm[off + pos[r]] = (int32_t)r;
 }
 
 __device__ inline uint64_t mix64(uint64_t z)
@@ -735,8 +772,9 @@ __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
     const int lane = threadIdx.x & 63;
     for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
          slice += (int64_t)gridDim.x * 4) {
-        const int64_t row = slice * 64 + lane;
-        if (row >= n) continue;
+        const int64_t rho = slice * 64 + lane;
+        if (rho >= n) continue;
+        const int64_t row = S.perm ? (int64_t)S.perm[rho] : rho;
         const int pid = S.spat[slice];
         const int32_t* d = S.pdel + S.pptr[pid];
         const int len = S.plen[pid];
@@ -841,6 +879,7 @@ int ensure_solver_state(Ctx* c, int mode)
 SellView sell_view(Ctx* c)
 {
     SellView v;
+    v.perm = c->sell.perm_on ? c->sell.perm.p : nullptr;
     v.sptr = c->sell.slice_ptr.p;
     v.spat = c->sell.slice_pat.p;
     v.pptr = c->sell.pat_ptr.p;
@@ -1006,8 +1045,34 @@ int build_sell(Ctx* c)
     INS_HIP_CHECK(c, hipMalloc((void**)&err, sizeof(int)));
     INS_HIP_CHECK(c, hipMemsetAsync(keys, 0, cap * sizeof(uint64_t), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
-    k_slice_union<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p,
-                                                                      udel, ulen, err);
+    // vector problems: slices hold rows of one component (ascending), so the
+    // rows of a slice share their column-offset list
+    S.perm_on = c->dpn > 1;
+    if (S.perm_on) {
+        INS_TRY(ensure(c, S.perm, n + 1));
+        int32_t *comp = nullptr, *flag = nullptr, *pos = nullptr;
+        INS_HIP_CHECK(c, hipMalloc((void**)&comp, (n + 1) * sizeof(int32_t)));
+        INS_HIP_CHECK(c, hipMalloc((void**)&flag, (n + 1) * sizeof(int32_t)));
+        INS_HIP_CHECK(c, hipMalloc((void**)&pos, (n + 1) * sizeof(int32_t)));
+        const int64_t nd = c->n_nodes * c->dpn;
+        k_row_comp<<<grid_for(nd, 256), 256, 0, c->stream>>>(nd, c->dof_id.p, c->dpn, comp);
+        int64_t off = 0;
+        for (int cc = 0; cc < c->dpn; cc++) {
+            k_comp_flag<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, flag);
+            int64_t tot = 0;
+            int st2 = exclusive_scan_i32(c, flag, pos, n, &tot);
+            if (st2 != INS_OK) return st2;
+            k_comp_place<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, pos, off, S.perm.p);
+            off += tot;
+        }
+        INS_HIP_CHECK(c, hipGetLastError());
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        hipFree(comp);
+        hipFree(flag);
+        hipFree(pos);
+    }
+    k_slice_union<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(
+        n, S.perm_on ? S.perm.p : nullptr, c->row_ptr.p, c->col.p, udel, ulen, err);
     const unsigned g = grid_for(nsl, 256);
     k_list_hash<<<g, 256, 0, c->stream>>>(nsl, udel, ulen, h);
     k_hash_insert<<<g, 256, 0, c->stream>>>(nsl, h, keys, reps, cap - 1, err);
