@@ -1,0 +1,52 @@
+// Poisson problem -Lap u = 2 pi^2 sin(pi x) sin(pi y) on the unit square,
+// u = 0 on the boundary, Q1 on an N x N mesh, written against the drop-in API
+// in the order of the reference's heat::PoissonDriver (heat/PoissonDriver.hpp
+// :185-210) and reference/04-heat/dirichlet.cpp:120-173.
+// Prints: numDoFs iterations relres max|u_h - u| L2(u_h - u)
+#include <cstdio>
+#include <cstdlib>
+
+#include "insilico/insilico.hpp"
+#include "insilico/post.hpp"
+
+int main(int argc, char* argv[])
+{
+    using namespace insilico;
+    const int N = argc > 1 ? std::atoi(argv[1]) : 64;
+    try {
+        Mesh mesh = unitCube(2, N, N);
+        Field field(mesh.numNodes(), 1);
+        constrainBoundary(mesh, field, [](const std::array<double, 3>&, std::size_t node, Field& f) {
+            f.constrainValue(node, 0, 0.0);
+        });
+        const std::size_t numDoFs = numberDoFsConsecutively(field);
+
+        base::Quadrature<2, base::QUAD> quadrature;   // 2*fieldDeg, PoissonDriver.hpp:191
+        base::FieldBinder<Mesh, Field> fieldBinder(mesh, field);
+        typedef base::FieldBinder<Mesh, Field>::TupleBinder<1, 1>::Type FTB;
+        heat::Laplace<> laplace(1.0);
+
+        base::solver::Hip solver(numDoFs);
+        base::asmb::stiffnessMatrixComputation<FTB>(quadrature, solver, fieldBinder, laplace);
+        base::asmb::bodyForceComputation<FTB>(
+            quadrature, solver, fieldBinder,
+            base::asmb::SineProductForce{2. * M_PI * M_PI, {M_PI, M_PI, 0.}});
+        solver.finishAssembly();
+        const int iters = solver.cgSolve(1e-10);
+        base::dof::setDoFsFromSolver(solver, field);
+
+        double maxErr = 0.;
+        for (std::size_t n = 0; n < mesh.numNodes(); n++) {
+            const double x = mesh.xyz[2 * n], y = mesh.xyz[2 * n + 1];
+            maxErr = std::max(maxErr, std::abs(field.value[n] - std::sin(M_PI * x) * std::sin(M_PI * y)));
+        }
+        const double l2 = l2Error(mesh, field, [](const double* x) {
+            return std::array<double, 3>{std::sin(M_PI * x[0]) * std::sin(M_PI * x[1]), 0., 0.};
+        });
+        std::printf("%zu %d %.6e %.6e %.6e\n", numDoFs, iters, solver.lastRelres(), maxErr, l2);
+    } catch (const Error& e) {
+        std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,4 +1,5 @@
-"""Build step: compiles the gfx950 HIP library in tree (make -C csrc)."""
+"""Build step: compiles the gfx950 HIP library in tree (make -C csrc) and the
+example C++ drivers linked against it (make -C apps)."""
 import os
 import subprocess
 
@@ -10,6 +11,10 @@ def build(jobs=8, force=True):
     if force:
         args.append("-B")
     subprocess.run(args, check=True)
+    apps = ["make", "-s", "-C", os.path.join(HERE, "apps")]
+    if force:
+        apps.append("-B")
+    subprocess.run(apps, check=True)
     return os.path.join(HERE, "libinsilico_hip.so")
 
 

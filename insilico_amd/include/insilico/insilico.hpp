@@ -1,0 +1,575 @@
+// Drop-in C++17 host layer over the C ABI (include/insilico_hip.h).
+//
+// It re-exposes the names and call order of the reference's template API for
+// the supported subset (Q1 QUAD4/HEX8, tensor Gauss-Legendre, heat::Laplace,
+// solid::HyperElastic<StVenant | NeoHookeanCompressible>) so that an
+// application written against the reference reads the same:
+//
+//   base::solver::Hip solver(numDoFs);                 // base/solver/Eigen3.hpp:71
+//   base::asmb::stiffnessMatrixComputation<FTB>(quadrature, solver,
+//                                               fieldBinder, kernel);
+//   base::asmb::bodyForceComputation<FTB>(quadrature, solver, fieldBinder, f);
+//   solver.finishAssembly();                           // device assembly runs here
+//   solver.cgSolve();
+//   base::dof::setDoFsFromSolver(solver, field);
+//
+// The assembly free functions do not loop over elements on the host: they
+// record what is to be assembled; finishAssembly() hands mesh, DoFs, kernel
+// and loads to the device once and runs the gfx950 kernels (element
+// integration, deterministic CSR gather, RHS).  Matrices assembled by hand
+// through insertToLHS/insertToRHS take the host triplet path of the reference
+// (base/solver/TripletContainer.hpp:305-352) and are uploaded by
+// finishAssembly().
+//
+// Errors: the reference aborts through VERIFY_MSG (base/verify.hpp:63-65);
+// this layer throws insilico::Error carrying the C ABI status.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "insilico_hip.h"
+
+namespace insilico {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(int s, const ins_ctx* c)
+{
+    if (s != INS_OK)
+        throw Error(s, std::string(ins_strerror(s)) + ": " + (c ? ins_last_error(c) : ""));
+}
+
+// ---------------------------------------------------------------------------
+// Discretisation (host setup; flattened base::Unstructured + fe::Field)
+enum DoFStatus { ACTIVE = 0, CONSTRAINED = 1 };  // base/dof/DegreeOfFreedom.hpp:33-38
+
+struct Mesh {
+    int dim = 3;
+    std::vector<double> xyz;     // node coordinates, dim per node
+    std::vector<int32_t> conn;   // element vertices, hierarchic order
+    int npe() const { return 1 << dim; }
+    std::size_t numNodes() const { return xyz.size() / dim; }
+    std::size_t numElements() const { return conn.size() / npe(); }
+};
+
+// tools/meshGeneration/unitCube/unitCube.hpp:85-106,217-257
+inline Mesh unitCube(int dim, int e1, int e2, int e3 = 1)
+{
+    Mesh m;
+    m.dim = dim;
+    const int n1 = e1 + 1, n2 = e2 + 1, n3 = dim == 3 ? e3 + 1 : 1;
+    const double h1 = 1.0 / e1, h2 = 1.0 / e2, h3 = dim == 3 ? 1.0 / e3 : 0.;
+    for (int i3 = 0; i3 < n3; i3++)
+        for (int i2 = 0; i2 < n2; i2++)
+            for (int i1 = 0; i1 < n1; i1++) {
+                m.xyz.push_back(h1 * i1);
+                m.xyz.push_back(h2 * i2);
+                if (dim == 3) m.xyz.push_back(h3 * i3);
+            }
+    static const int H2[4] = {0, 1, 3, 2};
+    static const int H3[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    const int npe = 1 << dim;
+    for (int i3 = 0; i3 < (dim == 3 ? e3 : 1); i3++)
+        for (int i2 = 0; i2 < e2; i2++)
+            for (int i1 = 0; i1 < e1; i1++) {
+                const int32_t base = i1 + i2 * n1 + i3 * n1 * n2;
+                int32_t lex[8], hier[8];
+                int c = 0;
+                for (int d3 = 0; d3 <= (dim == 3 ? 1 : 0); d3++)
+                    for (int d2 = 0; d2 <= 1; d2++)
+                        for (int d1 = 0; d1 <= 1; d1++) lex[c++] = base + d3 * n1 * n2 + d2 * n1 + d1;
+                for (int v = 0; v < npe; v++) hier[dim == 3 ? H3[v] : H2[v]] = lex[v];
+                m.conn.insert(m.conn.end(), hier, hier + npe);
+            }
+    return m;
+}
+
+// base::mesh::MeshBoundary::create: (element, face) pairs on the boundary,
+// faces from the element face table (base/mesh/ElementFaces.hpp:284-286,
+// 422-434); returns the face node lists
+inline std::vector<std::vector<int32_t>> boundaryFaces(const Mesh& m)
+{
+    static const int HEX[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                  {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+    static const int QUAD[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    const int nf = m.dim == 3 ? 6 : 4, nfn = m.dim == 3 ? 4 : 2, npe = m.npe();
+    std::map<std::vector<int32_t>, std::pair<int, std::vector<int32_t>>> seen;
+    for (std::size_t e = 0; e < m.numElements(); e++)
+        for (int f = 0; f < nf; f++) {
+            std::vector<int32_t> nodes(nfn);
+            for (int k = 0; k < nfn; k++)
+                nodes[k] = m.conn[e * npe + (m.dim == 3 ? HEX[f][k] : QUAD[f][k])];
+            std::vector<int32_t> key = nodes;
+            std::sort(key.begin(), key.end());
+            auto it = seen.find(key);
+            if (it == seen.end()) seen.emplace(key, std::make_pair(1, nodes));
+            else it->second.first++;
+        }
+    std::vector<std::vector<int32_t>> out;
+    for (auto& kv : seen)
+        if (kv.second.first == 1) out.push_back(kv.second.second);
+    return out;
+}
+
+// Field of DoF objects with `size` components per node (iso-parametric:
+// DoF = node, base/dof/copyConnectivity.hpp:34-60)
+struct Field {
+    int size = 1;
+    std::vector<int> status;        // per (node, comp)
+    std::vector<double> value;      // current value (DegreeOfFreedom values_[0])
+    std::vector<double> prescribed; // constraint value
+    std::vector<int32_t> index;     // solver row, -1 if constrained
+
+    Field() = default;
+    Field(std::size_t numNodes, int s)
+        : size(s), status(numNodes * s, ACTIVE), value(numNodes * s, 0.),
+          prescribed(numNodes * s, 0.), index(numNodes * s, -1) {}
+
+    // DegreeOfFreedom::constrainValue (base/dof/DegreeOfFreedom.hpp:232-242)
+    void constrainValue(std::size_t node, int comp, double v)
+    {
+        status[node * size + comp] = CONSTRAINED;
+        prescribed[node * size + comp] = v;
+    }
+    bool isActive(std::size_t node, int comp) const { return status[node * size + comp] == ACTIVE; }
+};
+
+// base/dof/numbering.hpp:45-66
+inline std::size_t numberDoFsConsecutively(Field& f)
+{
+    std::size_t counter = 0;
+    for (std::size_t k = 0; k < f.status.size(); k++)
+        f.index[k] = f.status[k] == ACTIVE ? (int32_t)counter++ : -1;
+    return counter;
+}
+
+// base::dof::scaleConstraints (base/dof/Distribute.hpp:76-80)
+inline void scaleConstraints(Field& f, double factor)
+{
+    for (std::size_t k = 0; k < f.status.size(); k++)
+        if (f.status[k] == CONSTRAINED) f.prescribed[k] *= factor;
+}
+
+// base::dof::constrainBoundary with a functor diriFun(x, node, field)
+// applied at the nodes of the boundary faces (base/dof/constrainBoundary.hpp:52-120)
+template <typename DIRIFUN>
+void constrainBoundary(const Mesh& m, Field& f, DIRIFUN diriFun)
+{
+    std::vector<char> done(m.numNodes(), 0);
+    for (const auto& face : boundaryFaces(m))
+        for (int32_t node : face) {
+            if (done[node]) continue;
+            done[node] = 1;
+            std::array<double, 3> x{0., 0., 0.};
+            for (int d = 0; d < m.dim; d++) x[d] = m.xyz[node * m.dim + d];
+            diriFun(x, (std::size_t)node, f);
+        }
+}
+
+}  // namespace insilico
+
+// ---------------------------------------------------------------------------
+// Quadrature, kernels and materials (tags with their parameters)
+namespace base {
+enum Shape { QUAD = 4, HEX = 8 };
+
+// base::Quadrature<DEG,SHAPE>: tensor Gauss-Legendre with (DEG+2)/2 points
+// per direction (base/quad/GaussLegendre.hpp:33-36)
+template <unsigned DEG, Shape SHAPE>
+struct Quadrature {
+    static constexpr int numPointsPerDir = (DEG + 2) / 2;
+};
+
+template <typename MESH, typename FIELD>
+struct FieldBinder {  // base/asmb/FieldBinder.hpp:107-188 (one field)
+    MESH& mesh;
+    FIELD& field;
+    FieldBinder(MESH& m, FIELD& f) : mesh(m), field(f) {}
+    template <int, int>
+    struct TupleBinder {
+        using Type = FieldBinder;
+    };
+};
+}  // namespace base
+
+namespace mat {
+// mat/Lame.hpp:28-40
+struct Lame {
+    static double lambda(double E, double nu) { return E * nu / (1. + nu) / (1. - 2. * nu); }
+    static double mu(double E, double nu) { return E / 2. / (1. + nu); }
+};
+namespace hypel {
+struct StVenant {
+    static constexpr int id = INS_STVENANT;
+    double lambda, mu;
+    StVenant(double l, double m) : lambda(l), mu(m) {}
+};
+struct NeoHookeanCompressible {
+    static constexpr int id = INS_NEOHOOKE;
+    double lambda, mu;
+    NeoHookeanCompressible(double l, double m) : lambda(l), mu(m) {}
+};
+}  // namespace hypel
+}  // namespace mat
+
+namespace heat {
+// heat::Laplace (heat/Laplace.hpp:85-97): a constant factor, or a
+// per-element conductivity set with setConductivityFunction
+template <typename FTB = void>
+struct Laplace {
+    double factor;
+    std::vector<double> perElement;
+    explicit Laplace(double f) : factor(f) {}
+    void setConductivityFunction(std::function<double(std::size_t)> k, std::size_t numElements)
+    {
+        perElement.resize(numElements);
+        for (std::size_t e = 0; e < numElements; e++) perElement[e] = k(e);
+    }
+};
+}  // namespace heat
+
+namespace solid {
+// solid::HyperElastic<MATERIAL, FTB> (solid/HyperElastic.hpp); per-element
+// Lame parameters are an extension (the reference holds one material)
+template <typename MATERIAL, typename FTB = void>
+struct HyperElastic {
+    MATERIAL material;
+    std::vector<double> perElement;  // lambda_e, mu_e pairs (optional)
+    explicit HyperElastic(const MATERIAL& m) : material(m) {}
+};
+}  // namespace solid
+
+// ---------------------------------------------------------------------------
+// SOLVER concept on the device (base/solver/Eigen3.hpp:64-342)
+namespace base {
+namespace solver {
+
+class Hip {
+public:
+    explicit Hip(std::size_t size, int device = 0, uint64_t hbmBudget = 0) : n_(size)
+    {
+        insilico::check(ins_create(&ctx_, device, hbmBudget), nullptr);
+    }
+    ~Hip()
+    {
+        if (ctx_) ins_destroy(ctx_);
+    }
+    Hip(const Hip&) = delete;
+    Hip& operator=(const Hip&) = delete;
+
+    ins_ctx* context() { return ctx_; }
+
+    // ---- host triplet path (Eigen3.hpp:82-124, TripletContainer dynamic mode)
+    template <typename MATRIX, typename RDOFS, typename CDOFS>
+    void insertToLHS(const MATRIX& m, const RDOFS& rows, const CDOFS& cols)
+    {
+        for (std::size_t i = 0; i < rows.size(); i++)
+            for (std::size_t j = 0; j < cols.size(); j++) {
+                if ((std::size_t)rows[i] >= n_ || (std::size_t)cols[j] >= n_)
+                    throw insilico::Error(INS_ERR_RANGE, "index out of bound");
+                triplets_[{(int32_t)rows[i], (int32_t)cols[j]}] += m(i, j);
+            }
+        hostMatrix_ = true;
+    }
+    template <typename VECTOR, typename DOFS>
+    void insertToRHS(const VECTOR& v, const DOFS& ids)
+    {
+        hostRhs_.resize(n_, 0.);
+        for (std::size_t i = 0; i < ids.size(); i++) {
+            if ((std::size_t)ids[i] >= n_) throw insilico::Error(INS_ERR_RANGE, "out of bound");
+            hostRhs_[ids[i]] += v[i];
+        }
+    }
+
+    // ---- device assembly requests (filled by base::asmb free functions)
+    template <typename MESH, typename FIELD>
+    void bind(const MESH& mesh, FIELD& field)
+    {
+        mesh_ = &mesh;
+        field_ = &field;
+    }
+    void requestMatrix(int kernel, int nq, std::vector<double> coef, int ncoef, bool incremental)
+    {
+        kernel_ = kernel;
+        nq_ = nq;
+        coef_ = std::move(coef);
+        ncoef_ = ncoef;
+        flags_ |= INS_ASM_MATRIX | INS_ASM_RHS;
+        if (incremental) flags_ |= INS_ASM_INCREMENTAL;
+    }
+    void requestResidual(int kernel, int nq, std::vector<double> coef, int ncoef)
+    {
+        if (!kernel_) {
+            kernel_ = kernel;
+            nq_ = nq;
+            coef_ = std::move(coef);
+            ncoef_ = ncoef;
+        }
+        flags_ |= INS_ASM_RHS | INS_ASM_RESIDUAL;
+    }
+    void requestBodyForce(int kind, const std::array<double, 4>& p)
+    {
+        bfKind_ = kind;
+        bf_ = p;
+        flags_ |= INS_ASM_RHS;
+    }
+    void requestTraction(std::vector<std::vector<int32_t>> faces, const std::array<double, 3>& t)
+    {
+        faces_ = std::move(faces);
+        traction_ = t;
+        flags_ |= INS_ASM_RHS;
+    }
+    void setPrecision(int p) { precision_ = p; }
+    // forget the recorded assembly requests and the solution (a new Newton
+    // iteration re-assembles into the same device context, pattern kept)
+    void reset()
+    {
+        flags_ = 0;
+        solved_ = false;
+        assembled_ = false;
+        hostRhs_.clear();
+        triplets_.clear();
+        hostMatrix_ = false;
+    }
+
+    // Eigen3::finishAssembly (base/solver/Eigen3.hpp:142-153)
+    void finishAssembly(bool = true)
+    {
+        if (hostMatrix_) {
+            uploadTriplets();
+        } else if (flags_) {
+            deviceAssembly();
+        }
+        if (!hostRhs_.empty()) {
+            if (!hostMatrix_ && flags_ & INS_ASM_RHS) insilico::check(ins_add_rhs(ctx_, hostRhs_.data()), ctx_);
+            else insilico::check(ins_set_rhs(ctx_, hostRhs_.data()), ctx_);
+        }
+        assembled_ = true;
+    }
+
+    // Eigen3::norm(): |b|/N, or |x|/N once solved (the solution overwrites b)
+    double norm() const
+    {
+        double v = 0.;
+        if (solved_) insilico::check(ins_solution_norm(ctx_, &v), ctx_);
+        else insilico::check(ins_rhs_norm(ctx_, &v), ctx_);
+        return v;
+    }
+
+    // Eigen3::cgSolve (base/solver/Eigen3.hpp:265-275): Jacobi-PCG
+    int cgSolve(double rtol = 1e-12, int mode = INS_SOLVE_FP64, int format = INS_FMT_SELL)
+    {
+        int it = 0;
+        double rel = 0.;
+        const int maxIt = (int)std::max<std::size_t>(2 * n_, 100);
+        insilico::check(ins_solve(ctx_, mode, format, rtol, maxIt, &it, &rel), ctx_);
+        x_.resize(n_);
+        insilico::check(ins_get_solution(ctx_, x_.data()), ctx_);
+        solved_ = true;
+        lastRelres_ = rel;
+        return it;
+    }
+    // direct solves of the reference (choleskySolve/luSolve) have no device
+    // factorisation here: they run the same PCG to a tight tolerance
+    void choleskySolve() { cgSolve(1e-13); }
+    void luSolve() { cgSolve(1e-13); }
+
+    double getValue(std::size_t i) const { return x_.at(i); }  // Eigen3.hpp:293-296
+    double lastRelres() const { return lastRelres_; }
+    std::size_t size() const { return n_; }
+
+    // Eigen3::debugLHS format (row col value), base/solver/Eigen3.hpp:311-317
+    template <typename OS>
+    void debugLHS(OS& out)
+    {
+        int64_t nnz = 0;
+        ins_stats st;
+        insilico::check(ins_get_stats(ctx_, &st), ctx_);
+        nnz = st.nnz;
+        std::vector<int32_t> rp(n_ + 1), ci(nnz);
+        std::vector<double> v(nnz);
+        insilico::check(ins_get_pattern(ctx_, rp.data(), ci.data()), ctx_);
+        insilico::check(ins_get_csr_values(ctx_, v.data()), ctx_);
+        for (std::size_t r = 0; r < n_; r++)
+            for (int32_t k = rp[r]; k < rp[r + 1]; k++) out << r << " " << ci[k] << " " << v[k] << "\n";
+    }
+
+    // DoF distribution (base/dof/Distribute.hpp:141-215) on the host field
+    template <typename FIELD>
+    void distribute(FIELD& f, bool add) const
+    {
+        for (std::size_t k = 0; k < f.status.size(); k++) {
+            if (f.status[k] == insilico::ACTIVE) f.value[k] = add ? f.value[k] + x_[f.index[k]] : x_[f.index[k]];
+            else f.value[k] = f.prescribed[k];
+        }
+    }
+
+private:
+    void uploadTriplets()
+    {
+        std::vector<int32_t> rp(n_ + 1, 0), ci;
+        std::vector<double> v;
+        ci.reserve(triplets_.size());
+        v.reserve(triplets_.size());
+        for (auto& t : triplets_) {
+            rp[t.first.first + 1]++;
+            ci.push_back(t.first.second);
+            v.push_back(t.second);
+        }
+        for (std::size_t i = 0; i < n_; i++) rp[i + 1] += rp[i];
+        insilico::check(ins_set_csr(ctx_, (int64_t)n_, rp.data(), ci.data(), v.data()), ctx_);
+        if (hostRhs_.empty()) hostRhs_.assign(n_, 0.);
+    }
+
+    void deviceAssembly()
+    {
+        if (!mesh_ || !field_) throw insilico::Error(INS_ERR_STATE, "no mesh/field bound");
+        const auto& m = *mesh_;
+        auto& f = *field_;
+        if (!meshSent_) {
+            insilico::check(ins_set_mesh(ctx_, m.dim, (int64_t)m.numNodes(), m.xyz.data(),
+                                         (int64_t)m.numElements(), m.npe(), m.conn.data()), ctx_);
+            insilico::check(ins_set_dofs(ctx_, f.size, f.index.data(), (int64_t)n_), ctx_);
+            int64_t nnz = 0;
+            insilico::check(ins_build_pattern(ctx_, &nnz), ctx_);
+            meshSent_ = true;
+        }
+        insilico::check(ins_set_prescribed(ctx_, f.prescribed.data()), ctx_);
+        insilico::check(ins_set_dof_values(ctx_, f.value.data()), ctx_);
+        insilico::check(ins_set_material(ctx_, kernel_, nq_, ncoef_, coef_.data()), ctx_);
+        insilico::check(ins_set_body_force(ctx_, bfKind_, bf_.data()), ctx_);
+        std::vector<int32_t> flat;
+        for (auto& fc : faces_) flat.insert(flat.end(), fc.begin(), fc.end());
+        const int nfn = faces_.empty() ? 0 : (int)faces_[0].size();
+        insilico::check(ins_set_traction(ctx_, (int64_t)faces_.size(), nfn, faces_.empty() ? nullptr : flat.data(),
+                                         traction_.data()), ctx_);
+        insilico::check(ins_assemble(ctx_, precision_, flags_), ctx_);
+    }
+
+    ins_ctx* ctx_ = nullptr;
+    std::size_t n_;
+    std::map<std::pair<int32_t, int32_t>, double> triplets_;
+    std::vector<double> hostRhs_;
+    bool hostMatrix_ = false, assembled_ = false, solved_ = false, meshSent_ = false;
+    const insilico::Mesh* mesh_ = nullptr;
+    insilico::Field* field_ = nullptr;
+    int kernel_ = 0, nq_ = 2, ncoef_ = 1, flags_ = 0, precision_ = INS_FP64;
+    std::vector<double> coef_;
+    int bfKind_ = INS_BF_NONE;
+    std::array<double, 4> bf_{0., 0., 0., 0.};
+    std::vector<std::vector<int32_t>> faces_;
+    std::array<double, 3> traction_{0., 0., 0.};
+    std::vector<double> x_;
+    double lastRelres_ = 0.;
+};
+
+}  // namespace solver
+
+// ---------------------------------------------------------------------------
+// Assembly free functions (base/asmb/StiffnessMatrix.hpp:52-73,
+// ForceIntegrator.hpp:40-56, BodyForce.hpp:67-82, NeumannForce.hpp:36-63)
+namespace asmb {
+
+namespace detail_ {
+template <typename FTB>
+std::vector<double> laplaceCoef(const heat::Laplace<FTB>& k, std::size_t ne)
+{
+    if (!k.perElement.empty()) return k.perElement;
+    return std::vector<double>(ne, k.factor);
+}
+template <typename MAT, typename FTB>
+std::vector<double> elasticCoef(const solid::HyperElastic<MAT, FTB>& k, std::size_t ne)
+{
+    if (!k.perElement.empty()) return k.perElement;
+    std::vector<double> c(2 * ne);
+    for (std::size_t e = 0; e < ne; e++) {
+        c[2 * e] = k.material.lambda;
+        c[2 * e + 1] = k.material.mu;
+    }
+    return c;
+}
+}  // namespace detail_
+
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename LFTB>
+void stiffnessMatrixComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                                const heat::Laplace<LFTB>& kernel, bool incremental = true)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestMatrix(INS_LAPLACE, Quadrature<DEG, SH>::numPointsPerDir,
+                         detail_::laplaceCoef(kernel, fieldBinder.mesh.numElements()), 1, incremental);
+}
+
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename MAT, typename KFTB>
+void stiffnessMatrixComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                                const solid::HyperElastic<MAT, KFTB>& kernel, bool incremental = true)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestMatrix(MAT::id, Quadrature<DEG, SH>::numPointsPerDir,
+                         detail_::elasticCoef(kernel, fieldBinder.mesh.numElements()), 2, incremental);
+}
+
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename MAT, typename KFTB>
+void computeResidualForces(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                           const solid::HyperElastic<MAT, KFTB>& kernel)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestResidual(MAT::id, Quadrature<DEG, SH>::numPointsPerDir,
+                           detail_::elasticCoef(kernel, fieldBinder.mesh.numElements()), 2);
+}
+
+// built-in body forces: constant vector, or A * prod_d sin(k_d x_d)
+struct ConstantForce {
+    std::array<double, 3> f;
+};
+struct SineProductForce {
+    double amplitude;
+    std::array<double, 3> k;
+};
+
+template <typename FTB, unsigned DEG, Shape SH, typename FB>
+void bodyForceComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                          const ConstantForce& f)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestBodyForce(INS_BF_CONST, {f.f[0], f.f[1], f.f[2], 0.});
+}
+
+template <typename FTB, unsigned DEG, Shape SH, typename FB>
+void bodyForceComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                          const SineProductForce& f)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestBodyForce(INS_BF_SINPROD, {f.amplitude, f.k[0], f.k[1], f.k[2]});
+}
+
+// constant traction on the given boundary faces
+template <typename FB>
+void neumannForceComputation(solver::Hip& solver, FB& fieldBinder,
+                             std::vector<std::vector<int32_t>> faces, const std::array<double, 3>& t)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestTraction(std::move(faces), t);
+}
+
+}  // namespace asmb
+
+namespace dof {
+// base/dof/Distribute.hpp:36-50
+template <typename FIELD>
+void setDoFsFromSolver(const solver::Hip& s, FIELD& f) { s.distribute(f, false); }
+template <typename FIELD>
+void addToDoFsFromSolver(const solver::Hip& s, FIELD& f) { s.distribute(f, true); }
+}  // namespace dof
+}  // namespace base

@@ -32,3 +32,16 @@ def test_binding_loads_and_reports_errors():
     assert L.ins_create(None, 0, 0) == -1
     assert L.ins_destroy(None) == -1
     assert L.ins_set_mesh(None, 3, 1, None, 1, 8, None) == -1
+
+
+def test_apps_built_and_print_usage():
+    """The example C++ drivers link against the HIP library; without
+    arguments they print their usage and touch no device."""
+    import subprocess
+    apps = os.path.join(ROOT, "insilico_amd", "apps")
+    for name in ("poisson", "linear_elastic", "compressible"):
+        exe = os.path.join(apps, name)
+        assert os.path.exists(exe), exe
+    for name in ("linear_elastic", "compressible"):
+        out = subprocess.run([os.path.join(apps, name)], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0 and "Usage" in out.stdout

@@ -1,0 +1,120 @@
+"""The example C++ drivers (insilico_amd/apps), written against the drop-in
+C++ layer, run as subprocesses on the GPU and checked against the oracle and
+the reference's golden output."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from insilico_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+APPS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "insilico_amd", "apps")
+
+
+def _run(*args, timeout=120):
+    exe = os.path.join(APPS, args[0])
+    if not os.path.exists(exe):
+        raise RuntimeError("%s not built (run __graft_entry__.build())" % exe)
+    out = subprocess.run([exe] + [str(a) for a in args[1:]], capture_output=True, text=True,
+                         timeout=timeout)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_poisson_app(oracle):
+    """64 x 64 Q1 Poisson through the C++ layer: same DoF count as the Python
+    problem, relative residual 1e-10, nodal error equal to the oracle's solve."""
+    ndof, it, rel, maxerr, l2 = _run("poisson", 64).split()
+    prob = P.poisson2d(64)
+    assert int(ndof) == prob.n_active
+    assert float(rel) <= 1e-10
+    rp, ci, val, b, _ = oracle.assemble(prob)
+    x, _, _ = oracle.pcg(rp, ci, val, b, tol=1e-13)
+    act = prob.dof_id >= 0
+    ex = np.sin(np.pi * prob.xyz[:, 0]) * np.sin(np.pi * prob.xyz[:, 1])
+    err_o = np.abs(x[prob.dof_id[act]] - ex[act]).max()
+    assert abs(float(maxerr) - err_o) <= 1e-8 * err_o + 1e-11, (maxerr, err_o)
+    assert float(l2) < 1e-3
+
+
+@pytest.mark.parametrize("dim,mesh,N", [(2, "quad.002.smf", 2), (2, "quad.005.smf", 5),
+                                        (2, "quad.010.smf", 10), (3, "cube.002.smf", 2),
+                                        (3, "cube.004.smf", 4), (3, "cube.008.smf", 8)])
+def test_linear_elastic_app_golden(golden, dim, mesh, N):
+    """reference/06-elastic/linearElastic{2D,3D}.ref.dat to the printed 6 digits."""
+    ref = dict(np.loadtxt(os.path.join(golden, "linearElastic%dD.ref.dat" % dim)))
+    err = float(_run("linear_elastic", os.path.join(golden, mesh), dim))
+    assert err == pytest.approx(ref[N], rel=3e-6), (N, err, ref[N])
+
+
+def newton_oracle(oracle, n, steps, pull, maxit, tol, mu, lam):
+    """The load-step / Newton loop of reference/06-elastic/compressible.cpp
+    :245-335 on the oracle: returns [(step, iter, |F|, |x| or None)], u."""
+    prob = P.neohooke3d(n, mu=mu, lam=lam)
+    prob.g[prob.meta["right_x"] * 3] = pull / steps
+    act = prob.dof_id >= 0
+    u = np.zeros(prob.n_nodes * 3)
+    hist = []
+    for step in range(steps):
+        prob.g *= 1.0 if step == 0 else (step + 1.0) / step
+        it = 0
+        while it < maxit:
+            rp, ci, val, b, _ = oracle.assemble(prob, u=u, incremental=True, residual=True)
+            conv1 = np.linalg.norm(b) / prob.n_active
+            if conv1 < tol:
+                hist.append((step, it, conv1, None))
+                break
+            x, _, _ = oracle.pcg(rp, ci, val, b, tol=1e-12)
+            u[act] += x[prob.dof_id[act]]
+            u[~act] = prob.g[~act]
+            conv2 = np.linalg.norm(x) / prob.n_active
+            hist.append((step, it, conv1, conv2))
+            it += 1
+            if conv2 < tol:
+                break
+    return hist, u
+
+
+def _parse_history(out):
+    hist, umax = [], None
+    for line in out.splitlines():
+        if line.startswith("# max |u_y|"):
+            umax = float(line.split()[-1])
+        if not line.strip() or line.startswith("#"):
+            continue
+        f = line.split()
+        hist.append((int(f[0]), int(f[1]), float(f[2]), float(f[3]) if len(f) > 3 else None))
+    return hist, umax
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_compressible_newton_app(oracle, prec):
+    """Newton history (|F|, |x| per iteration) and final displacement of the
+    neo-Hookean block against the oracle's Newton loop.  The first residual of
+    every Newton sequence is the same FP64 state on both sides (1e-10); the
+    later ones carry the CG tolerance and, with the FP32 tangent, its
+    rounding, so they are compared where they are above that noise."""
+    n, steps, pull, maxit, tol, mu, lam = 5, 3, 0.3, 10, 1e-9, 1.0, 10.0
+    args = ["compressible", n, steps, pull, maxit, tol, mu, lam] + ([prec] if prec == "fp32" else [])
+    hist, umax = _parse_history(_run(*args))
+    ref, u = newton_oracle(oracle, n, steps, pull, maxit, tol, mu, lam)
+    assert len(hist) >= steps
+    tol_conv = 1e-6 if prec == "fp64" else 1e-3
+    by_key = {(s, i): (c1, c2) for s, i, c1, c2 in ref}
+    first = {}
+    for s, i, c1, c2 in hist:
+        first.setdefault(s, (i, c1))
+        if (s, i) in by_key and c1 > 1e-7:
+            assert c1 == pytest.approx(by_key[(s, i)][0], rel=tol_conv), (s, i, c1, by_key[(s, i)])
+    for s, (i, c1) in first.items():
+        assert i == 0
+    # Newton converges quadratically: each step ends below the tolerance
+    last = {}
+    for s, i, c1, c2 in hist:
+        last[s] = (c1, c2)
+    for s, (c1, c2) in last.items():
+        assert (c2 is not None and c2 < tol) or c1 < tol, (s, c1, c2)
+    assert umax == pytest.approx(np.abs(u[1::3]).max(), rel=1e-6)
