@@ -39,9 +39,14 @@ def spmv_bytes(n, nnz, nslice_vals, fmt, vbytes, xbytes):
     return nnz * (vbytes + 4) + n * 4 + 2 * n * xbytes
 
 
-def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup):
+def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup, profile=False):
+    """warmup + steps passes of assembly + solve; with profile, every SpMV
+    launch of the timed steps is bracketed by HIP events on the context
+    stream (recorded back to back, read after each solve)."""
     from insilico_amd import abi
     lat, its, rels, asm, sol = [], [], [], [], []
+    spmv_ms, spmv_calls = 0.0, 0
+    ctx.set_profiling(profile)
     for s in range(warmup + steps):
         ctx.sync()
         t0 = time.perf_counter()
@@ -56,50 +61,52 @@ def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup):
             rels.append(rel)
             asm.append(st["assemble_ms"])
             sol.append(st["solve_ms"])
-    return lat, its, rels, asm, sol
-
-
-def profile_spmv(ctx, mode, fmt, precision, rtol):
-    """HIP-event timing of every SpMV launch of one solve (dominant kernel)."""
-    from insilico_amd import abi
-    ctx.assemble(precision, abi.ASM_MATRIX | abi.ASM_RHS)
-    ctx.set_profiling(True)
-    ctx.solve(mode, fmt, rtol)
+            spmv_ms += st["spmv_ms"]
+            spmv_calls += st["spmv_calls"]
     ctx.set_profiling(False)
-    st = ctx.stats()
-    return st["spmv_ms"] / max(st["spmv_calls"], 1), st["spmv_calls"]
+    return lat, its, rels, asm, sol, (spmv_ms / max(spmv_calls, 1), spmv_calls)
 
 
-def cpu_baseline(prob, ctx, iters_fp64, budget_s=20.0):
-    """The oracle (reference algorithm restated in C, -O3 -fopenmp) on the host
-    cores: assembly of a bounded element sample and a bounded number of
-    Jacobi-CG iterations on the full matrix, extrapolated to the full job."""
-    from oracle import pyoracle as O
+def _cpu_rate(prob, O, rp, ci, val, b, iters_fp64, nthreads, n_elem_sample, k_iters):
+    """Seconds per element of the oracle assembly (first n_elem_sample
+    elements, contiguous z-slabs) and seconds per Jacobi-CG iteration on the
+    full matrix, with nthreads OpenMP threads."""
     from insilico_amd import problems as P
-    ncores = len(os.sched_getaffinity(0))
-    # assembly sample: first E_s elements (contiguous z-slabs)
-    E_s = min(prob.n_elem, 131072)
+    E_s = min(prob.n_elem, n_elem_sample)
     sub = P.Problem(prob.dim, prob.xyz, prob.conn[:E_s], prob.dpn, prob.dof_id, prob.n_active,
                     prob.g, prob.kernel, prob.coef[:E_s], prob.nq, prob.bf_kind, prob.bf)
     rp_s, ci_s = O.pattern(sub)
-    _, _, _, _, t_asm_s = O.assemble(sub, rp_s, ci_s, nthreads=ncores)
-    t_elem = t_asm_s / E_s
-    # CG sample on the full system (values equal the oracle's to 1e-12)
+    _, _, _, _, t_asm_s = O.assemble(sub, rp_s, ci_s, nthreads=nthreads)
+    t0 = time.perf_counter()
+    O.pcg(rp, ci, val, b, tol=1e-30, maxit=k_iters, nthreads=nthreads)
+    t_it = (time.perf_counter() - t0) / (k_iters + 1)
+    t_total = prob.n_elem * (t_asm_s / E_s) + iters_fp64 * t_it
+    return t_total, t_asm_s, E_s, t_it
+
+
+def cpu_baseline(prob, ctx, iters_fp64):
+    """The oracle (reference algorithm restated in C, -O3 -fopenmp) on the host
+    cores: assembly of a bounded element sample and a bounded number of
+    Jacobi-CG iterations on the full matrix, extrapolated to the full job.
+    Two settings: all host cores (the NTHREADS=0 convention of the reference's
+    element loop) and one thread (its default NTHREADS=1)."""
+    from oracle import pyoracle as O
+    ncores = len(os.sched_getaffinity(0))
     rp, ci = ctx.get_pattern()
     val = ctx.get_csr_values()
     b = ctx.get_rhs()
-    k = 20
-    t0 = time.perf_counter()
-    O.pcg(rp, ci, val, b, tol=1e-30, maxit=k, nthreads=ncores)
-    t_it = (time.perf_counter() - t0) / (k + 1)
-    t_total = prob.n_elem * t_elem + iters_fp64 * t_it
+    t_all, t_asm_s, E_s, t_it = _cpu_rate(prob, O, rp, ci, val, b, iters_fp64, ncores, 131072, 20)
+    t_one, t_asm1, E_1, t_it1 = _cpu_rate(prob, O, rp, ci, val, b, iters_fp64, 1, 16384, 3)
     return {
-        "value": prob.n_active / t_total, "unit": "DOF/s", "cores": ncores, "kind": "port",
+        "value": prob.n_active / t_all, "unit": "DOF/s", "cores": ncores, "kind": "port",
         "sample": "oracle (reference algorithm in C, -O3 -fopenmp, %d threads): assembly of "
-                  "%d of %d elements (%.2f s) and %d Jacobi-CG iterations on the full "
+                  "%d of %d elements (%.2f s) and 20 Jacobi-CG iterations on the full "
                   "matrix (%.3f s/iter), extrapolated to %d elements + %d iterations"
-                  % (ncores, E_s, prob.n_elem, t_asm_s, k, t_it, prob.n_elem, iters_fp64),
-        "t_assemble_s": prob.n_elem * t_elem, "t_iter_s": t_it,
+                  % (ncores, E_s, prob.n_elem, t_asm_s, t_it, prob.n_elem, iters_fp64),
+        "t_assemble_s": prob.n_elem * t_asm_s / E_s, "t_iter_s": t_it,
+        "value_1thread": prob.n_active / t_one,
+        "sample_1thread": "1 thread: assembly of %d elements (%.2f s) and 3 CG iterations "
+                          "(%.3f s/iter), extrapolated likewise" % (E_1, t_asm1, t_it1),
     }
 
 
@@ -118,6 +125,21 @@ def read_traffic(kernel_tag):
     return None
 
 
+def copy_probe(n_bytes=1 << 30, reps=3):
+    """A plain device copy of known size (reads and writes n_bytes per launch),
+    run under the same rocprofv3 counter passes as the SpMV to calibrate
+    FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    src = torch.empty(n_bytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n_bytes, "copy_probe_bytes_written": n_bytes,
+                      "launches": reps}))
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -132,12 +154,19 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no per-launch SpMV events in the timed steps")
     ap.add_argument("--fmt", default="sell", choices=["sell", "csr"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only the copy kernel that calibrates the HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     from insilico_amd import abi, problems as P
 
@@ -160,29 +189,35 @@ def main():
     fmt = abi.FMT_SELL if args.fmt == "sell" else abi.FMT_CSR
 
     # plain FP64 path
-    lat64, its64, rel64, asm64, sol64 = run_path(ctx, prob, abi.SOLVE_FP64, abi.FMT_CSR, abi.FP64,
-                                                 args.rtol, args.steps, args.warmup)
+    lat64, its64, rel64, asm64, sol64, _ = run_path(ctx, prob, abi.SOLVE_FP64, abi.FMT_CSR,
+                                                    abi.FP64, args.rtol, args.steps, args.warmup)
     log("fp64 path: %.1f ms/step, %d iterations" % (statistics.mean(lat64), its64[0]))
     # mixed-precision path (headline)
-    latm, itsm, relm, asmm, solm = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
-                                            args.rtol, args.steps, args.warmup)
+    latm, itsm, relm, asmm, solm, _ = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
+                                               args.rtol, args.steps, args.warmup)
     log("mixed path: %.1f ms/step, %d iterations" % (statistics.mean(latm), itsm[0]))
     st = ctx.stats()
     ms_step = statistics.mean(latm)
     value = prob.n_active / (ms_step / 1e3)
     value64 = prob.n_active / (statistics.mean(lat64) / 1e3)
 
-    # dominant kernel: the SpMV of the mixed path
-    spmv_ms, spmv_calls = profile_spmv(ctx, abi.SOLVE_MIXED, fmt, abi.FP64, args.rtol)
+    # dominant kernel: the SpMV of the mixed path, timed with HIP event pairs
+    # around each launch over as many further steps (the events add about 8%
+    # to a step, so the steps that give `value` run without them)
+    spmv_ms, spmv_calls = 0.0, 0
+    if not args.no_profile:
+        _, _, _, _, _, (spmv_ms, spmv_calls) = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
+                                                        args.rtol, args.steps, 0, profile=True)
     n = prob.n_active
     sell_vals = ctx.stats()["sell_values"] or nnz
     fmt_name = "sell" if fmt == abi.FMT_SELL else "csr"
     alg = spmv_bytes(n, nnz, sell_vals, fmt_name, 4, 4)
-    achieved = alg / (spmv_ms / 1e3) / 1e9
+    achieved = round(alg / (spmv_ms / 1e3) / 1e9, 1) if spmv_calls else None
     traffic = read_traffic("mixed_" + fmt_name)
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_spmv_pq_%s<float,float,float> (q = A p fused with p.q)" % fmt_name,
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
+            "kernel": ("k_spmv_sell<float, float, float, 4>" if fmt_name == "sell" else
+                         "k_spmv_csr<LPR, float, float, float>") + " (q = A p with p.q partials)",
             "bytes_per_launch": alg, "avg_launch_ms": round(spmv_ms, 5),
             "launches_timed": spmv_calls}
 

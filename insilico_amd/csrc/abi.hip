@@ -113,8 +113,7 @@ int ins_create(ins_ctx** out, int device, uint64_t hbm_budget_bytes)
     c->device = device;
     c->budget = hbm_budget_bytes;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->evp0) != hipSuccess || hipEventCreate(&c->evp1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return INS_ERR_HIP;
     }
@@ -136,8 +135,7 @@ int ins_destroy(ins_ctx* h)
         if (c->evs[i]) hipEventDestroy(c->evs[i]);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
-    hipEventDestroy(c->evp0);
-    hipEventDestroy(c->evp1);
+    for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     hipStreamDestroy(c->stream);
     delete c;
     return INS_OK;

@@ -157,7 +157,10 @@ struct Ctx {
     DArr<int> dflag;
     bool have_solution = false;
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evp0 = nullptr, evp1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // profiling: one event pair per SpMV launch of a solve (grown on demand)
+    std::vector<hipEvent_t> prof_ev;
+    size_t prof_n = 0;
 };
 
 // allocation with budget accounting (count in elements)
