@@ -201,6 +201,42 @@ int ins_solution_norm(ins_ctx* ctx, double* norm);
  * components get their prescribed value. */
 int ins_distribute(ins_ctx* ctx, int op);
 
+/* ---- serving: many solves batched into shared launches --------------------
+ * No single reference counterpart: the reference solves one system per
+ * Eigen3::cgSolve call (base/solver/Eigen3.hpp:265-275).  A server runs the
+ * mixed-precision Jacobi-PCG of INS_SOLVE_MIXED / INS_FMT_SELL for up to
+ * max_slots assembled contexts at once: every iteration is three launches
+ * shared by all running requests, each request keeps its matrix, diagonal
+ * and iterates resident in its own context, and requests join and leave
+ * between chunks (continuous batching).  A request's iterates are the same
+ * bits alone, in any mix, and through ins_solve(ctx, INS_SOLVE_MIXED,
+ * INS_FMT_SELL, ...).  A server and the contexts attached to it are used by
+ * one host thread. */
+typedef struct ins_server ins_server;
+
+#define INS_SLOT_FREE      0
+#define INS_SLOT_RUNNING   1
+#define INS_SLOT_FINISHED  2
+
+int ins_server_create(ins_server** out, int device, int max_slots /* 1..64 */);
+int ins_server_destroy(ins_server* srv);
+const char* ins_server_last_error(const ins_server* srv);
+/* Admit an assembled context (ins_assemble done, its own stream may still be
+ * busy: the request starts after that work).  *slot receives its slot. */
+int ins_server_attach(ins_server* srv, ins_ctx* ctx, double rtol, int max_it,
+                      int* slot);
+/* Queue k batched iterations of every running request, then act on the
+ * stop / replacement flags of the chunk queued before (one chunk stays in
+ * flight).  *n_finished = requests that finished during this call. */
+int ins_server_step(ins_server* srv, int k, int* n_finished);
+/* state INS_SLOT_*; status INS_OK, INS_ERR_NOCONV or INS_ERR_BREAKDOWN */
+int ins_server_poll(ins_server* srv, int slot, int* state, int* status,
+                    int* iters, double* relres);
+/* Free the slot.  A finished request's context then holds its solution
+ * (ins_get_solution); detaching a running request cancels it. */
+int ins_server_detach(ins_server* srv, int slot);
+int ins_server_running(const ins_server* srv, int* n_running);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libinsilico_hip.so")
 INS_OK = 0
 ERRORS = {-1: "ARG", -2: "HIP", -3: "OOM", -4: "RANGE", -5: "PATTERN",
           -6: "BREAKDOWN", -7: "NOCONV", -8: "STATE", -9: "UNSUPPORTED"}
+ERRORS_INV = {v: k for k, v in ERRORS.items()}
 LAPLACE, STVENANT, NEOHOOKE = 1, 2, 3
 FP64, FP32 = 0, 1
 ASM_MATRIX, ASM_RHS, ASM_RESIDUAL, ASM_INCREMENTAL = 1, 2, 4, 8
@@ -31,8 +32,11 @@ SYMBOLS = [
     "ins_get_pattern", "ins_set_material", "ins_set_body_force", "ins_set_traction",
     "ins_assemble", "ins_get_csr_values", "ins_get_rhs", "ins_set_rhs", "ins_add_rhs",
     "ins_rhs_norm", "ins_set_csr", "ins_solve", "ins_get_solution", "ins_solution_norm",
-    "ins_distribute",
+    "ins_distribute", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
+    "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
+    "ins_server_running",
 ]
+SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 
 
 class Stats(C.Structure):
@@ -93,11 +97,21 @@ def load():
         "ins_get_solution": [vp, vp],
         "ins_solution_norm": [vp, C.POINTER(dbl)],
         "ins_distribute": [vp, i32],
+        "ins_server_create": [C.POINTER(vp), i32, i32],
+        "ins_server_destroy": [vp],
+        "ins_server_last_error": [vp],
+        "ins_server_attach": [vp, vp, dbl, i32, C.POINTER(i32)],
+        "ins_server_step": [vp, i32, C.POINTER(i32)],
+        "ins_server_poll": [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                            C.POINTER(dbl)],
+        "ins_server_detach": [vp, i32],
+        "ins_server_running": [vp, C.POINTER(i32)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
-        f.restype = C.c_char_p if name in ("ins_strerror", "ins_last_error") else C.c_int
+        f.restype = (C.c_char_p if name in ("ins_strerror", "ins_last_error",
+                                            "ins_server_last_error") else C.c_int)
     _lib = L
     return L
 
@@ -280,3 +294,73 @@ class Context:
         self.set_body_force(prob.bf_kind, prob.bf)
         self.set_traction(prob.faces, prob.traction)
         return self.build_pattern()
+
+
+class Server:
+    """Batched mixed-precision solves of several assembled contexts
+    (continuous batching: requests attach and finish between chunks)."""
+
+    def __init__(self, device=0, max_slots=64):
+        self.L = load()
+        h = C.c_void_p()
+        s = self.L.ins_server_create(C.byref(h), device, max_slots)
+        if s != INS_OK:
+            raise InsError(s, self.L.ins_strerror(s).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.ins_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, s):
+        if s != INS_OK:
+            raise InsError(s, self.L.ins_server_last_error(self.h).decode())
+        return s
+
+    def attach(self, ctx, rtol=1e-8, max_it=None):
+        slot = C.c_int()
+        self._chk(self.L.ins_server_attach(self.h, ctx.h, rtol,
+                                           max_it if max_it is not None else max(2 * ctx.n_active, 100),
+                                           C.byref(slot)))
+        return slot.value
+
+    def step(self, k=16):
+        nf = C.c_int()
+        self._chk(self.L.ins_server_step(self.h, k, C.byref(nf)))
+        return nf.value
+
+    def poll(self, slot):
+        st, status, it, rr = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.L.ins_server_poll(self.h, slot, C.byref(st), C.byref(status), C.byref(it),
+                                         C.byref(rr)))
+        return st.value, status.value, it.value, rr.value
+
+    def detach(self, slot):
+        self._chk(self.L.ins_server_detach(self.h, slot))
+
+    def running(self):
+        n = C.c_int()
+        self._chk(self.L.ins_server_running(self.h, C.byref(n)))
+        return n.value
+
+    def run_until_done(self, k=16, limit=1000000):
+        """step until no request is running"""
+        for _ in range(limit):
+            if self.running() == 0:
+                self.step(k)  # drain the last snapshots
+                return
+            self.step(k)
+        raise RuntimeError("server did not drain")

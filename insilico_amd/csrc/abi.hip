@@ -528,4 +528,60 @@ int ins_distribute(ins_ctx* h, int op)
     return distribute(c, op);
 }
 
+// ---- serving ------------------------------------------------------------------
+int ins_server_create(ins_server** out, int device, int max_slots)
+{
+    if (!out) return INS_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return INS_ERR_HIP;
+    if (device < 0 || device >= ndev) return INS_ERR_ARG;
+    Server* sv = nullptr;
+    const int s = server_create(&sv, device, max_slots);
+    *out = reinterpret_cast<ins_server*>(sv);
+    return s;
+}
+
+int ins_server_destroy(ins_server* h)
+{
+    if (!h) return INS_ERR_ARG;
+    server_destroy(reinterpret_cast<Server*>(h));
+    return INS_OK;
+}
+
+const char* ins_server_last_error(const ins_server* h)
+{
+    return server_error(reinterpret_cast<Server*>(const_cast<ins_server*>(h)));
+}
+
+int ins_server_attach(ins_server* h, ins_ctx* ctx, double rtol, int max_it, int* slot)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!h || !c) return INS_ERR_ARG;
+    CTX_CHECK(c);
+    return server_attach(reinterpret_cast<Server*>(h), c, rtol, max_it, slot);
+}
+
+int ins_server_step(ins_server* h, int k, int* n_finished)
+{
+    return server_step(reinterpret_cast<Server*>(h), k, n_finished);
+}
+
+int ins_server_poll(ins_server* h, int slot, int* state, int* status, int* iters,
+                    double* relres)
+{
+    return server_poll(reinterpret_cast<Server*>(h), slot, state, status, iters, relres);
+}
+
+int ins_server_detach(ins_server* h, int slot)
+{
+    return server_detach(reinterpret_cast<Server*>(h), slot);
+}
+
+int ins_server_running(const ins_server* h, int* n)
+{
+    if (!h || !n) return INS_ERR_ARG;
+    *n = server_running(reinterpret_cast<Server*>(const_cast<ins_server*>(h)));
+    return INS_OK;
+}
+
 }  // extern "C"

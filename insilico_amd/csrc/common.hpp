@@ -56,7 +56,7 @@ struct Scal {
     int iters;       // SpMVs performed inside the loop
     int restart;     // mixed: restart the direction after a replacement
     int pause;       // mixed: the residual dropped 10x, replace it now
-    int pad;
+    int fin;         // converged on the true (FP64) residual, or empty rhs
     double rref;     // mixed: |b - A x|^2 at the last replacement
     double delta2;   // mixed: replace when |r|^2 < delta2 * rref
 };
@@ -204,6 +204,16 @@ inline unsigned grid_for(int64_t work, int block, unsigned cap = 65535u * 8)
 }
 
 // ---- launchers provided by the translation units ---------------------------
+// solve.hip: serving (batched mixed-precision solves)
+struct Server;
+int server_create(Server** out, int device, int max_slots);
+void server_destroy(Server* sv);
+const char* server_error(Server* sv);
+int server_attach(Server* sv, Ctx* c, double rtol, int max_it, int* slot);
+int server_step(Server* sv, int k, int* n_finished);
+int server_poll(Server* sv, int s, int* state, int* status, int* iters, double* relres);
+int server_detach(Server* sv, int s);
+int server_running(Server* sv);
 // scan.hip
 int exclusive_scan_i32(Ctx* c, const int32_t* in, int32_t* out, int64_t n,
                        int64_t* total);
@@ -219,7 +229,7 @@ int compute_dinv(Ctx* c);
 // solve.hip
 int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
           double* relres);
-int build_sell(Ctx* c);
+int build_sell(Ctx* c, bool force);
 int distribute(Ctx* c, int op);
 int vec_norm2(Ctx* c, const double* v, int64_t n, double* out);
 

@@ -21,6 +21,7 @@
 // (FP64 values, FP64 arithmetic).  Convergence is only declared on that FP64
 // residual.
 #include <algorithm>
+#include <deque>
 #include <cmath>
 #include <cstdlib>
 
@@ -66,14 +67,24 @@ __device__ inline void block_allsum(double (&v)[NV])
     }
 }
 
+// A block's place in the decomposition of one solve: its index and the
+// number of blocks.  A plain launch uses blockIdx/gridDim; a batched launch
+// (several solves in one grid) gives every solve the decomposition it has
+// alone, so its sums, and hence its iterates, do not depend on the batch.
+struct Blk {
+    int id, n;
+};
+__device__ inline Blk grid_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
+
 // one partial per block
 template <int NV>
-__device__ inline void block_partials(double (&v)[NV], double* part, const int (&slot)[NV])
+__device__ inline void block_partials(double (&v)[NV], double* part, const int (&slot)[NV],
+                                      Blk b = grid_blk())
 {
     block_allsum<NV>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) part[slot[k] * MAXG + blockIdx.x] = v[k];
+        for (int k = 0; k < NV; k++) part[slot[k] * MAXG + b.id] = v[k];
 }
 
 // every block: the same total of cnt partials of the given slots
@@ -88,7 +99,7 @@ __device__ inline void totals(const double* part, int cnt, const int (&slot)[NV]
     block_allsum<NV>(t);
 }
 
-__device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+__device__ inline bool leader(Blk b = grid_blk()) { return b.id == 0 && threadIdx.x == 0; }
 
 // Workgroups are placed on the 8 XCDs round-robin by block id.  Each XCD
 // owns one contiguous eighth of the items (rows or slices) and its blocks
@@ -114,17 +125,17 @@ __device__ inline void st(T* p, T v, bool nt)
     else *p = v;
 }
 
-__device__ inline Sweep xcd_sweep(int64_t total, int64_t step)
+__device__ inline Sweep xcd_sweep(int64_t total, int64_t step, Blk b = grid_blk())
 {
-    const int g = gridDim.x;
+    const int g = b.n;
     Sweep w;
     if ((g % NXCD) || !g_xcd_mode) {
-        w.first = (int64_t)blockIdx.x * step;
+        w.first = (int64_t)b.id * step;
         w.end = total;
         w.stride = (int64_t)g * step;
         return w;
     }
-    const int xcd = blockIdx.x % NXCD, local = blockIdx.x / NXCD, nloc = g / NXCD;
+    const int xcd = b.id % NXCD, local = b.id / NXCD, nloc = g / NXCD;
     int64_t per = (total + NXCD - 1) / NXCD;
     per = (per + step - 1) / step * step;
     const int64_t lo = (int64_t)xcd * per;
@@ -242,15 +253,15 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
 // ---------------------------------------------------------------------------
 // K1: q = A p with per-block partials of p.q
 template <typename TV, typename TX, typename TA, int SGK>
-__global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
-                                                  const TX* __restrict__ p, TX* __restrict__ q,
-                                                  double* part, const Scal* sc)
+__device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
+                                      const TV* __restrict__ val, const TX* __restrict__ p,
+                                      TX* __restrict__ q, double* part, const Scal* sc)
 {
     if (sc->done | sc->pause) return;
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
     double pq = 0.;
-    const Sweep w = xcd_sweep(nsl, SB / 64);
+    const Sweep w = xcd_sweep(nsl, SB / 64, b);
     for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
         const int64_t rho = slice * 64 + lane;
         const bool live = rho < n;
@@ -264,7 +275,15 @@ __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const T
     }
     double v[1] = {pq};
     const int sl[1] = {P_PQ};
-    block_partials<1>(v, part, sl);
+    block_partials<1>(v, part, sl, b);
+}
+
+template <typename TV, typename TX, typename TA, int SGK>
+__global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
+                                                  const TX* __restrict__ p, TX* __restrict__ q,
+                                                  double* part, const Scal* sc)
+{
+    spmv_sell_body<TV, TX, TA, SGK>(grid_blk(), n, S, val, p, q, part, sc);
 }
 
 template <int LPR, typename TV, typename TX, typename TA>
@@ -293,11 +312,10 @@ __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __res
 
 // K2: alpha from the p.q partials; x += alpha p; r -= alpha q; r.r, r.z partials
 template <typename TX, typename TD>
-__global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x, TX* __restrict__ r,
-                                                  const TX* __restrict__ p,
-                                                  const TX* __restrict__ q,
-                                                  const TD* __restrict__ dinv, double* part,
-                                                  int g_pq, Scal* sc, int par)
+__device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* __restrict__ r,
+                                      const TX* __restrict__ p, const TX* __restrict__ q,
+                                      const TD* __restrict__ dinv, double* part, int g_pq,
+                                      Scal* sc, int par)
 {
     if (sc->done | sc->pause) return;
     double t[1];
@@ -305,7 +323,7 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
     totals<1>(part, g_pq, slq, t);
     const double pq = t[0];
     if (!(pq > 0.) || !isfinite(pq)) {
-        if (leader()) {
+        if (leader(b)) {
             sc->pq = pq;
             sc->breakdown = 1;
             sc->done = 1;
@@ -313,7 +331,7 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
         return;
     }
     const double ad = sc->rho[par] / pq;
-    if (leader()) {
+    if (leader(b)) {
         sc->pq = pq;
         sc->alpha = ad;
         sc->iters += 1;
@@ -321,7 +339,7 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
     const TX alpha = (TX)ad;
     const bool nt = g_nt;
     double rr = 0., rz = 0.;
-    const Sweep w = xcd_sweep(n, VB);
+    const Sweep w = xcd_sweep(n, VB, b);
     for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
         st(&x[i], x[i] + alpha * p[i], nt);
         const TX ri = r[i] - alpha * q[i];
@@ -332,15 +350,24 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
     }
     double v[2] = {rr, rz};
     const int sl[2] = {P_RR, P_RZ};
-    block_partials<2>(v, part, sl);
+    block_partials<2>(v, part, sl, b);
+}
+
+template <typename TX, typename TD>
+__global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x, TX* __restrict__ r,
+                                                  const TX* __restrict__ p,
+                                                  const TX* __restrict__ q,
+                                                  const TD* __restrict__ dinv, double* part,
+                                                  int g_pq, Scal* sc, int par)
+{
+    update_xr_body<TX, TD>(grid_blk(), n, x, r, p, q, dinv, part, g_pq, sc, par);
 }
 
 // K3: stop test; beta; p = D^-1 r + beta p
 template <typename TX, typename TD>
-__global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
-                                                 const TX* __restrict__ r,
-                                                 const TD* __restrict__ dinv, const double* part,
-                                                 int g, Scal* sc, int par)
+__device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
+                                     const TX* __restrict__ r, const TD* __restrict__ dinv,
+                                     const double* part, int g, Scal* sc, int par)
 {
     if (sc->done | sc->pause) return;
     double t[2];
@@ -350,7 +377,7 @@ __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
     const bool bad = !isfinite(rr);
     const bool conv = rr < sc->thr;
     const double rho = sc->rho[par];
-    if (leader()) {
+    if (leader(b)) {
         sc->rr = rr;
         if (bad) {
             sc->breakdown = 1;
@@ -367,9 +394,18 @@ __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
     }
     if (bad || conv) return;
     const TX beta = (TX)(rz / rho);
-    const Sweep w = xcd_sweep(n, VB);
+    const Sweep w = xcd_sweep(n, VB, b);
     for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride)
         st(&p[i], (TX)dinv[i] * r[i] + beta * p[i], g_nt);
+}
+
+template <typename TX, typename TD>
+__global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
+                                                 const TX* __restrict__ r,
+                                                 const TD* __restrict__ dinv, const double* part,
+                                                 int g, Scal* sc, int par)
+{
+    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, par);
 }
 
 // start: x = 0, r = b, p = D^-1 b; partials of b.b and r.z
@@ -412,6 +448,7 @@ __global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, doub
         sc->breakdown = 0;
         sc->delta2 = delta2;
         sc->done = (t[0] == 0.) ? 1 : 0;
+        sc->fin = sc->done;
         sc->restart = 0;
         sc->pause = 0;
         sc->rref = t[0];
@@ -508,6 +545,7 @@ __global__ __launch_bounds__(VB) void k_resid_fin(const double* part, int g, Sca
                 sc->done = 1;
             } else if (t[0] < sc->thr) {
                 sc->done = 1;
+                sc->fin = 1;
             } else {
                 // keep the search direction only while the recursion still
                 // tracks the true residual; once the FP32 recursion has
@@ -935,32 +973,35 @@ int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const
 }
 
 // |b - A x64| (and the residual replacement); then the finalising kernel
-int launch_resid(Ctx* c, int format, int replace, int par_next)
+int launch_resid(Ctx* c, int format, int replace, int par_next, hipStream_t stream = nullptr,
+                 Scal* scal = nullptr)
 {
     const int64_t n = c->n_active;
+    hipStream_t st = stream ? stream : c->stream;
+    Scal* sc = scal ? scal : c->scal.p;
     unsigned g;
     const bool v64 = c->matrix_precision == INS_FP64;
     if (format == INS_FMT_SELL) {
         g = sell_grid(n);
         if (v64)
-            k_resid_sell<double><<<g, SB, 0, c->stream>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
         else
-            k_resid_sell<float><<<g, SB, 0, c->stream>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
     } else {
         const int lpr = pick_lpr(c);
         g = csr_grid(n, lpr);
         if (v64) {
-            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else k_resid_csr<4, double><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else k_resid_csr<4, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
         } else {
-            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else k_resid_csr<4, float><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            else k_resid_csr<4, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
         }
     }
     INS_HIP_CHECK(c, hipGetLastError());
-    k_resid_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)g, c->scal.p, replace, par_next);
+    k_resid_fin<<<1, VB, 0, st>>>(c->partial.p, (int)g, sc, replace, par_next);
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }
@@ -1047,7 +1088,9 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
 }
 
 }  // namespace
-int build_sell(Ctx* c)
+// force: keep the SELL form even when its padding makes it larger than the
+// CSR values by more than 1/8 (the serving kernels need it)
+int build_sell(Ctx* c, bool force)
 {
     Sell& S = c->sell;
     const int64_t n = c->n_active;
@@ -1157,7 +1200,7 @@ int build_sell(Ctx* c)
     hipFree(err);
     S.nvals = nvals;
     // worth it only when the shared lists stay close to the rows' own lists
-    S.ok = (herr == 0) && (nvals <= c->nnz + c->nnz / 8 + 64 * 64) && nvals < INT32_MAX;
+    S.ok = (herr == 0) && (force || nvals <= c->nnz + c->nnz / 8 + 64 * 64) && nvals < INT32_MAX;
     if (S.ok) {
         INS_TRY(ensure(c, S.map, c->nnz));
         int* e2 = nullptr;
@@ -1286,7 +1329,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
         return fail(c, INS_ERR_STATE, "ins_solve before ins_assemble (matrix and rhs)");
     if (mode == INS_SOLVE_FP64 && c->matrix_precision != INS_FP64)
         return fail(c, INS_ERR_STATE, "FP64 solve needs an FP64 assembly");
-    if (format == INS_FMT_SELL && !c->sell.built) INS_TRY(build_sell(c));
+    if (format == INS_FMT_SELL && !c->sell.built) INS_TRY(build_sell(c, false));
     if (format == INS_FMT_SELL && !c->sell.ok) format = INS_FMT_CSR;
     if (format == INS_FMT_SELL && c->sell.stale) INS_TRY(fill_sell(c));
     INS_TRY(ensure_solver_state(c, mode));
@@ -1413,5 +1456,433 @@ int vec_norm2(Ctx* c, const double* v, int64_t n, double* out)
     *out = std::sqrt(c->h_scal->tt);
     return INS_OK;
 }
+
+// ===========================================================================
+// Serving: many mixed-precision solves batched into shared launches.
+//
+// Every running request keeps its matrix (SELL, FP32 and FP64 values), its
+// Jacobi diagonal and its iterates resident in its own context.  One chunk of
+// the server runs k iterations of all of them with three launches per
+// iteration (kb_spmv, kb_update_xr, kb_update_p); block b of a launch serves
+// the request whose block range holds b, with the block index and count that
+// request has when it is solved alone, so its sums and iterates are the same
+// alone and in any mix.  Residual replacements and the final residual run per
+// request, on the server stream, between chunks.
+namespace {
+
+constexpr int MAXS = 64;  // requests in one batch
+
+struct SlotDesc {
+    int64_t n;
+    SellView S;
+    const float* val;
+    const float* dinv;
+    float* x;
+    float* r;
+    float* p;
+    float* q;
+    double* part;
+    Scal* sc;
+    int g_spmv, g_vec, par0, pad;
+};
+
+struct BatchTab {
+    int nslot, pad;
+    int pre_spmv[MAXS + 1];  // first block of every request in kb_spmv
+    int pre_vec[MAXS + 1];   // ... in kb_update_xr / kb_update_p
+    SlotDesc d[MAXS];
+};
+
+// request of block bid: the last s with pre[s] <= bid
+__device__ inline int find_slot(const int* pre, int ns, int bid)
+{
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt)
+{
+    const int s = find_slot(bt->pre_spmv, bt->nslot, blockIdx.x);
+    const SlotDesc& d = bt->d[s];
+    spmv_sell_body<float, float, float, 4>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv}, d.n,
+                                           d.S, d.val, d.p, d.q, d.part, d.sc);
+}
+
+__global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
+{
+    const int s = find_slot(bt->pre_vec, bt->nslot, blockIdx.x);
+    const SlotDesc& d = bt->d[s];
+    const int par = (int)((it & 1) ^ d.par0);
+    update_xr_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, d.x, d.r,
+                                 d.p, d.q, d.dinv, d.part, d.g_spmv, d.sc, par);
+}
+
+__global__ __launch_bounds__(VB) void kb_update_p(const BatchTab* __restrict__ bt, int64_t it)
+{
+    const int s = find_slot(bt->pre_vec, bt->nslot, blockIdx.x);
+    const SlotDesc& d = bt->d[s];
+    const int par = (int)((it & 1) ^ d.par0);
+    update_p_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, d.p, d.r,
+                                d.dinv, d.part, d.g_vec, d.sc, par);
+}
+
+}  // namespace
+
+struct SlotH {
+    Ctx* c = nullptr;
+    int state = INS_SLOT_FREE;
+    int status = INS_OK;
+    double rtol = 0.;
+    int max_it = 0;
+    int64_t join_it = 0;      // index of the request's first batched iteration
+    int64_t join_chunk = 0;   // first chunk whose snapshot includes its start
+    int64_t ignore_upto = -1; // snapshots of chunks <= this predate its last replacement
+    int refinements = 0;
+    int iters = 0;
+    double relres = 0.;
+    bool relres_known = false;
+};
+
+struct Server {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int max_slots = 0;
+    std::string err;
+    std::vector<SlotH> slots;
+    Scal* scal = nullptr;                 // device, one per slot
+    static constexpr int NSNAP = 4;
+    Scal* h_snap = nullptr;               // pinned, NSNAP x max_slots
+    hipEvent_t snap_ev[NSNAP] = {};
+    std::deque<std::pair<int64_t, int>> inflight;  // (chunk, snapshot ring entry)
+    static constexpr int NTAB = 4;
+    BatchTab* h_tab = nullptr;            // pinned staging ring
+    BatchTab* d_tab = nullptr;            // device ring
+    hipEvent_t tab_ev[NTAB] = {};
+    int tab_cur = 0;
+    bool dirty = true;
+    int nrun = 0;
+    int grid_spmv = 0, grid_vec = 0;
+    int64_t it = 0;      // batched iterations launched
+    int64_t chunks = 0;  // chunks launched
+    hipEvent_t ev_sync = nullptr;
+};
+
+namespace {
+
+int sfail(Server* s, int code, const std::string& msg)
+{
+    s->err = msg;
+    return code;
+}
+
+#define SRV_HIP(s, call)                                                          \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return sfail((s), INS_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// work of request s finished on the server stream before later work of its context
+int hand_back(Server* sv, SlotH& h)
+{
+    SRV_HIP(sv, hipEventRecord(sv->ev_sync, sv->stream));
+    SRV_HIP(sv, hipStreamWaitEvent(h.c->stream, sv->ev_sync, 0));
+    return INS_OK;
+}
+
+int finish(Server* sv, int s, int status, const Scal* S)
+{
+    SlotH& h = sv->slots[s];
+    h.state = INS_SLOT_FINISHED;
+    h.status = status;
+    h.c->have_solution = true;
+    if (S) {
+        h.iters = S->iters;
+        h.relres = S->bb > 0. ? std::sqrt(S->tt / S->bb) : 0.;
+        h.relres_known = true;
+        h.c->stats.iterations = h.iters;
+        h.c->stats.relres = h.relres;
+    }
+    h.c->stats.refinements = h.refinements;
+    sv->nrun--;
+    sv->dirty = true;
+    return hand_back(sv, h);
+}
+
+// x64 += x32; r = b - A x64 (FP64); restart the direction if needed
+int replace(Server* sv, int s)
+{
+    SlotH& h = sv->slots[s];
+    Ctx* c = h.c;
+    const int64_t n = c->n_active;
+    const unsigned gv = vec_grid(n);
+    Scal* sc = sv->scal + s;
+    k_fold<<<gv, VB, 0, sv->stream>>>(n, c->x.p, c->x32.p);
+    SRV_HIP(sv, hipGetLastError());
+    const int par_next = (int)((sv->it - h.join_it) & 1);
+    if (launch_resid(c, INS_FMT_SELL, 1, par_next, sv->stream, sc) != INS_OK)
+        return sfail(sv, INS_ERR_HIP, c->err);
+    k_restart<<<gv, VB, 0, sv->stream>>>(n, c->p32.p, c->r32.p, c->dinv32.p, sc);
+    SRV_HIP(sv, hipGetLastError());
+    h.refinements++;
+    h.ignore_upto = sv->chunks - 1;
+    return INS_OK;
+}
+
+int process_snapshot(Server* sv, int64_t chunk, const Scal* snap)
+{
+    for (int s = 0; s < sv->max_slots; s++) {
+        SlotH& h = sv->slots[s];
+        if (h.state != INS_SLOT_RUNNING || chunk < h.join_chunk || chunk <= h.ignore_upto)
+            continue;
+        const Scal& S = snap[s];
+        if (S.fin) {
+            INS_TRY(finish(sv, s, INS_OK, &S));
+        } else if (S.breakdown) {
+            INS_TRY(finish(sv, s, INS_ERR_BREAKDOWN, &S));
+        } else if (S.done || S.pause) {
+            INS_TRY(replace(sv, s));
+        } else if (S.iters >= h.max_it) {
+            // iteration limit: fold what was computed and measure it
+            Ctx* c = h.c;
+            k_fold<<<vec_grid(c->n_active), VB, 0, sv->stream>>>(c->n_active, c->x.p, c->x32.p);
+            SRV_HIP(sv, hipGetLastError());
+            if (launch_resid(c, INS_FMT_SELL, 0, 0, sv->stream, sv->scal + s) != INS_OK)
+                return sfail(sv, INS_ERR_HIP, c->err);
+            h.iters = S.iters;
+            h.relres_known = false;
+            INS_TRY(finish(sv, s, INS_ERR_NOCONV, nullptr));
+        }
+    }
+    return INS_OK;
+}
+
+int upload_table(Server* sv)
+{
+    const int t = (sv->tab_cur + 1) % Server::NTAB;
+    SRV_HIP(sv, hipEventSynchronize(sv->tab_ev[t]));
+    BatchTab& T = sv->h_tab[t];
+    int ns = 0, gs = 0, gvv = 0;
+    for (int s = 0; s < sv->max_slots; s++) {
+        const SlotH& h = sv->slots[s];
+        if (h.state != INS_SLOT_RUNNING) continue;
+        Ctx* c = h.c;
+        SlotDesc& d = T.d[ns];
+        d.n = c->n_active;
+        d.S = sell_view(c);
+        d.val = c->sell.v32.p;
+        d.dinv = c->dinv32.p;
+        d.x = c->x32.p;
+        d.r = c->r32.p;
+        d.p = c->p32.p;
+        d.q = c->q32.p;
+        d.part = c->partial.p;
+        d.sc = sv->scal + s;
+        d.g_spmv = (int)sell_grid(d.n);
+        d.g_vec = (int)vec_grid(d.n);
+        d.par0 = (int)(h.join_it & 1);
+        d.pad = 0;
+        T.pre_spmv[ns] = gs;
+        T.pre_vec[ns] = gvv;
+        gs += d.g_spmv;
+        gvv += d.g_vec;
+        ns++;
+    }
+    T.nslot = ns;
+    T.pre_spmv[ns] = gs;
+    T.pre_vec[ns] = gvv;
+    sv->grid_spmv = gs;
+    sv->grid_vec = gvv;
+    SRV_HIP(sv, hipMemcpyAsync(sv->d_tab + t, &T, sizeof(BatchTab), hipMemcpyHostToDevice,
+                               sv->stream));
+    SRV_HIP(sv, hipEventRecord(sv->tab_ev[t], sv->stream));
+    sv->tab_cur = t;
+    sv->dirty = false;
+    return INS_OK;
+}
+
+int pop_snapshot(Server* sv)
+{
+    const auto f = sv->inflight.front();
+    sv->inflight.pop_front();
+    SRV_HIP(sv, hipEventSynchronize(sv->snap_ev[f.second]));
+    return process_snapshot(sv, f.first, sv->h_snap + (size_t)f.second * sv->max_slots);
+}
+
+}  // namespace
+
+int server_create(Server** out, int device, int max_slots)
+{
+    if (!out || max_slots < 1 || max_slots > MAXS) return INS_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return INS_ERR_HIP;
+    Server* sv = new (std::nothrow) Server();
+    if (!sv) return INS_ERR_OOM;
+    sv->device = device;
+    sv->max_slots = max_slots;
+    sv->slots.resize(max_slots);
+    bool ok = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc((void**)&sv->scal, max_slots * sizeof(Scal)) == hipSuccess &&
+              hipHostMalloc((void**)&sv->h_snap, Server::NSNAP * max_slots * sizeof(Scal), 0) ==
+                  hipSuccess &&
+              hipHostMalloc((void**)&sv->h_tab, Server::NTAB * sizeof(BatchTab), 0) == hipSuccess &&
+              hipMalloc((void**)&sv->d_tab, Server::NTAB * sizeof(BatchTab)) == hipSuccess &&
+              hipEventCreateWithFlags(&sv->ev_sync, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < Server::NSNAP; i++)
+        ok = hipEventCreateWithFlags(&sv->snap_ev[i], hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < Server::NTAB; i++)
+        ok = hipEventCreateWithFlags(&sv->tab_ev[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        server_destroy(sv);
+        return INS_ERR_HIP;
+    }
+    *out = sv;
+    return INS_OK;
+}
+
+void server_destroy(Server* sv)
+{
+    if (!sv) return;
+    (void)hipSetDevice(sv->device);
+    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+    for (auto& e : sv->snap_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : sv->tab_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (sv->ev_sync) (void)hipEventDestroy(sv->ev_sync);
+    if (sv->scal) (void)hipFree(sv->scal);
+    if (sv->d_tab) (void)hipFree(sv->d_tab);
+    if (sv->h_snap) (void)hipHostFree(sv->h_snap);
+    if (sv->h_tab) (void)hipHostFree(sv->h_tab);
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    delete sv;
+}
+
+const char* server_error(Server* sv) { return sv ? sv->err.c_str() : "null server"; }
+
+int server_attach(Server* sv, Ctx* c, double rtol, int max_it, int* slot)
+{
+    if (!sv || !c || !slot || !(rtol > 0.) || max_it < 1) return INS_ERR_ARG;
+    if (c->device != sv->device) return sfail(sv, INS_ERR_ARG, "context on another device");
+    if (!c->have_matrix || !c->have_rhs)
+        return sfail(sv, INS_ERR_STATE, "request context not assembled (matrix and rhs)");
+    for (int s = 0; s < sv->max_slots; s++)
+        if (sv->slots[s].state != INS_SLOT_FREE && sv->slots[s].c == c)
+            return sfail(sv, INS_ERR_STATE, "context already attached");
+    int s = -1;
+    for (int k = 0; k < sv->max_slots && s < 0; k++)
+        if (sv->slots[k].state == INS_SLOT_FREE) s = k;
+    if (s < 0) return sfail(sv, INS_ERR_STATE, "no free slot");
+    auto ctx_fail = [&](int st) { return sfail(sv, st, c->err); };
+    int st;
+    if ((!c->sell.built || !c->sell.ok) && (st = build_sell(c, true)) != INS_OK)
+        return ctx_fail(st);
+    if (!c->sell.ok)
+        return sfail(sv, INS_ERR_UNSUPPORTED, "matrix has no SELL form (irregular rows)");
+    if (c->sell.stale && (st = fill_sell(c)) != INS_OK) return ctx_fail(st);
+    if ((st = ensure_solver_state(c, INS_SOLVE_MIXED)) != INS_OK) return ctx_fail(st);
+    // the request's own setup (assembly, SELL) precedes its start
+    SRV_HIP(sv, hipEventRecord(sv->ev_sync, c->stream));
+    SRV_HIP(sv, hipStreamWaitEvent(sv->stream, sv->ev_sync, 0));
+    const int64_t n = c->n_active;
+    const unsigned gv = vec_grid(n);
+    SRV_HIP(sv, hipMemsetAsync(c->x.p, 0, n * sizeof(double), sv->stream));
+    k_init<float, float><<<gv, VB, 0, sv->stream>>>(n, c->b.p, c->x32.p, c->r32.p, c->p32.p,
+                                                    c->dinv32.p, c->partial.p);
+    k_init_fin<<<1, VB, 0, sv->stream>>>(c->partial.p, (int)gv, rtol * rtol, 0.01, sv->scal + s);
+    SRV_HIP(sv, hipGetLastError());
+    SlotH& h = sv->slots[s];
+    h = SlotH();
+    h.c = c;
+    h.state = INS_SLOT_RUNNING;
+    h.rtol = rtol;
+    h.max_it = max_it;
+    h.join_it = sv->it;
+    h.join_chunk = sv->chunks;
+    c->have_solution = false;
+    sv->nrun++;
+    sv->dirty = true;
+    *slot = s;
+    return INS_OK;
+}
+
+int server_step(Server* sv, int k, int* n_finished)
+{
+    if (!sv || k < 1) return INS_ERR_ARG;
+    int before = 0;
+    for (const auto& h : sv->slots) before += (h.state == INS_SLOT_FINISHED);
+    if (sv->nrun > 0) {
+        if (sv->dirty) INS_TRY(upload_table(sv));
+        const BatchTab* tab = sv->d_tab + sv->tab_cur;
+        for (int i = 0; i < k; i++) {
+            kb_spmv<<<sv->grid_spmv, SB, 0, sv->stream>>>(tab);
+            kb_update_xr<<<sv->grid_vec, VB, 0, sv->stream>>>(tab, sv->it);
+            kb_update_p<<<sv->grid_vec, VB, 0, sv->stream>>>(tab, sv->it);
+            sv->it++;
+        }
+        SRV_HIP(sv, hipGetLastError());
+        const int r = (int)(sv->chunks % Server::NSNAP);
+        SRV_HIP(sv, hipMemcpyAsync(sv->h_snap + (size_t)r * sv->max_slots, sv->scal,
+                                   sv->max_slots * sizeof(Scal), hipMemcpyDeviceToHost,
+                                   sv->stream));
+        SRV_HIP(sv, hipEventRecord(sv->snap_ev[r], sv->stream));
+        sv->inflight.emplace_back(sv->chunks, r);
+        sv->chunks++;
+    }
+    // keep one chunk queued behind the one being examined
+    while (sv->inflight.size() >= 2 || (!sv->inflight.empty() && sv->nrun == 0))
+        INS_TRY(pop_snapshot(sv));
+    int after = 0;
+    for (const auto& h : sv->slots) after += (h.state == INS_SLOT_FINISHED);
+    if (n_finished) *n_finished = after - before;
+    return INS_OK;
+}
+
+int server_poll(Server* sv, int s, int* state, int* status, int* iters, double* relres)
+{
+    if (!sv || s < 0 || s >= sv->max_slots) return INS_ERR_ARG;
+    SlotH& h = sv->slots[s];
+    if (h.state == INS_SLOT_FINISHED && !h.relres_known) {
+        Scal S;
+        SRV_HIP(sv, hipStreamSynchronize(sv->stream));
+        SRV_HIP(sv, hipMemcpy(&S, sv->scal + s, sizeof(Scal), hipMemcpyDeviceToHost));
+        h.relres = S.bb > 0. ? std::sqrt(S.tt / S.bb) : 0.;
+        h.relres_known = true;
+        h.c->stats.iterations = h.iters;
+        h.c->stats.relres = h.relres;
+    }
+    if (state) *state = h.state;
+    if (status) *status = h.status;
+    if (iters) *iters = h.state == INS_SLOT_FREE ? 0 : h.iters;
+    if (relres) *relres = h.relres;
+    return INS_OK;
+}
+
+int server_detach(Server* sv, int s)
+{
+    if (!sv || s < 0 || s >= sv->max_slots) return INS_ERR_ARG;
+    SlotH& h = sv->slots[s];
+    if (h.state == INS_SLOT_FREE) return sfail(sv, INS_ERR_STATE, "slot not in use");
+    if (h.state == INS_SLOT_RUNNING) {
+        // cancel: nothing of this request may still be queued when it leaves
+        h.state = INS_SLOT_FREE;
+        sv->nrun--;
+        sv->dirty = true;
+        SRV_HIP(sv, hipStreamSynchronize(sv->stream));
+        sv->inflight.clear();
+    } else if (h.status != INS_OK && h.status != INS_ERR_BREAKDOWN) {
+        // queued chunks may still iterate on its vectors (its stop flag is not set)
+        SRV_HIP(sv, hipStreamSynchronize(sv->stream));
+    }
+    h = SlotH();
+    return INS_OK;
+}
+
+int server_running(Server* sv) { return sv ? sv->nrun : 0; }
 
 }  // namespace ins

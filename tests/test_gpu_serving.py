@@ -1,0 +1,121 @@
+"""Serving: batched mixed-precision solves of several requests in shared
+launches.  Each request's solution is bit-identical alone, in a mix with
+staggered arrivals, and through the single-request ins_solve path, and it
+meets the tolerance checked by the oracle's FP64 SpMV."""
+import numpy as np
+import pytest
+
+from insilico_amd import abi, problems as P
+
+pytestmark = pytest.mark.gpu
+
+REQS = [
+    lambda: P.diffusion3d(8, seed=1),
+    lambda: P.elasticity3d(4, seed=2),
+    lambda: P.diffusion3d(14, seed=3),
+    lambda: P.elasticity3d(7, seed=4),
+    lambda: P.poisson2d(40),
+    lambda: P.diffusion3d(20, seed=5),
+]
+
+
+def _ctx(prob):
+    ctx = abi.Context(0)
+    ctx.load_problem(prob)
+    ctx.assemble(abi.FP64, abi.ASM_MATRIX | abi.ASM_RHS)
+    return ctx
+
+
+def _serve(srv, ctxs, stagger=0, k=8):
+    """attach the contexts (every `stagger` steps one more), run to the end;
+    returns [(x, iters, relres, status)]"""
+    slots = {}
+    pending = list(range(len(ctxs)))
+    steps = 0
+    while pending or srv.running():
+        if pending and (stagger == 0 or steps % stagger == 0):
+            while pending:
+                i = pending.pop(0)
+                slots[i] = srv.attach(ctxs[i], 1e-8)
+                if stagger:
+                    break
+        srv.step(k)
+        steps += 1
+        assert steps < 100000
+    srv.step(k)
+    out = []
+    for i in range(len(ctxs)):
+        st, status, it, rr = srv.poll(slots[i])
+        assert st == abi.SLOT_FINISHED
+        out.append((ctxs[i].get_solution(), it, rr, status))
+        srv.detach(slots[i])
+    return out
+
+
+def test_serving_alone_mix_and_single_identical(oracle):
+    probs = [f() for f in REQS]
+    ctxs = [_ctx(p) for p in probs]
+    with abi.Server(0, 16) as srv:
+        alone = [_serve(srv, [c])[0] for c in ctxs]
+    with abi.Server(0, 16) as srv:
+        mix = _serve(srv, ctxs, stagger=0)
+    with abi.Server(0, 8) as srv:
+        # arrivals spread over the run: requests join while others iterate
+        staggered = _serve(srv, ctxs, stagger=3, k=5)
+    for i, (p, c) in enumerate(zip(probs, ctxs)):
+        it, rr, st = c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+        single = c.get_solution()
+        xa, ia, ra, sa = alone[i]
+        assert sa == abi.INS_OK and ra <= 1e-8
+        for xm, im, rm, sm in (mix[i], staggered[i]):
+            assert sm == abi.INS_OK
+            assert im == ia and rm == ra
+            assert np.array_equal(xm, xa), (i, np.abs(xm - xa).max())
+        assert it == ia
+        assert np.array_equal(single, xa), (i, np.abs(single - xa).max())
+        orp, oci, oval, ob, _ = oracle.assemble(p)
+        rel = np.linalg.norm(ob - oracle.spmv(orp, oci, oval, xa)) / np.linalg.norm(ob)
+        assert rel <= 1e-8 * 1.001, (i, rel)
+    for c in ctxs:
+        c.close()
+
+
+def test_serving_iteration_limit_and_cancel():
+    c1 = _ctx(P.diffusion3d(12, seed=8))
+    c2 = _ctx(P.diffusion3d(10, seed=9))
+    with abi.Server(0, 4) as srv:
+        s1 = srv.attach(c1, 1e-8, max_it=10)
+        s2 = srv.attach(c2, 1e-8)
+        for _ in range(20):
+            srv.step(4)
+            if srv.poll(s1)[0] == abi.SLOT_FINISHED:
+                break
+        st, status, it, rr = srv.poll(s1)
+        assert st == abi.SLOT_FINISHED and status == abi.ERRORS_INV["NOCONV"]
+        assert it >= 10 and 0 < rr < 1
+        x1 = c1.get_solution()
+        assert np.all(np.isfinite(x1))
+        # cancel the other one while it runs, then the server is empty
+        st2, _, _, _ = srv.poll(s2)
+        assert st2 == abi.SLOT_RUNNING
+        srv.detach(s2)
+        srv.detach(s1)
+        assert srv.running() == 0
+        # slots are reusable: a fresh request on the freed slots converges
+        s3 = srv.attach(c2, 1e-8)
+        srv.run_until_done(8)
+        st3, status3, it3, rr3 = srv.poll(s3)
+        assert st3 == abi.SLOT_FINISHED and status3 == abi.INS_OK and rr3 <= 1e-8
+        srv.detach(s3)
+    c1.close()
+    c2.close()
+
+
+def test_serving_rejects_unassembled():
+    c = abi.Context(0)
+    c.load_problem(P.diffusion3d(4, seed=1))
+    with abi.Server(0, 2) as srv:
+        with pytest.raises(abi.InsError) as e:
+            srv.attach(c, 1e-8)
+        assert e.value.status == abi.ERRORS_INV["STATE"]
+    c.close()
