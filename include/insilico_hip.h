@@ -192,6 +192,13 @@ int ins_set_csr(ins_ctx* ctx, int64_t n, const int32_t* row_ptr,
  * last iterate is still available). */
 int ins_solve(ins_ctx* ctx, int mode, int format, double rtol, int max_it,
               int* iters, double* relres);
+/* Build what ins_solve(mode, format) needs (SELL form and its values,
+ * solver vectors) without solving, so that a later solve or server attach
+ * allocates nothing.  With INS_FMT_SELL the SELL form is kept even where its
+ * padding exceeds 1/8 of the CSR values (ins_solve alone would fall back to
+ * CSR there).  No reference counterpart (Eigen builds its solver state
+ * inside cgSolve, base/solver/Eigen3.hpp:265-275). */
+int ins_prepare(ins_ctx* ctx, int mode, int format);
 /* Eigen3::getValue for all rows (base/solver/Eigen3.hpp:293-296) */
 int ins_get_solution(ins_ctx* ctx, double* x);
 /* |solution|_2 / N as Eigen3::norm() reads it after cgSolve */

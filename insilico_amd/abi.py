@@ -32,7 +32,7 @@ SYMBOLS = [
     "ins_get_pattern", "ins_set_material", "ins_set_body_force", "ins_set_traction",
     "ins_assemble", "ins_get_csr_values", "ins_get_rhs", "ins_set_rhs", "ins_add_rhs",
     "ins_rhs_norm", "ins_set_csr", "ins_solve", "ins_get_solution", "ins_solution_norm",
-    "ins_distribute", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
+    "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
     "ins_server_running",
 ]
@@ -97,6 +97,7 @@ def load():
         "ins_get_solution": [vp, vp],
         "ins_solution_norm": [vp, C.POINTER(dbl)],
         "ins_distribute": [vp, i32],
+        "ins_prepare": [vp, i32, i32],
         "ins_server_create": [C.POINTER(vp), i32, i32],
         "ins_server_destroy": [vp],
         "ins_server_last_error": [vp],
@@ -259,6 +260,9 @@ class Context:
         if check:
             self._chk(s)
         return it.value, rr.value, s
+
+    def prepare(self, mode=SOLVE_MIXED, fmt=FMT_SELL):
+        self._chk(self.L.ins_prepare(self.h, mode, fmt))
 
     def get_solution(self):
         x = np.zeros(self.n_active)

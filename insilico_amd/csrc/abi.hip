@@ -112,6 +112,14 @@ int ins_create(ins_ctx** out, int device, uint64_t hbm_budget_bytes)
     if (!c) return INS_ERR_OOM;
     c->device = device;
     c->budget = hbm_budget_bytes;
+    // temporaries of the setup steps are stream-ordered allocations on the
+    // context stream (no device-wide synchronisation); keep their memory in
+    // the pool between calls
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -499,6 +507,15 @@ int ins_solve(ins_ctx* h, int mode, int format, double rtol, int max_it, int* it
     if (format != INS_FMT_CSR && format != INS_FMT_SELL) return fail(c, INS_ERR_ARG, "format");
     if (!(rtol > 0.) || max_it < 0) return fail(c, INS_ERR_ARG, "rtol/max_it");
     return solve(c, mode, format, rtol, max_it, iters, relres);
+}
+
+int ins_prepare(ins_ctx* h, int mode, int format)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (mode != INS_SOLVE_FP64 && mode != INS_SOLVE_MIXED) return fail(c, INS_ERR_ARG, "solve mode");
+    if (format != INS_FMT_CSR && format != INS_FMT_SELL) return fail(c, INS_ERR_ARG, "format");
+    return prepare(c, mode, format);
 }
 
 int ins_get_solution(ins_ctx* h, double* x)

@@ -81,6 +81,8 @@ struct Sell {
     bool ok = false;         // dictionary small enough to be worth it
     bool built = false;
     bool stale = true;       // values older than the last assembly
+    bool zero64 = false;     // v64 / v32 sized and zeroed for the current slices
+    bool zero32 = false;
 };
 
 struct Ctx {
@@ -230,6 +232,7 @@ int compute_dinv(Ctx* c);
 int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
           double* relres);
 int build_sell(Ctx* c, bool force);
+int prepare(Ctx* c, int mode, int format);
 int distribute(Ctx* c, int op);
 int vec_norm2(Ctx* c, const double* v, int64_t n, double* out);
 

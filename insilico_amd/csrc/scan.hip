@@ -90,8 +90,8 @@ int scan_i64(Ctx* c, const int64_t* in, int64_t n, int64_t* out)
     }
     int64_t* sums = nullptr;
     int64_t* soff = nullptr;
-    INS_HIP_CHECK(c, hipMalloc((void**)&sums, nb * sizeof(int64_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&soff, (nb + 1) * sizeof(int64_t)));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&sums, nb * sizeof(int64_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&soff, (nb + 1) * sizeof(int64_t), c->stream));
     k_chunk_sums<int64_t><<<(unsigned)nb, SB, 0, c->stream>>>(in, n, sums);
     INS_HIP_CHECK(c, hipGetLastError());
     int s = scan_i64(c, sums, nb, soff);
@@ -100,8 +100,8 @@ int scan_i64(Ctx* c, const int64_t* in, int64_t n, int64_t* out)
         INS_HIP_CHECK(c, hipGetLastError());
     }
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    hipFree(sums);
-    hipFree(soff);
+    (void)hipFreeAsync(sums, c->stream);
+    (void)hipFreeAsync(soff, c->stream);
     return s;
 }
 
@@ -115,8 +115,8 @@ int exclusive_scan_i32(Ctx* c, const int32_t* in, int32_t* out, int64_t n,
     const int64_t nb = (n + SCHUNK - 1) / SCHUNK;
     int64_t* sums = nullptr;
     int64_t* soff = nullptr;
-    INS_HIP_CHECK(c, hipMalloc((void**)&sums, (nb + 1) * sizeof(int64_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&soff, (nb + 1) * sizeof(int64_t)));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&sums, (nb + 1) * sizeof(int64_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&soff, (nb + 1) * sizeof(int64_t), c->stream));
     if (nb > 0) {
         k_chunk_sums<int32_t><<<(unsigned)nb, SB, 0, c->stream>>>(in, n, sums);
         INS_HIP_CHECK(c, hipGetLastError());
@@ -128,8 +128,8 @@ int exclusive_scan_i32(Ctx* c, const int32_t* in, int32_t* out, int64_t n,
                                     hipMemcpyDeviceToHost, c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (tot > INT32_MAX) {
-        hipFree(sums);
-        hipFree(soff);
+        (void)hipFreeAsync(sums, c->stream);
+        (void)hipFreeAsync(soff, c->stream);
         return fail(c, INS_ERR_UNSUPPORTED, "index total exceeds int32 range");
     }
     if (nb > 0) {
@@ -140,8 +140,8 @@ int exclusive_scan_i32(Ctx* c, const int32_t* in, int32_t* out, int64_t n,
     INS_HIP_CHECK(c, hipMemcpyAsync(out + n, &t32, sizeof(int32_t), hipMemcpyHostToDevice,
                                     c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    hipFree(sums);
-    hipFree(soff);
+    (void)hipFreeAsync(sums, c->stream);
+    (void)hipFreeAsync(soff, c->stream);
     *total = tot;
     return INS_OK;
 }

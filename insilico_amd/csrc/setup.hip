@@ -184,8 +184,8 @@ int build_adjacency(Ctx* c, const int32_t* items, int64_t n_items, int per_item,
     INS_TRY(ensure(c, list, nent));
     int32_t* cnt = nullptr;
     int* flags = nullptr;
-    INS_HIP_CHECK(c, hipMalloc((void**)&cnt, (n_nodes + 1) * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&flags, 2 * sizeof(int)));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&cnt, (n_nodes + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&flags, 2 * sizeof(int), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(cnt, 0, (n_nodes + 1) * sizeof(int32_t), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(flags, 0, 2 * sizeof(int), c->stream));
     k_adj_count<<<grid_for(nent, BLK), BLK, 0, c->stream>>>(items, nent, n_nodes, cnt, flags);
@@ -204,8 +204,8 @@ int build_adjacency(Ctx* c, const int32_t* items, int64_t n_items, int per_item,
     INS_HIP_CHECK(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost,
                                     c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    hipFree(cnt);
-    hipFree(flags);
+    (void)hipFreeAsync(cnt, c->stream);
+    (void)hipFreeAsync(flags, c->stream);
     if (hflags[0]) return fail(c, INS_ERR_RANGE, "node id out of range in connectivity");
     if (max_deg) *max_deg = hflags[1];
     return INS_OK;
@@ -222,9 +222,9 @@ int build_pattern(Ctx* c)
     int32_t* nb_cnt = nullptr;
     int32_t* row_len = nullptr;
     int* flags = nullptr;
-    INS_HIP_CHECK(c, hipMalloc((void**)&nb_cnt, (nn + 1) * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&row_len, (c->n_active + 1) * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&flags, 2 * sizeof(int)));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&nb_cnt, (nn + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&row_len, (c->n_active + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&flags, 2 * sizeof(int), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(flags, 0, 2 * sizeof(int), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(row_len, 0, (c->n_active + 1) * sizeof(int32_t), c->stream));
     const unsigned g = grid_for(nn, NB_BLK, 1u << 20);
@@ -265,9 +265,9 @@ int build_pattern(Ctx* c)
     INS_HIP_CHECK(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost,
                                     c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    hipFree(nb_cnt);
-    hipFree(row_len);
-    hipFree(flags);
+    (void)hipFreeAsync(nb_cnt, c->stream);
+    (void)hipFreeAsync(row_len, c->stream);
+    (void)hipFreeAsync(flags, c->stream);
     if (hflags[0] & 2)
         return fail(c, INS_ERR_UNSUPPORTED, "more neighbour nodes than supported (96)");
     c->max_nb = hflags[1];

@@ -1104,15 +1104,15 @@ int build_sell(Ctx* c, bool force)
     int* err = nullptr;
     uint64_t cap = 1024;
     while (cap < (uint64_t)(2 * nsl + 2)) cap <<= 1;
-    INS_HIP_CHECK(c, hipMalloc((void**)&udel, (size_t)nsl * MAXU * sizeof(int32_t) + 8));
-    INS_HIP_CHECK(c, hipMalloc((void**)&ulen, (nsl + 1) * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&h, nsl * sizeof(uint64_t) + 8));
-    INS_HIP_CHECK(c, hipMalloc((void**)&keys, cap * sizeof(uint64_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&reps, cap * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&slot_of, nsl * sizeof(int32_t) + 8));
-    INS_HIP_CHECK(c, hipMalloc((void**)&used, (cap + 1) * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&slot_id, (cap + 1) * sizeof(int32_t)));
-    INS_HIP_CHECK(c, hipMalloc((void**)&err, sizeof(int)));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&udel, (size_t)nsl * MAXU * sizeof(int32_t) + 8, c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&ulen, (nsl + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&h, nsl * sizeof(uint64_t) + 8, c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&keys, cap * sizeof(uint64_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&reps, cap * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&slot_of, nsl * sizeof(int32_t) + 8, c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&used, (cap + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&slot_id, (cap + 1) * sizeof(int32_t), c->stream));
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&err, sizeof(int), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(keys, 0, cap * sizeof(uint64_t), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
     // vector problems: slices hold rows of one component (ascending), so the
@@ -1121,9 +1121,9 @@ int build_sell(Ctx* c, bool force)
     if (S.perm_on) {
         INS_TRY(ensure(c, S.perm, n + 1));
         int32_t *comp = nullptr, *flag = nullptr, *pos = nullptr;
-        INS_HIP_CHECK(c, hipMalloc((void**)&comp, (n + 1) * sizeof(int32_t)));
-        INS_HIP_CHECK(c, hipMalloc((void**)&flag, (n + 1) * sizeof(int32_t)));
-        INS_HIP_CHECK(c, hipMalloc((void**)&pos, (n + 1) * sizeof(int32_t)));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&comp, (n + 1) * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&flag, (n + 1) * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&pos, (n + 1) * sizeof(int32_t), c->stream));
         const int64_t nd = c->n_nodes * c->dpn;
         k_row_comp<<<grid_for(nd, 256), 256, 0, c->stream>>>(nd, c->dof_id.p, c->dpn, comp);
         int64_t off = 0;
@@ -1137,9 +1137,9 @@ int build_sell(Ctx* c, bool force)
         }
         INS_HIP_CHECK(c, hipGetLastError());
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        hipFree(comp);
-        hipFree(flag);
-        hipFree(pos);
+        (void)hipFreeAsync(comp, c->stream);
+        (void)hipFreeAsync(flag, c->stream);
+        (void)hipFreeAsync(pos, c->stream);
     }
     k_slice_union<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(
         n, S.perm_on ? S.perm.p : nullptr, c->row_ptr.p, c->col.p, udel, ulen, err);
@@ -1171,8 +1171,8 @@ int build_sell(Ctx* c, bool force)
     {
         int32_t* sv = nullptr;
         int32_t* sp32 = nullptr;
-        INS_HIP_CHECK(c, hipMalloc((void**)&sv, (nsl + 1) * sizeof(int32_t)));
-        INS_HIP_CHECK(c, hipMalloc((void**)&sp32, (nsl + 1) * sizeof(int32_t)));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&sv, (nsl + 1) * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&sp32, (nsl + 1) * sizeof(int32_t), c->stream));
         k_slice_vals<<<grid_for(nsl, 256), 256, 0, c->stream>>>(nsl, ulen, sv);
         int64_t tv = 0;
         s = exclusive_scan_i32(c, sv, sp32, nsl, &tv);
@@ -1180,8 +1180,8 @@ int build_sell(Ctx* c, bool force)
         k_widen<<<grid_for(nsl + 1, 256), 256, 0, c->stream>>>(nsl + 1, sp32, S.slice_ptr.p);
         INS_HIP_CHECK(c, hipGetLastError());
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        hipFree(sv);
-        hipFree(sp32);
+        (void)hipFreeAsync(sv, c->stream);
+        (void)hipFreeAsync(sp32, c->stream);
     }
     int herr = 0;
     int64_t nvals = 0;
@@ -1189,22 +1189,22 @@ int build_sell(Ctx* c, bool force)
     INS_HIP_CHECK(c, hipMemcpyAsync(&nvals, S.slice_ptr.p + nsl, sizeof(int64_t),
                                     hipMemcpyDeviceToHost, c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    hipFree(udel);
-    hipFree(ulen);
-    hipFree(h);
-    hipFree(keys);
-    hipFree(reps);
-    hipFree(slot_of);
-    hipFree(used);
-    hipFree(slot_id);
-    hipFree(err);
+    (void)hipFreeAsync(udel, c->stream);
+    (void)hipFreeAsync(ulen, c->stream);
+    (void)hipFreeAsync(h, c->stream);
+    (void)hipFreeAsync(keys, c->stream);
+    (void)hipFreeAsync(reps, c->stream);
+    (void)hipFreeAsync(slot_of, c->stream);
+    (void)hipFreeAsync(used, c->stream);
+    (void)hipFreeAsync(slot_id, c->stream);
+    (void)hipFreeAsync(err, c->stream);
     S.nvals = nvals;
     // worth it only when the shared lists stay close to the rows' own lists
     S.ok = (herr == 0) && (force || nvals <= c->nnz + c->nnz / 8 + 64 * 64) && nvals < INT32_MAX;
     if (S.ok) {
         INS_TRY(ensure(c, S.map, c->nnz));
         int* e2 = nullptr;
-        INS_HIP_CHECK(c, hipMalloc((void**)&e2, sizeof(int)));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&e2, sizeof(int), c->stream));
         INS_HIP_CHECK(c, hipMemsetAsync(e2, 0, sizeof(int), c->stream));
         k_sell_map<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, sell_view(c), c->row_ptr.p,
                                                                        c->col.p, S.map.p, e2);
@@ -1212,7 +1212,7 @@ int build_sell(Ctx* c, bool force)
         int he = 0;
         INS_HIP_CHECK(c, hipMemcpyAsync(&he, e2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        hipFree(e2);
+        (void)hipFreeAsync(e2, c->stream);
         if (he) S.ok = false;
         // values: zero once, the padding is never written afterwards
         release(c, S.v64);
@@ -1220,6 +1220,8 @@ int build_sell(Ctx* c, bool force)
     }
     S.built = true;
     S.stale = true;
+    S.zero64 = false;
+    S.zero32 = false;
     return INS_OK;
 }
 
@@ -1228,13 +1230,17 @@ int fill_sell(Ctx* c)
 {
     Sell& S = c->sell;
     const bool f64 = c->matrix_precision == INS_FP64;
-    if (f64 && !S.v64.p) {
+    // a new slice structure: size the value arrays to it and zero them once
+    // (the scatter writes the stored entries only; padding must read 0)
+    if (f64 && !S.zero64) {
         INS_TRY(ensure(c, S.v64, S.nvals));
         INS_HIP_CHECK(c, hipMemsetAsync(S.v64.p, 0, S.nvals * sizeof(double), c->stream));
+        S.zero64 = true;
     }
-    if (!S.v32.p) {
+    if (!S.zero32) {
         INS_TRY(ensure(c, S.v32, S.nvals));
         INS_HIP_CHECK(c, hipMemsetAsync(S.v32.p, 0, S.nvals * sizeof(float), c->stream));
+        S.zero32 = true;
     }
     k_sell_scatter<<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(
         c->nnz, S.map.p, f64 ? c->val64.p : nullptr, c->val32.p, f64 ? S.v64.p : nullptr,
@@ -1432,6 +1438,18 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     else if (!(rel <= rtol))
         status = fail(c, INS_ERR_NOCONV, "iteration limit reached before rtol");
     return status;
+}
+
+int prepare(Ctx* c, int mode, int format)
+{
+    if (!c->have_matrix) return fail(c, INS_ERR_STATE, "ins_prepare before ins_assemble");
+    if (format == INS_FMT_SELL) {
+        if (!c->sell.built || !c->sell.ok) INS_TRY(build_sell(c, true));
+        if (c->sell.ok && c->sell.stale) INS_TRY(fill_sell(c));
+    }
+    INS_TRY(ensure_solver_state(c, mode));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return INS_OK;
 }
 
 int distribute(Ctx* c, int op)

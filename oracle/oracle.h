@@ -12,6 +12,17 @@
  * product order of tensor-product shape functions, per-quadrature-point
  * accumulation, element order of the global sums).  Eigen-internal arithmetic
  * (3x3 inverse, CG loop) is restated from the published algorithms.
+ *
+ * Pinning (tests/test_oracle.py): Gauss rules by their literal constants and
+ * polynomial exactness; shape functions by partition of unity, the Kronecker
+ * property and the SMF vertex order; element matrices by closed forms,
+ * rigid-body modes, symmetry/PSD and finite differences of the residual; the
+ * pattern by sparsity.1.ref.dat and closed-form nnz; assembly and lifting by
+ * brute force and the linearElastic{2D,3D}.ref.dat L2 errors; orc_pcg by its
+ * residual and a direct solve.  Parity unpinned: the exact iterate sequence
+ * and iteration count of Eigen 3.2's ConjugateGradient (x0 = 0 per Eigen's
+ * documentation), and last-bit rounding of Eigen's 3x3
+ * computeInverseAndDetWithCheck; no fixture of the reference fixes them.
  */
 #ifndef INSILICO_ORACLE_H
 #define INSILICO_ORACLE_H

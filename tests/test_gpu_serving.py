@@ -119,3 +119,41 @@ def test_serving_rejects_unassembled():
             srv.attach(c, 1e-8)
         assert e.value.status == abi.ERRORS_INV["STATE"]
     c.close()
+
+
+def test_context_reuse_for_a_larger_problem():
+    """A context set up again for a larger problem (as the serving loop reuses
+    its contexts) sizes and zeroes its SELL values anew: its solution equals
+    that of a fresh context, bit for bit, through ins_solve and the server.
+    ins_prepare(SELL) keeps the SELL form for these small, padding-heavy
+    matrices (ins_solve alone would choose CSR), as the server does."""
+    small, big = P.elasticity3d(4, seed=6), P.elasticity3d(9, seed=7)
+    reused = _ctx(small)
+    reused.prepare(abi.SOLVE_MIXED, abi.FMT_SELL)
+    reused.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+    reused.load_problem(big)
+    reused.assemble(abi.FP64, abi.ASM_MATRIX | abi.ASM_RHS)
+    reused.prepare(abi.SOLVE_MIXED, abi.FMT_SELL)
+    it_r, rr_r, _ = reused.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+    x_r = reused.get_solution()
+    fresh = _ctx(big)
+    fresh.prepare(abi.SOLVE_MIXED, abi.FMT_SELL)
+    it_f, rr_f, _ = fresh.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+    x_f = fresh.get_solution()
+    assert it_r == it_f and np.array_equal(x_r, x_f)
+    # and through the server, after a smaller request on the same context
+    reused.load_problem(small)
+    reused.assemble(abi.FP64, abi.ASM_MATRIX | abi.ASM_RHS)
+    b_f, v_f = fresh.get_rhs(), fresh.get_csr_values()
+    with abi.Server(0, 2) as srv:
+        _serve(srv, [reused])
+        reused.load_problem(big)
+        reused.assemble(abi.FP64, abi.ASM_MATRIX | abi.ASM_RHS)
+        b_r, v_r = reused.get_rhs(), reused.get_csr_values()
+        assert np.array_equal(b_r, b_f), np.abs(b_r - b_f).max()
+        assert np.array_equal(v_r, v_f), np.abs(v_r - v_f).max()
+        (x_s, it_s, rr_s, st_s), = _serve(srv, [reused])
+    assert st_s == abi.INS_OK and it_s == it_f, (st_s, it_s, it_f)
+    assert np.array_equal(x_s, x_f), (np.abs(x_s - x_f).max(), np.abs(x_f).max(), rr_s, rr_f)
+    reused.close()
+    fresh.close()
