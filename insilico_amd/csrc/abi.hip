@@ -138,7 +138,6 @@ int ins_destroy(ins_ctx* h)
     release_all(c);
     if (c->h_scal) hipHostFree(c->h_scal);
     if (c->h_snap) hipHostFree(c->h_snap);
-    if (c->chunk_graph) hipGraphExecDestroy(c->chunk_graph);
     for (int i = 0; i < 2; i++)
         if (c->evs[i]) hipEventDestroy(c->evs[i]);
     hipEventDestroy(c->ev0);

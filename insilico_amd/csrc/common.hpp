@@ -55,8 +55,12 @@ struct Scal {
     int breakdown;
     int iters;       // SpMVs performed inside the loop
     int restart;     // mixed: restart the direction after a replacement
-    int pause;       // mixed: the residual dropped 10x, replace it now
+    int pause;       // mixed: the residual dropped 10x, replace it now (for the host)
     int fin;         // converged on the true (FP64) residual, or empty rhs
+    int pause_at;    // mixed: launches of iteration >= pause_at wait for the
+                     // replacement (set by iteration pause_at - 1, so no block
+                     // of the launch that sets it sees it)
+    int pad;
     double rref;     // mixed: |b - A x|^2 at the last replacement
     double delta2;   // mixed: replace when |r|^2 < delta2 * rref
 };
@@ -152,10 +156,6 @@ struct Ctx {
     Scal* h_scal = nullptr;  // pinned
     Scal* h_snap = nullptr;  // pinned, 2 snapshots for the pipelined host loop
     hipEvent_t evs[2] = {nullptr, nullptr};
-    // captured chunk of PCG iterations (kernel launches only)
-    hipGraphExec_t chunk_graph = nullptr;
-    int chunk_key[4] = {0, 0, 0, 0};   // mode, format, chunk, n (low bits)
-    const void* chunk_ptrs[14] = {};
     DArr<int> dflag;
     bool have_solution = false;
 

@@ -21,6 +21,7 @@
 // (FP64 values, FP64 arithmetic).  Convergence is only declared on that FP64
 // residual.
 #include <algorithm>
+#include <climits>
 #include <deque>
 #include <cmath>
 #include <cstdlib>
@@ -255,9 +256,9 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
 template <typename TV, typename TX, typename TA, int SGK>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
-                                      TX* __restrict__ q, double* part, const Scal* sc)
+                                      TX* __restrict__ q, double* part, const Scal* sc, int it)
 {
-    if (sc->done | sc->pause) return;
+    if (sc->done || it >= sc->pause_at) return;
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
     double pq = 0.;
@@ -281,9 +282,9 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
 template <typename TV, typename TX, typename TA, int SGK>
 __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
                                                   const TX* __restrict__ p, TX* __restrict__ q,
-                                                  double* part, const Scal* sc)
+                                                  double* part, const Scal* sc, int it)
 {
-    spmv_sell_body<TV, TX, TA, SGK>(grid_blk(), n, S, val, p, q, part, sc);
+    spmv_sell_body<TV, TX, TA, SGK>(grid_blk(), n, S, val, p, q, part, sc, it);
 }
 
 template <int LPR, typename TV, typename TX, typename TA>
@@ -291,9 +292,9 @@ __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __res
                                                  const int32_t* __restrict__ col,
                                                  const TV* __restrict__ val,
                                                  const TX* __restrict__ p, TX* __restrict__ q,
-                                                 double* part, const Scal* sc)
+                                                 double* part, const Scal* sc, int it)
 {
-    if (sc->done | sc->pause) return;
+    if (sc->done || it >= sc->pause_at) return;
     const int sub = threadIdx.x & (LPR - 1);
     double pq = 0.;
     const Sweep w = xcd_sweep(n, SB / LPR);
@@ -315,9 +316,10 @@ template <typename TX, typename TD>
 __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* __restrict__ r,
                                       const TX* __restrict__ p, const TX* __restrict__ q,
                                       const TD* __restrict__ dinv, double* part, int g_pq,
-                                      Scal* sc, int par)
+                                      Scal* sc, int it)
 {
-    if (sc->done | sc->pause) return;
+    if (sc->done || it >= sc->pause_at) return;
+    const int par = it & 1;
     double t[1];
     const int slq[1] = {P_PQ};
     totals<1>(part, g_pq, slq, t);
@@ -358,18 +360,19 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
                                                   const TX* __restrict__ p,
                                                   const TX* __restrict__ q,
                                                   const TD* __restrict__ dinv, double* part,
-                                                  int g_pq, Scal* sc, int par)
+                                                  int g_pq, Scal* sc, int it)
 {
-    update_xr_body<TX, TD>(grid_blk(), n, x, r, p, q, dinv, part, g_pq, sc, par);
+    update_xr_body<TX, TD>(grid_blk(), n, x, r, p, q, dinv, part, g_pq, sc, it);
 }
 
 // K3: stop test; beta; p = D^-1 r + beta p
 template <typename TX, typename TD>
 __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
                                      const TX* __restrict__ r, const TD* __restrict__ dinv,
-                                     const double* part, int g, Scal* sc, int par)
+                                     const double* part, int g, Scal* sc, int it)
 {
-    if (sc->done | sc->pause) return;
+    if (sc->done || it >= sc->pause_at) return;
+    const int par = it & 1;
     double t[2];
     const int sl[2] = {P_RR, P_RZ};
     totals<2>(part, g, sl, t);
@@ -389,7 +392,10 @@ __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
             sc->beta = rz / rho;
             // mixed precision: ask for a residual replacement once the
             // recursive residual dropped by 1/delta since the last one
-            if (rr < sc->delta2 * sc->rref) sc->pause = 1;
+            if (rr < sc->delta2 * sc->rref) {
+                sc->pause = 1;
+                sc->pause_at = it + 1;
+            }
         }
     }
     if (bad || conv) return;
@@ -403,9 +409,9 @@ template <typename TX, typename TD>
 __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
                                                  const TX* __restrict__ r,
                                                  const TD* __restrict__ dinv, const double* part,
-                                                 int g, Scal* sc, int par)
+                                                 int g, Scal* sc, int it)
 {
-    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, par);
+    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, it);
 }
 
 // start: x = 0, r = b, p = D^-1 b; partials of b.b and r.z
@@ -451,6 +457,7 @@ __global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, doub
         sc->fin = sc->done;
         sc->restart = 0;
         sc->pause = 0;
+        sc->pause_at = INT_MAX;
         sc->rref = t[0];
     }
 }
@@ -553,6 +560,7 @@ __global__ __launch_bounds__(VB) void k_resid_fin(const double* part, int g, Sca
                 sc->restart = (sc->done || t[0] > 64. * sc->rr) ? 1 : 0;
                 sc->done = 0;
                 sc->pause = 0;
+                sc->pause_at = INT_MAX;
                 sc->rho[par_next] = t[1];
                 sc->rr = t[0];
                 sc->rref = t[0];
@@ -940,32 +948,32 @@ int pick_lpr(Ctx* c)
 // K1; returns the number of partials written
 template <typename TV, typename TX, typename TA>
 int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
-                int* g_out)
+                int* g_out, int it)
 {
     const int64_t n = c->n_active;
     if (format == INS_FMT_SELL) {
         const unsigned g = sell_grid(n);
         static const int sg = env_int("INS_SG", 4);
         if (sg == 7)
-            k_spmv_sell<TV, TX, TA, 7><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_sell<TV, TX, TA, 7><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
         else if (sg == 2)
-            k_spmv_sell<TV, TX, TA, 2><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_sell<TV, TX, TA, 2><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
         else if (sg == 4)
-            k_spmv_sell<TV, TX, TA, 4><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_sell<TV, TX, TA, 4><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
         else
-            k_spmv_sell<TV, TX, TA, 3><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_sell<TV, TX, TA, 3><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
         *g_out = (int)g;
     } else {
         const int lpr = pick_lpr(c);
         const unsigned g = csr_grid(n, lpr);
         if (lpr == 32)
-            k_spmv_csr<32, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_csr<32, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
         else if (lpr == 16)
-            k_spmv_csr<16, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_csr<16, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
         else if (lpr == 8)
-            k_spmv_csr<8, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_csr<8, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
         else
-            k_spmv_csr<4, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p);
+            k_spmv_csr<4, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
         *g_out = (int)g;
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1071,16 +1079,16 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
     const int64_t n = c->n_active;
     const unsigned gv = vec_grid(n);
     for (int i = 0; i < k; i++) {
-        const int par = (int)(*it_launched & 1);
+        const int it = (int)*it_launched;
         int g1 = 0;
         {
             ProfScope ps(c);
-            INS_TRY((launch_spmv<TV, TX, TX>(c, format, csr_val, sell_val, p, q, &g1)));
+            INS_TRY((launch_spmv<TV, TX, TX>(c, format, csr_val, sell_val, p, q, &g1, it)));
         }
         k_update_xr<TX, TX><<<gv, VB, 0, c->stream>>>(n, x, r, p, q, dinv, c->partial.p, g1,
-                                                      c->scal.p, par);
+                                                      c->scal.p, it);
         k_update_p<TX, TX><<<gv, VB, 0, c->stream>>>(n, p, r, dinv, c->partial.p, (int)gv,
-                                                     c->scal.p, par);
+                                                     c->scal.p, it);
         (*it_launched)++;
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1250,47 +1258,6 @@ int fill_sell(Ctx* c)
     return INS_OK;
 }
 
-// A chunk of iterations captured once as a HIP graph (kernels only) and
-// replayed; re-captured when the solve configuration or buffers change.
-int launch_chunk_graph(Ctx* c, bool mixed, int format, int chunk, int64_t* launched)
-{
-    const int key[4] = {mixed ? 1 : 0, format, chunk, (int)(c->n_active & 0x7fffffff)};
-    const void* ptrs[14] = {c->p32.p, c->p.p, c->q32.p, c->q.p, c->r32.p, c->r.p, c->x32.p, c->x.p,
-                            c->dinv32.p, c->dinv64.p, c->partial.p, c->scal.p,
-                            format == INS_FMT_SELL ? (const void*)(mixed ? (const void*)c->sell.v32.p : (const void*)c->sell.v64.p)
-                                                   : (const void*)(mixed ? (const void*)c->val32.p : (const void*)c->val64.p),
-                            format == INS_FMT_SELL ? (const void*)c->sell.slice_ptr.p : (const void*)c->row_ptr.p};
-    bool same = c->chunk_graph != nullptr;
-    for (int i = 0; i < 4 && same; i++) same = key[i] == c->chunk_key[i];
-    for (int i = 0; i < 14 && same; i++) same = ptrs[i] == c->chunk_ptrs[i];
-    if (!same) {
-        if (c->chunk_graph) {
-            (void)hipGraphExecDestroy(c->chunk_graph);
-            c->chunk_graph = nullptr;
-        }
-        hipGraph_t g = nullptr;
-        INS_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int64_t dummy = 0;
-        int s;
-        if (mixed)
-            s = run_chunk<float, float>(c, format, c->val32.p, format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
-                                        c->x32.p, c->r32.p, c->p32.p, c->q32.p, c->dinv32.p, chunk, &dummy);
-        else
-            s = run_chunk<double, double>(c, format, c->val64.p, format == INS_FMT_SELL ? c->sell.v64.p : nullptr,
-                                          c->x.p, c->r.p, c->p.p, c->q.p, c->dinv64.p, chunk, &dummy);
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (s != INS_OK) return s;
-        if (e != hipSuccess) return fail(c, INS_ERR_HIP, "graph capture failed");
-        INS_HIP_CHECK(c, hipGraphInstantiate(&c->chunk_graph, g, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(g);
-        for (int i = 0; i < 4; i++) c->chunk_key[i] = key[i];
-        for (int i = 0; i < 14; i++) c->chunk_ptrs[i] = ptrs[i];
-    }
-    INS_HIP_CHECK(c, hipGraphLaunch(c->chunk_graph, c->stream));
-    *launched += chunk;
-    return INS_OK;
-}
-
 // Host loop.  Chunks of iterations are enqueued two ahead, each followed by
 // an asynchronous copy of the device scalars into a pinned snapshot; the host
 // waits only for the older snapshot, so the GPU never idles while the host
@@ -1347,8 +1314,6 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     c->stats.spmv_ms = 0.;
     c->stats.spmv_calls = 0;
     c->prof_n = 0;
-    static const int graph_knob = env_int("INS_GRAPH", 0);
-    const bool use_graph = graph_knob && !c->profiling && (chunk % 2 == 0);
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     int refinements = 0;
     int64_t launched = 0;
@@ -1372,9 +1337,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     while (!stop) {
         while (pipe.inflight < 2 && pipe.enq < max_it) {
             const int k = std::min(chunk, max_it - pipe.enq);
-            if (use_graph && k == chunk && (launched & 1) == 0) {
-                INS_TRY(launch_chunk_graph(c, mixed, format, chunk, &launched));
-            } else if (mixed)
+            if (mixed)
                 INS_TRY((run_chunk<float, float>(c, format, c->val32.p,
                                                  format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
                                                  c->x32.p, c->r32.p, c->p32.p, c->q32.p,
@@ -1501,7 +1464,8 @@ struct SlotDesc {
     float* q;
     double* part;
     Scal* sc;
-    int g_spmv, g_vec, par0, pad;
+    int64_t join;  // batched iteration of the request's first launch
+    int g_spmv, g_vec;
 };
 
 struct BatchTab {
@@ -1523,30 +1487,29 @@ __device__ inline int find_slot(const int* pre, int ns, int bid)
     return lo;
 }
 
-__global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt)
+__global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, int64_t it)
 {
     const int s = find_slot(bt->pre_spmv, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
     spmv_sell_body<float, float, float, 4>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv}, d.n,
-                                           d.S, d.val, d.p, d.q, d.part, d.sc);
+                                           d.S, d.val, d.p, d.q, d.part, d.sc,
+                                           (int)(it - d.join));
 }
 
 __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
 {
     const int s = find_slot(bt->pre_vec, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
-    const int par = (int)((it & 1) ^ d.par0);
     update_xr_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, d.x, d.r,
-                                 d.p, d.q, d.dinv, d.part, d.g_spmv, d.sc, par);
+                                 d.p, d.q, d.dinv, d.part, d.g_spmv, d.sc, (int)(it - d.join));
 }
 
 __global__ __launch_bounds__(VB) void kb_update_p(const BatchTab* __restrict__ bt, int64_t it)
 {
     const int s = find_slot(bt->pre_vec, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
-    const int par = (int)((it & 1) ^ d.par0);
     update_p_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, d.p, d.r,
-                                d.dinv, d.part, d.g_vec, d.sc, par);
+                                d.dinv, d.part, d.g_vec, d.sc, (int)(it - d.join));
 }
 
 }  // namespace
@@ -1703,8 +1666,7 @@ int upload_table(Server* sv)
         d.sc = sv->scal + s;
         d.g_spmv = (int)sell_grid(d.n);
         d.g_vec = (int)vec_grid(d.n);
-        d.par0 = (int)(h.join_it & 1);
-        d.pad = 0;
+        d.join = h.join_it;
         T.pre_spmv[ns] = gs;
         T.pre_vec[ns] = gvv;
         gs += d.g_spmv;
@@ -1838,7 +1800,7 @@ int server_step(Server* sv, int k, int* n_finished)
         if (sv->dirty) INS_TRY(upload_table(sv));
         const BatchTab* tab = sv->d_tab + sv->tab_cur;
         for (int i = 0; i < k; i++) {
-            kb_spmv<<<sv->grid_spmv, SB, 0, sv->stream>>>(tab);
+            kb_spmv<<<sv->grid_spmv, SB, 0, sv->stream>>>(tab, sv->it);
             kb_update_xr<<<sv->grid_vec, VB, 0, sv->stream>>>(tab, sv->it);
             kb_update_p<<<sv->grid_vec, VB, 0, sv->stream>>>(tab, sv->it);
             sv->it++;

@@ -157,3 +157,22 @@ def test_context_reuse_for_a_larger_problem():
     assert np.array_equal(x_s, x_f), (np.abs(x_s - x_f).max(), np.abs(x_f).max(), rr_s, rr_f)
     reused.close()
     fresh.close()
+
+
+def test_serving_large_batch_equals_alone():
+    """Twelve requests of several hundred blocks each in one batch: the
+    blocks of a request start far apart in time, and the request must still
+    take the same path as alone (its replacement flag is an iteration index
+    that no block of the launch that sets it can see)."""
+    probs = ([P.elasticity3d(18, seed=20 + i) for i in range(6)] +
+             [P.diffusion3d(40, seed=40 + i) for i in range(6)])
+    ctxs = [_ctx(p) for p in probs]
+    with abi.Server(0, 16) as srv:
+        mix = _serve(srv, ctxs, stagger=0, k=8)
+    for c, (xm, im, rm, sm) in zip(ctxs, mix):
+        assert sm == abi.INS_OK and rm <= 1e-8
+        c.prepare(abi.SOLVE_MIXED, abi.FMT_SELL)
+        it, rr, st = c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+        assert it == im and np.array_equal(c.get_solution(), xm)
+    for c in ctxs:
+        c.close()
