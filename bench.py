@@ -140,6 +140,83 @@ def copy_probe(n_bytes=1 << 30, reps=3):
                       "launches": reps}))
 
 
+def run_newton(ctx, prob, pull, steps, tangent_precision, mode, fmt, tol=1e-8, max_newton=25):
+    """Load stepping + full Newton of reference/06-elastic/compressible.cpp
+    :245-335 on the device: per iteration the FP64 internal-force residual
+    and the tangent (tangent_precision) with incremental lifting g - u, a
+    Jacobi-PCG solve of the increment, u += du.  A step ends when
+    |R|/|R0| <= tol (R0: the step's first residual).  Linear tolerance
+    eta = min(1e-3, |R|/|R0|) (inexact Newton; the final accuracy is set by
+    the FP64 residual)."""
+    from insilico_amd import abi
+    right = prob.meta["right_x"]
+    g = np.zeros(prob.n_nodes * 3)
+    g[right * 3] = pull / steps
+    ctx.set_prescribed(g)
+    ctx.set_dof_values(None)
+    flags = abi.ASM_MATRIX | abi.ASM_RHS | abi.ASM_RESIDUAL | abi.ASM_INCREMENTAL
+    hist, lin_its, t_asm, t_sol = [], 0, 0.0, 0.0
+    ctx.sync()
+    t0 = time.perf_counter()
+    for step in range(steps):
+        if step > 0:
+            ctx.scale_prescribed((step + 1.0) / step)
+        r0 = None
+        for it in range(max_newton + 1):
+            ctx.assemble(tangent_precision, flags)
+            t_asm += ctx.stats()["assemble_ms"]
+            r = ctx.rhs_norm() * prob.n_active
+            r0 = r if r0 is None else r0
+            if r <= tol * r0 or it == max_newton:
+                hist.append((step, it, r / r0))
+                break
+            eta = min(1e-3, r / r0)
+            k, _, st = ctx.solve(mode, fmt, eta, check=False)
+            if st not in (abi.INS_OK,):
+                raise RuntimeError("linear solve failed: %s" % abi.ERRORS.get(st, st))
+            lin_its += k
+            t_sol += ctx.stats()["solve_ms"]
+            ctx.distribute(abi.DIST_ADD)
+    ctx.sync()
+    total = time.perf_counter() - t0
+    u = ctx.get_dof_values()
+    newton = sum(h[1] for h in hist)
+    return {"seconds": round(total, 3), "newton_iterations": newton,
+            "per_step": [h[1] for h in hist], "final_rel_residual": [float("%.3e" % h[2]) for h in hist],
+            "linear_iterations": lin_its, "assemble_ms_total": round(t_asm, 1),
+            "solve_ms_total": round(t_sol, 1), "max_u_x": float(np.abs(u[0::3]).max()),
+            "max_u_y": float(np.abs(u[1::3]).max()),
+            "dof_per_s": round(prob.n_active * newton / total, 1)}
+
+
+def newton_main(args):
+    from insilico_amd import abi, problems as P
+    n = args.n or 64
+    prob = P.neohooke3d(n, mu=1.0, lam=10.0)
+    log("neo-Hookean %d^3: %d dof" % (n, prob.n_active))
+    ctx = abi.Context(0)
+    nnz = ctx.load_problem(prob)
+    res = {}
+    for name, prec, mode, fmt in (("fp32_tangent_mixed", abi.FP32, abi.SOLVE_MIXED, abi.FMT_SELL),
+                                  ("fp64", abi.FP64, abi.SOLVE_FP64, abi.FMT_CSR)):
+        res[name] = run_newton(ctx, prob, 0.2, 10, prec, mode, fmt)
+        log("%s: %.2f s, %d Newton iterations" % (name, res[name]["seconds"],
+                                                  res[name]["newton_iterations"]))
+    a, b = res["fp32_tangent_mixed"], res["fp64"]
+    out = {"metric": "Newton load stepping: solved DOF/s (Newton iterations x DoF / time)",
+           "value": a["dof_per_s"], "unit": "DOF/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f32 tangent + f32 sweeps / f64 residual", "data": "synthetic",
+           "config": {"workload": "compressible neo-Hookean (mu=1, lambda=10) %d^3 Q1 hex, clamped "
+                                  "x=0, u_x on x=1 ramped to 0.2 over 10 steps, full Newton to "
+                                  "|R|/|R0| <= 1e-8 per step" % n,
+                      "n_dof": prob.n_active, "nnz": nnz},
+           "fp32_tangent_mixed": a, "fp64": b,
+           "speedup_vs_fp64": round(b["seconds"] / a["seconds"], 3),
+           "max_u_y_rel_diff": abs(a["max_u_y"] - b["max_u_y"]) / max(b["max_u_y"], 1e-300)}
+    print(json.dumps(out))
+    ctx.close()
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -149,7 +226,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="diffusion", choices=["diffusion", "elasticity"])
+    ap.add_argument("--config", default="diffusion", choices=["diffusion", "elasticity", "newton"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--rtol", type=float, default=1e-8)
@@ -166,6 +243,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.config == "newton":
+        newton_main(args)
         return
 
     from insilico_amd import abi, problems as P
@@ -208,6 +288,28 @@ def main():
     if not args.no_profile:
         _, _, _, _, _, (spmv_ms, spmv_calls) = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
                                                         args.rtol, args.steps, 0, profile=True)
+    fp32_integration = None
+    if args.config == "elasticity":
+        # FP32 element integration feeding the same mixed solve; its result is
+        # measured against the FP64-integrated system (host SpMV)
+        lat32, its32, rel32, asm32, _, _ = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP32,
+                                                    args.rtol, args.steps, args.warmup)
+        x32 = ctx.get_solution()
+        ctx.assemble(abi.FP64, abi.ASM_MATRIX | abi.ASM_RHS)
+        import scipy.sparse as sp
+        rp, ci = ctx.get_pattern()
+        A = sp.csr_matrix((ctx.get_csr_values(), ci, rp), shape=(prob.n_active, prob.n_active))
+        b64 = ctx.get_rhs()
+        rel_vs64 = float(np.linalg.norm(b64 - A @ x32) / np.linalg.norm(b64))
+        fp32_integration = {"ms_per_step": round(statistics.mean(lat32), 3),
+                            "value": round(prob.n_active / (statistics.mean(lat32) / 1e3), 1),
+                            "assemble_ms": round(statistics.mean(asm32), 3),
+                            "iterations": int(statistics.median(its32)),
+                            "relres_own_system": max(rel32),
+                            "relres_vs_fp64_system": rel_vs64}
+        log("fp32 integration: %.1f ms/step, relres vs FP64 system %.2e"
+            % (statistics.mean(lat32), rel_vs64))
+        del A
     n = prob.n_active
     sell_vals = ctx.stats()["sell_values"] or nnz
     fmt_name = "sell" if fmt == abi.FMT_SELL else "csr"
@@ -247,6 +349,8 @@ def main():
         "setup_ms": round(setup_ms, 3), "host_generate_s": round(t_gen, 3),
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if fp32_integration is not None:
+        out["fp32_integration"] = fp32_integration
     print(json.dumps(out))
     ctx.close()
 

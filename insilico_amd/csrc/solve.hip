@@ -102,6 +102,12 @@ __device__ inline void totals(const double* part, int cnt, const int (&slot)[NV]
 
 __device__ inline bool leader(Blk b = grid_blk()) { return b.id == 0 && threadIdx.x == 0; }
 
+// index of the calling wavefront within its block, as a wave-uniform value:
+// slice indices derived from it stay in scalar registers, so the per-slice
+// offset, pattern and column-offset reads go through the scalar cache rather
+// than through one vector load per lane
+__device__ inline int wave_in_block() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Workgroups are placed on the 8 XCDs round-robin by block id.  Each XCD
 // owns one contiguous eighth of the items (rows or slices) and its blocks
 // sweep that eighth together in grid-stride order: 8 sequential streams,
@@ -263,7 +269,7 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     const int64_t nsl = (n + 63) / 64;
     double pq = 0.;
     const Sweep w = xcd_sweep(nsl, SB / 64, b);
-    for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
+    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
         const int64_t rho = slice * 64 + lane;
         const bool live = rho < n;
         const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
@@ -486,7 +492,7 @@ __global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const 
     const int64_t nsl = (n + 63) / 64;
     double tt = 0., rz = 0.;
     const Sweep w = xcd_sweep(nsl, SB / 64);
-    for (int64_t slice = w.first + (threadIdx.x >> 6); slice < w.end; slice += w.stride) {
+    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
         const int64_t rho = slice * 64 + lane;
         const bool live = rho < n;
         const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
@@ -618,7 +624,7 @@ __global__ __launch_bounds__(256) void k_slice_union(int64_t n, const int32_t* _
 {
     const int64_t nsl = (n + 63) / 64;
     const int lane = threadIdx.x & 63;
-    for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
+    for (int64_t slice = (int64_t)blockIdx.x * 4 + wave_in_block(); slice < nsl;
          slice += (int64_t)gridDim.x * 4) {
         const int64_t rho = slice * 64 + lane;
         int64_t row = 0;
@@ -816,7 +822,7 @@ __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
 {
     const int64_t nsl = (n + 63) / 64;
     const int lane = threadIdx.x & 63;
-    for (int64_t slice = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); slice < nsl;
+    for (int64_t slice = (int64_t)blockIdx.x * 4 + wave_in_block(); slice < nsl;
          slice += (int64_t)gridDim.x * 4) {
         const int64_t rho = slice * 64 + lane;
         if (rho >= n) continue;
@@ -1453,20 +1459,50 @@ namespace {
 
 constexpr int MAXS = 64;  // requests in one batch
 
+// Device pointers in the table are declared as global memory in the device
+// pass, so the kernels that read them use global and scalar loads (as for
+// kernel arguments) rather than flat ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define INS_GLOBAL __attribute__((address_space(1)))
+#else
+#define INS_GLOBAL
+#endif
+
+struct GSellView {
+    const int32_t INS_GLOBAL* perm;
+    const int64_t INS_GLOBAL* sptr;
+    const int32_t INS_GLOBAL* spat;
+    const int32_t INS_GLOBAL* pptr;
+    const int32_t INS_GLOBAL* plen;
+    const int32_t INS_GLOBAL* pdel;
+};
+
 struct SlotDesc {
     int64_t n;
-    SellView S;
-    const float* val;
-    const float* dinv;
-    float* x;
-    float* r;
-    float* p;
-    float* q;
-    double* part;
-    Scal* sc;
+    GSellView S;
+    const float INS_GLOBAL* val;
+    const float INS_GLOBAL* dinv;
+    float INS_GLOBAL* x;
+    float INS_GLOBAL* r;
+    float INS_GLOBAL* p;
+    float INS_GLOBAL* q;
+    double INS_GLOBAL* part;
+    Scal INS_GLOBAL* sc;
     int64_t join;  // batched iteration of the request's first launch
     int g_spmv, g_vec;
 };
+
+__device__ inline SellView sell_of(const GSellView& g)
+{
+    SellView v;
+    v.perm = (const int32_t*)g.perm;
+    v.sptr = (const int64_t*)g.sptr;
+    v.spat = (const int32_t*)g.spat;
+    v.pptr = (const int32_t*)g.pptr;
+    v.plen = (const int32_t*)g.plen;
+    v.pdel = (const int32_t*)g.pdel;
+    return v;
+}
 
 struct BatchTab {
     int nslot, pad;
@@ -1492,7 +1528,8 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
     const int s = find_slot(bt->pre_spmv, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
     spmv_sell_body<float, float, float, 4>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv}, d.n,
-                                           d.S, d.val, d.p, d.q, d.part, d.sc,
+                                           sell_of(d.S), (const float*)d.val, (const float*)d.p,
+                                           (float*)d.q, (double*)d.part, (const Scal*)d.sc,
                                            (int)(it - d.join));
 }
 
@@ -1500,16 +1537,19 @@ __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ 
 {
     const int s = find_slot(bt->pre_vec, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
-    update_xr_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, d.x, d.r,
-                                 d.p, d.q, d.dinv, d.part, d.g_spmv, d.sc, (int)(it - d.join));
+    update_xr_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, (float*)d.x,
+                                 (float*)d.r, (const float*)d.p, (const float*)d.q,
+                                 (const float*)d.dinv, (double*)d.part, d.g_spmv, (Scal*)d.sc,
+                                 (int)(it - d.join));
 }
 
 __global__ __launch_bounds__(VB) void kb_update_p(const BatchTab* __restrict__ bt, int64_t it)
 {
     const int s = find_slot(bt->pre_vec, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
-    update_p_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, d.p, d.r,
-                                d.dinv, d.part, d.g_vec, d.sc, (int)(it - d.join));
+    update_p_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, (float*)d.p,
+                                (const float*)d.r, (const float*)d.dinv, (const double*)d.part,
+                                d.g_vec, (Scal*)d.sc, (int)(it - d.join));
 }
 
 }  // namespace
@@ -1655,15 +1695,23 @@ int upload_table(Server* sv)
         Ctx* c = h.c;
         SlotDesc& d = T.d[ns];
         d.n = c->n_active;
-        d.S = sell_view(c);
-        d.val = c->sell.v32.p;
-        d.dinv = c->dinv32.p;
-        d.x = c->x32.p;
-        d.r = c->r32.p;
-        d.p = c->p32.p;
-        d.q = c->q32.p;
-        d.part = c->partial.p;
-        d.sc = sv->scal + s;
+        const SellView sv0 = sell_view(c);
+#define INS_AS(field, ptr) field = (decltype(field))(ptr)
+        INS_AS(d.S.perm, sv0.perm);
+        INS_AS(d.S.sptr, sv0.sptr);
+        INS_AS(d.S.spat, sv0.spat);
+        INS_AS(d.S.pptr, sv0.pptr);
+        INS_AS(d.S.plen, sv0.plen);
+        INS_AS(d.S.pdel, sv0.pdel);
+        INS_AS(d.val, c->sell.v32.p);
+        INS_AS(d.dinv, c->dinv32.p);
+        INS_AS(d.x, c->x32.p);
+        INS_AS(d.r, c->r32.p);
+        INS_AS(d.p, c->p32.p);
+        INS_AS(d.q, c->q32.p);
+        INS_AS(d.part, c->partial.p);
+        INS_AS(d.sc, sv->scal + s);
+#undef INS_AS
         d.g_spmv = (int)sell_grid(d.n);
         d.g_vec = (int)vec_grid(d.n);
         d.join = h.join_it;

@@ -1,0 +1,133 @@
+"""Error behaviour and determinism of the C ABI on the GPU: every failure is
+a status code with a message (the reference aborts through VERIFY_MSG,
+base/verify.hpp:63-65), never a crash; repeated work gives the same bits."""
+import numpy as np
+import pytest
+
+from insilico_amd import abi, problems as P
+
+pytestmark = pytest.mark.gpu
+
+ERR = abi.ERRORS_INV
+
+
+def _status(fn):
+    with pytest.raises(abi.InsError) as e:
+        fn()
+    return e.value.status
+
+
+def test_calls_out_of_order():
+    prob = P.diffusion3d(4, seed=1)
+    with abi.Context(0) as c:
+        assert _status(lambda: c.set_dofs(1, prob.dof_id, prob.n_active)) == ERR["STATE"]
+        c.set_mesh(prob.xyz, prob.conn)
+        c.set_dofs(1, prob.dof_id, prob.n_active)
+        c.set_material(prob.kernel, prob.nq, prob.coef)
+        c.nnz = 0
+        assert _status(lambda: c.assemble(abi.FP64)) == ERR["STATE"]      # no pattern
+        c.build_pattern()
+        assert _status(lambda: c.solve(abi.SOLVE_FP64, abi.FMT_CSR, 1e-8)) == ERR["STATE"]
+        assert _status(lambda: c.get_solution()) == ERR["STATE"]
+        assert _status(lambda: c.distribute(abi.DIST_SET)) == ERR["STATE"]
+
+
+def test_bad_ids_and_arguments():
+    prob = P.diffusion3d(4, seed=1)
+    with abi.Context(0) as c:
+        c.set_mesh(prob.xyz, prob.conn)
+        bad = prob.dof_id.copy()
+        k = np.nonzero(prob.dof_id >= 0)[0][0]        # out of range
+        bad[k] = prob.n_active
+        assert _status(lambda: c.set_dofs(1, bad, prob.n_active)) == ERR["RANGE"]
+        dup = prob.dof_id.copy()
+        a, b = np.nonzero(prob.dof_id >= 0)[0][:2]
+        dup[b] = dup[a]                               # used twice
+        assert _status(lambda: c.set_dofs(1, dup, prob.n_active)) == ERR["RANGE"]
+        conn = prob.conn.copy()
+        conn[0, 0] = prob.n_nodes                     # node out of range
+        assert _status(lambda: c.set_mesh(prob.xyz, conn)) == ERR["RANGE"]
+        assert _status(lambda: c.set_mesh(prob.xyz, prob.conn[:, :4])) == ERR["UNSUPPORTED"]
+        c.set_mesh(prob.xyz, prob.conn)
+        c.set_dofs(1, prob.dof_id, prob.n_active)
+        c.build_pattern()
+        c.set_material(prob.kernel, prob.nq, prob.coef)
+        c.assemble(abi.FP64)
+        assert _status(lambda: c.assemble(7)) == ERR["ARG"]
+        assert _status(lambda: c.solve(5, abi.FMT_CSR, 1e-8)) == ERR["ARG"]
+        assert _status(lambda: c.solve(abi.SOLVE_FP64, abi.FMT_CSR, -1.0)) == ERR["ARG"]
+
+
+def test_iteration_limit_and_breakdown():
+    prob = P.diffusion3d(10, seed=2)
+    with abi.Context(0) as c:
+        c.load_problem(prob)
+        c.assemble(abi.FP64)
+        for mode in (abi.SOLVE_FP64, abi.SOLVE_MIXED):
+            it, rel, st = c.solve(mode, abi.FMT_SELL, 1e-8, max_it=5, check=False)
+            assert st == ERR["NOCONV"] and 0 < rel < 100  # CG residuals need not fall monotonically
+            assert np.all(np.isfinite(c.get_solution()))   # last iterate available
+        # an indefinite matrix: p.Ap <= 0 is a breakdown, reported, not a crash
+        rp, ci = c.get_pattern()
+        val = c.get_csr_values()
+        diag = np.repeat(np.arange(rp.shape[0] - 1), np.diff(rp)) == ci
+        neg = val.copy()
+        neg[diag] *= -1.0
+        c.set_csr(rp, ci, neg)
+        c.set_rhs(np.ones(prob.n_active))
+        it, rel, st = c.solve(abi.SOLVE_FP64, abi.FMT_CSR, 1e-8, check=False)
+        assert st == ERR["BREAKDOWN"]
+
+
+def test_repeat_solves_are_bitwise_equal():
+    prob = P.elasticity3d(6, seed=3)
+    with abi.Context(0) as c:
+        c.load_problem(prob)
+        out = []
+        for _ in range(2):
+            c.assemble(abi.FP64)
+            v, b = c.get_csr_values(), c.get_rhs()
+            res = []
+            for mode, fmt in ((abi.SOLVE_FP64, abi.FMT_CSR), (abi.SOLVE_MIXED, abi.FMT_SELL)):
+                it, rel, _ = c.solve(mode, fmt, 1e-8)
+                res.append((it, rel, c.get_solution()))
+            out.append((v, b, res))
+        (v0, b0, r0), (v1, b1, r1) = out
+        assert np.array_equal(v0, v1) and np.array_equal(b0, b1)
+        for (i0, e0, x0), (i1, e1, x1) in zip(r0, r1):
+            assert i0 == i1 and e0 == e1 and np.array_equal(x0, x1)
+
+
+def test_non_monotone_numbering(oracle):
+    """DoF ids in a shuffled order: the pattern is that of the permuted
+    system (rows sorted), values and RHS match the oracle's."""
+    prob = P.diffusion3d(6, seed=4)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(prob.n_active).astype(np.int32)
+    act = prob.dof_id >= 0
+    prob.dof_id = prob.dof_id.copy()
+    prob.dof_id[act] = perm[prob.dof_id[act]]
+    with abi.Context(0) as c:
+        c.load_problem(prob)
+        c.assemble(abi.FP64)
+        rp, ci = c.get_pattern()
+        orp, oci, oval, ob, _ = oracle.assemble(prob)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+        assert np.max(np.abs(c.get_csr_values() - oval)) <= 1e-12 * np.max(np.abs(oval))
+        assert np.max(np.abs(c.get_rhs() - ob)) <= 1e-12 * np.max(np.abs(ob))
+        it, rel, _ = c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+        x = c.get_solution()
+        r = ob - oracle.spmv(orp, oci, oval, x)
+        assert np.linalg.norm(r) / np.linalg.norm(ob) <= 1e-8 * 1.001
+
+
+def test_sell_and_csr_agree():
+    prob = P.elasticity3d(8, seed=5)
+    with abi.Context(0) as c:
+        c.load_problem(prob)
+        c.assemble(abi.FP64)
+        xs = []
+        for fmt in (abi.FMT_CSR, abi.FMT_SELL):
+            c.solve(abi.SOLVE_FP64, fmt, 1e-10)
+            xs.append(c.get_solution())
+        assert np.linalg.norm(xs[0] - xs[1]) <= 1e-7 * np.linalg.norm(xs[0])
