@@ -45,7 +45,7 @@ def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup, profile=False
     stream (recorded back to back, read after each solve)."""
     from insilico_amd import abi
     lat, its, rels, asm, sol = [], [], [], [], []
-    spmv_ms, spmv_calls = 0.0, 0
+    spmv_ms, spmv_calls, rep_ms, rep_calls = 0.0, 0, 0.0, 0
     ctx.set_profiling(profile)
     for s in range(warmup + steps):
         ctx.sync()
@@ -63,8 +63,11 @@ def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup, profile=False
             sol.append(st["solve_ms"])
             spmv_ms += st["spmv_ms"]
             spmv_calls += st["spmv_calls"]
+            rep_ms += st["rep_ms"]
+            rep_calls += st["rep_calls"]
     ctx.set_profiling(False)
-    return lat, its, rels, asm, sol, (spmv_ms / max(spmv_calls, 1), spmv_calls)
+    return lat, its, rels, asm, sol, (spmv_ms / max(spmv_calls, 1), spmv_calls,
+                                      rep_ms / max(rep_calls, 1), rep_calls)
 
 
 def _cpu_rate(prob, O, rp, ci, val, b, iters_fp64, nthreads, n_elem_sample, k_iters):
@@ -110,10 +113,11 @@ def cpu_baseline(prob, ctx, iters_fp64):
     }
 
 
-def read_traffic(kernel_tag):
+def read_traffic(workload_tag, kernel_tag):
     """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", "spmv_counters_diffusion128.md")
+    rocprofv3 counter summary of this workload (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated); None when there is none."""
+    path = os.path.join(ROOT, "profiles", "spmv_counters_%s.md" % workload_tag)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -272,6 +276,10 @@ def main():
     lat64, its64, rel64, asm64, sol64, _ = run_path(ctx, prob, abi.SOLVE_FP64, abi.FMT_CSR,
                                                     abi.FP64, args.rtol, args.steps, args.warmup)
     log("fp64 path: %.1f ms/step, %d iterations" % (statistics.mean(lat64), its64[0]))
+    # FP64 PCG over the SELL storage (the best FP64 storage, for reference)
+    lat64s, its64s, rel64s, _, _, _ = run_path(ctx, prob, abi.SOLVE_FP64, abi.FMT_SELL, abi.FP64,
+                                               args.rtol, args.steps, args.warmup)
+    log("fp64 sell path: %.1f ms/step, %d iterations" % (statistics.mean(lat64s), its64s[0]))
     # mixed-precision path (headline)
     latm, itsm, relm, asmm, solm, _ = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
                                                args.rtol, args.steps, args.warmup)
@@ -284,10 +292,10 @@ def main():
     # dominant kernel: the SpMV of the mixed path, timed with HIP event pairs
     # around each launch over as many further steps (the events add about 8%
     # to a step, so the steps that give `value` run without them)
-    spmv_ms, spmv_calls = 0.0, 0
+    spmv_ms, spmv_calls, rep_ms, rep_calls = 0.0, 0, 0.0, 0
     if not args.no_profile:
-        _, _, _, _, _, (spmv_ms, spmv_calls) = run_path(ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64,
-                                                        args.rtol, args.steps, 0, profile=True)
+        _, _, _, _, _, (spmv_ms, spmv_calls, rep_ms, rep_calls) = run_path(
+            ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64, args.rtol, args.steps, 0, profile=True)
     fp32_integration = None
     if args.config == "elasticity":
         # FP32 element integration feeding the same mixed solve; its result is
@@ -315,13 +323,15 @@ def main():
     fmt_name = "sell" if fmt == abi.FMT_SELL else "csr"
     alg = spmv_bytes(n, nnz, sell_vals, fmt_name, 4, 4)
     achieved = round(alg / (spmv_ms / 1e3) / 1e9, 1) if spmv_calls else None
-    traffic = read_traffic("mixed_" + fmt_name)
+    traffic = read_traffic("%s%d" % (args.config, args.n or (128 if args.config == "diffusion" else 96)),
+                           "mixed_" + fmt_name)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
             "kernel": ("k_spmv_sell<float, float, float, 4>" if fmt_name == "sell" else
                          "k_spmv_csr<LPR, float, float, float>") + " (q = A p with p.q partials)",
             "bytes_per_launch": alg, "avg_launch_ms": round(spmv_ms, 5),
-            "launches_timed": spmv_calls}
+            "launches_timed": spmv_calls,
+            "replacement_launch_ms": round(rep_ms, 5), "replacement_launches": rep_calls}
 
     log("spmv: %.4f ms/launch over %d launches" % (spmv_ms, spmv_calls))
     cpu = None
@@ -346,6 +356,10 @@ def main():
         "fp64_iterations": int(statistics.median(its64)), "fp64_relres": max(rel64),
         "fp64_assemble_ms": round(statistics.mean(asm64), 3),
         "mixed_over_fp64": round(value / value64, 3),
+        "fp64_sell_value": round(prob.n_active / (statistics.mean(lat64s) / 1e3), 1),
+        "fp64_sell_ms_per_step": round(statistics.mean(lat64s), 3),
+        "fp64_sell_relres": max(rel64s),
+        "mixed_over_fp64_sell": round(statistics.mean(lat64s) / ms_step, 3),
         "setup_ms": round(setup_ms, 3), "host_generate_s": round(t_gen, 3),
         "roofline": roof, "cpu_baseline": cpu,
     }

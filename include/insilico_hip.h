@@ -101,6 +101,9 @@ typedef struct ins_stats {
     int64_t sell_values;    /* padded value count of the SELL storage (0 = none) */
     int     sell_patterns;  /* distinct column-offset lists of the SELL storage  */
     int     reserved;
+    double  rep_ms;         /* profiled: total time of the residual-replacement
+                               launches of the dominant kernel (FP64 residual) */
+    int64_t rep_calls;
 } ins_stats;
 
 /* ---- context ------------------------------------------------------------ */

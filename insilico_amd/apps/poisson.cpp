@@ -24,10 +24,12 @@ int main(int argc, char* argv[])
         base::Quadrature<2, base::QUAD> quadrature;   // 2*fieldDeg, PoissonDriver.hpp:191
         base::FieldBinder<Mesh, Field> fieldBinder(mesh, field);
         typedef base::FieldBinder<Mesh, Field>::TupleBinder<1, 1>::Type FTB;
-        heat::Laplace<> laplace(1.0);
+        // heat::PoissonDriver's kernel (heat/PoissonDriver.hpp:78, :89-104)
+        typedef mat::thermal::IsotropicConstant Material;
+        heat::Static<Material, FTB> kernel{Material(1.0)};
 
         base::solver::Hip solver(numDoFs);
-        base::asmb::stiffnessMatrixComputation<FTB>(quadrature, solver, fieldBinder, laplace);
+        base::asmb::stiffnessMatrixComputation<FTB>(quadrature, solver, fieldBinder, kernel);
         base::asmb::bodyForceComputation<FTB>(
             quadrature, solver, fieldBinder,
             base::asmb::SineProductForce{2. * M_PI * M_PI, {M_PI, M_PI, 0.}});

@@ -545,8 +545,8 @@ __global__ __launch_bounds__(BLK) void k_gather(
     const int32_t* __restrict__ adj, const int32_t* __restrict__ nb_ptr,
     const int32_t* __restrict__ nb, const int32_t* __restrict__ nboff,
     const int32_t* __restrict__ dof_id, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ col, int monotone, KE ke, double* __restrict__ v64,
-    float* __restrict__ v32, int* __restrict__ err)
+    const int32_t* __restrict__ col, const int32_t* __restrict__ conn, int monotone, KE ke,
+    double* __restrict__ v64, float* __restrict__ v32, int* __restrict__ err)
 {
     const int G = 1 << lgG;
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -567,24 +567,24 @@ __global__ __launch_bounds__(BLK) void k_gather(
         for (int a = 0; a < S; a++)
 #pragma unroll
             for (int b = 0; b < S; b++) v[a][b] = (TA)0;
-        int32_t ia = as, ib = adj_ptr[m];
-        const int32_t eb = adj_ptr[m + 1];
-        while (ia < ae && ib < eb) {
-            const int32_t xa = adj[ia], xb = adj[ib];
-            const int32_t ea = xa >> lg_npe, ebb = xb >> lg_npe;
-            if (ea == ebb) {
-                const int la = xa & mask, lb = xb & mask;
+        // the elements shared by n and m, in ascending element order: walk
+        // n's (sorted) elements and find m among each element's vertices;
+        // the loads are independent of each other and the same for all the
+        // lanes of node n
+        for (int32_t ia = as; ia < ae; ia++) {
+            const int32_t xa = adj[ia];
+            const int64_t e = xa >> lg_npe;
+            const int la = xa & mask;
+            const int32_t* __restrict__ ce = conn + e * NPE;
+            int lb = -1;
 #pragma unroll
-                for (int cr = 0; cr < S; cr++)
+            for (int t = 0; t < NPE; t++)
+                if (ce[t] == (int32_t)m) lb = t;
+            if (lb < 0) continue;
 #pragma unroll
-                    for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(ea, la, cr, lb, cc);
-                ia++;
-                ib++;
-            } else if (ea < ebb) {
-                ia++;
-            } else {
-                ib++;
-            }
+            for (int cr = 0; cr < S; cr++)
+#pragma unroll
+                for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(e, la, cr, lb, cc);
         }
         const int32_t off = nboff[nbs + j];
 #pragma unroll
@@ -949,13 +949,13 @@ int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int in
         if (precision == INS_FP64)
             k_gather<NPE, S, KE, double><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
-                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->monotone ? 1 : 0, ke, v64,
-                v32, derr);
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, c->monotone ? 1 : 0,
+                ke, v64, v32, derr);
         else
             k_gather<NPE, S, KE, float><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
-                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->monotone ? 1 : 0, ke, nullptr,
-                v32, derr);
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, c->monotone ? 1 : 0,
+                ke, nullptr, v32, derr);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     if (do_rhs) {

@@ -54,13 +54,13 @@ struct Scal {
     int done;        // 1 = stop (converged, breakdown or empty rhs)
     int breakdown;
     int iters;       // SpMVs performed inside the loop
-    int restart;     // mixed: restart the direction after a replacement
-    int pause;       // mixed: the residual dropped 10x, replace it now (for the host)
+    int restart;     // mixed: the replacement restarts the direction
     int fin;         // converged on the true (FP64) residual, or empty rhs
-    int pause_at;    // mixed: launches of iteration >= pause_at wait for the
-                     // replacement (set by iteration pause_at - 1, so no block
-                     // of the launch that sets it sees it)
-    int pad;
+    int rep_at;      // mixed: iteration that replaces the residual (set by the
+                     // iteration before it, so no block of the launch that
+                     // sets it sees it)
+    int conv32;      // mixed: the FP32 recursion reached the tolerance
+    int nrep;        // mixed: residual replacements so far
     double rref;     // mixed: |b - A x|^2 at the last replacement
     double delta2;   // mixed: replace when |r|^2 < delta2 * rref
 };

@@ -26,6 +26,8 @@
 #include <cmath>
 #include <cstdlib>
 
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 
 namespace ins {
@@ -258,17 +260,53 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
 }
 
 // ---------------------------------------------------------------------------
-// K1: q = A p with per-block partials of p.q
-template <typename TV, typename TX, typename TA, int SGK>
+// Operands of the residual replacement of mixed precision, carried by K1:
+// t = b - A x64 with the FP64 (or, after an FP32 assembly, FP32) values in
+// FP64 arithmetic; r32 = (float)t.
+template <typename TVR>
+struct Rep {
+    const TVR* val;
+    const double* x64;
+    const double* b;
+    float* r32;
+    const float* dinv32;
+};
+
+// K1: q = A p with per-block partials of p.q; at the replacement iteration
+// (it == rep_at) the FP64 true residual instead, with partials of t.t and
+// of r32.z
+template <typename TV, typename TX, typename TA, int SGK, typename TVR>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
-                                      TX* __restrict__ q, double* part, const Scal* sc, int it)
+                                      TX* __restrict__ q, double* part, const Scal* sc, int it,
+                                      const Rep<TVR>& R)
 {
-    if (sc->done || it >= sc->pause_at) return;
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
-    double pq = 0.;
     const Sweep w = xcd_sweep(nsl, SB / 64, b);
+    if (it == sc->rep_at) {
+        double tt = 0., rz = 0.;
+        for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+            const int64_t rho = slice * 64 + lane;
+            const bool live = rho < n;
+            const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
+            const double s = sell_row<TVR, double, double, 2>(row, live, n, lane, slice, S, R.val,
+                                                              R.x64);
+            if (live) {
+                const double ti = R.b[row] - s;
+                tt += ti * ti;
+                const float rf = (float)ti;
+                R.r32[row] = rf;
+                rz += (double)rf * (double)(R.dinv32[row] * rf);
+            }
+        }
+        double v[2] = {tt, rz};
+        const int sl[2] = {P_TT, P_RZ};
+        block_partials<2>(v, part, sl, b);
+        return;
+    }
+    if (sc->done) return;
+    double pq = 0.;
     for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
         const int64_t rho = slice * 64 + lane;
         const bool live = rho < n;
@@ -285,25 +323,42 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     block_partials<1>(v, part, sl, b);
 }
 
-template <typename TV, typename TX, typename TA, int SGK>
+template <typename TV, typename TX, typename TA, int SGK, typename TVR>
 __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
                                                   const TX* __restrict__ p, TX* __restrict__ q,
-                                                  double* part, const Scal* sc, int it)
+                                                  double* part, const Scal* sc, int it, Rep<TVR> R)
 {
-    spmv_sell_body<TV, TX, TA, SGK>(grid_blk(), n, S, val, p, q, part, sc, it);
+    spmv_sell_body<TV, TX, TA, SGK, TVR>(grid_blk(), n, S, val, p, q, part, sc, it, R);
 }
 
-template <int LPR, typename TV, typename TX, typename TA>
+template <int LPR, typename TV, typename TX, typename TA, typename TVR>
 __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ col,
                                                  const TV* __restrict__ val,
                                                  const TX* __restrict__ p, TX* __restrict__ q,
-                                                 double* part, const Scal* sc, int it)
+                                                 double* part, const Scal* sc, int it, Rep<TVR> R)
 {
-    if (sc->done || it >= sc->pause_at) return;
     const int sub = threadIdx.x & (LPR - 1);
-    double pq = 0.;
     const Sweep w = xcd_sweep(n, SB / LPR);
+    if (it == sc->rep_at) {
+        double tt = 0., rz = 0.;
+        for (int64_t row = w.first + threadIdx.x / LPR; row < w.end; row += w.stride) {
+            const double s = csr_row<LPR, TVR, double, double>(row, sub, rp, col, R.val, R.x64);
+            if (sub == 0) {
+                const double ti = R.b[row] - s;
+                tt += ti * ti;
+                const float rf = (float)ti;
+                R.r32[row] = rf;
+                rz += (double)rf * (double)(R.dinv32[row] * rf);
+            }
+        }
+        double v[2] = {tt, rz};
+        const int sl[2] = {P_TT, P_RZ};
+        block_partials<2>(v, part, sl);
+        return;
+    }
+    if (sc->done) return;
+    double pq = 0.;
     for (int64_t row = w.first + threadIdx.x / LPR; row < w.end; row += w.stride) {
         const TA s = csr_row<LPR, TV, TX, TA>(row, sub, rp, col, val, p);
         if (sub == 0) {
@@ -317,6 +372,20 @@ __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __res
     block_partials<1>(v, part, sl);
 }
 
+// 16-byte vector of a scalar type (4 floats or 2 doubles)
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int W = 4;
+};
+template <>
+struct V16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int W = 2;
+};
+
 // K2: alpha from the p.q partials; x += alpha p; r -= alpha q; r.r, r.z partials
 template <typename TX, typename TD>
 __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* __restrict__ r,
@@ -324,7 +393,36 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
                                       const TD* __restrict__ dinv, double* part, int g_pq,
                                       Scal* sc, int it)
 {
-    if (sc->done || it >= sc->pause_at) return;
+    if (it == sc->rep_at) {
+        // replacement iteration: accept or continue from the true residual
+        // of K1 (one block; the others have nothing to do)
+        if (b.id != 0) return;
+        double t[2];
+        const int sl[2] = {P_TT, P_RZ};
+        totals<2>(part, g_pq, sl, t);
+        if (threadIdx.x == 0) {
+            sc->tt = t[0];
+            sc->nrep += 1;
+            if (!isfinite(t[0])) {
+                sc->breakdown = 1;
+                sc->done = 1;
+            } else if (t[0] < sc->thr) {
+                sc->done = 1;
+                sc->fin = 1;
+            } else {
+                // keep the direction while the recursion still tracks the
+                // true residual; once the FP32 recursion has converged or
+                // drifted, restart from p = D^-1 r
+                sc->restart = (sc->conv32 || t[0] > 64. * sc->rr) ? 1 : 0;
+                sc->rho[(it + 1) & 1] = t[1];
+                sc->rr = t[0];
+                sc->rref = t[0];
+            }
+            sc->conv32 = 0;
+        }
+        return;
+    }
+    if (sc->done) return;
     const int par = it & 1;
     double t[1];
     const int slq[1] = {P_PQ};
@@ -344,14 +442,34 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
         sc->alpha = ad;
         sc->iters += 1;
     }
+    static_assert(sizeof(TX) == sizeof(TD), "vectors and diagonal share a type");
+    typedef typename V16<TX>::type VT;
+    constexpr int W = V16<TX>::W;
     const TX alpha = (TX)ad;
-    const bool nt = g_nt;
     double rr = 0., rz = 0.;
-    const Sweep w = xcd_sweep(n, VB, b);
+    // 16-byte accesses over the full vectors, then the n % W tail
+    const int64_t nv = n / W;
+    const Sweep w = xcd_sweep(nv, VB, b);
+    VT* __restrict__ xv = reinterpret_cast<VT*>(x);
+    VT* __restrict__ rv = reinterpret_cast<VT*>(r);
+    const VT* __restrict__ pv = reinterpret_cast<const VT*>(p);
+    const VT* __restrict__ qv = reinterpret_cast<const VT*>(q);
+    const VT* __restrict__ dv = reinterpret_cast<const VT*>(dinv);
     for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
-        st(&x[i], x[i] + alpha * p[i], nt);
+        const VT ri = rv[i] - alpha * qv[i];
+        xv[i] = xv[i] + alpha * pv[i];
+        rv[i] = ri;
+        const VT zi = dv[i] * ri;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            rr += (double)ri[k] * (double)ri[k];
+            rz += (double)ri[k] * (double)zi[k];
+        }
+    }
+    for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB) {
+        x[i] = x[i] + alpha * p[i];
         const TX ri = r[i] - alpha * q[i];
-        st(&r[i], ri, nt);
+        r[i] = ri;
         const TX zi = (TX)dinv[i] * ri;
         rr += (double)ri * (double)ri;
         rz += (double)ri * (double)zi;
@@ -371,13 +489,33 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
     update_xr_body<TX, TD>(grid_blk(), n, x, r, p, q, dinv, part, g_pq, sc, it);
 }
 
-// K3: stop test; beta; p = D^-1 r + beta p
+// K3: stop test; beta; p = D^-1 r + beta p.  In mixed precision, once the
+// recursive residual fell below delta^2 of the last true one, or below the
+// tolerance, the next iteration is a replacement: every block folds its rows
+// of x32 into x64 here and the leader sets rep_at.  At the replacement
+// iteration K3 restarts the direction if K2 asked for it.
 template <typename TX, typename TD>
 __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
                                      const TX* __restrict__ r, const TD* __restrict__ dinv,
-                                     const double* part, int g, Scal* sc, int it)
+                                     const double* part, int g, Scal* sc, int it,
+                                     double* __restrict__ x64, TX* __restrict__ x32)
 {
-    if (sc->done || it >= sc->pause_at) return;
+    typedef typename V16<TX>::type VT;
+    constexpr int W = V16<TX>::W;
+    constexpr bool mixed = sizeof(TX) == 4;
+    const int64_t nv = n / W;
+    const Sweep w = xcd_sweep(nv, VB, b);
+    VT* __restrict__ pv = reinterpret_cast<VT*>(p);
+    const VT* __restrict__ rv = reinterpret_cast<const VT*>(r);
+    const VT* __restrict__ dv = reinterpret_cast<const VT*>(dinv);
+    if (it == sc->rep_at) {
+        if (!sc->restart || sc->done) return;
+        for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) pv[i] = dv[i] * rv[i];
+        for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB)
+            p[i] = (TX)dinv[i] * r[i];
+        return;
+    }
+    if (sc->done) return;
     const int par = it & 1;
     double t[2];
     const int sl[2] = {P_RR, P_RZ};
@@ -385,39 +523,45 @@ __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
     const double rr = t[0], rz = t[1];
     const bool bad = !isfinite(rr);
     const bool conv = rr < sc->thr;
+    const bool pause = mixed && !bad && !conv && rr < sc->delta2 * sc->rref;
     const double rho = sc->rho[par];
     if (leader(b)) {
         sc->rr = rr;
         if (bad) {
             sc->breakdown = 1;
             sc->done = 1;
-        } else if (conv) {
+        } else if (conv && !mixed) {
             sc->done = 1;
-        } else {
+        } else if (conv || pause) {
+            sc->conv32 = conv ? 1 : 0;
+            sc->rep_at = it + 1;
+        }
+        if (!bad && !conv) {
             sc->rho[par ^ 1] = rz;
             sc->beta = rz / rho;
-            // mixed precision: ask for a residual replacement once the
-            // recursive residual dropped by 1/delta since the last one
-            if (rr < sc->delta2 * sc->rref) {
-                sc->pause = 1;
-                sc->pause_at = it + 1;
-            }
         }
     }
-    if (bad || conv) return;
+    if (bad) return;
+    if (mixed && (conv || pause)) {
+        for (int64_t i = (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB) {
+            x64[i] += (double)x32[i];
+            x32[i] = (TX)0;
+        }
+    }
+    if (conv) return;
     const TX beta = (TX)(rz / rho);
-    const Sweep w = xcd_sweep(n, VB, b);
-    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride)
-        st(&p[i], (TX)dinv[i] * r[i] + beta * p[i], g_nt);
+    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) pv[i] = dv[i] * rv[i] + beta * pv[i];
+    for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB)
+        p[i] = (TX)dinv[i] * r[i] + beta * p[i];
 }
 
 template <typename TX, typename TD>
 __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
                                                  const TX* __restrict__ r,
                                                  const TD* __restrict__ dinv, const double* part,
-                                                 int g, Scal* sc, int it)
+                                                 int g, Scal* sc, int it, double* x64, TX* x32)
 {
-    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, it);
+    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, it, x64, x32);
 }
 
 // start: x = 0, r = b, p = D^-1 b; partials of b.b and r.z
@@ -462,13 +606,15 @@ __global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, doub
         sc->done = (t[0] == 0.) ? 1 : 0;
         sc->fin = sc->done;
         sc->restart = 0;
-        sc->pause = 0;
-        sc->pause_at = INT_MAX;
+        sc->rep_at = INT_MAX;
+        sc->conv32 = 0;
+        sc->nrep = 0;
         sc->rref = t[0];
     }
 }
 
-// ---- mixed precision: residual replacement ------------------------------------
+// ---- true residual at the end of a solve -------------------------------------------
+// x64 += x32 (mixed precision, when a solve stops at its iteration limit)
 __global__ __launch_bounds__(VB) void k_fold(int64_t n, double* __restrict__ x64,
                                              float* __restrict__ x32)
 {
@@ -479,18 +625,15 @@ __global__ __launch_bounds__(VB) void k_fold(int64_t n, double* __restrict__ x64
     }
 }
 
-// t = b - A x (FP64); partials of t.t and, when replacing, r32 = (float)t and r.z
+// t = b - A x in FP64; partials of t.t
 template <typename TV>
 __global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const TV* __restrict__ val,
                                                    const double* __restrict__ x,
-                                                   const double* __restrict__ b,
-                                                   float* __restrict__ r32,
-                                                   const float* __restrict__ dinv32,
-                                                   double* part, int replace)
+                                                   const double* __restrict__ b, double* part)
 {
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
-    double tt = 0., rz = 0.;
+    double tt = 0.;
     const Sweep w = xcd_sweep(nsl, SB / 64);
     for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
         const int64_t rho = slice * 64 + lane;
@@ -500,16 +643,11 @@ __global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const 
         if (live) {
             const double ti = b[row] - s;
             tt += ti * ti;
-            if (replace) {
-                const float rf = (float)ti;
-                r32[row] = rf;
-                rz += (double)rf * (double)(dinv32[row] * rf);
-            }
         }
     }
-    double v[2] = {tt, rz};
-    const int sl[2] = {P_TT, P_RZ};
-    block_partials<2>(v, part, sl);
+    double v[1] = {tt};
+    const int sl[1] = {P_TT};
+    block_partials<1>(v, part, sl);
 }
 
 template <int LPR, typename TV>
@@ -517,72 +655,30 @@ __global__ __launch_bounds__(SB) void k_resid_csr(int64_t n, const int32_t* __re
                                                   const int32_t* __restrict__ col,
                                                   const TV* __restrict__ val,
                                                   const double* __restrict__ x,
-                                                  const double* __restrict__ b,
-                                                  float* __restrict__ r32,
-                                                  const float* __restrict__ dinv32,
-                                                  double* part, int replace)
+                                                  const double* __restrict__ b, double* part)
 {
     const int sub = threadIdx.x & (LPR - 1);
-    double tt = 0., rz = 0.;
+    double tt = 0.;
     const Sweep w = xcd_sweep(n, SB / LPR);
     for (int64_t row = w.first + threadIdx.x / LPR; row < w.end; row += w.stride) {
         const double s = csr_row<LPR, TV, double, double>(row, sub, rp, col, val, x);
         if (sub == 0) {
             const double ti = b[row] - s;
             tt += ti * ti;
-            if (replace) {
-                const float rf = (float)ti;
-                r32[row] = rf;
-                rz += (double)rf * (double)(dinv32[row] * rf);
-            }
         }
     }
-    double v[2] = {tt, rz};
-    const int sl[2] = {P_TT, P_RZ};
-    block_partials<2>(v, part, sl);
+    double v[1] = {tt};
+    const int sl[1] = {P_TT};
+    block_partials<1>(v, part, sl);
 }
 
-// after a residual: record |b - A x|^2; when replacing, restart the
-// recursion from the replaced residual (rho of the next iteration's slot)
-__global__ __launch_bounds__(VB) void k_resid_fin(const double* part, int g, Scal* sc, int replace,
-                                                  int par_next)
+// |b - A x|^2 into the scalars
+__global__ __launch_bounds__(VB) void k_resid_fin(const double* part, int g, Scal* sc)
 {
-    double t[2];
-    const int sl[2] = {P_TT, P_RZ};
-    totals<2>(part, g, sl, t);
-    if (threadIdx.x == 0) {
-        sc->tt = t[0];
-        if (replace) {
-            if (!isfinite(t[0])) {
-                sc->breakdown = 1;
-                sc->done = 1;
-            } else if (t[0] < sc->thr) {
-                sc->done = 1;
-                sc->fin = 1;
-            } else {
-                // keep the search direction only while the recursion still
-                // tracks the true residual; once the FP32 recursion has
-                // stopped or drifted, restart from p = D^-1 r
-                sc->restart = (sc->done || t[0] > 64. * sc->rr) ? 1 : 0;
-                sc->done = 0;
-                sc->pause = 0;
-                sc->pause_at = INT_MAX;
-                sc->rho[par_next] = t[1];
-                sc->rr = t[0];
-                sc->rref = t[0];
-            }
-        }
-    }
-}
-
-// p = D^-1 r after a residual replacement that asked for a restart
-__global__ __launch_bounds__(VB) void k_restart(int64_t n, float* __restrict__ p,
-                                                const float* __restrict__ r,
-                                                const float* __restrict__ dinv, const Scal* sc)
-{
-    if (!sc->restart || sc->done) return;
-    const Sweep w = xcd_sweep(n, VB);
-    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) p[i] = dinv[i] * r[i];
+    double t[1];
+    const int sl[1] = {P_TT};
+    totals<1>(part, g, sl, t);
+    if (threadIdx.x == 0) sc->tt = t[0];
 }
 
 __global__ __launch_bounds__(VB) void k_norm2(int64_t n, const double* __restrict__ v, double* part)
@@ -952,43 +1048,79 @@ int pick_lpr(Ctx* c)
 }
 
 // K1; returns the number of partials written
-template <typename TV, typename TX, typename TA>
-int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
-                int* g_out, int it)
+// Launch of the dominant kernel on the context stream.  With profiling on,
+// the launch itself records an event pair around the kernel (no separate
+// marker packets in the stream), read once the solve has finished.
+template <typename F, typename... Args>
+int launch_timed(Ctx* c, F kernel, unsigned grid, unsigned block, Args... args)
+{
+    if (!c->profiling) {
+        kernel<<<grid, block, 0, c->stream>>>(args...);
+        return INS_OK;
+    }
+    const size_t k = c->prof_n;
+    while (c->prof_ev.size() < 2 * (k + 1)) {
+        hipEvent_t e = nullptr;
+        INS_HIP_CHECK(c, hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+    }
+    c->prof_n++;
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, c->stream, c->prof_ev[2 * k],
+                          c->prof_ev[2 * k + 1], 0, args...);
+    return INS_OK;
+}
+
+template <typename TV, typename TX, typename TA, typename TVR>
+int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
+                  int* g_out, int it, const TVR* rep_val)
 {
     const int64_t n = c->n_active;
+    const Scal* sc = c->scal.p;
+    const Rep<TVR> R{rep_val, c->x.p, c->b.p, c->r32.p, c->dinv32.p};
     if (format == INS_FMT_SELL) {
         const unsigned g = sell_grid(n);
-        static const int sg = env_int("INS_SG", 4);
-        if (sg == 7)
-            k_spmv_sell<TV, TX, TA, 7><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
-        else if (sg == 2)
-            k_spmv_sell<TV, TX, TA, 2><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
-        else if (sg == 4)
-            k_spmv_sell<TV, TX, TA, 4><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
-        else
-            k_spmv_sell<TV, TX, TA, 3><<<g, SB, 0, c->stream>>>(n, sell_view(c), sell_val, p, q, c->partial.p, c->scal.p, it);
+        INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR>, g, SB, n, sell_view(c), sell_val,
+                             p, q, c->partial.p, sc, it, R));
         *g_out = (int)g;
     } else {
         const int lpr = pick_lpr(c);
         const unsigned g = csr_grid(n, lpr);
+        const int32_t* rp = c->row_ptr.p;
+        const int32_t* ci = c->col.p;
         if (lpr == 32)
-            k_spmv_csr<32, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
+            INS_TRY(launch_timed(c, k_spmv_csr<32, TV, TX, TA, TVR>, g, SB, n, rp, ci, csr_val, p, q, c->partial.p, sc, it, R));
         else if (lpr == 16)
-            k_spmv_csr<16, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
+            INS_TRY(launch_timed(c, k_spmv_csr<16, TV, TX, TA, TVR>, g, SB, n, rp, ci, csr_val, p, q, c->partial.p, sc, it, R));
         else if (lpr == 8)
-            k_spmv_csr<8, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
+            INS_TRY(launch_timed(c, k_spmv_csr<8, TV, TX, TA, TVR>, g, SB, n, rp, ci, csr_val, p, q, c->partial.p, sc, it, R));
         else
-            k_spmv_csr<4, TV, TX, TA><<<g, SB, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, csr_val, p, q, c->partial.p, c->scal.p, it);
+            INS_TRY(launch_timed(c, k_spmv_csr<4, TV, TX, TA, TVR>, g, SB, n, rp, ci, csr_val, p, q, c->partial.p, sc, it, R));
         *g_out = (int)g;
     }
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }
 
+// K1 launch; the values of the residual replacement are the FP64 ones when
+// the assembly was FP64 (the FP64 solve never replaces)
+template <typename TV, typename TX, typename TA>
+int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
+                int* g_out, int it)
+{
+    const bool sell = format == INS_FMT_SELL;
+    if constexpr (sizeof(TX) == 8)
+        return launch_spmv_r<TV, TX, TA, double>(c, format, csr_val, sell_val, p, q, g_out, it,
+                                                 sell ? c->sell.v64.p : c->val64.p);
+    if (c->matrix_precision == INS_FP64)
+        return launch_spmv_r<TV, TX, TA, double>(c, format, csr_val, sell_val, p, q, g_out, it,
+                                                 sell ? c->sell.v64.p : c->val64.p);
+    return launch_spmv_r<TV, TX, TA, float>(c, format, csr_val, sell_val, p, q, g_out, it,
+                                            sell ? c->sell.v32.p : c->val32.p);
+}
+
 // |b - A x64| (and the residual replacement); then the finalising kernel
-int launch_resid(Ctx* c, int format, int replace, int par_next, hipStream_t stream = nullptr,
-                 Scal* scal = nullptr)
+// true residual |b - A x64|^2 into the scalars
+int launch_resid(Ctx* c, int format, hipStream_t stream = nullptr, Scal* scal = nullptr)
 {
     const int64_t n = c->n_active;
     hipStream_t st = stream ? stream : c->stream;
@@ -998,24 +1130,24 @@ int launch_resid(Ctx* c, int format, int replace, int par_next, hipStream_t stre
     if (format == INS_FMT_SELL) {
         g = sell_grid(n);
         if (v64)
-            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, c->partial.p);
         else
-            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, c->partial.p);
     } else {
         const int lpr = pick_lpr(c);
         g = csr_grid(n, lpr);
         if (v64) {
-            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else k_resid_csr<4, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->partial.p);
+            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->partial.p);
+            else k_resid_csr<4, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->partial.p);
         } else {
-            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
-            else k_resid_csr<4, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->r32.p, c->dinv32.p, c->partial.p, replace);
+            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->partial.p);
+            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->partial.p);
+            else k_resid_csr<4, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->partial.p);
         }
     }
     INS_HIP_CHECK(c, hipGetLastError());
-    k_resid_fin<<<1, VB, 0, st>>>(c->partial.p, (int)g, sc, replace, par_next);
+    k_resid_fin<<<1, VB, 0, st>>>(c->partial.p, (int)g, sc);
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }
@@ -1028,38 +1160,11 @@ int read_scal(Ctx* c)
     return INS_OK;
 }
 
-// profiling of the dominant kernel (the SpMV of K1): an event pair around
-// every launch, recorded on the context stream without waiting, so the
-// launches run back to back as in an unprofiled solve.  The pairs are read
-// once the solve has finished (prof_collect).
-struct ProfScope {
-    Ctx* c;
-    size_t k = 0;
-    bool on;
-    ProfScope(Ctx* c_) : c(c_), on(c_->profiling)
-    {
-        if (!on) return;
-        k = c->prof_n;
-        while (c->prof_ev.size() < 2 * (k + 1)) {
-            hipEvent_t e = nullptr;
-            if (hipEventCreate(&e) != hipSuccess) {
-                on = false;
-                return;
-            }
-            c->prof_ev.push_back(e);
-        }
-        c->prof_n++;
-        (void)hipEventRecord(c->prof_ev[2 * k], c->stream);
-    }
-    ~ProfScope()
-    {
-        if (on) (void)hipEventRecord(c->prof_ev[2 * k + 1], c->stream);
-    }
-};
-
-// Sum of the recorded SpMV durations.  Launches enqueued ahead of a stop or a
-// pause find the flag set and return after a few microseconds; they are
-// told apart by a duration under a quarter of the median and left out.
+// Sum of the recorded K1 durations, in two classes: the SpMV launches, and
+// the replacement launches (FP64 true residual: about twice the bytes and
+// time, more than 1.6x the median).  Launches enqueued after the stop find
+// the flag set and return within a few microseconds (under a quarter of the
+// median); they are left out.
 int prof_collect(Ctx* c)
 {
     std::vector<float> ms(c->prof_n, 0.f);
@@ -1068,11 +1173,16 @@ int prof_collect(Ctx* c)
     std::vector<float> sorted(ms);
     std::sort(sorted.begin(), sorted.end());
     const float med = sorted.empty() ? 0.f : sorted[sorted.size() / 2];
-    for (float v : ms)
-        if (v >= 0.25f * med) {
+    for (float v : ms) {
+        if (v < 0.25f * med) continue;
+        if (v > 1.6f * med) {
+            c->stats.rep_ms += v;
+            c->stats.rep_calls += 1;
+        } else {
             c->stats.spmv_ms += v;
             c->stats.spmv_calls += 1;
         }
+    }
     c->prof_n = 0;
     return INS_OK;
 }
@@ -1087,14 +1197,12 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
     for (int i = 0; i < k; i++) {
         const int it = (int)*it_launched;
         int g1 = 0;
-        {
-            ProfScope ps(c);
-            INS_TRY((launch_spmv<TV, TX, TX>(c, format, csr_val, sell_val, p, q, &g1, it)));
-        }
+        INS_TRY((launch_spmv<TV, TX, TX>(c, format, csr_val, sell_val, p, q, &g1, it)));
         k_update_xr<TX, TX><<<gv, VB, 0, c->stream>>>(n, x, r, p, q, dinv, c->partial.p, g1,
                                                       c->scal.p, it);
         k_update_p<TX, TX><<<gv, VB, 0, c->stream>>>(n, p, r, dinv, c->partial.p, (int)gv,
-                                                     c->scal.p, it);
+                                                     c->scal.p, it, c->x.p,
+                                                     sizeof(TX) == 4 ? x : nullptr);
         (*it_launched)++;
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1267,9 +1375,8 @@ int fill_sell(Ctx* c)
 // Host loop.  Chunks of iterations are enqueued two ahead, each followed by
 // an asynchronous copy of the device scalars into a pinned snapshot; the host
 // waits only for the older snapshot, so the GPU never idles while the host
-// decides.  Once a snapshot shows the stop (done) or, in mixed precision, the
-// replacement request (pause), the iterations already enqueued behind it exit
-// at once.
+// decides.  Once a snapshot shows the stop, the iterations already enqueued
+// behind it exit at once.
 struct Pipe {
     Ctx* c;
     int inflight = 0;
@@ -1319,9 +1426,10 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     const int chunk = mixed ? mchunk : CHUNK;
     c->stats.spmv_ms = 0.;
     c->stats.spmv_calls = 0;
+    c->stats.rep_ms = 0.;
+    c->stats.rep_calls = 0;
     c->prof_n = 0;
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
-    int refinements = 0;
     int64_t launched = 0;
     // replace the residual after every 10x reduction (delta = 0.1)
     const double delta2 = 0.01;
@@ -1338,11 +1446,16 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     }
     INS_HIP_CHECK(c, hipGetLastError());
     INS_TRY(read_scal(c));
+    // The device decides everything, replacements included; the host keeps
+    // two chunks queued and stops queueing once a snapshot shows the stop.
+    // The iteration limit counts SpMV iterations (replacement iterations,
+    // one per replacement, come on top).
     Pipe pipe{c};
     bool stop = c->h_scal->done != 0;
+    int reps_seen = 0;
     while (!stop) {
-        while (pipe.inflight < 2 && pipe.enq < max_it) {
-            const int k = std::min(chunk, max_it - pipe.enq);
+        while (pipe.inflight < 2 && pipe.enq < max_it + reps_seen) {
+            const int k = std::min(chunk, max_it + reps_seen - pipe.enq);
             if (mixed)
                 INS_TRY((run_chunk<float, float>(c, format, c->val32.p,
                                                  format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
@@ -1359,31 +1472,19 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
         if (pipe.inflight == 0) break;  // iteration limit
         Scal S;
         INS_TRY(pipe.pop(&S));
-        if (S.breakdown) break;
-        if (!(S.done || S.pause)) continue;
-        INS_TRY(pipe.drain());
-        if (!mixed) break;  // FP64: converged on the recursive residual
-        // residual replacement: x64 += x32; r = b - A x64 in FP64
-        k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
-        INS_HIP_CHECK(c, hipGetLastError());
-        INS_TRY(launch_resid(c, format, 1, (int)(launched & 1)));
-        k_restart<<<gv, VB, 0, c->stream>>>(n, c->p32.p, c->r32.p, c->dinv32.p, c->scal.p);
-        INS_HIP_CHECK(c, hipGetLastError());
-        refinements++;
-        INS_TRY(read_scal(c));
-        pipe.enq = c->h_scal->iters;
-        if (c->h_scal->done || c->h_scal->breakdown) stop = true;
+        reps_seen = S.nrep;
+        if (S.done) stop = true;
     }
     INS_TRY(pipe.drain());
     if (mixed) {
         INS_TRY(read_scal(c));
-        if (!(c->h_scal->done && !c->h_scal->breakdown)) {
+        if (!c->h_scal->fin && !c->h_scal->breakdown) {
             // iteration limit: fold what was computed and measure it
             k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
-            INS_TRY(launch_resid(c, format, 0, 0));
+            INS_TRY(launch_resid(c, format));
         }
     } else {
-        INS_TRY(launch_resid(c, format, 0, 0));
+        INS_TRY(launch_resid(c, format));
     }
     INS_TRY(read_scal(c));
     INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
@@ -1395,7 +1496,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     const double rel = S.bb > 0. ? std::sqrt(S.tt / S.bb) : 0.;
     c->stats.solve_ms = ms;
     c->stats.iterations = S.iters;
-    c->stats.refinements = refinements;
+    c->stats.refinements = S.nrep;
     c->stats.relres = rel;
     if (iters) *iters = S.iters;
     if (relres) *relres = rel;
@@ -1488,6 +1589,9 @@ struct SlotDesc {
     float INS_GLOBAL* q;
     double INS_GLOBAL* part;
     Scal INS_GLOBAL* sc;
+    const double INS_GLOBAL* v64;  // FP64 slice values (residual replacement)
+    double INS_GLOBAL* x64;
+    const double INS_GLOBAL* b;
     int64_t join;  // batched iteration of the request's first launch
     int g_spmv, g_vec;
 };
@@ -1527,10 +1631,12 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
 {
     const int s = find_slot(bt->pre_spmv, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
-    spmv_sell_body<float, float, float, 4>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv}, d.n,
-                                           sell_of(d.S), (const float*)d.val, (const float*)d.p,
-                                           (float*)d.q, (double*)d.part, (const Scal*)d.sc,
-                                           (int)(it - d.join));
+    const Rep<double> R{(const double*)d.v64, (const double*)d.x64, (const double*)d.b,
+                        (float*)d.r, (const float*)d.dinv};
+    spmv_sell_body<float, float, float, 4, double>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv},
+                                                   d.n, sell_of(d.S), (const float*)d.val,
+                                                   (const float*)d.p, (float*)d.q, (double*)d.part,
+                                                   (const Scal*)d.sc, (int)(it - d.join), R);
 }
 
 __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
@@ -1549,7 +1655,8 @@ __global__ __launch_bounds__(VB) void kb_update_p(const BatchTab* __restrict__ b
     const SlotDesc& d = bt->d[s];
     update_p_body<float, float>(Blk{(int)blockIdx.x - bt->pre_vec[s], d.g_vec}, d.n, (float*)d.p,
                                 (const float*)d.r, (const float*)d.dinv, (const double*)d.part,
-                                d.g_vec, (Scal*)d.sc, (int)(it - d.join));
+                                d.g_vec, (Scal*)d.sc, (int)(it - d.join), (double*)d.x64,
+                                (float*)d.x);
 }
 
 }  // namespace
@@ -1562,7 +1669,6 @@ struct SlotH {
     int max_it = 0;
     int64_t join_it = 0;      // index of the request's first batched iteration
     int64_t join_chunk = 0;   // first chunk whose snapshot includes its start
-    int64_t ignore_upto = -1; // snapshots of chunks <= this predate its last replacement
     int refinements = 0;
     int iters = 0;
     double relres = 0.;
@@ -1623,6 +1729,7 @@ int finish(Server* sv, int s, int status, const Scal* S)
     h.status = status;
     h.c->have_solution = true;
     if (S) {
+        h.refinements = S->nrep;
         h.iters = S->iters;
         h.relres = S->bb > 0. ? std::sqrt(S->tt / S->bb) : 0.;
         h.relres_known = true;
@@ -1635,45 +1742,22 @@ int finish(Server* sv, int s, int status, const Scal* S)
     return hand_back(sv, h);
 }
 
-// x64 += x32; r = b - A x64 (FP64); restart the direction if needed
-int replace(Server* sv, int s)
-{
-    SlotH& h = sv->slots[s];
-    Ctx* c = h.c;
-    const int64_t n = c->n_active;
-    const unsigned gv = vec_grid(n);
-    Scal* sc = sv->scal + s;
-    k_fold<<<gv, VB, 0, sv->stream>>>(n, c->x.p, c->x32.p);
-    SRV_HIP(sv, hipGetLastError());
-    const int par_next = (int)((sv->it - h.join_it) & 1);
-    if (launch_resid(c, INS_FMT_SELL, 1, par_next, sv->stream, sc) != INS_OK)
-        return sfail(sv, INS_ERR_HIP, c->err);
-    k_restart<<<gv, VB, 0, sv->stream>>>(n, c->p32.p, c->r32.p, c->dinv32.p, sc);
-    SRV_HIP(sv, hipGetLastError());
-    h.refinements++;
-    h.ignore_upto = sv->chunks - 1;
-    return INS_OK;
-}
-
 int process_snapshot(Server* sv, int64_t chunk, const Scal* snap)
 {
     for (int s = 0; s < sv->max_slots; s++) {
         SlotH& h = sv->slots[s];
-        if (h.state != INS_SLOT_RUNNING || chunk < h.join_chunk || chunk <= h.ignore_upto)
-            continue;
+        if (h.state != INS_SLOT_RUNNING || chunk < h.join_chunk) continue;
         const Scal& S = snap[s];
         if (S.fin) {
             INS_TRY(finish(sv, s, INS_OK, &S));
         } else if (S.breakdown) {
             INS_TRY(finish(sv, s, INS_ERR_BREAKDOWN, &S));
-        } else if (S.done || S.pause) {
-            INS_TRY(replace(sv, s));
         } else if (S.iters >= h.max_it) {
             // iteration limit: fold what was computed and measure it
             Ctx* c = h.c;
             k_fold<<<vec_grid(c->n_active), VB, 0, sv->stream>>>(c->n_active, c->x.p, c->x32.p);
             SRV_HIP(sv, hipGetLastError());
-            if (launch_resid(c, INS_FMT_SELL, 0, 0, sv->stream, sv->scal + s) != INS_OK)
+            if (launch_resid(c, INS_FMT_SELL, sv->stream, sv->scal + s) != INS_OK)
                 return sfail(sv, INS_ERR_HIP, c->err);
             h.iters = S.iters;
             h.relres_known = false;
@@ -1711,6 +1795,9 @@ int upload_table(Server* sv)
         INS_AS(d.q, c->q32.p);
         INS_AS(d.part, c->partial.p);
         INS_AS(d.sc, sv->scal + s);
+        INS_AS(d.v64, c->sell.v64.p);
+        INS_AS(d.x64, c->x.p);
+        INS_AS(d.b, c->b.p);
 #undef INS_AS
         d.g_spmv = (int)sell_grid(d.n);
         d.g_vec = (int)vec_grid(d.n);
@@ -1812,6 +1899,8 @@ int server_attach(Server* sv, Ctx* c, double rtol, int max_it, int* slot)
         return ctx_fail(st);
     if (!c->sell.ok)
         return sfail(sv, INS_ERR_UNSUPPORTED, "matrix has no SELL form (irregular rows)");
+    if (c->matrix_precision != INS_FP64)
+        return sfail(sv, INS_ERR_UNSUPPORTED, "serving needs an FP64 assembly (FP64 residual)");
     if (c->sell.stale && (st = fill_sell(c)) != INS_OK) return ctx_fail(st);
     if ((st = ensure_solver_state(c, INS_SOLVE_MIXED)) != INS_OK) return ctx_fail(st);
     // the request's own setup (assembly, SELL) precedes its start

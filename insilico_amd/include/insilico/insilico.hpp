@@ -238,7 +238,26 @@ struct Laplace {
         for (std::size_t e = 0; e < numElements; e++) perElement[e] = k(e);
     }
 };
+// heat::Static<MATERIAL> (heat/Static.hpp:98-...), the kernel of
+// heat::PoissonDriver (heat/PoissonDriver.hpp:78).  With a constant isotropic
+// conductivity K = k I and dK/du = 0 (mat/thermal/IsotropicConstant.hpp:42-52)
+// its tangent is k times the Laplace matrix, and it is assembled as such.
+template <typename MATERIAL, typename FTB = void>
+struct Static {
+    MATERIAL material;
+    explicit Static(const MATERIAL& m) : material(m) {}
+};
 }  // namespace heat
+
+namespace mat {
+namespace thermal {
+// mat::thermal::IsotropicConstant (mat/thermal/IsotropicConstant.hpp:42-52)
+struct IsotropicConstant {
+    double conductivity;
+    explicit IsotropicConstant(double k = 1.0) : conductivity(k) {}
+};
+}  // namespace thermal
+}  // namespace mat
 
 namespace solid {
 // solid::HyperElastic<MATERIAL, FTB> (solid/HyperElastic.hpp); per-element
@@ -509,6 +528,15 @@ void stiffnessMatrixComputation(const Quadrature<DEG, SH>&, solver::Hip& solver,
     solver.bind(fieldBinder.mesh, fieldBinder.field);
     solver.requestMatrix(INS_LAPLACE, Quadrature<DEG, SH>::numPointsPerDir,
                          detail_::laplaceCoef(kernel, fieldBinder.mesh.numElements()), 1, incremental);
+}
+
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename SFTB>
+void stiffnessMatrixComputation(const Quadrature<DEG, SH>& q, solver::Hip& solver, FB& fieldBinder,
+                                const heat::Static<mat::thermal::IsotropicConstant, SFTB>& kernel,
+                                bool incremental = true)
+{
+    stiffnessMatrixComputation<FTB>(q, solver, fieldBinder,
+                                    heat::Laplace<>(kernel.material.conductivity), incremental);
 }
 
 template <typename FTB, unsigned DEG, Shape SH, typename FB, typename MAT, typename KFTB>
