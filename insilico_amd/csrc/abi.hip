@@ -75,6 +75,7 @@ static void release_all(Ctx* c)
     release(c, c->x); release(c, c->r); release(c, c->p); release(c, c->q); release(c, c->t64);
     release(c, c->x32); release(c, c->r32); release(c, c->p32); release(c, c->q32);
     release(c, c->partial); release(c, c->scal); release(c, c->dflag);
+    release(c, c->xp); release(c, c->bp); release(c, c->d64p); release(c, c->d32p);
 }
 
 static void invalidate_pattern(Ctx* c)

@@ -152,6 +152,16 @@ struct Ctx {
     DArr<double> x, r, p, q, t64;
     DArr<float> x32, r32, p32, q32;
     DArr<double> partial;
+    // SELL solves of vector problems run in the component-major numbering:
+    // permuted copies of b and the Jacobi diagonal, and the FP64 solution
+    DArr<double> xp, bp, d64p;
+    DArr<float> d32p;
+    // vectors of the current solve in its numbering (set by solve_vectors)
+    double* sx64 = nullptr;
+    const double* sb = nullptr;
+    const double* sd64 = nullptr;
+    const float* sd32 = nullptr;
+    bool sperm = false;
     DArr<Scal> scal;
     Scal* h_scal = nullptr;  // pinned
     Scal* h_snap = nullptr;  // pinned, 2 snapshots for the pipelined host loop

@@ -775,6 +775,70 @@ __global__ void k_comp_place(int64_t n, const int32_t* __restrict__ comp, int c,
         if (comp[r] == c) perm[off + pos[r]] = (int32_t)r;
 }
 
+// inverse of the slice-row permutation
+__global__ void k_iperm(int64_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ iperm)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        iperm[perm[r]] = (int32_t)r;
+}
+
+// row lengths of the permuted matrix P A P^T
+__global__ void k_prow_len(int64_t n, const int32_t* __restrict__ perm,
+                           const int32_t* __restrict__ rp, int32_t* __restrict__ len)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t row = perm[r];
+        len[r] = rp[row + 1] - rp[row];
+    }
+}
+
+// columns of the permuted matrix, ascending, with the original CSR entry of
+// each.  Permuted indices order the rows by component, then by row, so a
+// row's permuted columns are its columns of component 0 (ascending), then of
+// component 1, ...: one pass per component gives them sorted.
+__global__ void k_prow_fill(int64_t n, int dpn, const int32_t* __restrict__ perm,
+                            const int32_t* __restrict__ iperm, const int32_t* __restrict__ comp,
+                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ prp, int32_t* __restrict__ pcol,
+                            int32_t* __restrict__ pent)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t row = perm[r];
+        const int32_t k0 = rp[row], k1 = rp[row + 1];
+        int32_t o = prp[r];
+        for (int cc = 0; cc < dpn; cc++)
+            for (int32_t k = k0; k < k1; k++) {
+                const int32_t cl = col[k];
+                if (comp[cl] != cc) continue;
+                pcol[o] = iperm[cl];
+                pent[o] = k;
+                o++;
+            }
+    }
+}
+
+// x[perm[r]] = xp[r] / xp[r] = x[perm[r]]
+template <typename T>
+__global__ void k_perm_out(int64_t n, const int32_t* __restrict__ perm, const T* __restrict__ xp,
+                           T* __restrict__ x)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        x[perm[r]] = xp[r];
+}
+
+template <typename T>
+__global__ void k_perm_in(int64_t n, const int32_t* __restrict__ perm, const T* __restrict__ x,
+                          T* __restrict__ xp)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        xp[r] = x[perm[r]];
+}
+
 __device__ inline uint64_t mix64(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
@@ -914,6 +978,7 @@ __global__ void k_widen(int64_t n, const int32_t* __restrict__ a, int64_t* __res
 __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
                                                   const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ ent,
                                                   int32_t* __restrict__ map, int* err)
 {
     const int64_t nsl = (n + 63) / 64;
@@ -930,8 +995,10 @@ __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
         int32_t k = rp[row];
         const int32_t e = rp[row + 1];
         for (int j = 0; j < len && k < e; j++)
-            if (col[k] - (int32_t)row == d[j])
-                map[k++] = (int32_t)(base + (int64_t)(j >> 2) * 256 + (j & 3));
+            if (col[k] - (int32_t)row == d[j]) {
+                map[ent ? ent[k] : k] = (int32_t)(base + (int64_t)(j >> 2) * 256 + (j & 3));
+                k++;
+            }
         if (k != e) atomicOr(err, 8);
     }
 }
@@ -1024,16 +1091,61 @@ int ensure_solver_state(Ctx* c, int mode)
     return INS_OK;
 }
 
+// the slices index rows of the (permuted, for vector problems) solve
+// numbering directly
 SellView sell_view(Ctx* c)
 {
     SellView v;
-    v.perm = c->sell.perm_on ? c->sell.perm.p : nullptr;
+    v.perm = nullptr;
     v.sptr = c->sell.slice_ptr.p;
     v.spat = c->sell.slice_pat.p;
     v.pptr = c->sell.pat_ptr.p;
     v.plen = c->sell.pat_len.p;
     v.pdel = c->sell.pat_delta.p;
     return v;
+}
+
+// The vectors of a solve in its numbering: the context's own for CSR solves
+// and scalar problems; for SELL solves of vector problems, copies of b and
+// of the Jacobi diagonal in the component-major numbering of the slices, and
+// a separate FP64 solution (solve_vectors_out writes it back).
+int solve_vectors(Ctx* c, int format, int mode, hipStream_t st)
+{
+    const int64_t n = c->n_active;
+    c->sperm = format == INS_FMT_SELL && c->sell.perm_on;
+    if (!c->sperm) {
+        c->sx64 = c->x.p;
+        c->sb = c->b.p;
+        c->sd64 = c->dinv64.p;
+        c->sd32 = c->dinv32.p;
+        return INS_OK;
+    }
+    INS_TRY(ensure(c, c->xp, n));
+    INS_TRY(ensure(c, c->bp, n));
+    const int32_t* perm = c->sell.perm.p;
+    k_perm_in<double><<<grid_for(n, 256), 256, 0, st>>>(n, perm, c->b.p, c->bp.p);
+    if (mode == INS_SOLVE_MIXED) {
+        INS_TRY(ensure(c, c->d32p, n));
+        k_perm_in<float><<<grid_for(n, 256), 256, 0, st>>>(n, perm, c->dinv32.p, c->d32p.p);
+    } else {
+        INS_TRY(ensure(c, c->d64p, n));
+        k_perm_in<double><<<grid_for(n, 256), 256, 0, st>>>(n, perm, c->dinv64.p, c->d64p.p);
+    }
+    INS_HIP_CHECK(c, hipGetLastError());
+    c->sx64 = c->xp.p;
+    c->sb = c->bp.p;
+    c->sd64 = c->d64p.p;
+    c->sd32 = c->d32p.p;
+    return INS_OK;
+}
+
+int solve_vectors_out(Ctx* c, hipStream_t st)
+{
+    if (!c->sperm) return INS_OK;
+    const int64_t n = c->n_active;
+    k_perm_out<double><<<grid_for(n, 256), 256, 0, st>>>(n, c->sell.perm.p, c->xp.p, c->x.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
 }
 
 // lanes per row of the CSR kernels, from the mean row length
@@ -1076,7 +1188,7 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
 {
     const int64_t n = c->n_active;
     const Scal* sc = c->scal.p;
-    const Rep<TVR> R{rep_val, c->x.p, c->b.p, c->r32.p, c->dinv32.p};
+    const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32};
     if (format == INS_FMT_SELL) {
         const unsigned g = sell_grid(n);
         INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR>, g, SB, n, sell_view(c), sell_val,
@@ -1130,20 +1242,20 @@ int launch_resid(Ctx* c, int format, hipStream_t stream = nullptr, Scal* scal = 
     if (format == INS_FMT_SELL) {
         g = sell_grid(n);
         if (v64)
-            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v64.p, c->x.p, c->b.p, c->partial.p);
+            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v64.p, c->sx64, c->sb, c->partial.p);
         else
-            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v32.p, c->x.p, c->b.p, c->partial.p);
+            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v32.p, c->sx64, c->sb, c->partial.p);
     } else {
         const int lpr = pick_lpr(c);
         g = csr_grid(n, lpr);
         if (v64) {
-            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->partial.p);
-            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->partial.p);
-            else k_resid_csr<4, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->x.p, c->b.p, c->partial.p);
+            if (lpr == 16) k_resid_csr<16, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->sx64, c->sb, c->partial.p);
+            else if (lpr == 8) k_resid_csr<8, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->sx64, c->sb, c->partial.p);
+            else k_resid_csr<4, double><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val64.p, c->sx64, c->sb, c->partial.p);
         } else {
-            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->partial.p);
-            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->partial.p);
-            else k_resid_csr<4, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->x.p, c->b.p, c->partial.p);
+            if (lpr == 16) k_resid_csr<16, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->sx64, c->sb, c->partial.p);
+            else if (lpr == 8) k_resid_csr<8, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->sx64, c->sb, c->partial.p);
+            else k_resid_csr<4, float><<<g, SB, 0, st>>>(n, c->row_ptr.p, c->col.p, c->val32.p, c->sx64, c->sb, c->partial.p);
         }
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1201,7 +1313,7 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
         k_update_xr<TX, TX><<<gv, VB, 0, c->stream>>>(n, x, r, p, q, dinv, c->partial.p, g1,
                                                       c->scal.p, it);
         k_update_p<TX, TX><<<gv, VB, 0, c->stream>>>(n, p, r, dinv, c->partial.p, (int)gv,
-                                                     c->scal.p, it, c->x.p,
+                                                     c->scal.p, it, c->sx64,
                                                      sizeof(TX) == 4 ? x : nullptr);
         (*it_launched)++;
     }
@@ -1237,15 +1349,22 @@ int build_sell(Ctx* c, bool force)
     INS_HIP_CHECK(c, hipMallocAsync((void**)&err, sizeof(int), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(keys, 0, cap * sizeof(uint64_t), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
-    // vector problems: slices hold rows of one component (ascending), so the
-    // rows of a slice share their column-offset list
+    // Vector problems: the slices are built on P A P^T, the matrix in the
+    // numbering that orders the rows by component (then by row), so that the
+    // 64 rows of a slice are consecutive nodes of one component, their
+    // column offsets are shared, and the gathers and stores of a slice are
+    // contiguous.  SELL solves of such problems run on permuted vectors.
     S.perm_on = c->dpn > 1;
+    const int32_t* brp = c->row_ptr.p;   // CSR the slices are built from
+    const int32_t* bcol = c->col.p;
+    int32_t *prp = nullptr, *pcol = nullptr, *pent = nullptr;
     if (S.perm_on) {
         INS_TRY(ensure(c, S.perm, n + 1));
-        int32_t *comp = nullptr, *flag = nullptr, *pos = nullptr;
+        int32_t *comp = nullptr, *flag = nullptr, *pos = nullptr, *iperm = nullptr;
         INS_HIP_CHECK(c, hipMallocAsync((void**)&comp, (n + 1) * sizeof(int32_t), c->stream));
         INS_HIP_CHECK(c, hipMallocAsync((void**)&flag, (n + 1) * sizeof(int32_t), c->stream));
         INS_HIP_CHECK(c, hipMallocAsync((void**)&pos, (n + 1) * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&iperm, (n + 1) * sizeof(int32_t), c->stream));
         const int64_t nd = c->n_nodes * c->dpn;
         k_row_comp<<<grid_for(nd, 256), 256, 0, c->stream>>>(nd, c->dof_id.p, c->dpn, comp);
         int64_t off = 0;
@@ -1257,14 +1376,28 @@ int build_sell(Ctx* c, bool force)
             k_comp_place<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, pos, off, S.perm.p);
             off += tot;
         }
+        k_iperm<<<grid_for(n, 256), 256, 0, c->stream>>>(n, S.perm.p, iperm);
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&prp, (n + 1) * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&pcol, (size_t)c->nnz * sizeof(int32_t) + 8, c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&pent, (size_t)c->nnz * sizeof(int32_t) + 8, c->stream));
+        k_prow_len<<<grid_for(n, 256), 256, 0, c->stream>>>(n, S.perm.p, c->row_ptr.p, flag);
+        int64_t pnnz = 0;
+        int st3 = exclusive_scan_i32(c, flag, prp, n, &pnnz);
+        if (st3 != INS_OK) return st3;
+        k_prow_fill<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->dpn, S.perm.p, iperm, comp,
+                                                             c->row_ptr.p, c->col.p, prp, pcol,
+                                                             pent);
         INS_HIP_CHECK(c, hipGetLastError());
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         (void)hipFreeAsync(comp, c->stream);
         (void)hipFreeAsync(flag, c->stream);
         (void)hipFreeAsync(pos, c->stream);
+        (void)hipFreeAsync(iperm, c->stream);
+        brp = prp;
+        bcol = pcol;
     }
-    k_slice_union<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(
-        n, S.perm_on ? S.perm.p : nullptr, c->row_ptr.p, c->col.p, udel, ulen, err);
+    k_slice_union<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, nullptr, brp, bcol,
+                                                                      udel, ulen, err);
     const unsigned g = grid_for(nsl, 256);
     k_list_hash<<<g, 256, 0, c->stream>>>(nsl, udel, ulen, h);
     k_hash_insert<<<g, 256, 0, c->stream>>>(nsl, h, keys, reps, cap - 1, err);
@@ -1328,17 +1461,19 @@ int build_sell(Ctx* c, bool force)
         int* e2 = nullptr;
         INS_HIP_CHECK(c, hipMallocAsync((void**)&e2, sizeof(int), c->stream));
         INS_HIP_CHECK(c, hipMemsetAsync(e2, 0, sizeof(int), c->stream));
-        k_sell_map<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, sell_view(c), c->row_ptr.p,
-                                                                       c->col.p, S.map.p, e2);
+        k_sell_map<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, sell_view(c), brp, bcol,
+                                                                       pent, S.map.p, e2);
         INS_HIP_CHECK(c, hipGetLastError());
         int he = 0;
         INS_HIP_CHECK(c, hipMemcpyAsync(&he, e2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         (void)hipFreeAsync(e2, c->stream);
         if (he) S.ok = false;
-        // values: zero once, the padding is never written afterwards
-        release(c, S.v64);
-        release(c, S.v32);
+    }
+    if (prp) {
+        (void)hipFreeAsync(prp, c->stream);
+        (void)hipFreeAsync(pcol, c->stream);
+        (void)hipFreeAsync(pent, c->stream);
     }
     S.built = true;
     S.stale = true;
@@ -1430,17 +1565,18 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     c->stats.rep_calls = 0;
     c->prof_n = 0;
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    INS_TRY(solve_vectors(c, format, mode, c->stream));
     int64_t launched = 0;
     // replace the residual after every 10x reduction (delta = 0.1)
     const double delta2 = 0.01;
     if (!mixed) {
-        k_init<double, double><<<gv, VB, 0, c->stream>>>(n, c->b.p, c->x.p, c->r.p, c->p.p,
-                                                         c->dinv64.p, c->partial.p);
+        k_init<double, double><<<gv, VB, 0, c->stream>>>(n, c->sb, c->sx64, c->r.p, c->p.p,
+                                                         c->sd64, c->partial.p);
         k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, 0., c->scal.p);
     } else {
-        INS_HIP_CHECK(c, hipMemsetAsync(c->x.p, 0, n * sizeof(double), c->stream));
-        k_init<float, float><<<gv, VB, 0, c->stream>>>(n, c->b.p, c->x32.p, c->r32.p, c->p32.p,
-                                                       c->dinv32.p, c->partial.p);
+        INS_HIP_CHECK(c, hipMemsetAsync(c->sx64, 0, n * sizeof(double), c->stream));
+        k_init<float, float><<<gv, VB, 0, c->stream>>>(n, c->sb, c->x32.p, c->r32.p, c->p32.p,
+                                                       c->sd32, c->partial.p);
         k_init_fin<<<1, VB, 0, c->stream>>>(c->partial.p, (int)gv, rtol * rtol, delta2,
                                             c->scal.p);
     }
@@ -1460,11 +1596,11 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
                 INS_TRY((run_chunk<float, float>(c, format, c->val32.p,
                                                  format == INS_FMT_SELL ? c->sell.v32.p : nullptr,
                                                  c->x32.p, c->r32.p, c->p32.p, c->q32.p,
-                                                 c->dinv32.p, k, &launched)));
+                                                 c->sd32, k, &launched)));
             else
                 INS_TRY((run_chunk<double, double>(c, format, c->val64.p,
                                                    format == INS_FMT_SELL ? c->sell.v64.p : nullptr,
-                                                   c->x.p, c->r.p, c->p.p, c->q.p, c->dinv64.p, k,
+                                                   c->sx64, c->r.p, c->p.p, c->q.p, c->sd64, k,
                                                    &launched)));
             pipe.enq += k;
             INS_TRY(pipe.issue_snapshot());
@@ -1480,12 +1616,13 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
         INS_TRY(read_scal(c));
         if (!c->h_scal->fin && !c->h_scal->breakdown) {
             // iteration limit: fold what was computed and measure it
-            k_fold<<<gv, VB, 0, c->stream>>>(n, c->x.p, c->x32.p);
+            k_fold<<<gv, VB, 0, c->stream>>>(n, c->sx64, c->x32.p);
             INS_TRY(launch_resid(c, format));
         }
     } else {
         INS_TRY(launch_resid(c, format));
     }
+    INS_TRY(solve_vectors_out(c, c->stream));
     INS_TRY(read_scal(c));
     INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     INS_HIP_CHECK(c, hipEventSynchronize(c->ev1));
@@ -1725,6 +1862,7 @@ int hand_back(Server* sv, SlotH& h)
 int finish(Server* sv, int s, int status, const Scal* S)
 {
     SlotH& h = sv->slots[s];
+    if (solve_vectors_out(h.c, sv->stream) != INS_OK) return sfail(sv, INS_ERR_HIP, h.c->err);
     h.state = INS_SLOT_FINISHED;
     h.status = status;
     h.c->have_solution = true;
@@ -1755,7 +1893,7 @@ int process_snapshot(Server* sv, int64_t chunk, const Scal* snap)
         } else if (S.iters >= h.max_it) {
             // iteration limit: fold what was computed and measure it
             Ctx* c = h.c;
-            k_fold<<<vec_grid(c->n_active), VB, 0, sv->stream>>>(c->n_active, c->x.p, c->x32.p);
+            k_fold<<<vec_grid(c->n_active), VB, 0, sv->stream>>>(c->n_active, c->sx64, c->x32.p);
             SRV_HIP(sv, hipGetLastError());
             if (launch_resid(c, INS_FMT_SELL, sv->stream, sv->scal + s) != INS_OK)
                 return sfail(sv, INS_ERR_HIP, c->err);
@@ -1788,7 +1926,7 @@ int upload_table(Server* sv)
         INS_AS(d.S.plen, sv0.plen);
         INS_AS(d.S.pdel, sv0.pdel);
         INS_AS(d.val, c->sell.v32.p);
-        INS_AS(d.dinv, c->dinv32.p);
+        INS_AS(d.dinv, c->sd32);
         INS_AS(d.x, c->x32.p);
         INS_AS(d.r, c->r32.p);
         INS_AS(d.p, c->p32.p);
@@ -1796,8 +1934,8 @@ int upload_table(Server* sv)
         INS_AS(d.part, c->partial.p);
         INS_AS(d.sc, sv->scal + s);
         INS_AS(d.v64, c->sell.v64.p);
-        INS_AS(d.x64, c->x.p);
-        INS_AS(d.b, c->b.p);
+        INS_AS(d.x64, c->sx64);
+        INS_AS(d.b, c->sb);
 #undef INS_AS
         d.g_spmv = (int)sell_grid(d.n);
         d.g_vec = (int)vec_grid(d.n);
@@ -1908,9 +2046,11 @@ int server_attach(Server* sv, Ctx* c, double rtol, int max_it, int* slot)
     SRV_HIP(sv, hipStreamWaitEvent(sv->stream, sv->ev_sync, 0));
     const int64_t n = c->n_active;
     const unsigned gv = vec_grid(n);
-    SRV_HIP(sv, hipMemsetAsync(c->x.p, 0, n * sizeof(double), sv->stream));
-    k_init<float, float><<<gv, VB, 0, sv->stream>>>(n, c->b.p, c->x32.p, c->r32.p, c->p32.p,
-                                                    c->dinv32.p, c->partial.p);
+    if ((st = solve_vectors(c, INS_FMT_SELL, INS_SOLVE_MIXED, sv->stream)) != INS_OK)
+        return ctx_fail(st);
+    SRV_HIP(sv, hipMemsetAsync(c->sx64, 0, n * sizeof(double), sv->stream));
+    k_init<float, float><<<gv, VB, 0, sv->stream>>>(n, c->sb, c->x32.p, c->r32.p, c->p32.p,
+                                                    c->sd32, c->partial.p);
     k_init_fin<<<1, VB, 0, sv->stream>>>(c->partial.p, (int)gv, rtol * rtol, 0.01, sv->scal + s);
     SRV_HIP(sv, hipGetLastError());
     SlotH& h = sv->slots[s];
