@@ -76,6 +76,7 @@ static void release_all(Ctx* c)
     release(c, c->x32); release(c, c->r32); release(c, c->p32); release(c, c->q32);
     release(c, c->partial); release(c, c->scal); release(c, c->dflag);
     release(c, c->xp); release(c, c->bp); release(c, c->d64p); release(c, c->d32p);
+    release(c, c->pinfo);
 }
 
 static void invalidate_pattern(Ctx* c)

@@ -4,8 +4,8 @@
 //     into a staging buffer:
 //       - heat::Laplace: one element per lane, the Q1 local-gradient table of
 //         all quadrature points in LDS, 36 (3-D) / 10 (2-D) unique entries of
-//         the symmetric element matrix kept in registers, written in 64-element
-//         slabs ([e/64][entry][e%64]) so the stores are coalesced;
+//         the symmetric element matrix kept in registers, written element-major
+//         ([e][entry]) so the gather reads whole sectors;
 //       - solid::HyperElastic: one wavefront per element; lanes < nqp build
 //         geometry, F, S and the Voigt C of their quadrature point, all 64
 //         lanes then form the effective elasticity A_iJkL of every point, and
@@ -190,9 +190,14 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
                 for (int n = 0; n < NPE; n++) fb[n] += f * phi[n] * w * det;
             }
         }
-        T* dst = ke + (e >> 6) * (int64_t)NK * 64 + (e & 63);
+        // element-major: the NK entries of an element are contiguous, so the
+        // gather, which reads the entries of the few elements around a node,
+        // uses whole sectors
+        typedef T T2 __attribute__((ext_vector_type(2)));
+        static_assert(NK % 2 == 0, "pairs of entries");
+        T2* dst = reinterpret_cast<T2*>(ke + e * NK);
 #pragma unroll
-        for (int k = 0; k < NK; k++) dst[k * 64] = K[k];
+        for (int k = 0; k < NK / 2; k++) dst[k] = (T2){K[2 * k], K[2 * k + 1]};
         if (fe) {
 #pragma unroll
             for (int n = 0; n < NPE; n++) fe[e * NPE + n] = (double)fb[n];
@@ -524,7 +529,7 @@ struct KeLaplace {
         const int i = a < b ? a : b, j = a < b ? b : a;
         const int k = i * NPE - (i * (i - 1)) / 2 + (j - i);
         constexpr int NK = NPE * (NPE + 1) / 2;
-        return (double)p[((e >> 6) * NK + k) * 64 + (e & 63)];
+        return (double)p[e * NK + k];
     }
 };
 
@@ -545,8 +550,9 @@ __global__ __launch_bounds__(BLK) void k_gather(
     const int32_t* __restrict__ adj, const int32_t* __restrict__ nb_ptr,
     const int32_t* __restrict__ nb, const int32_t* __restrict__ nboff,
     const int32_t* __restrict__ dof_id, const int32_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ col, const int32_t* __restrict__ conn, int monotone, KE ke,
-    double* __restrict__ v64, float* __restrict__ v32, int* __restrict__ err)
+    const int32_t* __restrict__ col, const int32_t* __restrict__ conn,
+    const uint32_t* __restrict__ pinfo, int monotone, KE ke, double* __restrict__ v64,
+    float* __restrict__ v32, int* __restrict__ err)
 {
     const int G = 1 << lgG;
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -571,20 +577,36 @@ __global__ __launch_bounds__(BLK) void k_gather(
         // n's (sorted) elements and find m among each element's vertices;
         // the loads are independent of each other and the same for all the
         // lanes of node n
-        for (int32_t ia = as; ia < ae; ia++) {
-            const int32_t xa = adj[ia];
-            const int64_t e = xa >> lg_npe;
-            const int la = xa & mask;
-            const int32_t* __restrict__ ce = conn + e * NPE;
-            int lb = -1;
+        if (pinfo) {
+            // precomputed at pattern set-up: one word per pair
+            const uint32_t w = pinfo[nbs + j];
+            for (int32_t i = 0; i < ae - as; i++) {
+                const uint32_t nib = (w >> (4 * i)) & 15u;
+                if (!nib) continue;
+                const int32_t xa = adj[as + i];
+                const int64_t e = xa >> lg_npe;
+                const int la = xa & mask, lb = (int)(nib & 7u);
 #pragma unroll
-            for (int t = 0; t < NPE; t++)
-                if (ce[t] == (int32_t)m) lb = t;
-            if (lb < 0) continue;
+                for (int cr = 0; cr < S; cr++)
 #pragma unroll
-            for (int cr = 0; cr < S; cr++)
+                    for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(e, la, cr, lb, cc);
+            }
+        } else {
+            for (int32_t ia = as; ia < ae; ia++) {
+                const int32_t xa = adj[ia];
+                const int64_t e = xa >> lg_npe;
+                const int la = xa & mask;
+                const int32_t* __restrict__ ce = conn + e * NPE;
+                int lb = -1;
 #pragma unroll
-                for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(e, la, cr, lb, cc);
+                for (int t = 0; t < NPE; t++)
+                    if (ce[t] == (int32_t)m) lb = t;
+                if (lb < 0) continue;
+#pragma unroll
+                for (int cr = 0; cr < S; cr++)
+#pragma unroll
+                    for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(e, la, cr, lb, cc);
+            }
         }
         const int32_t off = nboff[nbs + j];
 #pragma unroll
@@ -949,13 +971,13 @@ int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int in
         if (precision == INS_FP64)
             k_gather<NPE, S, KE, double><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
-                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, c->monotone ? 1 : 0,
-                ke, v64, v32, derr);
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p,
+                c->have_pinfo ? c->pinfo.p : nullptr, c->monotone ? 1 : 0, ke, v64, v32, derr);
         else
             k_gather<NPE, S, KE, float><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
-                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, c->monotone ? 1 : 0,
-                ke, nullptr, v32, derr);
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p,
+                c->have_pinfo ? c->pinfo.p : nullptr, c->monotone ? 1 : 0, ke, nullptr, v32, derr);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     if (do_rhs) {

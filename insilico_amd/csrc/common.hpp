@@ -118,6 +118,12 @@ struct Ctx {
     bool have_pattern = false;
     int64_t nnz = 0;
     int max_nb = 0;         // max neighbour nodes of a node
+    int max_adj = 0;        // max elements of a node
+    // per (node, neighbour) pair, when every node has at most 8 elements:
+    // 4 bits per element of the node (in adjacency order): bit 3 = the
+    // neighbour is a vertex of it, bits 0-2 = its local index there
+    DArr<uint32_t> pinfo;
+    bool have_pinfo = false;
     DArr<int32_t> adj_ptr, adj;   // node -> element*npe + local, by element
     DArr<int32_t> nb_ptr, nb, nboff;
     DArr<int32_t> row_ptr, col;

@@ -211,6 +211,39 @@ int build_adjacency(Ctx* c, const int32_t* items, int64_t n_items, int per_item,
     return INS_OK;
 }
 
+// Shared elements of every (node, neighbour) pair, for nodes of at most 8
+// elements: for element i of the node (adjacency order, ascending element
+// id) nibble i holds 8 + the neighbour's local index in it, or 0.  Lets the
+// assembly gather skip the search on every assembly.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_pair_info(int64_t n_nodes, int lgG, int lg_npe,
+                                                   const int32_t* __restrict__ adj_ptr,
+                                                   const int32_t* __restrict__ adj,
+                                                   const int32_t* __restrict__ conn,
+                                                   const int32_t* __restrict__ nb_ptr,
+                                                   const int32_t* __restrict__ nb,
+                                                   uint32_t* __restrict__ pinfo)
+{
+    const int G = 1 << lgG;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = gt >> lgG;
+    const int j0 = (int)(gt & (G - 1));
+    if (n >= n_nodes) return;
+    const int32_t nbs = nb_ptr[n], nbe = nb_ptr[n + 1];
+    const int32_t as = adj_ptr[n], ae = adj_ptr[n + 1];
+    for (int j = j0; j < nbe - nbs; j += G) {
+        const int32_t m = nb[nbs + j];
+        uint32_t w = 0;
+        for (int32_t i = 0; i < ae - as; i++) {
+            const int64_t e = adj[as + i] >> lg_npe;
+#pragma unroll
+            for (int t = 0; t < NPE; t++)
+                if (conn[e * NPE + t] == m) w |= (uint32_t)(8 + t) << (4 * i);
+        }
+        pinfo[nbs + j] = w;
+    }
+}
+
 int build_pattern(Ctx* c)
 {
     const int64_t nn = c->n_nodes;
@@ -271,7 +304,26 @@ int build_pattern(Ctx* c)
     if (hflags[0] & 2)
         return fail(c, INS_ERR_UNSUPPORTED, "more neighbour nodes than supported (96)");
     c->max_nb = hflags[1];
+    c->max_adj = maxadj;
     c->nnz = nnz;
+    c->have_pinfo = false;
+    if (maxadj <= 8) {
+        int G = 1;
+        while (G < c->max_nb && G < 64) G <<= 1;
+        int lgG = 0;
+        while ((1 << lgG) < G) lgG++;
+        INS_TRY(ensure(c, c->pinfo, (size_t)nbtot + 1));
+        const int64_t threads = nn << lgG;
+        const unsigned gp = (unsigned)((threads + 255) / 256);
+        if (c->npe == 8)
+            k_pair_info<8><<<gp, 256, 0, c->stream>>>(nn, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p,
+                                                       c->conn.p, c->nb_ptr.p, c->nb.p, c->pinfo.p);
+        else
+            k_pair_info<4><<<gp, 256, 0, c->stream>>>(nn, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p,
+                                                       c->conn.p, c->nb_ptr.p, c->nb.p, c->pinfo.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+        c->have_pinfo = true;
+    }
     c->have_pattern = true;
     return INS_OK;
 }
