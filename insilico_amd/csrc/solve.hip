@@ -796,27 +796,48 @@ __global__ void k_prow_len(int64_t n, const int32_t* __restrict__ perm,
 
 // columns of the permuted matrix, ascending, with the original CSR entry of
 // each.  Permuted indices order the rows by component, then by row, so a
-// row's permuted columns are its columns of component 0 (ascending), then of
-// component 1, ...: one pass per component gives them sorted.
-__global__ void k_prow_fill(int64_t n, int dpn, const int32_t* __restrict__ perm,
-                            const int32_t* __restrict__ iperm, const int32_t* __restrict__ comp,
-                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                            const int32_t* __restrict__ prp, int32_t* __restrict__ pcol,
-                            int32_t* __restrict__ pent)
+// row's permuted columns are its columns of component 0 (in their order),
+// then of component 1, ...  One wavefront per row: ballots give every entry
+// its rank among the entries of its component.
+__global__ __launch_bounds__(256) void k_prow_fill(int64_t n, int dpn,
+                                                   const int32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ iperm,
+                                                   const int32_t* __restrict__ comp,
+                                                   const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ prp,
+                                                   int32_t* __restrict__ pcol,
+                                                   int32_t* __restrict__ pent)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-         r += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t r = (int64_t)blockIdx.x * 4 + wave_in_block(); r < n;
+         r += (int64_t)gridDim.x * 4) {
         const int32_t row = perm[r];
         const int32_t k0 = rp[row], k1 = rp[row + 1];
-        int32_t o = prp[r];
-        for (int cc = 0; cc < dpn; cc++)
-            for (int32_t k = k0; k < k1; k++) {
-                const int32_t cl = col[k];
-                if (comp[cl] != cc) continue;
-                pcol[o] = iperm[cl];
-                pent[o] = k;
-                o++;
+        const int32_t o = prp[r];
+        int cnt[4] = {0, 0, 0, 0};
+        for (int32_t kb = k0; kb < k1; kb += 64) {
+            const int32_t k = kb + lane;
+            const int cc = (k < k1) ? comp[col[k]] : -1;
+            for (int c = 0; c < dpn; c++) cnt[c] += __popcll(__ballot(cc == c));
+        }
+        int base[4] = {0, 0, 0, 0};
+        for (int c = 1; c < dpn; c++) base[c] = base[c - 1] + cnt[c - 1];
+        for (int32_t kb = k0; kb < k1; kb += 64) {
+            const int32_t k = kb + lane;
+            const int32_t cl = (k < k1) ? col[k] : 0;
+            const int cc = (k < k1) ? comp[cl] : -1;
+            for (int c = 0; c < dpn; c++) {
+                const uint64_t m = __ballot(cc == c);
+                if (cc == c) {
+                    const int32_t d = o + base[c] + __popcll(m & below);
+                    pcol[d] = iperm[cl];
+                    pent[d] = k;
+                }
+                base[c] += __popcll(m);
             }
+        }
     }
 }
 
@@ -975,31 +996,38 @@ __global__ void k_widen(int64_t n, const int32_t* __restrict__ a, int64_t* __res
 
 // position of every CSR entry inside the slices: lane = row, walk the
 // slice's offset list and the row's columns together (setup, once)
+// Slice position of every CSR entry: one wavefront per row, lanes over the
+// row's entries, each finding its column offset in the slice's (ascending)
+// offset list by bisection.
 __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
                                                   const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ col,
                                                   const int32_t* __restrict__ ent,
                                                   int32_t* __restrict__ map, int* err)
 {
-    const int64_t nsl = (n + 63) / 64;
     const int lane = threadIdx.x & 63;
-    for (int64_t slice = (int64_t)blockIdx.x * 4 + wave_in_block(); slice < nsl;
-         slice += (int64_t)gridDim.x * 4) {
-        const int64_t rho = slice * 64 + lane;
-        if (rho >= n) continue;
-        const int64_t row = S.perm ? (int64_t)S.perm[rho] : rho;
+    for (int64_t rho = (int64_t)blockIdx.x * 4 + wave_in_block(); rho < n;
+         rho += (int64_t)gridDim.x * 4) {
+        const int64_t slice = rho >> 6;
+        const int lp = (int)(rho & 63);
         const int pid = S.spat[slice];
         const int32_t* d = S.pdel + S.pptr[pid];
         const int len = S.plen[pid];
-        const int64_t base = S.sptr[slice] + lane * 4;
-        int32_t k = rp[row];
-        const int32_t e = rp[row + 1];
-        for (int j = 0; j < len && k < e; j++)
-            if (col[k] - (int32_t)row == d[j]) {
-                map[ent ? ent[k] : k] = (int32_t)(base + (int64_t)(j >> 2) * 256 + (j & 3));
-                k++;
+        const int64_t base = S.sptr[slice] + lp * 4;
+        const int32_t k0 = rp[rho], k1 = rp[rho + 1];
+        for (int32_t k = k0 + lane; k < k1; k += 64) {
+            const int32_t delta = col[k] - (int32_t)rho;
+            int lo = 0, hi = len;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (d[mid] < delta) lo = mid + 1;
+                else hi = mid;
             }
-        if (k != e) atomicOr(err, 8);
+            if (lo < len && d[lo] == delta)
+                map[ent ? ent[k] : k] = (int32_t)(base + (int64_t)(lo >> 2) * 256 + (lo & 3));
+            else
+                atomicOr(err, 8);
+        }
     }
 }
 
@@ -1384,9 +1412,9 @@ int build_sell(Ctx* c, bool force)
         int64_t pnnz = 0;
         int st3 = exclusive_scan_i32(c, flag, prp, n, &pnnz);
         if (st3 != INS_OK) return st3;
-        k_prow_fill<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->dpn, S.perm.p, iperm, comp,
-                                                             c->row_ptr.p, c->col.p, prp, pcol,
-                                                             pent);
+        k_prow_fill<<<grid_for(n, 4, 1u << 16), 256, 0, c->stream>>>(n, c->dpn, S.perm.p, iperm,
+                                                                       comp, c->row_ptr.p,
+                                                                       c->col.p, prp, pcol, pent);
         INS_HIP_CHECK(c, hipGetLastError());
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         (void)hipFreeAsync(comp, c->stream);
@@ -1461,8 +1489,8 @@ int build_sell(Ctx* c, bool force)
         int* e2 = nullptr;
         INS_HIP_CHECK(c, hipMallocAsync((void**)&e2, sizeof(int), c->stream));
         INS_HIP_CHECK(c, hipMemsetAsync(e2, 0, sizeof(int), c->stream));
-        k_sell_map<<<grid_for(nsl, 4, 1u << 16), 256, 0, c->stream>>>(n, sell_view(c), brp, bcol,
-                                                                       pent, S.map.p, e2);
+        k_sell_map<<<grid_for(n, 4, 1u << 16), 256, 0, c->stream>>>(n, sell_view(c), brp, bcol,
+                                                                     pent, S.map.p, e2);
         INS_HIP_CHECK(c, hipGetLastError());
         int he = 0;
         INS_HIP_CHECK(c, hipMemcpyAsync(&he, e2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
