@@ -16,6 +16,11 @@ the hot path (it runs once per problem, on the host):
   :34-60), components interleaved, CONSTRAINED components skipped
   (base/dof/numbering.hpp:45-66).
 * random element fields: SplitMix64 + Box-Muller, drawn in element order.
+* ``q2_field``: DoF objects of a degree-2 Lagrange field on a Q1 mesh, as
+  base/dof/IndexMap.hpp:222-280 delegates to
+  base/dof/generateDoFIndicesFromFaces.hpp (vertex, edge, face and cell
+  passes over the elements in order; shared n-faces keep the number of their
+  first appearance, keyed by the sorted vertex ids).
 """
 from dataclasses import dataclass, field
 import math
@@ -53,10 +58,22 @@ class Problem:
     traction: tuple = (0.0, 0.0, 0.0)
     name: str = ""
     meta: dict = field(default_factory=dict)
+    # degree-2 field on the Q1 geometry above: dof_id and g are then indexed
+    # by field node (fconn), faces stay geometric and ffaces lists the field
+    # nodes of each Neumann face in its hierarchic order
+    deg: int = 1
+    fconn: np.ndarray = None   # (n_elem, 3**dim) int32, hierarchic order
+    fxyz: np.ndarray = None    # (n_fnodes, dim) support points
+    ffaces: np.ndarray = None  # (n_faces, 3**(dim-1)) int32
 
     @property
     def n_nodes(self):
         return self.xyz.shape[0]
+
+    @property
+    def n_fnodes(self):
+        """field nodes (DoF objects): geometry nodes for Q1"""
+        return self.fxyz.shape[0] if self.deg == 2 else self.xyz.shape[0]
 
     @property
     def n_elem(self):
@@ -130,6 +147,98 @@ def boundary_faces(conn, dim):
     on = cnt[inv.ravel()] == 1
     idx = np.nonzero(on)[0]
     return idx // nf, idx % nf, fn[idx].astype(np.int32)
+
+
+# element n-faces in hierarchic vertex numbers, in the order of the element's
+# DoF array (base/fe/Policies.hpp:43-77: vertices, edges, faces, cell) and of
+# the hierarchic shape functions (base/mesh/HierarchicOrder.hpp:201-452)
+HEX_EDGES = np.array([[0, 1], [1, 2], [2, 3], [3, 0], [4, 5], [5, 6],
+                      [6, 7], [7, 4], [0, 4], [1, 5], [2, 6], [3, 7]])
+
+
+def _first_appearance(keys):
+    """number distinct rows of `keys` (m, k) by their first appearance"""
+    _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    rank = np.empty(first.shape[0], np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(first.shape[0])
+    return rank[inv.ravel()], first.shape[0]
+
+
+def q2_field(xyz, conn, dim):
+    """Degree-2 field nodes of a Q1 mesh: (fconn (E, 3**dim), fxyz).
+
+    generateDoFIndicesFromFaces (base/dof/generateDoFIndicesFromFaces.hpp)
+    runs one pass per n-face kind: vertices, then edges (QUAD/HEX edge tables
+    of base/mesh/ElementFaces.hpp:282-286,339-347), then faces (HEX face
+    table :421-434; the cell of a QUAD), then the HEX cell.  Each pass walks
+    the elements in order and their n-faces in table order; an n-face seen
+    before (same sorted vertex ids) keeps its number, a new one takes the
+    next number.  A pass of co-dimension 0 (the cell) numbers every element
+    anew.  Support points: the Q1 geometry at the Lagrange points, i.e. the
+    vertex, the edge midpoint, the face and cell centres, summed in vertex
+    order (base/dof/constrainBoundary.hpp:77-110 with base/geometry.hpp)."""
+    conn = np.asarray(conn, np.int64)
+    E = conn.shape[0]
+    nv = 2 ** dim
+    edges = QUAD_EDGES if dim == 2 else HEX_EDGES
+    faces = None if dim == 2 else HEX_FACES
+    ncol = 3 ** dim
+    fconn = np.empty((E, ncol), np.int64)
+    pts = []
+    start = 0
+    # vertices
+    r, m = _first_appearance(conn.reshape(-1, 1))
+    fconn[:, :nv] = r.reshape(E, nv)
+    vx = np.empty((m, dim))
+    vx[r] = xyz[conn.reshape(-1)]
+    pts.append(vx)
+    start += m
+    col = nv
+    groups = [edges] + ([faces] if faces is not None else [])
+    for tab in groups:
+        verts = conn[:, tab]                                   # (E, nf, k)
+        key = np.sort(verts, axis=2).reshape(-1, tab.shape[1])
+        r, m = _first_appearance(key)
+        fconn[:, col:col + tab.shape[0]] = start + r.reshape(E, tab.shape[0])
+        c = np.empty((m, dim))
+        sm = xyz[verts[..., 0]]
+        for j in range(1, tab.shape[1]):
+            sm = sm + xyz[verts[..., j]]
+        c[r] = (sm * (1.0 / tab.shape[1])).reshape(-1, dim)
+        pts.append(c)
+        start += m
+        col += tab.shape[0]
+    # cell
+    fconn[:, col] = start + np.arange(E)
+    sm = xyz[conn[:, 0]]
+    for j in range(1, nv):
+        sm = sm + xyz[conn[:, j]]
+    pts.append(sm * (1.0 / nv))
+    return fconn.astype(np.int32), np.concatenate(pts, axis=0)
+
+
+def q2_face_nodes(fconn, elem, face, dim):
+    """Field nodes of the Neumann faces (element, local face) in the face's
+    own hierarchic order: LINE3 = (v0, v1, mid) for a QUAD edge, QUAD9 =
+    (w0..w3, edges (w0,w1), (w1,w2), (w2,w3), (w3,w0), centre) for a HEX face
+    whose vertices w are in the face table order (ElementFaces.hpp)."""
+    elem = np.asarray(elem)
+    face = np.asarray(face)
+    if dim == 2:
+        a, b = QUAD_EDGES[face, 0], QUAD_EDGES[face, 1]
+        return np.stack([fconn[elem, a], fconn[elem, b], fconn[elem, 4 + face]], 1).astype(np.int32)
+    lut = {}
+    for i, (a, b) in enumerate(HEX_EDGES):
+        lut[(a, b)] = lut[(b, a)] = i
+    out = np.empty((elem.shape[0], 9), np.int32)
+    for f in range(6):
+        sel = face == f
+        if not sel.any():
+            continue
+        w = HEX_FACES[f]
+        cols = list(w) + [8 + lut[(w[i], w[(i + 1) % 4])] for i in range(4)] + [20 + f]
+        out[sel] = fconn[elem[sel]][:, cols]
+    return out
 
 
 def number_dofs(constrained):
@@ -240,6 +349,41 @@ def neohooke3d(n=64, mu=1.0, lam=10.0):
     g = np.zeros(xyz.shape[0] * 3)
     p = Problem(3, xyz, conn, 3, dof, na, g, NEOHOOKE, coef, 2, name="neohooke3d-%d" % n)
     p.meta["right_x"] = right
+    return p
+
+
+def pulled_sheet_q2(n=20, disp_controlled=True, E=1000.0, nu=0.3, dim=2):
+    """reference/06-elastic/compressible.cpp:108-240 with
+    reference/06-elastic/PulledSheet.hpp: neo-Hookean block (0,1)^dim, Q2
+    field on the Q1 unit-square/cube mesh, Gauss rule of degree 3 (2 points
+    per direction, volume and surface); x = 0 clamped (all components);
+    x = 1 either u_x prescribed (displacement control, value set per load
+    step) or a traction t_x = value * (y - 1/2) (set per load step)."""
+    xyz, conn = unit_cube(dim, n)
+    fconn, fxyz = q2_field(xyz, conn, dim)
+    el, fa, bf = boundary_faces(conn, dim)
+    fnodes = q2_face_nodes(fconn, el, fa, dim)
+    bnodes = np.unique(fnodes)
+    tol = 1e-5
+    con = np.zeros((fxyz.shape[0], dim), bool)
+    left = bnodes[np.abs(fxyz[bnodes, 0]) < tol]
+    right = bnodes[np.abs(fxyz[bnodes, 0] - 1.0) < tol]
+    con[left, :] = True
+    if disp_controlled:
+        con[right, 0] = True
+    dof, na = number_dofs(con)
+    lam, mu = lame(E, nu)
+    coef = np.tile(np.array([[lam, mu]]), (conn.shape[0], 1))
+    on_right = np.all(np.abs(xyz[bf, 0] - 1.0) < tol, axis=1)
+    p = Problem(dim, xyz, conn, dim, dof, na, np.zeros(fxyz.shape[0] * dim), NEOHOOKE,
+                coef, 2, name="pulledSheetQ2-%d%s" % (n, "D" if disp_controlled else "N"),
+                deg=2, fconn=fconn, fxyz=fxyz)
+    if not disp_controlled:
+        p.faces = bf[on_right].astype(np.int32)
+        p.ffaces = fnodes[on_right]
+        p.traction = (0.0, 0.0, 0.0)
+        p.meta["traction_linear"] = (1, 0.5)
+    p.meta.update(right_x=right, E=E, lam=lam, mu=mu)
     return p
 
 

@@ -116,13 +116,124 @@ void orc_q1_shape(int dim, const double* xi, double* phi, double* dphi)
     }
 }
 
+/* Hierarchic look-up table H (lexicographic position -> hierarchic index) of
+ * the (K+1)^dim tensor-product element, K = 1 or 2:
+ * base/mesh/HierarchicOrder.hpp:201-270 (QUAD: vertices {0, K, K(K+2),
+ * K(K+1)}, then the edges (0,1),(1,2),(2,3),(3,0) walked from their first
+ * vertex, then the interior) and :345-452 (HEX: 8 vertices, 12 edges, 6
+ * faces spanned by (v1,v2,v3) = (0,1,3),(4,5,7),(0,1,4),(1,2,5),(2,3,6),
+ * (3,0,7), then the interior). */
+static void hier_table(int dim, int K, int* H)
+{
+    const int n1 = K + 1, dZ = K * n1 * n1;
+    int V[8] = {0, K, K * (K + 2), K * (K + 1), dZ, K + dZ, K * (K + 2) + dZ, K * (K + 1) + dZ};
+    static const int E2[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    static const int E3[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                  {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    static const int F3[6][3] = {{0, 1, 3}, {4, 5, 7}, {0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}};
+    int ctr = 0;
+    const int nv = dim == 2 ? 4 : 8, ne = dim == 2 ? 4 : 12;
+    for (int i = 0; i < nv; i++) H[V[i]] = ctr++;
+    for (int i = 0; i < ne; i++) {
+        const int a = V[dim == 2 ? E2[i][0] : E3[i][0]], b = V[dim == 2 ? E2[i][1] : E3[i][1]];
+        const int delta = (b - a) / K;
+        for (int n = 0; n < K - 1; n++) H[a + delta + n * delta] = ctr++;
+    }
+    if (dim == 2) {
+        const int d1 = (V[1] - V[0]) / K, d2 = (V[3] - V[0]) / K;
+        for (int m2 = 0; m2 < K - 1; m2++)
+            for (int m1 = 0; m1 < K - 1; m1++) H[V[0] + d1 + d2 + m1 * d1 + m2 * d2] = ctr++;
+        return;
+    }
+    for (int f = 0; f < 6; f++) {
+        const int a = V[F3[f][0]];
+        const int d1 = (V[F3[f][1]] - a) / K, d2 = (V[F3[f][2]] - a) / K;
+        for (int m2 = 0; m2 < K - 1; m2++)
+            for (int m1 = 0; m1 < K - 1; m1++) H[a + d1 + d2 + m1 * d1 + m2 * d2] = ctr++;
+    }
+    const int d1 = (V[1] - V[0]) / K, d2 = (V[3] - V[0]) / K, d3 = (V[4] - V[0]) / K;
+    for (int m3 = 0; m3 < K - 1; m3++)
+        for (int m2 = 0; m2 < K - 1; m2++)
+            for (int m1 = 0; m1 < K - 1; m1++)
+                H[V[0] + d1 + d2 + d3 + m1 * d1 + m2 * d2 + m3 * d3] = ctr++;
+}
+
+/* 1-D Lagrange functions of degree 1 (base/sfun/Lagrange1D.ipp:46-58) and
+ * degree 2 (:76-91; support points 0, 1/2, 1 in that order). */
+static void lagrange1d(int K, double x, double* f, double* g)
+{
+    if (K == 1) {
+        f[0] = 1. - x; f[1] = x;
+        g[0] = -1.;    g[1] = 1.;
+        return;
+    }
+    f[0] = (1. - x) * (1. - 2. * x);
+    f[1] = 4. * x * (1. - x);
+    f[2] = x * (2. * x - 1.);
+    g[0] = 4. * x - 3.;
+    g[1] = 4. - 8. * x;
+    g[2] = 4. * x - 1.;
+}
+
+/* Tensor-product Lagrange shape functions of degree K (1 or 2) on QUAD/HEX
+ * in hierarchic order: the recursive products of
+ * base/sfun/TensorProduct.hpp:225-320 (value = lower * outer; gradient
+ * components below the last = outer * lower-gradient, last = outer-gradient
+ * * lower), then out[H[n]] = in[n] (:40-57).  For K = 1 this is
+ * orc_q1_shape. */
+void orc_lagrange_shape(int dim, int K, const double* xi, double* phi, double* dphi)
+{
+    const int n1 = K + 1;
+    double f[3][3], g[3][3];
+    for (int d = 0; d < dim; d++) lagrange1d(K, xi[d], f[d], g[d]);
+    double lf[27], lg[27][3];
+    /* 1-D */
+    int m = n1;
+    for (int i = 0; i < n1; i++) { lf[i] = f[0][i]; lg[i][0] = g[0][i]; }
+    for (int d = 1; d < dim; d++) {
+        double tf[27], tg[27][3];
+        int c = 0;
+        for (int o = 0; o < n1; o++)
+            for (int i = 0; i < m; i++) {
+                tf[c] = lf[i] * f[d][o];
+                for (int a = 0; a < d; a++) tg[c][a] = f[d][o] * lg[i][a];
+                tg[c][d] = g[d][o] * lf[i];
+                c++;
+            }
+        m = c;
+        for (int i = 0; i < m; i++) {
+            lf[i] = tf[i];
+            for (int a = 0; a <= d; a++) lg[i][a] = tg[i][a];
+        }
+    }
+    int H[27];
+    hier_table(dim, K, H);
+    for (int n = 0; n < m; n++) {
+        if (phi) phi[H[n]] = lf[n];
+        if (dphi)
+            for (int a = 0; a < dim; a++) dphi[H[n] * dim + a] = lg[n][a];
+    }
+}
+
 /* Jacobi matrix J(i,a) = sum_n X_n[i] dphi_n[a]: base/geometry.hpp:143-177.
  * Contra-variant basis inv(J^T) with det(J^T) by the cofactor (adjugate)
  * method, as Eigen's computeInverseAndDetWithCheck does for 2x2/3x3:
  * base/geometry.hpp:420-445.  grad_x phi_n = inv(J^T) dphi_n:
  * base/LagrangeShapeFun.hpp:182-200.  Returns detJ. */
+static double contra_field(int dim, const double* X, const double* dphi, int nf,
+                           const double* fdphi, double* gradx);
+
 double orc_contravariant(int dim, const double* X, const double* dphi,
                          double* gradx)
+{
+    return contra_field(dim, X, dphi, 1 << dim, dphi, gradx);
+}
+
+/* as orc_contravariant, with the inverse applied to the local gradients of
+ * nf field functions (Q2 on Q1 geometry: base/LagrangeShapeFun.hpp:182-200
+ * maps every field gradient by the geometry's contra-variant basis) */
+static double contra_field(int dim, const double* X, const double* dphi, int nf,
+                           const double* fdphi, double* gradx)
 {
     const int npe = 1 << dim;
     double J[3][3] = {{0}};
@@ -151,14 +262,24 @@ double orc_contravariant(int dim, const double* X, const double* dphi,
         for (int r = 0; r < 3; r++)
             for (int c = 0; c < 3; c++) inv[r][c] = cof[c][r] * id;
     }
-    for (int n = 0; n < npe; n++)
+    for (int n = 0; n < nf; n++)
         for (int i = 0; i < dim; i++) {
             double s = 0.;
-            for (int a = 0; a < dim; a++) s += inv[i][a] * dphi[n * dim + a];
+            for (int a = 0; a < dim; a++) s += inv[i][a] * fdphi[n * dim + a];
             gradx[n * dim + i] = s;
         }
     return det;
 }
+
+/* field nodes of an element: the geometry nodes (iso-parametric Q1) or the
+ * degree-2 field nodes */
+static inline int field_npe(const orc_problem* p) { return p->deg == 2 ? p->fnpe : p->npe; }
+static inline const int32_t* field_conn(const orc_problem* p)
+{
+    return p->deg == 2 ? p->fconn : p->conn;
+}
+#define ORC_MAXF 27            /* field nodes per element */
+#define ORC_MAXE (ORC_MAXF * 3) /* element DoFs */
 
 /* 3x3 determinant: mat/TensorAlgebra.hpp:55-60 */
 static double det3(const double A[3][3])
@@ -266,26 +387,39 @@ static double body_force(const orc_problem* p, const double* x, int comp)
  * their integrators are separate element loops in the reference. */
 int orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe)
 {
-    const int dim = p->dim, npe = p->npe, s = p->dpn, nee = npe * s;
-    double X[8 * 3], U[8 * 3];
-    for (int n = 0; n < npe; n++) {
-        const int32_t node = p->conn[e * npe + n];
+    const int dim = p->dim, gnpe = p->npe, s = p->dpn;
+    const int npe = field_npe(p), nee = npe * s, K = p->deg == 2 ? 2 : 1;
+    const int32_t* fc = field_conn(p);
+    double X[8 * 3], U[ORC_MAXE];
+    for (int n = 0; n < gnpe; n++) {
+        const int32_t node = p->conn[e * gnpe + n];
         for (int d = 0; d < dim; d++) X[n * dim + d] = p->xyz[(int64_t)node * dim + d];
+    }
+    for (int n = 0; n < npe; n++) {
+        const int32_t node = fc[e * npe + n];
         for (int c = 0; c < s; c++) U[n * s + c] = p->u ? p->u[(int64_t)node * s + c] : 0.;
     }
     double wq[64], xq[64 * 3];
     const int nq = orc_tensor_rule(dim, p->nq, wq, xq);
     if (nq < 0) return -1;
     if (Ke) for (int i = 0; i < nee * nee; i++) Ke[i] = 0.;
-    double fb[24], fr[24];
+    double fb[ORC_MAXE], fr[ORC_MAXE];
     for (int i = 0; i < nee; i++) { fb[i] = 0.; fr[i] = 0.; }
 
     for (int q = 0; q < nq; q++) {
         const double* xi = xq + q * dim;
         const double w = wq[q];
-        double phi[8], dphi[8 * 3], gx[8 * 3];
-        orc_q1_shape(dim, xi, phi, dphi);
-        const double detJ = orc_contravariant(dim, X, dphi, gx);
+        double gphi[8], gdphi[8 * 3];
+        double phi[ORC_MAXF], dphi[ORC_MAXF * 3], gx[ORC_MAXF * 3];
+        orc_q1_shape(dim, xi, gphi, gdphi);
+        double detJ;
+        if (K == 1) {
+            for (int n = 0; n < gnpe; n++) phi[n] = gphi[n];
+            detJ = orc_contravariant(dim, X, gdphi, gx);
+        } else {
+            orc_lagrange_shape(dim, K, xi, phi, dphi);
+            detJ = contra_field(dim, X, gdphi, npe, dphi, gx);
+        }
         if (p->kernel == ORC_LAPLACE && Ke) {
             const double kappa = p->coef ? p->coef[e * p->ncoef] : 1.0;
             const double scalar = kappa * detJ * w;
@@ -349,8 +483,8 @@ int orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe)
             /* Jacobian = det(J) (base/geometry.hpp:227-251,480-489); x(xi) by
              * base/geometry.hpp:95-130 */
             double x[3] = {0, 0, 0};
-            for (int n = 0; n < npe; n++)
-                for (int d = 0; d < dim; d++) x[d] += X[n * dim + d] * phi[n];
+            for (int n = 0; n < gnpe; n++)
+                for (int d = 0; d < dim; d++) x[d] += X[n * dim + d] * gphi[n];
             for (int n = 0; n < npe; n++)
                 for (int c = 0; c < s; c++)
                     fb[n * s + c] += body_force(p, x, c) * phi[n] * w * detJ;
@@ -408,8 +542,10 @@ int orc_unit_cube(int dim, int e1, int e2, int e3, double* xyz, int32_t* conn)
 static int active_ids(const orc_problem* p, int64_t e, int32_t* ids, int* loc)
 {
     int na = 0;
-    for (int n = 0; n < p->npe; n++) {
-        const int64_t node = p->conn[e * p->npe + n];
+    const int npe = field_npe(p);
+    const int32_t* fc = field_conn(p);
+    for (int n = 0; n < npe; n++) {
+        const int64_t node = fc[e * npe + n];
         for (int c = 0; c < p->dpn; c++) {
             const int32_t id = p->dof_id[node * p->dpn + c];
             if (id >= 0) { ids[na] = id; loc[na] = n * p->dpn + c; na++; }
@@ -434,8 +570,8 @@ int orc_pattern(const orc_problem* p, int64_t* nnz_out, int32_t* row_ptr,
 {
     const int64_t n = p->n_active;
     int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
-    int32_t ids[24];
-    int loc[24];
+    int32_t ids[ORC_MAXE];
+    int loc[ORC_MAXE];
     for (int64_t e = 0; e < p->n_elem; e++) {
         const int na = active_ids(p, e, ids, loc);
         for (int r = 0; r < na; r++) cnt[ids[r] + 1] += na;
@@ -489,18 +625,20 @@ static int64_t triplet_find(const triplet* t, int64_t nnz, int32_t r, int32_t c)
 static void face_force(const orc_problem* p, int64_t f, double* out)
 {
     const int dim = p->dim, nfn = p->nfn, s = p->dpn;
+    const int K = p->deg == 2 ? 2 : 1, ffn = K == 2 ? p->ffn : nfn;
     double X[4 * 3];
     for (int n = 0; n < nfn; n++)
         for (int d = 0; d < dim; d++)
             X[n * dim + d] = p->xyz[(int64_t)p->faces[f * nfn + n] * dim + d];
     double wq[16], xq[16 * 2];
     const int nq = orc_tensor_rule(dim - 1, p->nq, wq, xq);
-    for (int i = 0; i < nfn * s; i++) out[i] = 0.;
+    for (int i = 0; i < ffn * s; i++) out[i] = 0.;
     for (int q = 0; q < nq; q++) {
-        double phi[4], dphi[4 * 2];
+        double phi[9], dphi[4 * 2];
         double detG;
         if (dim == 3) {
             orc_q1_shape(2, xq + q * 2, phi, dphi);
+            if (K == 2) orc_lagrange_shape(2, 2, xq + q * 2, phi, NULL);
             double J[3][2] = {{0}};
             for (int n = 0; n < 4; n++)
                 for (int i = 0; i < 3; i++)
@@ -512,11 +650,27 @@ static void face_force(const orc_problem* p, int64_t f, double* out)
         } else {
             const double eta = xq[q];
             phi[0] = 1. - eta; phi[1] = eta;
+            if (K == 2) {
+                /* LINE3 hierarchic order: ends first, then the midpoint
+                 * (base/mesh/HierarchicOrder.hpp:131-150) */
+                double f[3], g[3];
+                lagrange1d(2, eta, f, g);
+                phi[0] = f[0]; phi[1] = f[2]; phi[2] = f[1];
+            }
             const double tx = X[2] - X[0], ty = X[3] - X[1];
             detG = sqrt(tx * tx + ty * ty);
         }
-        for (int n = 0; n < nfn; n++)
-            for (int c = 0; c < s; c++) out[n * s + c] += p->traction[c] * phi[n] * wq[q] * detG;
+        double t[3] = {p->traction[0], p->traction[1], p->traction[2]};
+        if (p->tr_kind == 1) {
+            /* surface point x(eta) of the Q1 face geometry */
+            double gp[4], gd[4 * 2], xs = 0.;
+            if (dim == 3) orc_q1_shape(2, xq + q * 2, gp, gd);
+            else { gp[0] = 1. - xq[q]; gp[1] = xq[q]; }
+            for (int n = 0; n < nfn; n++) xs += X[n * dim + p->tr_axis] * gp[n];
+            for (int c = 0; c < s; c++) t[c] = p->traction[c] * (xs - p->tr_x0);
+        }
+        for (int n = 0; n < ffn; n++)
+            for (int c = 0; c < s; c++) out[n * s + c] += t[c] * phi[n] * wq[q] * detG;
     }
 }
 
@@ -532,7 +686,8 @@ int orc_assemble(const orc_problem* p, const int32_t* row_ptr,
 {
     const int64_t n = p->n_active;
     const int64_t nnz = row_ptr[n];
-    const int s = p->dpn, npe = p->npe, nee = npe * s;
+    const int s = p->dpn, npe = field_npe(p), nee = npe * s;
+    const int32_t* fc = field_conn(p);
     triplet* T = (triplet*)malloc((size_t)(nnz ? nnz : 1) * sizeof(triplet));
     for (int64_t i = 0; i < n; i++)
         for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
@@ -546,14 +701,14 @@ int orc_assemble(const orc_problem* p, const int32_t* row_ptr,
 #pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
 #endif
     for (int64_t e = 0; e < p->n_elem; e++) {
-        double Ke[24 * 24];
+        double Ke[ORC_MAXE * ORC_MAXE];
         orc_element(p, e, Ke, NULL);
-        int32_t ids[24];
-        int loc[24];
-        int st[24];
-        double gv[24];
+        int32_t ids[ORC_MAXE];
+        int loc[ORC_MAXE];
+        int st[ORC_MAXE];
+        double gv[ORC_MAXE];
         for (int nn = 0; nn < npe; nn++) {
-            const int64_t node = p->conn[e * npe + nn];
+            const int64_t node = fc[e * npe + nn];
             for (int c = 0; c < s; c++) {
                 const int64_t k = node * s + c;
                 st[nn * s + c] = p->dof_id[k] >= 0;
@@ -584,20 +739,22 @@ int orc_assemble(const orc_problem* p, const int32_t* row_ptr,
     /* force loops, serial as in the reference */
     if (p->residual || p->bf_kind != ORC_BF_NONE) {
         for (int64_t e = 0; e < p->n_elem; e++) {
-            double Fe[24];
+            double Fe[ORC_MAXE];
             orc_element(p, e, NULL, Fe);
-            int32_t ids[24];
-            int loc[24];
+            int32_t ids[ORC_MAXE];
+            int loc[ORC_MAXE];
             const int na = active_ids(p, e, ids, loc);
             for (int r = 0; r < na; r++) b[ids[r]] += Fe[loc[r]];
         }
     }
+    const int ffn = p->deg == 2 ? p->ffn : p->nfn;
+    const int32_t* ffc = p->deg == 2 ? p->ffaces : p->faces;
     for (int64_t f = 0; f < p->n_faces; f++) {
-        double ff[4 * 3];
+        double ff[9 * 3];
         face_force(p, f, ff);
-        for (int nn = 0; nn < p->nfn; nn++)
+        for (int nn = 0; nn < ffn; nn++)
             for (int c = 0; c < s; c++) {
-                const int32_t id = p->dof_id[(int64_t)p->faces[f * p->nfn + nn] * s + c];
+                const int32_t id = p->dof_id[(int64_t)ffc[f * ffn + nn] * s + c];
                 if (id >= 0) b[id] += ff[nn * s + c];
             }
     }

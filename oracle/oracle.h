@@ -19,7 +19,13 @@
  * rigid-body modes, symmetry/PSD and finite differences of the residual; the
  * pattern by sparsity.1.ref.dat and closed-form nnz; assembly and lifting by
  * brute force and the linearElastic{2D,3D}.ref.dat L2 errors; orc_pcg by its
- * residual and a direct solve.  Parity unpinned: the exact iterate sequence
+ * residual and a direct solve.  Degree-2 fields on Q1 geometry
+ * (tests/test_oracle_q2.py): orc_lagrange_shape by the Kronecker property at
+ * the hierarchic support points, reproduction of x^2 and finite differences;
+ * Q2 elements by closed-form energies, rigid-body modes and finite differences
+ * of the residual; Q2 assembly, lifting and the varying traction by the
+ * Newton histories compRefOut{D,N}.dat (every printed digit above the
+ * round-off floor).  Parity unpinned: the exact iterate sequence
  * and iteration count of Eigen 3.2's ConjugateGradient (x0 = 0 per Eigen's
  * documentation), and last-bit rounding of Eigen's 3x3
  * computeInverseAndDetWithCheck; no fixture of the reference fixes them.
@@ -63,11 +69,25 @@ typedef struct orc_problem {
     double traction[3];
     int residual;            /* add -F_int(u) (Newton residual) to the RHS */
     int nthreads;            /* 1 = reference default (serial) */
+    /* Fields of degree 2 on the Q1 geometry above (deg = 2).  Then conn/npe
+     * and faces/nfn are the geometry, and dof_id, g, u are indexed by field
+     * node.  deg 0 or 1: iso-parametric Q1 (field node = geometry node). */
+    int deg;
+    int fnpe;                /* field nodes per element: 9 (QUAD9) / 27 (HEX27) */
+    const int32_t* fconn;    /* n_elem*fnpe field node ids, hierarchic order */
+    int ffn;                 /* field nodes per Neumann face: 3 (LINE3) / 9 (QUAD9) */
+    const int32_t* ffaces;   /* n_faces*ffn field node ids, hierarchic face order */
+    /* traction varying along the face: t_c = traction[c] * (x[tr_axis] - tr_x0)
+     * when tr_kind = 1 (as the pulled-sheet Neumann function); 0 = constant */
+    int tr_kind;
+    int tr_axis;
+    double tr_x0;
 } orc_problem;
 
 int  orc_gauss1d(int npts, double* w, double* x);
 int  orc_tensor_rule(int dim, int npts, double* w, double* xi);
 void orc_q1_shape(int dim, const double* xi, double* phi, double* dphi);
+void orc_lagrange_shape(int dim, int deg, const double* xi, double* phi, double* dphi);
 double orc_contravariant(int dim, const double* X, const double* dphi,
                          double* gradx);
 int  orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe);

@@ -32,6 +32,9 @@ class Problem(C.Structure):
         ("n_faces", C.c_int64), ("nfn", C.c_int), ("faces", C.c_void_p),
         ("traction", C.c_double * 3), ("residual", C.c_int),
         ("nthreads", C.c_int),
+        ("deg", C.c_int), ("fnpe", C.c_int), ("fconn", C.c_void_p),
+        ("ffn", C.c_int), ("ffaces", C.c_void_p),
+        ("tr_kind", C.c_int), ("tr_axis", C.c_int), ("tr_x0", C.c_double),
     ]
 
 
@@ -50,6 +53,8 @@ def lib():
         L.orc_tensor_rule.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
         L.orc_q1_shape.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.orc_q1_shape.restype = None
+        L.orc_lagrange_shape.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p]
+        L.orc_lagrange_shape.restype = None
         L.orc_contravariant.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.orc_contravariant.restype = C.c_double
         L.orc_element.argtypes = [C.POINTER(Problem), C.c_int64, _f64p, _f64p]
@@ -118,8 +123,22 @@ class OracleProblem:
             p.faces = _ptr(faces)
             for i, v in enumerate(prob.traction):
                 p.traction[i] = v
+            lin = getattr(prob, "meta", {}).get("traction_linear")
+            if lin is not None:
+                p.tr_kind, p.tr_axis, p.tr_x0 = 1, int(lin[0]), float(lin[1])
         p.residual = int(residual)
         p.nthreads = nthreads
+        if getattr(prob, "deg", 1) == 2:
+            fconn = np.ascontiguousarray(prob.fconn, np.int32)
+            self.keep.append(fconn)
+            p.deg = 2
+            p.fnpe = fconn.shape[1]
+            p.fconn = _ptr(fconn)
+            if prob.ffaces is not None and len(prob.ffaces):
+                ff = np.ascontiguousarray(prob.ffaces, np.int32)
+                self.keep.append(ff)
+                p.ffn = ff.shape[1]
+                p.ffaces = _ptr(ff)
         self.p = p
 
     def ref(self):
@@ -153,7 +172,8 @@ def assemble(prob, rp=None, ci=None, **kw):
 
 def element(prob, e, **kw):
     op = OracleProblem(prob, **kw)
-    nee = prob.conn.shape[1] * prob.dpn
+    fc = prob.fconn if getattr(prob, "deg", 1) == 2 else prob.conn
+    nee = fc.shape[1] * prob.dpn
     Ke = np.zeros(nee * nee, np.float64)
     Fe = np.zeros(nee, np.float64)
     lib().orc_element(op.ref(), e, Ke, Fe)
@@ -189,6 +209,14 @@ def q1_shape(dim, xi):
     phi = np.zeros(2 ** dim)
     dphi = np.zeros(2 ** dim * dim)
     lib().orc_q1_shape(dim, np.ascontiguousarray(xi, np.float64), phi, dphi)
+    return phi, dphi.reshape(-1, dim)
+
+
+def lagrange_shape(dim, deg, xi):
+    nf = (deg + 1) ** dim
+    phi = np.zeros(nf)
+    dphi = np.zeros(nf * dim)
+    lib().orc_lagrange_shape(dim, deg, np.ascontiguousarray(xi, np.float64), phi, dphi)
     return phi, dphi.reshape(-1, dim)
 
 

@@ -1,0 +1,236 @@
+"""CPU tests of the degree-2 (Q2 field on Q1 geometry) part of the oracle and
+of the host-side Q2 DoF numbering.
+
+Pins: the reference's DoF-handler sparsity of a Q2 field on square_20.smf
+(reference/03-doFHandler/sparsity.2.ref.dat), and the Newton histories of the
+Q2 compressible pulled sheet under displacement and traction control
+(reference/06-elastic/compRefOut{D,N}.dat), besides closed-form properties of
+the shape functions and element matrices."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from insilico_amd import problems as P
+
+
+def _support_points(dim):
+    """Lagrange points of the hierarchic Q2 element from its n-faces:
+    vertices in hierarchic order, edge midpoints in edge-table order, face
+    centres in face-table order, then the cell centre."""
+    if dim == 2:
+        V = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+        groups = [P.QUAD_EDGES]
+    else:
+        V = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                      [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+        groups = [P.HEX_EDGES, P.HEX_FACES]
+    pts = [V]
+    for tab in groups:
+        pts.append(V[tab].mean(axis=1))
+    pts.append(V.mean(axis=0)[None, :])
+    return np.concatenate(pts)
+
+
+def test_degree1_equals_q1(oracle):
+    rng = np.random.default_rng(3)
+    for dim in (2, 3):
+        for _ in range(5):
+            xi = rng.random(dim)
+            a, da = oracle.q1_shape(dim, xi)
+            b, db = oracle.lagrange_shape(dim, 1, xi)
+            assert np.array_equal(a, b) and np.array_equal(da, db)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q2_shape_invariants(oracle, dim):
+    """Kronecker property at the hierarchic support points, partition of
+    unity, reproduction of x and x^2, gradients by central differences."""
+    S = _support_points(dim)
+    assert S.shape[0] == 3 ** dim
+    Phi = np.array([oracle.lagrange_shape(dim, 2, s)[0] for s in S])
+    assert np.allclose(Phi, np.eye(3 ** dim), atol=1e-15)
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    for _ in range(4):
+        xi = rng.random(dim)
+        phi, dphi = oracle.lagrange_shape(dim, 2, xi)
+        assert abs(phi.sum() - 1.0) < 1e-14
+        assert np.allclose(dphi.sum(axis=0), 0.0, atol=1e-13)
+        for d in range(dim):
+            assert abs(phi @ S[:, d] - xi[d]) < 1e-14
+            assert abs(phi @ S[:, d] ** 2 - xi[d] ** 2) < 1e-14
+            e = np.zeros(dim)
+            e[d] = h
+            fd = (oracle.lagrange_shape(dim, 2, xi + e)[0]
+                  - oracle.lagrange_shape(dim, 2, xi - e)[0]) / (2 * h)
+            assert np.allclose(fd, dphi[:, d], atol=1e-8)
+
+
+def test_q2_numbering_golden_sparsity2(oracle, golden):
+    """reference/03-doFHandler/sparsity.2.ref.dat: DoF-object connectivity of
+    a Q2 field on square_20.smf (vertex, edge, cell passes)."""
+    xyz, conn = P.read_smf(os.path.join(golden, "square_20.smf"), 2)
+    fconn, fxyz = P.q2_field(xyz, conn, 2)
+    nf = fxyz.shape[0]
+    assert nf == 441 + 840 + 400
+    prob = P.Problem(2, xyz, conn, 1, np.arange(nf, dtype=np.int32), nf, np.zeros(nf),
+                     P.LAPLACE, np.ones((conn.shape[0], 1)), deg=2, fconn=fconn, fxyz=fxyz)
+    rp, ci = oracle.pattern(prob)
+    ref = np.loadtxt(os.path.join(golden, "sparsity.2.ref.dat"), dtype=np.int64)
+    rows = np.repeat(np.arange(nf), np.diff(rp))
+    assert np.array_equal(np.stack([rows, ci], 1), ref)
+
+
+def test_q2_numbering_3d_counts():
+    """HEX27 on an n^3 cube: (2n+1)^3 field nodes, every vertex/edge/face
+    shared consistently (each field node's support point is unique)."""
+    n = 3
+    xyz, conn = P.unit_cube(3, n)
+    fconn, fxyz = P.q2_field(xyz, conn, 3)
+    assert fxyz.shape[0] == (2 * n + 1) ** 3
+    key = np.round(fxyz * 2 * n).astype(np.int64)
+    assert np.unique(key, axis=0).shape[0] == fxyz.shape[0]
+    # each element's field nodes sit at its Lagrange points
+    S = _support_points(3)
+    for e in (0, 7, conn.shape[0] - 1):
+        x0 = xyz[conn[e, 0]]
+        assert np.allclose(fxyz[fconn[e]], x0 + S / n)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q2_laplace_element_closed_form(oracle, dim):
+    """Affine element, 3-point rule (exact for Q2 stiffness): zero row sums,
+    symmetry, and u^T K u = int |grad u|^2 for a quadratic u."""
+    h = np.array([0.5, 0.25, 2.0])[:dim]
+    V = _support_points(dim)[: 2 ** dim] * h
+    S = _support_points(dim) * h
+    conn = np.arange(2 ** dim, dtype=np.int32)[None, :]
+    fconn = np.arange(3 ** dim, dtype=np.int32)[None, :]
+    nf = 3 ** dim
+    prob = P.Problem(dim, V, conn, 1, np.arange(nf, dtype=np.int32), nf, np.zeros(nf),
+                     P.LAPLACE, np.full((1, 1), 1.7), nq=3, deg=2, fconn=fconn, fxyz=S)
+    K, _ = oracle.element(prob, 0)
+    assert np.allclose(K, K.T, atol=1e-13)
+    assert np.allclose(K.sum(axis=1), 0.0, atol=1e-12)
+    # u = x^2 + x y (+ z^2): grad = (2x + y, x (, 2z))
+    x = S[:, 0]
+    y = S[:, 1]
+    u = x * x + x * y + (S[:, 2] ** 2 if dim == 3 else 0.0)
+    # exact integral of |grad u|^2 over the box by tensor Gauss (3 points, exact)
+    w, xi = oracle.tensor_rule(dim, 3)
+    X = xi * h
+    gx = 2 * X[:, 0] + X[:, 1]
+    gy = X[:, 0]
+    g2 = gx ** 2 + gy ** 2 + ((2 * X[:, 2]) ** 2 if dim == 3 else 0.0)
+    exact = 1.7 * np.sum(w * g2) * np.prod(h)
+    assert abs(u @ K @ u - exact) < 1e-12 * exact
+
+
+@pytest.mark.parametrize("kernel", [P.STVENANT, P.NEOHOOKE])
+def test_q2_elastic_element_rigid_modes_and_fd(oracle, kernel):
+    """Q2 HEX27 on a distorted Q1 hexahedron: translations and the linearised
+    rotations are in the kernel of the tangent at u = 0; the tangent is the
+    derivative of the residual (central differences at a random u)."""
+    rng = np.random.default_rng(11)
+    V = _support_points(3)[:8] + 0.1 * rng.standard_normal((8, 3))
+    conn = np.arange(8, dtype=np.int32)[None, :]
+    fconn = np.arange(27, dtype=np.int32)[None, :]
+    # field support points on the distorted geometry (Q1 map of the Lagrange points)
+    S = np.array([oracle.q1_shape(3, s)[0] @ V for s in _support_points(3)])
+    lam, mu = P.lame(100.0, 0.3)
+    prob = P.Problem(3, V, conn, 3, np.arange(81, dtype=np.int32), 81, np.zeros(81),
+                     kernel, np.array([[lam, mu]]), nq=2, deg=2, fconn=fconn, fxyz=S)
+    K, _ = oracle.element(prob, 0)
+    assert np.allclose(K, K.T, atol=1e-9 * abs(K).max())
+    modes = [np.tile(np.eye(3)[d], 27) for d in range(3)]
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        m = np.zeros((27, 3))
+        m[:, a] = -S[:, b]
+        m[:, b] = S[:, a]
+        modes.append(m.ravel())
+    for m in modes:
+        assert np.abs(K @ m).max() < 1e-9 * abs(K).max()
+    u = 0.01 * rng.standard_normal(81)
+    Ku, _ = oracle.element(prob, 0, u=u)
+    h = 1e-6
+    for j in (0, 40, 80):
+        e = np.zeros(81)
+        e[j] = h
+        _, fp = oracle.element(prob, 0, u=u + e, residual=True)
+        _, fm = oracle.element(prob, 0, u=u - e, residual=True)
+        # Fe = -F_int, so dFe/du = -K
+        fd = -(fp - fm) / (2 * h)
+        assert np.allclose(fd, Ku[:, j], atol=1e-6 * abs(Ku).max())
+
+
+def _newton_oracle(oracle, disp_controlled):
+    """reference/06-elastic/compressible.cpp:245-335 driven by the oracle:
+    5 load steps, ramp (step+1)/step of the prescribed u_x or of the
+    traction, full Newton with incremental lifting, direct solves,
+    |F| = ||b||/N and |x| = ||dx||/N (Eigen3::norm)."""
+    prob = P.pulled_sheet_q2(20, disp_controlled)
+    n, dim = prob.n_active, prob.dim
+    u = np.zeros(prob.n_fnodes * dim)
+    g = np.zeros_like(u)
+    con = prob.dof_id < 0
+    act = ~con
+    E, tol, steps = 1000.0, 1e-12, 5
+    hist = []
+    for step in range(steps):
+        if disp_controlled:
+            g[prob.meta["right_x"] * dim] = 3.0 / steps * (step + 1)
+        else:
+            prob.traction = (300.0 * (step + 1) / steps, 0.0, 0.0)
+        prob.g = g.copy()
+        for it in range(30):
+            rp, ci, val, b, _ = oracle.assemble(prob, u=u, incremental=True, residual=True)
+            c1 = np.linalg.norm(b) / n
+            if c1 < tol * E:
+                hist.append((step, it, c1, None))
+                break
+            x = spla.spsolve(sp.csr_matrix((val, ci, rp), shape=(n, n)).tocsc(), b)
+            u[act] += x[prob.dof_id[act]]
+            u[con] = g[con]
+            c2 = np.linalg.norm(x) / n
+            hist.append((step, it, c1, c2))
+            if c2 < tol:
+                break
+    return n, hist
+
+
+def _read_compref(path):
+    rows = []
+    with open(path) as fh:
+        head = fh.readline()
+        ndof = int(head.split()[-1])
+        for ln in fh:
+            if ln.startswith("#") or not ln.strip():
+                continue
+            v = ln.split()
+            rows.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]) if len(v) > 3 else None))
+    return ndof, rows
+
+
+@pytest.mark.parametrize("mode", ["D", "N"])
+def test_q2_newton_golden_compref(oracle, golden, mode):
+    """Every printed |F| and |x| of compRefOut{D,N}.dat to the printed digits
+    (5e-3 relative for |F| in [1e-12, 1e-10), next to the round-off floor;
+    below 1e-12 only the convergence test against tolerance*E = 1e-9)."""
+    ndof, ref = _read_compref(os.path.join(golden, "compRefOut%s.dat" % mode))
+    n, hist = _newton_oracle(oracle, mode == "D")
+    assert n == ndof
+    assert [(s, i) for s, i, _, _ in hist] == [(s, i) for s, i, _, _ in ref]
+    for (s, i, f, x), (_, _, fr, xr) in zip(hist, ref):
+        if fr >= 1e-10:
+            assert abs(f - fr) <= 1e-5 * fr, (s, i, f, fr)
+        elif fr >= 1e-12:
+            assert abs(f - fr) <= 5e-3 * fr, (s, i, f, fr)
+        else:
+            assert f < 1e-9
+        assert (x is None) == (xr is None)
+        if x is not None:
+            assert abs(x - xr) <= (1e-5 if xr > 1e-10 else 1e-3) * xr, (s, i, x, xr)
