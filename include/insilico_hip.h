@@ -126,6 +126,18 @@ int ins_sync(ins_ctx* ctx);
 int ins_set_mesh(ins_ctx* ctx, int dim, int64_t n_nodes, const double* xyz,
                  int64_t n_elem, int npe, const int32_t* conn);
 
+/* Field of degree 2 on the Q1 mesh just set (base::fe::Basis<shape,2> of a
+ * Lagrange field on a geomDeg-1 mesh, e.g. reference/06-elastic/
+ * compressible.cpp:108-110): field_conn[e*field_npe + k] = field node (DoF
+ * object) k of element e in hierarchic order, field_npe = 9 (QUAD9) or 27
+ * (HEX27), numbered as base/dof/IndexMap.hpp:222-280 with
+ * base/dof/generateDoFIndicesFromFaces.hpp does (vertices, edges, faces,
+ * cells).  Call after ins_set_mesh; the DoFs, prescribed and current values
+ * are then indexed by field node.  Neumann faces are not supported on a
+ * degree-2 field. */
+int ins_set_field(ins_ctx* ctx, int degree, int64_t n_field_nodes, int field_npe,
+                  const int32_t* field_conn);
+
 /* DoFs: dof_id[node*dpn+c] = active row index (base/dof/numbering.hpp:45-66)
  * or -1 for a CONSTRAINED component (base/dof/DegreeOfFreedom.hpp:232-242).
  * Ids must be a permutation of 0..n_active-1. */

@@ -34,7 +34,7 @@ SYMBOLS = [
     "ins_rhs_norm", "ins_set_csr", "ins_solve", "ins_get_solution", "ins_solution_norm",
     "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
-    "ins_server_running",
+    "ins_server_running", "ins_set_field",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 
@@ -77,6 +77,7 @@ def load():
         "ins_set_profiling": [vp, i32],
         "ins_sync": [vp],
         "ins_set_mesh": [vp, i32, i64, vp, i64, i32, vp],
+        "ins_set_field": [vp, i32, i64, i32, vp],
         "ins_set_dofs": [vp, i32, vp, i64],
         "ins_set_prescribed": [vp, vp],
         "ins_scale_prescribed": [vp, dbl],
@@ -163,6 +164,12 @@ class Context:
         self._chk(self.L.ins_set_mesh(self.h, xyz.shape[1], xyz.shape[0], _p(xyz),
                                       conn.shape[0], conn.shape[1], _p(conn)))
         self.n_nodes, self.dim = xyz.shape
+
+    def set_field(self, degree, n_field_nodes, fconn):
+        """degree-2 field on the mesh just set (field nodes in hierarchic order)"""
+        fc = np.ascontiguousarray(fconn, np.int32)
+        self._chk(self.L.ins_set_field(self.h, degree, n_field_nodes, fc.shape[1], _p(fc)))
+        self.n_nodes = n_field_nodes
 
     def set_dofs(self, dpn, dof_id, n_active):
         d = np.ascontiguousarray(dof_id, np.int32)
@@ -293,6 +300,8 @@ class Context:
     def load_problem(self, prob):
         """Mesh, DoFs, prescribed values, kernel and loads of a Problem."""
         self.set_mesh(prob.xyz, prob.conn)
+        if getattr(prob, "deg", 1) == 2:
+            self.set_field(2, prob.n_fnodes, prob.fconn)
         self.set_dofs(prob.dpn, prob.dof_id, prob.n_active)
         self.set_prescribed(prob.g)
         self.set_material(prob.kernel, prob.nq, prob.coef)

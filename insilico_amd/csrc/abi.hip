@@ -62,7 +62,7 @@ static int download(Ctx* c, T* h, const DArr<T>& a, size_t n)
 
 static void release_all(Ctx* c)
 {
-    release(c, c->xyz); release(c, c->conn); release(c, c->dof_id); release(c, c->rowmap);
+    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->dof_id); release(c, c->rowmap);
     release(c, c->g); release(c, c->u); release(c, c->adj_ptr); release(c, c->adj);
     release(c, c->nb_ptr); release(c, c->nb); release(c, c->nboff); release(c, c->row_ptr);
     release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
@@ -221,11 +221,46 @@ int ins_set_mesh(ins_ctx* h, int dim, int64_t n_nodes, const double* xyz, int64_
     c->lg_npe = (npe == 8) ? 3 : 2;
     c->n_nodes = n_nodes;
     c->n_elem = n_elem;
+    c->deg = 1;
+    c->gnpe = npe;
+    c->n_gnodes = n_nodes;
     INS_TRY(upload(c, c->xyz, xyz, (size_t)n_nodes * dim));
     INS_TRY(upload(c, c->conn, conn, (size_t)n_elem * npe));
     invalidate_pattern(c);
     c->have_dofs = false;
     c->n_faces = 0;
+    return INS_OK;
+}
+
+int ins_set_field(ins_ctx* h, int degree, int64_t n_field_nodes, int field_npe,
+                  const int32_t* field_conn)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (c->n_elem == 0) return fail(c, INS_ERR_STATE, "ins_set_field before ins_set_mesh");
+    if (degree != 2) return fail(c, INS_ERR_UNSUPPORTED, "only fields of degree 2 on a Q1 mesh");
+    const int want = (c->dim == 2) ? 9 : 27;
+    if (field_npe != want || n_field_nodes <= 0 || !field_conn)
+        return fail(c, INS_ERR_ARG, "ins_set_field: field_npe must be 9 (2-D) or 27 (3-D)");
+    if (c->deg == 2) return fail(c, INS_ERR_STATE, "ins_set_field called twice for one mesh");
+    if (n_field_nodes > INT32_MAX || c->n_elem * field_npe > INT32_MAX ||
+        c->n_elem > (INT32_MAX >> 5))
+        return fail(c, INS_ERR_UNSUPPORTED, "field exceeds int32 indexing");
+    for (int64_t i = 0; i < c->n_elem * field_npe; i++)
+        if (field_conn[i] < 0 || field_conn[i] >= n_field_nodes)
+            return fail(c, INS_ERR_RANGE, "field connectivity refers to a node out of range");
+    if (c->n_faces > 0) return fail(c, INS_ERR_UNSUPPORTED, "Neumann faces on a degree-2 field");
+    // the Q1 connectivity becomes the geometry
+    std::swap(c->gconn, c->conn);
+    c->gnpe = c->npe;
+    c->n_gnodes = c->n_nodes;
+    c->deg = 2;
+    c->npe = field_npe;
+    c->lg_npe = ceil_lg2(field_npe);
+    c->n_nodes = n_field_nodes;
+    INS_TRY(upload(c, c->conn, field_conn, (size_t)c->n_elem * field_npe));
+    invalidate_pattern(c);
+    c->have_dofs = false;
     return INS_OK;
 }
 
@@ -386,6 +421,7 @@ int ins_set_traction(ins_ctx* h, int64_t n_faces, int nfn, const int32_t* faces,
     }
     if (n_faces < 0 || !faces || !traction || nfn != (1 << (c->dim - 1)))
         return fail(c, INS_ERR_ARG, "ins_set_traction: bad arguments");
+    if (c->deg == 2) return fail(c, INS_ERR_UNSUPPORTED, "Neumann faces on a degree-2 field");
     for (int64_t i = 0; i < n_faces * nfn; i++)
         if (faces[i] < 0 || faces[i] >= c->n_nodes)
             return fail(c, INS_ERR_RANGE, "face refers to a node out of range");
@@ -393,8 +429,8 @@ int ins_set_traction(ins_ctx* h, int64_t n_faces, int nfn, const int32_t* faces,
     c->nfn = nfn;
     for (int i = 0; i < 3; i++) c->traction[i] = i < c->dim ? traction[i] : 0.;
     INS_TRY(upload(c, c->faces, faces, (size_t)n_faces * nfn));
-    INS_TRY(build_adjacency(c, c->faces.p, n_faces, nfn, c->n_nodes, c->fadj_ptr, c->fadj,
-                            nullptr));
+    INS_TRY(build_adjacency(c, c->faces.p, n_faces, nfn, ceil_lg2(nfn), c->n_nodes,
+                            c->fadj_ptr, c->fadj, nullptr));
     return INS_OK;
 }
 

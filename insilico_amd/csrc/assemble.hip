@@ -519,6 +519,411 @@ __global__ __launch_bounds__(256) void k_elem_elastic(
 }
 
 // ---------------------------------------------------------------------------
+// Degree-2 Lagrange fields on Q1 geometry (QUAD9 / HEX27 on QUAD4 / HEX8).
+// Table layout: geometry dphi [NQ][GN][DIM], geometry phi [NQ][GN], field
+// dphi [NQ][NF][DIM], field phi [NQ][NF], weights [NQ].
+template <int DIM, int NQ>
+struct TabQ2 {
+    static constexpr int GN = 1 << DIM;
+    static constexpr int NF = IPow<3, DIM>::value;
+    static constexpr int GD = 0;
+    static constexpr int GP = GD + NQ * GN * DIM;
+    static constexpr int FD = GP + NQ * GN;
+    static constexpr int FP = FD + NQ * NF * DIM;
+    static constexpr int W = FP + NQ * NF;
+    static constexpr int SIZE = W + NQ;
+};
+
+// J from the Q1 geometry, inv(J^T) and detJ, with the arithmetic of phys_grad
+// (base/geometry.hpp:143-177,420-445)
+template <int DIM, typename T>
+__device__ inline T geo_inverse(const T* X, const T* gd, T (&inv)[DIM][DIM])
+{
+    constexpr int GN = 1 << DIM;
+    T J[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int a = 0; a < DIM; a++) J[i][a] = (T)0;
+#pragma unroll
+    for (int n = 0; n < GN; n++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int a = 0; a < DIM; a++) J[i][a] += X[n * DIM + i] * gd[n * DIM + a];
+    T m[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int a = 0; a < DIM; a++) m[i][a] = J[a][i];
+    return inverse_det<DIM, T>(m, inv);
+}
+
+// 16x16x4 matrix-core products of the element kernels: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; element r of the result holds column l%16
+// of row row(l, r): 4*(l/16) + r for FP32, l/16 + 4*r for FP64 (measured on
+// gfx950 with a rank-one product)
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 op(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 op(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// heat::Laplace on a degree-2 field as one matrix product per element:
+// with G[(q,i)][M] = (grad_x phi_M)_i at quadrature point q and s_q =
+// kappa detJ_q w_q, Ke = G^T diag(s) G (the sum over q and i of
+// base/kernel/Laplace.hpp:100-151).  One wavefront per element: lanes
+// < NQ build inv(J^T) and s_q, the wave writes G (NQ*DIM x NF, zero-padded
+// to 16-wide tiles) into LDS, and the upper tiles of Ke come from 16x16x4
+// matrix-core steps over the NQ*DIM rows of G.  The symmetric half is
+// written element-major, as the Q1 kernel does.
+template <int DIM, int NQ1, typename T>
+struct LapQ2 {
+    static constexpr int NQ = IPow<NQ1, DIM>::value;
+    static constexpr int GN = 1 << DIM;
+    static constexpr int NF = IPow<3, DIM>::value;
+    static constexpr int KD = NQ * DIM;
+    static constexpr int KP = (KD + 3) / 4 * 4;
+    static constexpr int NT = (NF + 15) / 16;
+    static constexpr int NFP = NT * 16;
+    static constexpr int NK = NF * (NF + 1) / 2;
+    static constexpr int G = 0;                  // [KP][NFP]
+    static constexpr int KS = G + KP * NFP;      // [NFP][NFP]
+    static constexpr int INV = KS + NFP * NFP;   // [NQ][DIM][DIM]
+    static constexpr int SC = INV + NQ * DIM * DIM;
+    static constexpr int DET = SC + NQ;
+    static constexpr int FQ = DET + NQ;
+    static constexpr int X = FQ + NQ;            // [GN][DIM]
+    static constexpr int SIZE = X + GN * DIM;
+    static constexpr int EPB = 1;                // elements (wavefronts) per block
+};
+
+template <int DIM, int NQ1, typename T>
+__global__ __launch_bounds__(64) void k_elem_laplace_q2(
+    int64_t E, const int32_t* __restrict__ gconn, const double* __restrict__ xyz,
+    const double* __restrict__ coef, int ncoef, const T* __restrict__ tab, Bf bf,
+    T* __restrict__ ke, double* __restrict__ fe)
+{
+    using L = LapQ2<DIM, NQ1, T>;
+    using TB = TabQ2<DIM, L::NQ>;
+    constexpr int NQ = L::NQ, GN = L::GN, NF = L::NF, NFP = L::NFP, EPB = L::EPB;
+    // the tables stay in global memory (a few KB, cache-resident): the LDS
+    // holds G and the element matrix
+    const T* __restrict__ s_tab = tab;
+    __shared__ T s_all[EPB * L::SIZE];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    T* sm = s_all + wave * L::SIZE;
+    for (int64_t base = (int64_t)blockIdx.x * EPB; base < E; base += (int64_t)gridDim.x * EPB) {
+        const int64_t e = base + wave;
+        const bool valid = e < E;
+        if (valid && lane < GN * DIM) {
+            const int64_t node = gconn[e * GN + lane / DIM];
+            sm[L::X + lane] = (T)xyz[node * DIM + lane % DIM];
+        }
+        __syncthreads();
+        if (valid && lane < NQ) {
+            const int q = lane;
+            T inv[DIM][DIM];
+            const T det = geo_inverse<DIM, T>(sm + L::X, s_tab + TB::GD + q * GN * DIM, inv);
+#pragma unroll
+            for (int i = 0; i < DIM; i++)
+#pragma unroll
+                for (int a = 0; a < DIM; a++) sm[L::INV + (q * DIM + i) * DIM + a] = inv[i][a];
+            const T w = s_tab[TB::W + q];
+            sm[L::SC + q] = (T)coef[e * ncoef] * det * w;
+            sm[L::DET + q] = det;
+            if (fe && bf.kind != INS_BF_NONE) {
+                T x[3] = {0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < GN; n++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) x[d] += sm[L::X + n * DIM + d] * s_tab[TB::GP + q * GN + n];
+                sm[L::FQ + q] = body_force<T>(bf, x, DIM, 0);
+            }
+        }
+        __syncthreads();
+        // G[(q,i)][M] = sum_a inv_q[i][a] dphi_M[a] (zero in the padding)
+        if (valid) {
+            for (int idx = lane; idx < L::KP * NFP; idx += 64) {
+                const int k = idx / NFP, M = idx - (idx / NFP) * NFP;
+                T v = (T)0;
+                if (k < L::KD && M < NF) {
+                    const int q = k / DIM, i = k - q * DIM;
+                    const T* fd = s_tab + TB::FD + (q * NF + M) * DIM;
+                    const T* iv = sm + L::INV + (q * DIM + i) * DIM;
+#pragma unroll
+                    for (int a = 0; a < DIM; a++) v += iv[a] * fd[a];
+                }
+                sm[L::G + idx] = v;
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+            for (int ti = 0; ti < L::NT; ti++)
+#pragma unroll
+                for (int tj = ti; tj < L::NT; tj++) {
+                    typename Mfma<T>::v4 acc = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll 4
+                    for (int s0 = 0; s0 < L::KP; s0 += 4) {
+                        const int k = s0 + k4;
+                        const int q = k < L::KD ? k / DIM : 0;
+                        const T a = sm[L::G + k * NFP + ti * 16 + r16];
+                        const T b = sm[L::G + k * NFP + tj * 16 + r16] * sm[L::SC + q];
+                        acc = Mfma<T>::op(a, b, acc);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        sm[L::KS + (ti * 16 + Mfma<T>::row(lane, r)) * NFP + tj * 16 + r16] = acc[r];
+                }
+        }
+        __syncthreads();
+        if (valid) {
+            T* dst = ke + e * (int64_t)L::NK;
+            int off = 0;
+            for (int M = 0; M < NF; M++) {
+                for (int N = M + lane; N < NF; N += 64) dst[off + N - M] = sm[L::KS + M * NFP + N];
+                off += NF - M;
+            }
+            if (fe && lane < NF) {
+                T fb = (T)0;
+                if (bf.kind != INS_BF_NONE)
+                    for (int q = 0; q < NQ; q++)
+                        fb += sm[L::FQ + q] * s_tab[TB::FP + q * NF + lane] * s_tab[TB::W + q] *
+                              sm[L::DET + q];
+                fe[e * NF + lane] = (double)fb;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// solid::HyperElastic on a degree-2 field (solid/HyperElastic.hpp:110-179
+// tangent, :210-257 residual, :282-310 effective elasticity), the Q1 kernel's
+// three phases with NF field functions on the Q1 geometry: lanes < NQ build
+// the geometry, F, S and C of their point; the wave forms A_iJkL; lane pairs
+// (M,N) accumulate the S x S blocks.
+template <int DIM, int NQ1>
+struct ElQ2 {
+    static constexpr int GN = 1 << DIM;
+    static constexpr int NF = IPow<3, DIM>::value;
+    static constexpr int NQ = IPow<NQ1, DIM>::value;
+    static constexpr int S = DIM;
+    static constexpr int NEE = NF * S;
+    static constexpr int NA = S * S * S * S;
+    static constexpr int X = 0;                 // [GN][3]
+    static constexpr int U = X + GN * 3;        // [NF][3]
+    static constexpr int GX = U + NF * 3;       // [NQ][NF][DIM]
+    static constexpr int DW = GX + NQ * NF * DIM;
+    static constexpr int F = DW + NQ;           // [NQ][9]
+    static constexpr int SS = F + NQ * 9;       // [NQ][9]
+    static constexpr int C = SS + NQ * 9;       // [NQ][36]
+    static constexpr int A = C + NQ * 36;       // [NQ][NA]
+    static constexpr int P = A + NQ * NA;       // [NQ][9] first Piola-Kirchhoff
+    static constexpr int FQ = P + NQ * 9;       // [NQ][3] body force at the point
+    static constexpr int DET = FQ + NQ * 3;     // [NQ]
+    static constexpr int SIZE = DET + NQ;
+};
+
+template <int DIM, int NQ1, typename T, int MAT, bool WANT_K>
+__global__ __launch_bounds__(64) void k_elem_elastic_q2(
+    int64_t E, const int32_t* __restrict__ gconn, const int32_t* __restrict__ fconn,
+    const double* __restrict__ xyz, const double* __restrict__ coef,
+    const double* __restrict__ u, int ncoef, const T* __restrict__ tab, Bf bf, int want_resid,
+    T* __restrict__ ke, double* __restrict__ fe)
+{
+    using L = ElQ2<DIM, NQ1>;
+    using TB = TabQ2<DIM, L::NQ>;
+    constexpr int GN = L::GN, NF = L::NF, NQ = L::NQ, S = L::S, NEE = L::NEE;
+    const T* __restrict__ s_tab = tab;   // tables read through the cache
+    __shared__ T sm[L::SIZE];
+    const int lane = threadIdx.x;
+    for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
+        if (lane < GN * 3) {
+            const int n = lane / 3, d = lane % 3;
+            const int64_t node = gconn[e * GN + n];
+            sm[L::X + lane] = d < DIM ? (T)xyz[node * DIM + d] : (T)0;
+        }
+        for (int i = lane; i < NF * 3; i += 64) {
+            const int n = i / 3, d = i % 3;
+            const int64_t node = fconn[e * NF + n];
+            sm[L::U + i] = (d < S && u) ? (T)u[node * S + d] : (T)0;
+        }
+        __syncthreads();
+        if (lane < NQ) {
+            const int q = lane;
+            T Xg[GN * DIM];
+#pragma unroll
+            for (int n = 0; n < GN; n++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) Xg[n * DIM + d] = sm[L::X + n * 3 + d];
+            T inv[DIM][DIM];
+            const T det = geo_inverse<DIM, T>(Xg, s_tab + TB::GD + q * GN * DIM, inv);
+            const T w = s_tab[TB::W + q];
+            const T dw = det * w;
+            sm[L::DW + q] = dw;
+            sm[L::DET + q] = det;
+            T GU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int f = 0; f < NF; f++) {
+                const T* fd = s_tab + TB::FD + (q * NF + f) * DIM;
+                T g[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; i++) {
+                    T s = (T)0;
+#pragma unroll
+                    for (int a = 0; a < DIM; a++) s += inv[i][a] * fd[a];
+                    g[i] = s;
+                    sm[L::GX + (q * NF + f) * DIM + i] = s;
+                }
+#pragma unroll
+                for (int J = 0; J < DIM; J++)
+#pragma unroll
+                    for (int i = 0; i < S; i++) GU[J][i] += g[J] * sm[L::U + f * 3 + i];
+            }
+            T F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+            for (int J = 0; J < DIM; J++)
+#pragma unroll
+                for (int i = 0; i < S; i++) F[i][J] += GU[J][i];
+            const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
+            T Sm[3][3], Cm[6][6];
+            material<MAT, T>(lam, mu, F, Sm, Cm, WANT_K);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    sm[L::F + q * 9 + i * 3 + j] = F[i][j];
+                    sm[L::SS + q * 9 + i * 3 + j] = Sm[i][j];
+                }
+            if (WANT_K) {
+#pragma unroll
+                for (int i = 0; i < 6; i++)
+#pragma unroll
+                    for (int j = 0; j < 6; j++) sm[L::C + q * 36 + i * 6 + j] = Cm[i][j];
+            }
+            if (want_resid) {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int J = 0; J < 3; J++) {
+                        T t = (T)0;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) t += F[i][k] * Sm[k][J];
+                        sm[L::P + q * 9 + i * 3 + J] = t;
+                    }
+            }
+            if (bf.kind != INS_BF_NONE) {
+                T x[3] = {0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < GN; n++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) x[d] += Xg[n * DIM + d] * s_tab[TB::GP + q * GN + n];
+#pragma unroll
+                for (int c = 0; c < S; c++) sm[L::FQ + q * 3 + c] = body_force<T>(bf, x, DIM, c);
+            }
+        }
+        __syncthreads();
+        if (WANT_K) {
+            for (int idx = lane; idx < NQ * L::NA; idx += 64) {
+                const int q = idx / L::NA;
+                int r = idx % L::NA;
+                const int L_ = r % S; r /= S;
+                const int k = r % S; r /= S;
+                const int J = r % S; r /= S;
+                const int i = r;
+                const T* F = sm + L::F + q * 9;
+                const T* Cq = sm + L::C + q * 36;
+                T res = (i == k) ? sm[L::SS + q * 9 + J * 3 + L_] : (T)0;
+#pragma unroll
+                for (int A = 0; A < S; A++) {
+                    const int v1 = voigt_idx(A, J);
+#pragma unroll
+                    for (int B = 0; B < S; B++)
+                        res += F[i * 3 + A] * Cq[v1 * 6 + voigt_idx(B, L_)] * F[k * 3 + B];
+                }
+                sm[L::A + idx] = res;
+            }
+            __syncthreads();
+            for (int pr = lane; pr < NF * NF; pr += 64) {
+                const int M = pr / NF, N = pr % NF;
+                T K[S][S];
+#pragma unroll
+                for (int i = 0; i < S; i++)
+#pragma unroll
+                    for (int k = 0; k < S; k++) K[i][k] = (T)0;
+                for (int q = 0; q < NQ; q++) {
+                    T gM[S], gN[S];
+#pragma unroll
+                    for (int d = 0; d < S; d++) {
+                        gM[d] = sm[L::GX + (q * NF + M) * DIM + d];
+                        gN[d] = sm[L::GX + (q * NF + N) * DIM + d];
+                    }
+                    const T dw = sm[L::DW + q];
+                    const T* A = sm + L::A + q * L::NA;
+#pragma unroll
+                    for (int i = 0; i < S; i++)
+#pragma unroll
+                        for (int k = 0; k < S; k++) {
+                            T s = (T)0;
+#pragma unroll
+                            for (int J = 0; J < S; J++)
+#pragma unroll
+                                for (int L_ = 0; L_ < S; L_++)
+                                    s += gM[J] * A[((i * S + J) * S + k) * S + L_] * gN[L_];
+                            s *= dw;
+                            K[i][k] += s;
+                        }
+                }
+                T* dst = ke + e * (int64_t)(NEE * NEE) + (M * NF + N) * S * S;
+#pragma unroll
+                for (int i = 0; i < S; i++)
+#pragma unroll
+                    for (int k = 0; k < S; k++) dst[i * S + k] = K[i][k];
+            }
+        }
+        // element force vector, one DoF per lane, each sum over the points in
+        // order: body force f phi w detJ, internal force (P grad phi) detJ w
+        if (fe) {
+            for (int j = lane; j < NEE; j += 64) {
+                const int M = j / S, i = j - (j / S) * S;
+                T sb = (T)0, sr = (T)0;
+                if (bf.kind != INS_BF_NONE)
+                    for (int q = 0; q < NQ; q++)
+                        sb += sm[L::FQ + q * 3 + i] * s_tab[TB::FP + q * NF + M] * s_tab[TB::W + q] *
+                              sm[L::DET + q];
+                if (want_resid)
+                    for (int q = 0; q < NQ; q++) {
+                        T t = (T)0;
+#pragma unroll
+                        for (int J = 0; J < S; J++)
+                            t += sm[L::P + q * 9 + i * 3 + J] * sm[L::GX + (q * NF + M) * DIM + J];
+                        t *= sm[L::DW + q];
+                        sr += t;
+                    }
+                fe[e * NEE + j] = (double)sb - (double)sr;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // staged element-matrix accessors
 template <typename T, int NPE>
 struct KeLaplace {
@@ -551,9 +956,10 @@ __global__ __launch_bounds__(BLK) void k_gather(
     const int32_t* __restrict__ nb, const int32_t* __restrict__ nboff,
     const int32_t* __restrict__ dof_id, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const int32_t* __restrict__ conn,
-    const uint32_t* __restrict__ pinfo, int monotone, KE ke, double* __restrict__ v64,
-    float* __restrict__ v32, int* __restrict__ err)
+    const typename PairWord<NPE>::type* __restrict__ pinfo, int monotone, KE ke,
+    double* __restrict__ v64, float* __restrict__ v32, int* __restrict__ err)
 {
+    constexpr int BITS = PairWord<NPE>::bits, FLAG = PairWord<NPE>::flag;
     const int G = 1 << lgG;
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = gt >> lgG;
@@ -579,13 +985,13 @@ __global__ __launch_bounds__(BLK) void k_gather(
         // lanes of node n
         if (pinfo) {
             // precomputed at pattern set-up: one word per pair
-            const uint32_t w = pinfo[nbs + j];
+            const auto w = pinfo[nbs + j];
             for (int32_t i = 0; i < ae - as; i++) {
-                const uint32_t nib = (w >> (4 * i)) & 15u;
+                const int nib = (int)((w >> (BITS * i)) & (2 * FLAG - 1));
                 if (!nib) continue;
                 const int32_t xa = adj[as + i];
                 const int64_t e = xa >> lg_npe;
-                const int la = xa & mask, lb = (int)(nib & 7u);
+                const int la = xa & mask, lb = nib & (FLAG - 1);
 #pragma unroll
                 for (int cr = 0; cr < S; cr++)
 #pragma unroll
@@ -865,6 +1271,80 @@ void q1_shape(int dim, const double* xi, double* phi, double* dphi)
     }
 }
 
+// Lagrange shape functions of degree 2 in hierarchic order
+// (base/sfun/Lagrange1D.ipp:76-91 in 1-D, tensor products of
+// base/sfun/TensorProduct.hpp:225-320, reorder out[H[n]] = in[n] with the
+// table of base/mesh/HierarchicOrder.hpp:201-270 (QUAD) and :345-452 (HEX):
+// vertices, edges, faces, interior).
+void lagrange2_shape(int dim, const double* xi, double* phi, double* dphi)
+{
+    static const int H2[9] = {0, 4, 1, 7, 8, 5, 3, 6, 2};
+    static const int H3[27] = {0, 8, 1, 11, 20, 9, 3, 10, 2, 16, 22, 17, 25, 26, 23, 19, 24, 18,
+                               4, 12, 5, 15, 21, 13, 7, 14, 6};
+    double f[3][3], g[3][3];
+    for (int d = 0; d < dim; d++) {
+        const double x = xi[d];
+        f[d][0] = (1. - x) * (1. - 2. * x);
+        f[d][1] = 4. * x * (1. - x);
+        f[d][2] = x * (2. * x - 1.);
+        g[d][0] = 4. * x - 3.;
+        g[d][1] = 4. - 8. * x;
+        g[d][2] = 4. * x - 1.;
+    }
+    double lf[27], lg[27][3];
+    int m = 3;
+    for (int i = 0; i < 3; i++) {
+        lf[i] = f[0][i];
+        lg[i][0] = g[0][i];
+    }
+    for (int d = 1; d < dim; d++) {
+        double tf[27], tg[27][3];
+        int c = 0;
+        for (int o = 0; o < 3; o++)
+            for (int i = 0; i < m; i++) {
+                tf[c] = lf[i] * f[d][o];
+                for (int a = 0; a < d; a++) tg[c][a] = f[d][o] * lg[i][a];
+                tg[c][d] = g[d][o] * lf[i];
+                c++;
+            }
+        m = c;
+        for (int i = 0; i < m; i++) {
+            lf[i] = tf[i];
+            for (int a = 0; a <= d; a++) lg[i][a] = tg[i][a];
+        }
+    }
+    const int* H = dim == 2 ? H2 : H3;
+    for (int n = 0; n < m; n++) {
+        phi[H[n]] = lf[n];
+        for (int a = 0; a < dim; a++) dphi[H[n] * dim + a] = lg[n][a];
+    }
+}
+
+// degree-2 field table: geometry dphi, geometry phi, field dphi, field phi, w
+void fill_tables_q2(int dim, int nq1, std::vector<double>& t)
+{
+    std::vector<double> w, xi;
+    const int nq = tensor_rule(dim, nq1, w, xi);
+    const int gn = 1 << dim, nf = dim == 2 ? 9 : 27;
+    const size_t GD = 0, GP = GD + (size_t)nq * gn * dim, FD = GP + (size_t)nq * gn,
+                 FP = FD + (size_t)nq * nf * dim, W = FP + (size_t)nq * nf;
+    t.assign(W + nq, 0.);
+    for (int q = 0; q < nq; q++) {
+        double phi[8], dphi[24], fphi[27], fdphi[81];
+        q1_shape(dim, &xi[q * dim], phi, dphi);
+        lagrange2_shape(dim, &xi[q * dim], fphi, fdphi);
+        for (int n = 0; n < gn; n++) {
+            for (int d = 0; d < dim; d++) t[GD + (q * gn + n) * dim + d] = dphi[n * dim + d];
+            t[GP + q * gn + n] = phi[n];
+        }
+        for (int n = 0; n < nf; n++) {
+            for (int d = 0; d < dim; d++) t[FD + (q * nf + n) * dim + d] = fdphi[n * dim + d];
+            t[FP + q * nf + n] = fphi[n];
+        }
+        t[W + q] = w[q];
+    }
+}
+
 // volume table followed by the face table (dphi, phi, w on faces)
 void fill_tables(int dim, int nq1, std::vector<double>& t, int* face_off, int* nqf)
 {
@@ -914,6 +1394,47 @@ int launch_laplace(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
     return INS_OK;
 }
 
+template <int DIM, typename T>
+int launch_laplace_q2(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
+{
+    const unsigned g = grid_for(c->n_elem, 1, 1u << 22);
+    switch (c->nq) {
+    case 1: k_elem_laplace_q2<DIM, 1, T><<<g, 64, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 2: k_elem_laplace_q2<DIM, 2, T><<<g, 64, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 3: k_elem_laplace_q2<DIM, 3, T><<<g, 64, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
+    }
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
+
+template <int DIM, int NQ1, typename T, int MAT>
+int launch_elastic_q2_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+{
+    const unsigned g = grid_for(c->n_elem, 1, 1u << 22);
+    if (want_k)
+        k_elem_elastic_q2<DIM, NQ1, T, MAT, true><<<g, 64, 0, c->stream>>>(
+            c->n_elem, c->gconn.p, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf,
+            want_r, ke, fe);
+    else
+        k_elem_elastic_q2<DIM, NQ1, T, MAT, false><<<g, 64, 0, c->stream>>>(
+            c->n_elem, c->gconn.p, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf,
+            want_r, ke, fe);
+    INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
+
+template <int DIM, typename T, int MAT>
+int launch_elastic_q2(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+{
+    switch (c->nq) {
+    case 1: return launch_elastic_q2_nq<DIM, 1, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 2: return launch_elastic_q2_nq<DIM, 2, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 3: return launch_elastic_q2_nq<DIM, 3, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
+    }
+}
+
 template <int DIM, int NQ1, typename T, int MAT>
 int launch_elastic_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
 {
@@ -944,6 +1465,11 @@ int launch_elastic(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, d
 template <typename T, int DIM>
 int launch_elastic_mat(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
 {
+    if (c->deg == 2) {
+        if (c->kernel == INS_STVENANT)
+            return launch_elastic_q2<DIM, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
+        return launch_elastic_q2<DIM, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
+    }
     if (c->kernel == INS_STVENANT)
         return launch_elastic<DIM, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
     return launch_elastic<DIM, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
@@ -968,16 +1494,21 @@ int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int in
         const unsigned g = (unsigned)((threads + BLK - 1) / BLK);
         double* v64 = (precision == INS_FP64) ? c->val64.p : nullptr;
         float* v32 = c->val32.p;
+        const typename PairWord<NPE>::type* pw = nullptr;
+        if (c->have_pinfo) {
+            if constexpr (NPE <= 8) pw = c->pinfo.p;
+            else pw = c->pinfo64.p;
+        }
         if (precision == INS_FP64)
             k_gather<NPE, S, KE, double><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
-                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p,
-                c->have_pinfo ? c->pinfo.p : nullptr, c->monotone ? 1 : 0, ke, v64, v32, derr);
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, pw,
+                c->monotone ? 1 : 0, ke, v64, v32, derr);
         else
             k_gather<NPE, S, KE, float><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
-                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p,
-                c->have_pinfo ? c->pinfo.p : nullptr, c->monotone ? 1 : 0, ke, nullptr, v32, derr);
+                c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, pw,
+                c->monotone ? 1 : 0, ke, nullptr, v32, derr);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     if (do_rhs) {
@@ -990,12 +1521,50 @@ int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int in
     return INS_OK;
 }
 
+template <typename T, int NF>
+int gather_q2_t(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int incremental,
+                const double* fe, int precision, int* derr)
+{
+    const T* st = (const T*)c->stage_ke.p;
+    const int s = c->dpn;
+    constexpr int D = NF == 27 ? 3 : 2;
+    if (elastic)
+        return launch_gather_rhs<NF, D>(c, KeElastic<T, NF, D>{st}, do_matrix, do_rhs, lift,
+                                        incremental, fe, nullptr, precision, derr);
+    if (s == 1)
+        return launch_gather_rhs<NF, 1>(c, KeLaplace<T, NF>{st}, do_matrix, do_rhs, lift,
+                                        incremental, fe, nullptr, precision, derr);
+    if (s == D)
+        return launch_gather_rhs<NF, D>(c, KeLaplace<T, NF>{st}, do_matrix, do_rhs, lift,
+                                        incremental, fe, nullptr, precision, derr);
+    return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
+}
+
+int gather_q2(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int incremental,
+              const double* fe, int precision, int* derr)
+{
+    if (c->npe == 27)
+        return precision == INS_FP64
+                   ? gather_q2_t<double, 27>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr)
+                   : gather_q2_t<float, 27>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr);
+    return precision == INS_FP64
+               ? gather_q2_t<double, 9>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr)
+               : gather_q2_t<float, 9>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr);
+}
+
 }  // namespace
 
 int make_tables(Ctx* c)
 {
     std::vector<double> t;
-    fill_tables(c->dim, c->nq, t, &c->tab_face_off, &c->tab_nqf);
+    if (c->deg == 2) {
+        fill_tables_q2(c->dim, c->nq, t);
+        c->tab_face_off = 0;
+        c->tab_nqf = 0;
+    } else {
+        fill_tables(c->dim, c->nq, t, &c->tab_face_off, &c->tab_nqf);
+    }
+    c->tab_deg = c->deg;
     std::vector<float> tf(t.begin(), t.end());
     INS_TRY(ensure(c, c->tab64, t.size()));
     INS_TRY(ensure(c, c->tab32, t.size()));
@@ -1036,6 +1605,7 @@ int assemble(Ctx* c, int precision, int flags)
     Bf bf;
     bf.kind = c->bf_kind;
     for (int i = 0; i < 4; i++) bf.p[i] = c->bf[i];
+    if (c->tab_deg != c->deg) INS_TRY(make_tables(c));
 
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     int* derr = nullptr;
@@ -1068,7 +1638,15 @@ int assemble(Ctx* c, int precision, int flags)
     if (!elastic) {
         if (do_matrix || need_fe) {
             if (!do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
-            if (precision == INS_FP64) {
+            if (c->deg == 2) {
+                if (precision == INS_FP64) {
+                    if (c->dim == 3) INS_TRY((launch_laplace_q2<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
+                    else INS_TRY((launch_laplace_q2<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
+                } else {
+                    if (c->dim == 3) INS_TRY((launch_laplace_q2<3, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
+                    else INS_TRY((launch_laplace_q2<2, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
+                }
+            } else if (precision == INS_FP64) {
                 if (c->dim == 3) INS_TRY((launch_laplace<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
                 else INS_TRY((launch_laplace<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
             } else {
@@ -1108,7 +1686,9 @@ int assemble(Ctx* c, int precision, int flags)
 
     // 2./3. gather into CSR and RHS
     const int lift = do_matrix;
-    if (!elastic) {
+    if (c->deg == 2) {
+        INS_TRY(gather_q2(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr));
+    } else if (!elastic) {
         if (precision == INS_FP64) {
             if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
             else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));

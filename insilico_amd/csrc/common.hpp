@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "insilico_hip.h"
@@ -98,11 +99,18 @@ struct Ctx {
     bool profiling = false;
     ins_stats stats{};
 
-    // mesh
+    // mesh.  conn/npe/n_nodes describe the field nodes (DoF objects) the
+    // pattern, the gather and the RHS work on; gconn/gnpe the Q1 geometry of
+    // the element kernels.  Iso-parametric Q1 (deg 1): the two coincide and
+    // gconn is empty.  adj entries are element << lg_npe | local.
     int dim = 0, npe = 0, lg_npe = 0;
     int64_t n_nodes = 0, n_elem = 0;
     DArr<double> xyz;
     DArr<int32_t> conn;
+    int deg = 1, gnpe = 0;
+    int64_t n_gnodes = 0;
+    DArr<int32_t> gconn;
+    const int32_t* geo_conn() const { return deg == 2 ? gconn.p : conn.p; }
 
     // dofs
     int dpn = 0;
@@ -120,9 +128,12 @@ struct Ctx {
     int max_nb = 0;         // max neighbour nodes of a node
     int max_adj = 0;        // max elements of a node
     // per (node, neighbour) pair, when every node has at most 8 elements:
-    // 4 bits per element of the node (in adjacency order): bit 3 = the
-    // neighbour is a vertex of it, bits 0-2 = its local index there
+    // one field per element of the node (in adjacency order) holding the
+    // neighbour's local index there plus a presence bit; 4-bit fields in a
+    // 32-bit word for up to 8 nodes per element (pinfo), 6-bit fields in a
+    // 64-bit word for the degree-2 elements (pinfo64)
     DArr<uint32_t> pinfo;
+    DArr<uint64_t> pinfo64;
     bool have_pinfo = false;
     DArr<int32_t> adj_ptr, adj;   // node -> element*npe + local, by element
     DArr<int32_t> nb_ptr, nb, nboff;
@@ -134,6 +145,7 @@ struct Ctx {
     DArr<double> tab64;     // quadrature tables (dphi, phi, w)
     DArr<float> tab32;
     int tab_face_off = 0, tab_nqf = 0;
+    int tab_deg = 0;        // field degree the tables were built for
     int bf_kind = INS_BF_NONE;
     double bf[4] = {0, 0, 0, 0};
     int64_t n_faces = 0;
@@ -179,6 +191,16 @@ struct Ctx {
     // profiling: one event pair per SpMV launch of a solve (grown on demand)
     std::vector<hipEvent_t> prof_ev;
     size_t prof_n = 0;
+};
+
+// word of the per-(node, neighbour) shared-element record: 4-bit fields
+// (presence bit 8 + local index) for elements of up to 8 nodes, 6-bit fields
+// (presence bit 32 + local index) for the 9- and 27-node elements
+template <int NPE>
+struct PairWord {
+    typedef typename std::conditional<(NPE <= 8), uint32_t, uint64_t>::type type;
+    static constexpr int bits = NPE <= 8 ? 4 : 6;
+    static constexpr int flag = 1 << (bits - 1);
 };
 
 // allocation with budget accounting (count in elements)
@@ -237,8 +259,14 @@ int exclusive_scan_i32(Ctx* c, const int32_t* in, int32_t* out, int64_t n,
                        int64_t* total);
 // setup.hip
 int build_adjacency(Ctx* c, const int32_t* items, int64_t n_items, int per_item,
-                    int64_t n_nodes, DArr<int32_t>& ptr, DArr<int32_t>& list,
-                    int* max_deg);
+                    int lg_stride, int64_t n_nodes, DArr<int32_t>& ptr,
+                    DArr<int32_t>& list, int* max_deg);
+inline int ceil_lg2(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
 int build_pattern(Ctx* c);
 // assemble.hip
 int make_tables(Ctx* c);

@@ -352,6 +352,62 @@ def neohooke3d(n=64, mu=1.0, lam=10.0):
     return p
 
 
+def _q2_constrained(fxyz, fnodes_bnd, pred, dpn):
+    con = np.zeros((fxyz.shape[0], dpn), bool)
+    sel = fnodes_bnd[pred(fxyz[fnodes_bnd])]
+    con[sel, :] = True
+    return con
+
+
+def diffusion3d_q2(n=32, seed=12345):
+    """The 3-D diffusion problem with a degree-2 field: Q2 (HEX27) on the Q1
+    unit-cube mesh, k_e = exp(g_e), u = 0 on x = 0, f = 1, 3 Gauss points per
+    direction (kernel degree 2*fieldDeg = 4, heat/PoissonDriver.hpp:191)."""
+    xyz, conn = unit_cube(3, n)
+    fconn, fxyz = q2_field(xyz, conn, 3)
+    el, fa, _ = boundary_faces(conn, 3)
+    bn = np.unique(q2_face_nodes(fconn, el, fa, 3))
+    con = _q2_constrained(fxyz, bn, lambda x: np.abs(x[:, 0]) < 1e-12, 1)
+    dof, na = number_dofs(con)
+    k = np.exp(normal(seed, conn.shape[0]))[:, None]
+    return Problem(3, xyz, conn, 1, dof, na, np.zeros(fxyz.shape[0]), LAPLACE, k, 3,
+                   BF_CONST, (1.0, 0.0, 0.0, 0.0), name="diffusion3dQ2-%d" % n,
+                   meta={"seed": seed}, deg=2, fconn=fconn, fxyz=fxyz)
+
+
+def poisson2d_q2(n=16):
+    """-Lap u = 2 pi^2 sin(pi x) sin(pi y), u = 0 on the boundary, Q2 (QUAD9),
+    3 Gauss points per direction."""
+    xyz, conn = unit_cube(2, n)
+    fconn, fxyz = q2_field(xyz, conn, 2)
+    el, fa, _ = boundary_faces(conn, 2)
+    bn = np.unique(q2_face_nodes(fconn, el, fa, 2))
+    con = np.zeros((fxyz.shape[0], 1), bool)
+    con[bn] = True
+    dof, na = number_dofs(con)
+    pi = math.pi
+    return Problem(2, xyz, conn, 1, dof, na, np.zeros(fxyz.shape[0]), LAPLACE,
+                   np.ones((conn.shape[0], 1)), 3, BF_SINPROD, (2 * pi * pi, pi, pi, 0.0),
+                   name="poisson2dQ2-%d" % n, deg=2, fconn=fconn, fxyz=fxyz)
+
+
+def elasticity3d_q2(n=8, seed=12345, nu=0.3, kernel=STVENANT):
+    """Elastic block with a degree-2 field: E_e ~ U[0.5, 2], clamped at z = 0,
+    body force (0, 0, -1), 2 Gauss points per direction."""
+    xyz, conn = unit_cube(3, n)
+    fconn, fxyz = q2_field(xyz, conn, 3)
+    el, fa, _ = boundary_faces(conn, 3)
+    bn = np.unique(q2_face_nodes(fconn, el, fa, 3))
+    con = _q2_constrained(fxyz, bn, lambda x: np.abs(x[:, 2]) < 1e-12, 3)
+    dof, na = number_dofs(con)
+    E = 0.5 + 1.5 * uniform01(seed, conn.shape[0])
+    lam, mu = lame(E, nu)
+    return Problem(3, xyz, conn, 3, dof, na, np.zeros(fxyz.shape[0] * 3), kernel,
+                   np.stack([lam, mu], axis=1), 2, BF_CONST, (0.0, 0.0, -1.0, 0.0),
+                   name="elasticity3dQ2-%d" % n, meta={"seed": seed}, deg=2, fconn=fconn,
+                   fxyz=fxyz)
+
+
 def pulled_sheet_q2(n=20, disp_controlled=True, E=1000.0, nu=0.3, dim=2):
     """reference/06-elastic/compressible.cpp:108-240 with
     reference/06-elastic/PulledSheet.hpp: neo-Hookean block (0,1)^dim, Q2
