@@ -46,6 +46,7 @@ def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup, profile=False
     from insilico_amd import abi
     lat, its, rels, asm, sol = [], [], [], [], []
     spmv_ms, spmv_calls, rep_ms, rep_calls = 0.0, 0, 0.0, 0
+    run_path.element_ms = []
     ctx.set_profiling(profile)
     for s in range(warmup + steps):
         ctx.sync()
@@ -65,6 +66,8 @@ def run_path(ctx, prob, mode, fmt, precision, rtol, steps, warmup, profile=False
             spmv_calls += st["spmv_calls"]
             rep_ms += st["rep_ms"]
             rep_calls += st["rep_calls"]
+            if profile:
+                run_path.element_ms.append(st["element_ms"])
     ctx.set_profiling(False)
     return lat, its, rels, asm, sol, (spmv_ms / max(spmv_calls, 1), spmv_calls,
                                       rep_ms / max(rep_calls, 1), rep_calls)
@@ -75,9 +78,10 @@ def _cpu_rate(prob, O, rp, ci, val, b, iters_fp64, nthreads, n_elem_sample, k_it
     elements, contiguous z-slabs) and seconds per Jacobi-CG iteration on the
     full matrix, with nthreads OpenMP threads."""
     from insilico_amd import problems as P
+    import dataclasses
     E_s = min(prob.n_elem, n_elem_sample)
-    sub = P.Problem(prob.dim, prob.xyz, prob.conn[:E_s], prob.dpn, prob.dof_id, prob.n_active,
-                    prob.g, prob.kernel, prob.coef[:E_s], prob.nq, prob.bf_kind, prob.bf)
+    sub = dataclasses.replace(prob, conn=prob.conn[:E_s], coef=prob.coef[:E_s], faces=None,
+                              fconn=prob.fconn[:E_s] if prob.deg == 2 else None)
     rp_s, ci_s = O.pattern(sub)
     _, _, _, _, t_asm_s = O.assemble(sub, rp_s, ci_s, nthreads=nthreads)
     t0 = time.perf_counter()
@@ -251,6 +255,29 @@ def fp32_integration_run(ctx, prob, fmt, args):
             "relres_vs_fp64_system": rel_vs64}
 
 
+MFMA_FP64_PEAK_TFS = 78.6
+
+
+def element_roofline(prob, element_ms):
+    """Element-kernel figures of the FP64 assembly.  For the degree-2
+    Laplace kernel (one matrix-core product Ke = G^T diag(s) G per element)
+    the algorithmic count is 2 NF^2 NQ dim flops per element (NF = 27 field
+    functions, NQ Gauss points); the bound is the FP64 matrix-core peak."""
+    out = {"avg_launch_ms": round(element_ms, 4)}
+    if prob.deg == 2 and prob.kernel == 1:
+        nf = 3 ** prob.dim
+        nq = prob.nq ** prob.dim
+        flops = 2.0 * nf * nf * nq * prob.dim * prob.n_elem
+        nk = nf * (nf + 1) // 2
+        out.update({"bound": "mfma", "kernel": "k_elem_laplace_q2<%d, %d, double>" % (prob.dim, prob.nq),
+                    "flops_per_launch": flops, "unit": "TFLOP/s",
+                    "achieved": round(flops / (element_ms / 1e3) / 1e12, 2),
+                    "peak": MFMA_FP64_PEAK_TFS,
+                    "frac": round(flops / (element_ms / 1e3) / 1e12 / MFMA_FP64_PEAK_TFS, 4),
+                    "bytes_written": prob.n_elem * nk * 8})
+    return out
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -260,7 +287,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="diffusion", choices=["diffusion", "elasticity", "newton"])
+    ap.add_argument("--config", default="diffusion",
+                    choices=["diffusion", "elasticity", "newton", "q2"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--rtol", type=float, default=1e-8)
@@ -289,6 +317,11 @@ def main():
         prob = P.diffusion3d(args.n or 128, seed=args.seed)
         workload = ("3-D diffusion %d^3 Q1 hex, k=exp(N(0,1)) per element, u=0 on x=0, "
                     "assemble + Jacobi-PCG to %g" % (args.n or 128, args.rtol))
+    elif args.config == "q2":
+        prob = P.diffusion3d_q2(args.n or 64, seed=args.seed)
+        workload = ("3-D diffusion %d^3 Q1 hex with a Q2 (HEX27) field, k=exp(N(0,1)) per "
+                    "element, u=0 on x=0, 3x3x3 Gauss, assemble + Jacobi-PCG to %g"
+                    % (args.n or 64, args.rtol))
     else:
         prob = P.elasticity3d(args.n or 96, seed=args.seed)
         workload = ("3-D linear elasticity (StVenant at u=0) %d^3 Q1 hex, E~U[0.5,2] per "
@@ -326,14 +359,16 @@ def main():
     if not args.no_profile:
         _, _, _, _, _, (spmv_ms, spmv_calls, rep_ms, rep_calls) = run_path(
             ctx, prob, abi.SOLVE_MIXED, fmt, abi.FP64, args.rtol, args.steps, 0, profile=True)
+    element_ms = statistics.mean(run_path.element_ms) if run_path.element_ms else None
     fp32_integration = fp32_integration_run(ctx, prob, fmt, args)
     n = prob.n_active
-    sell_vals = ctx.stats()["sell_values"] or nnz
-    fmt_name = "sell" if fmt == abi.FMT_SELL else "csr"
+    sell_vals = ctx.stats()["sell_values"]
+    # the library falls back to CSR when the SELL padding would not pay
+    fmt_name = "sell" if (fmt == abi.FMT_SELL and sell_vals) else "csr"
     alg = spmv_bytes(n, nnz, sell_vals, fmt_name, 4, 4)
     achieved = round(alg / (spmv_ms / 1e3) / 1e9, 1) if spmv_calls else None
-    traffic = read_traffic("%s%d" % (args.config, args.n or (128 if args.config == "diffusion" else 96)),
-                           "mixed_" + fmt_name)
+    default_n = {"diffusion": 128, "elasticity": 96, "q2": 64}[args.config]
+    traffic = read_traffic("%s%d" % (args.config, args.n or default_n), "mixed_" + fmt_name)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
             "kernel": ("k_spmv_sell<float, float, float, 4>" if fmt_name == "sell" else
@@ -374,6 +409,8 @@ def main():
     }
     if fp32_integration is not None:
         out["fp32_integration"] = fp32_integration
+    if element_ms:
+        out["element_kernel"] = element_roofline(prob, element_ms)
     print(json.dumps(out))
     ctx.close()
 

@@ -104,6 +104,8 @@ typedef struct ins_stats {
     double  rep_ms;         /* profiled: total time of the residual-replacement
                                launches of the dominant kernel (FP64 residual) */
     int64_t rep_calls;
+    double  element_ms;     /* profiled: element kernels of the last
+                               ins_assemble (integration only, no gather) */
 } ins_stats;
 
 /* ---- context ------------------------------------------------------------ */

@@ -47,7 +47,8 @@ class Stats(C.Structure):
                 ("device_bytes", C.c_int64), ("spmv_ms", C.c_double),
                 ("spmv_calls", C.c_int64), ("sell_values", C.c_int64),
                 ("sell_patterns", C.c_int), ("reserved", C.c_int),
-                ("rep_ms", C.c_double), ("rep_calls", C.c_int64)]
+                ("rep_ms", C.c_double), ("rep_calls", C.c_int64),
+                ("element_ms", C.c_double)]
 
 
 class InsError(RuntimeError):

@@ -62,7 +62,7 @@ static int download(Ctx* c, T* h, const DArr<T>& a, size_t n)
 
 static void release_all(Ctx* c)
 {
-    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->dof_id); release(c, c->rowmap);
+    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->dof_id); release(c, c->rowmap);
     release(c, c->g); release(c, c->u); release(c, c->adj_ptr); release(c, c->adj);
     release(c, c->nb_ptr); release(c, c->nb); release(c, c->nboff); release(c, c->row_ptr);
     release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
@@ -123,7 +123,8 @@ int ins_create(ins_ctx** out, int device, uint64_t hbm_budget_bytes)
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev_el[0]) != hipSuccess || hipEventCreate(&c->ev_el[1]) != hipSuccess) {
         delete c;
         return INS_ERR_HIP;
     }
@@ -144,6 +145,8 @@ int ins_destroy(ins_ctx* h)
         if (c->evs[i]) hipEventDestroy(c->evs[i]);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
+    if (c->ev_el[0]) hipEventDestroy(c->ev_el[0]);
+    if (c->ev_el[1]) hipEventDestroy(c->ev_el[1]);
     for (hipEvent_t e : c->prof_ev) hipEventDestroy(e);
     hipStreamDestroy(c->stream);
     delete c;
@@ -224,11 +227,111 @@ int ins_set_mesh(ins_ctx* h, int dim, int64_t n_nodes, const double* xyz, int64_
     c->deg = 1;
     c->gnpe = npe;
     c->n_gnodes = n_nodes;
+    c->have_order = false;
     INS_TRY(upload(c, c->xyz, xyz, (size_t)n_nodes * dim));
     INS_TRY(upload(c, c->conn, conn, (size_t)n_elem * npe));
     invalidate_pattern(c);
     c->have_dofs = false;
     c->n_faces = 0;
+    return INS_OK;
+}
+
+// Solver storage order of the degree-2 field nodes: each node takes the
+// class of its first appearance in an element (vertex; edge along local axis
+// 0, 1, 2; face normal to local axis 2, 1, 0; cell) and the support point
+// there (the mean of the element vertices spanning its n-face); nodes are
+// ordered by class, then by (z, y, x) quantised to 1e-7 of the bounding-box
+// diagonal.  On a structured mesh the rows of one class and line then have
+// equal column-offset lists, which the SELL slices share.
+static int order_field_nodes(Ctx* c, const int32_t* fconn)
+{
+    const int dim = c->dim, gn = c->gnpe, nf = c->npe;
+    std::vector<double> X((size_t)c->n_gnodes * dim);
+    std::vector<int32_t> G((size_t)c->n_elem * gn);
+    INS_HIP_CHECK(c, hipMemcpyAsync(X.data(), c->xyz.p, X.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipMemcpyAsync(G.data(), c->gconn.p, G.size() * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    // n-faces of the hierarchic element and their class
+    static const int E2[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    static const int E3[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                  {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    static const int F3[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                 {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+    static const int EC3[12] = {1, 2, 1, 2, 1, 2, 1, 2, 3, 3, 3, 3};
+    static const int FC3[6] = {4, 4, 5, 6, 5, 6};
+    const int64_t nn = c->n_nodes;
+    std::vector<int8_t> cls((size_t)nn, -1);
+    std::vector<double> pt((size_t)nn * 3, 0.);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = 0; i < (size_t)c->n_gnodes; i++)
+        for (int d = 0; d < dim; d++) {
+            lo[d] = std::min(lo[d], X[i * dim + d]);
+            hi[d] = std::max(hi[d], X[i * dim + d]);
+        }
+    for (int64_t e = 0; e < c->n_elem; e++) {
+        const int32_t* ge = &G[(size_t)e * gn];
+        for (int k = 0; k < nf; k++) {
+            const int32_t node = fconn[e * nf + k];
+            if (cls[node] >= 0) continue;
+            int vs[8], nv = 0, cl = 0;
+            if (k < gn) {
+                vs[nv++] = k;
+            } else if (dim == 2) {
+                if (k < 8) {
+                    vs[nv++] = E2[k - 4][0];
+                    vs[nv++] = E2[k - 4][1];
+                    cl = 1 + (k - 4) % 2;
+                } else {
+                    for (int v = 0; v < 4; v++) vs[nv++] = v;
+                    cl = 3;
+                }
+            } else if (k < 20) {
+                vs[nv++] = E3[k - 8][0];
+                vs[nv++] = E3[k - 8][1];
+                cl = EC3[k - 8];
+            } else if (k < 26) {
+                for (int v = 0; v < 4; v++) vs[nv++] = F3[k - 20][v];
+                cl = FC3[k - 20];
+            } else {
+                for (int v = 0; v < 8; v++) vs[nv++] = v;
+                cl = 7;
+            }
+            cls[node] = (int8_t)cl;
+            for (int d = 0; d < dim; d++) {
+                double sum = 0.;
+                for (int v = 0; v < nv; v++) sum += X[(size_t)ge[vs[v]] * dim + d];
+                pt[(size_t)node * 3 + d] = sum / nv;
+            }
+        }
+    }
+    double diag = 0.;
+    for (int d = 0; d < dim; d++) diag += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    const double tol = 1e-7 * (diag > 0. ? std::sqrt(diag) : 1.);
+    struct Key {
+        int64_t k[4];
+        int32_t node;
+    };
+    std::vector<Key> keys((size_t)nn);
+    for (int64_t i = 0; i < nn; i++) {
+        Key& K = keys[(size_t)i];
+        K.node = (int32_t)i;
+        K.k[0] = cls[i] >= 0 ? cls[i] : 8;
+        for (int d = 0; d < 3; d++) {
+            const int dd = 2 - d;  // z, y, x
+            K.k[1 + d] = dd < dim ? (int64_t)std::llround((pt[(size_t)i * 3 + dd] - lo[dd]) / tol) : 0;
+        }
+    }
+    std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+        for (int j = 0; j < 4; j++)
+            if (a.k[j] != b.k[j]) return a.k[j] < b.k[j];
+        return a.node < b.node;
+    });
+    std::vector<int32_t> order((size_t)nn);
+    for (int64_t i = 0; i < nn; i++) order[(size_t)i] = keys[(size_t)i].node;
+    INS_TRY(upload(c, c->node_order, order.data(), order.size()));
+    c->have_order = true;
     return INS_OK;
 }
 
@@ -259,6 +362,7 @@ int ins_set_field(ins_ctx* h, int degree, int64_t n_field_nodes, int field_npe,
     c->lg_npe = ceil_lg2(field_npe);
     c->n_nodes = n_field_nodes;
     INS_TRY(upload(c, c->conn, field_conn, (size_t)c->n_elem * field_npe));
+    INS_TRY(order_field_nodes(c, field_conn));
     invalidate_pattern(c);
     c->have_dofs = false;
     return INS_OK;

@@ -1635,6 +1635,7 @@ int assemble(Ctx* c, int precision, int flags)
     double* fe = need_fe ? c->stage_fe.p : nullptr;
 
     // 1. element kernels
+    if (c->profiling) INS_HIP_CHECK(c, hipEventRecord(c->ev_el[0], c->stream));
     if (!elastic) {
         if (do_matrix || need_fe) {
             if (!do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
@@ -1671,6 +1672,7 @@ int assemble(Ctx* c, int precision, int flags)
             else INS_TRY((launch_elastic_mat<double, 2>(c, c->tab64.p, bf, 0, want_r, nullptr, fe)));
         }
     }
+    if (c->profiling) INS_HIP_CHECK(c, hipEventRecord(c->ev_el[1], c->stream));
     // faces
     double* ff = nullptr;
     if (do_rhs && c->n_faces > 0) {
@@ -1723,6 +1725,12 @@ int assemble(Ctx* c, int precision, int flags)
     float ms = 0.f;
     INS_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.assemble_ms = ms;
+    c->stats.element_ms = 0.;
+    if (c->profiling) {
+        float me = 0.f;
+        INS_HIP_CHECK(c, hipEventElapsedTime(&me, c->ev_el[0], c->ev_el[1]));
+        c->stats.element_ms = me;
+    }
     if (herr) return fail(c, INS_ERR_PATTERN, "element entry outside the pattern");
     if (do_matrix) c->have_matrix = true;
     if (do_rhs) c->have_rhs = true;

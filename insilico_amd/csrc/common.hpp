@@ -111,6 +111,10 @@ struct Ctx {
     int64_t n_gnodes = 0;
     DArr<int32_t> gconn;
     const int32_t* geo_conn() const { return deg == 2 ? gconn.p : conn.p; }
+    // solver storage order of the field nodes (degree 2): by n-face class,
+    // then by support point; used to build the SELL slices
+    DArr<int32_t> node_order;
+    bool have_order = false;
 
     // dofs
     int dpn = 0;
@@ -188,6 +192,7 @@ struct Ctx {
     bool have_solution = false;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_el[2] = {nullptr, nullptr};  // profiled element kernels
     // profiling: one event pair per SpMV launch of a solve (grown on demand)
     std::vector<hipEvent_t> prof_ev;
     size_t prof_n = 0;

@@ -701,7 +701,7 @@ __global__ __launch_bounds__(VB) void k_norm2_fin(const double* part, int g, Sca
 }
 
 // ---- SELL construction --------------------------------------------------------
-constexpr int MAXU = 160;  // longest per-slice offset list kept
+constexpr int MAXU = 512;  // longest per-slice offset list kept
 
 __device__ inline int wave_min_i(int v)
 {
@@ -773,6 +773,59 @@ __global__ void k_comp_place(int64_t n, const int32_t* __restrict__ comp, int c,
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += (int64_t)gridDim.x * blockDim.x)
         if (comp[r] == c) perm[off + pos[r]] = (int32_t)r;
+}
+
+// rows of component c in the solver order of the field nodes
+__global__ void k_ord_flag(int64_t nn, const int32_t* __restrict__ order,
+                           const int32_t* __restrict__ dof_id, int dpn, int c,
+                           int32_t* __restrict__ flag)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+         i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = dof_id[(int64_t)order[i] * dpn + c] >= 0;
+}
+
+__global__ void k_ord_place(int64_t nn, const int32_t* __restrict__ order,
+                            const int32_t* __restrict__ dof_id, int dpn, int c,
+                            const int32_t* __restrict__ pos, int64_t off,
+                            int32_t* __restrict__ perm)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t id = dof_id[(int64_t)order[i] * dpn + c];
+        if (id >= 0) perm[off + pos[i]] = id;
+    }
+}
+
+// columns of the permuted matrix for any permutation: one wavefront per row,
+// each entry's rank among the row's permuted columns by counting (columns of
+// a row are distinct, rows hold a few hundred entries at most)
+__global__ __launch_bounds__(256) void k_prow_fill_sorted(int64_t n,
+                                                          const int32_t* __restrict__ perm,
+                                                          const int32_t* __restrict__ iperm,
+                                                          const int32_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ col,
+                                                          const int32_t* __restrict__ prp,
+                                                          int32_t* __restrict__ pcol,
+                                                          int32_t* __restrict__ pent)
+{
+    const int lane = threadIdx.x & 63;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + wave_in_block(); r < n;
+         r += (int64_t)gridDim.x * 4) {
+        const int32_t row = perm[r];
+        const int32_t k0 = rp[row], k1 = rp[row + 1];
+        const int32_t o = prp[r];
+        for (int32_t kb = k0; kb < k1; kb += 64) {
+            const int32_t k = kb + lane;
+            const int32_t pc = (k < k1) ? iperm[col[k]] : INT32_MAX;
+            int32_t rank = 0;
+            for (int32_t i = k0; i < k1; i++) rank += iperm[col[i]] < pc;
+            if (k < k1) {
+                pcol[o + rank] = pc;
+                pent[o + rank] = k;
+            }
+        }
+    }
 }
 
 // inverse of the slice-row permutation
@@ -1382,26 +1435,43 @@ int build_sell(Ctx* c, bool force)
     // 64 rows of a slice are consecutive nodes of one component, their
     // column offsets are shared, and the gathers and stores of a slice are
     // contiguous.  SELL solves of such problems run on permuted vectors.
-    S.perm_on = c->dpn > 1;
+    // Degree-2 fields: the slices are built in the solver order of the field
+    // nodes (grouped by n-face class, then by support point), in which the
+    // rows of a slice are neighbours of one kind with shared column offsets;
+    // the face-based DoF numbering itself scatters them.
+    const bool ordered = c->have_order;
+    S.perm_on = c->dpn > 1 || ordered;
     const int32_t* brp = c->row_ptr.p;   // CSR the slices are built from
     const int32_t* bcol = c->col.p;
     int32_t *prp = nullptr, *pcol = nullptr, *pent = nullptr;
     if (S.perm_on) {
         INS_TRY(ensure(c, S.perm, n + 1));
+        const int64_t nn = c->n_nodes;
+        const int64_t nf = (nn > n ? nn : n) + 1;
         int32_t *comp = nullptr, *flag = nullptr, *pos = nullptr, *iperm = nullptr;
         INS_HIP_CHECK(c, hipMallocAsync((void**)&comp, (n + 1) * sizeof(int32_t), c->stream));
-        INS_HIP_CHECK(c, hipMallocAsync((void**)&flag, (n + 1) * sizeof(int32_t), c->stream));
-        INS_HIP_CHECK(c, hipMallocAsync((void**)&pos, (n + 1) * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&flag, nf * sizeof(int32_t), c->stream));
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&pos, nf * sizeof(int32_t), c->stream));
         INS_HIP_CHECK(c, hipMallocAsync((void**)&iperm, (n + 1) * sizeof(int32_t), c->stream));
         const int64_t nd = c->n_nodes * c->dpn;
         k_row_comp<<<grid_for(nd, 256), 256, 0, c->stream>>>(nd, c->dof_id.p, c->dpn, comp);
         int64_t off = 0;
         for (int cc = 0; cc < c->dpn; cc++) {
-            k_comp_flag<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, flag);
             int64_t tot = 0;
-            int st2 = exclusive_scan_i32(c, flag, pos, n, &tot);
-            if (st2 != INS_OK) return st2;
-            k_comp_place<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, pos, off, S.perm.p);
+            if (ordered) {
+                k_ord_flag<<<grid_for(nn, 256), 256, 0, c->stream>>>(nn, c->node_order.p,
+                                                                     c->dof_id.p, c->dpn, cc, flag);
+                int st2 = exclusive_scan_i32(c, flag, pos, nn, &tot);
+                if (st2 != INS_OK) return st2;
+                k_ord_place<<<grid_for(nn, 256), 256, 0, c->stream>>>(
+                    nn, c->node_order.p, c->dof_id.p, c->dpn, cc, pos, off, S.perm.p);
+            } else {
+                k_comp_flag<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, flag);
+                int st2 = exclusive_scan_i32(c, flag, pos, n, &tot);
+                if (st2 != INS_OK) return st2;
+                k_comp_place<<<grid_for(n, 256), 256, 0, c->stream>>>(n, comp, cc, pos, off,
+                                                                      S.perm.p);
+            }
             off += tot;
         }
         k_iperm<<<grid_for(n, 256), 256, 0, c->stream>>>(n, S.perm.p, iperm);
@@ -1412,9 +1482,12 @@ int build_sell(Ctx* c, bool force)
         int64_t pnnz = 0;
         int st3 = exclusive_scan_i32(c, flag, prp, n, &pnnz);
         if (st3 != INS_OK) return st3;
-        k_prow_fill<<<grid_for(n, 4, 1u << 16), 256, 0, c->stream>>>(n, c->dpn, S.perm.p, iperm,
-                                                                       comp, c->row_ptr.p,
-                                                                       c->col.p, prp, pcol, pent);
+        if (ordered)
+            k_prow_fill_sorted<<<grid_for(n, 4, 1u << 16), 256, 0, c->stream>>>(
+                n, S.perm.p, iperm, c->row_ptr.p, c->col.p, prp, pcol, pent);
+        else
+            k_prow_fill<<<grid_for(n, 4, 1u << 16), 256, 0, c->stream>>>(
+                n, c->dpn, S.perm.p, iperm, comp, c->row_ptr.p, c->col.p, prp, pcol, pent);
         INS_HIP_CHECK(c, hipGetLastError());
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
         (void)hipFreeAsync(comp, c->stream);
