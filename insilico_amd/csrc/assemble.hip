@@ -587,11 +587,14 @@ struct Mfma<float> {
 // heat::Laplace on a degree-2 field as one matrix product per element:
 // with G[(q,i)][M] = (grad_x phi_M)_i at quadrature point q and s_q =
 // kappa detJ_q w_q, Ke = G^T diag(s) G (the sum over q and i of
-// base/kernel/Laplace.hpp:100-151).  One wavefront per element: lanes
-// < NQ build inv(J^T) and s_q, the wave writes G (NQ*DIM x NF, zero-padded
-// to 16-wide tiles) into LDS, and the upper tiles of Ke come from 16x16x4
-// matrix-core steps over the NQ*DIM rows of G.  The symmetric half is
-// written element-major, as the Q1 kernel does.
+// base/kernel/Laplace.hpp:100-151).  One wavefront per element, four per
+// block sharing the reference-gradient table in LDS.  Lanes < NQ build
+// inv(J^T) and s_q of their point; then for each group of 4 rows of G every
+// lane forms its G entries of the 16-wide column tiles (3 multiply-adds on
+// the table) and the upper tiles of Ke advance by one 16x16x4 matrix-core
+// step each, with independent accumulators.  The symmetric half is written
+// element-major straight from the accumulators (16 consecutive entries per
+// row segment).
 template <int DIM, int NQ1, typename T>
 struct LapQ2 {
     static constexpr int NQ = IPow<NQ1, DIM>::value;
@@ -602,33 +605,32 @@ struct LapQ2 {
     static constexpr int NT = (NF + 15) / 16;
     static constexpr int NFP = NT * 16;
     static constexpr int NK = NF * (NF + 1) / 2;
-    static constexpr int G = 0;                  // [KP][NFP]
-    static constexpr int KS = G + KP * NFP;      // [NFP][NFP]
-    static constexpr int INV = KS + NFP * NFP;   // [NQ][DIM][DIM]
+    static constexpr int INV = 0;                // [NQ][DIM][DIM]
     static constexpr int SC = INV + NQ * DIM * DIM;
     static constexpr int DET = SC + NQ;
     static constexpr int FQ = DET + NQ;
     static constexpr int X = FQ + NQ;            // [GN][DIM]
     static constexpr int SIZE = X + GN * DIM;
-    static constexpr int EPB = 1;                // elements (wavefronts) per block
+    static constexpr int WPB = 4;                // wavefronts (elements) per block
 };
 
 template <int DIM, int NQ1, typename T>
-__global__ __launch_bounds__(64) void k_elem_laplace_q2(
+__global__ __launch_bounds__(256) void k_elem_laplace_q2(
     int64_t E, const int32_t* __restrict__ gconn, const double* __restrict__ xyz,
     const double* __restrict__ coef, int ncoef, const T* __restrict__ tab, Bf bf,
     T* __restrict__ ke, double* __restrict__ fe)
 {
     using L = LapQ2<DIM, NQ1, T>;
     using TB = TabQ2<DIM, L::NQ>;
-    constexpr int NQ = L::NQ, GN = L::GN, NF = L::NF, NFP = L::NFP, EPB = L::EPB;
-    // the tables stay in global memory (a few KB, cache-resident): the LDS
-    // holds G and the element matrix
-    const T* __restrict__ s_tab = tab;
-    __shared__ T s_all[EPB * L::SIZE];
+    constexpr int NQ = L::NQ, GN = L::GN, NF = L::NF, NT = L::NT, WPB = L::WPB;
+    __shared__ T s_fd[NQ * NF * DIM];            // reference gradients of the field
+    __shared__ T s_all[WPB * L::SIZE];
+    for (int i = threadIdx.x; i < NQ * NF * DIM; i += blockDim.x) s_fd[i] = tab[TB::FD + i];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, k4 = lane >> 4;
     T* sm = s_all + wave * L::SIZE;
-    for (int64_t base = (int64_t)blockIdx.x * EPB; base < E; base += (int64_t)gridDim.x * EPB) {
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * WPB; base < E; base += (int64_t)gridDim.x * WPB) {
         const int64_t e = base + wave;
         const bool valid = e < E;
         if (valid && lane < GN * DIM) {
@@ -639,12 +641,12 @@ __global__ __launch_bounds__(64) void k_elem_laplace_q2(
         if (valid && lane < NQ) {
             const int q = lane;
             T inv[DIM][DIM];
-            const T det = geo_inverse<DIM, T>(sm + L::X, s_tab + TB::GD + q * GN * DIM, inv);
+            const T det = geo_inverse<DIM, T>(sm + L::X, tab + TB::GD + q * GN * DIM, inv);
 #pragma unroll
             for (int i = 0; i < DIM; i++)
 #pragma unroll
                 for (int a = 0; a < DIM; a++) sm[L::INV + (q * DIM + i) * DIM + a] = inv[i][a];
-            const T w = s_tab[TB::W + q];
+            const T w = tab[TB::W + q];
             sm[L::SC + q] = (T)coef[e * ncoef] * det * w;
             sm[L::DET + q] = det;
             if (fe && bf.kind != INS_BF_NONE) {
@@ -652,60 +654,65 @@ __global__ __launch_bounds__(64) void k_elem_laplace_q2(
 #pragma unroll
                 for (int n = 0; n < GN; n++)
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) x[d] += sm[L::X + n * DIM + d] * s_tab[TB::GP + q * GN + n];
+                    for (int d = 0; d < DIM; d++) x[d] += sm[L::X + n * DIM + d] * tab[TB::GP + q * GN + n];
                 sm[L::FQ + q] = body_force<T>(bf, x, DIM, 0);
             }
         }
         __syncthreads();
-        // G[(q,i)][M] = sum_a inv_q[i][a] dphi_M[a] (zero in the padding)
         if (valid) {
-            for (int idx = lane; idx < L::KP * NFP; idx += 64) {
-                const int k = idx / NFP, M = idx - (idx / NFP) * NFP;
-                T v = (T)0;
-                if (k < L::KD && M < NF) {
-                    const int q = k / DIM, i = k - q * DIM;
-                    const T* fd = s_tab + TB::FD + (q * NF + M) * DIM;
-                    const T* iv = sm + L::INV + (q * DIM + i) * DIM;
+            typename Mfma<T>::v4 acc[NT * (NT + 1) / 2];
 #pragma unroll
-                    for (int a = 0; a < DIM; a++) v += iv[a] * fd[a];
-                }
-                sm[L::G + idx] = v;
-            }
-        }
-        __syncthreads();
-        if (valid) {
-            const int r16 = lane & 15, k4 = lane >> 4;
+            for (int t = 0; t < NT * (NT + 1) / 2; t++) acc[t] = (typename Mfma<T>::v4){0, 0, 0, 0};
+#pragma clang loop unroll_count(3)
+            for (int s0 = 0; s0 < L::KP; s0 += 4) {
+                const int k = s0 + k4;
+                const bool kin = k < L::KD;
+                const int q = kin ? k / DIM : 0, i = kin ? k - q * DIM : 0;
+                const T* iv = sm + L::INV + (q * DIM + i) * DIM;
+                const T sq = sm[L::SC + q];
+                T g[NT];
 #pragma unroll
-            for (int ti = 0; ti < L::NT; ti++)
+                for (int t = 0; t < NT; t++) {
+                    const int M = t * 16 + r16;
+                    T v = (T)0;
+                    if (kin && M < NF) {
+                        const T* fd = s_fd + (q * NF + M) * DIM;
 #pragma unroll
-                for (int tj = ti; tj < L::NT; tj++) {
-                    typename Mfma<T>::v4 acc = {(T)0, (T)0, (T)0, (T)0};
-#pragma unroll 4
-                    for (int s0 = 0; s0 < L::KP; s0 += 4) {
-                        const int k = s0 + k4;
-                        const int q = k < L::KD ? k / DIM : 0;
-                        const T a = sm[L::G + k * NFP + ti * 16 + r16];
-                        const T b = sm[L::G + k * NFP + tj * 16 + r16] * sm[L::SC + q];
-                        acc = Mfma<T>::op(a, b, acc);
+                        for (int a = 0; a < DIM; a++) v += iv[a] * fd[a];
                     }
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        sm[L::KS + (ti * 16 + Mfma<T>::row(lane, r)) * NFP + tj * 16 + r16] = acc[r];
+                    g[t] = v;
                 }
-        }
-        __syncthreads();
-        if (valid) {
-            T* dst = ke + e * (int64_t)L::NK;
-            int off = 0;
-            for (int M = 0; M < NF; M++) {
-                for (int N = M + lane; N < NF; N += 64) dst[off + N - M] = sm[L::KS + M * NFP + N];
-                off += NF - M;
+                int t = 0;
+#pragma unroll
+                for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                    for (int tj = ti; tj < NT; tj++) {
+                        acc[t] = Mfma<T>::op(g[ti], g[tj] * sq, acc[t]);
+                        t++;
+                    }
             }
+            // rows M of tile ti, columns N of tile tj, upper half only
+            T* dst = ke + e * (int64_t)L::NK;
+            int t = 0;
+#pragma unroll
+            for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                for (int tj = ti; tj < NT; tj++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int M = ti * 16 + Mfma<T>::row(lane, r);
+                        const int N = tj * 16 + r16;
+                        if (M < NF && N < NF && M <= N)
+                            dst[M * NF - (M * (M - 1)) / 2 + (N - M)] = acc[t][r];
+                    }
+                    t++;
+                }
             if (fe && lane < NF) {
                 T fb = (T)0;
                 if (bf.kind != INS_BF_NONE)
+#pragma unroll 3
                     for (int q = 0; q < NQ; q++)
-                        fb += sm[L::FQ + q] * s_tab[TB::FP + q * NF + lane] * s_tab[TB::W + q] *
+                        fb += sm[L::FQ + q] * tab[TB::FP + q * NF + lane] * tab[TB::W + q] *
                               sm[L::DET + q];
                 fe[e * NF + lane] = (double)fb;
             }
@@ -1397,11 +1404,11 @@ int launch_laplace(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
 template <int DIM, typename T>
 int launch_laplace_q2(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
 {
-    const unsigned g = grid_for(c->n_elem, 1, 1u << 22);
+    const unsigned g = grid_for(c->n_elem, 4, 1u << 20);
     switch (c->nq) {
-    case 1: k_elem_laplace_q2<DIM, 1, T><<<g, 64, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
-    case 2: k_elem_laplace_q2<DIM, 2, T><<<g, 64, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
-    case 3: k_elem_laplace_q2<DIM, 3, T><<<g, 64, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 1: k_elem_laplace_q2<DIM, 1, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 2: k_elem_laplace_q2<DIM, 2, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 3: k_elem_laplace_q2<DIM, 3, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
     default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
     }
     INS_HIP_CHECK(c, hipGetLastError());
