@@ -172,6 +172,17 @@ int ins_get_pattern(ins_ctx* ctx, int32_t* row_ptr, int32_t* col_idx);
  * (base/quad/GaussLegendre.hpp:33-36: nq = (DEG+2)/2). */
 int ins_set_material(ins_ctx* ctx, int kernel, int nq, int ncoef,
                      const double* elem_coef);
+/* Time-step terms of a linear multi-step method, as
+ * BoundaryValueProblem::applyTimeIntegrator adds them
+ * (base/BoundaryValueProblem.hpp:337-354 -> base/time/ReactionTerms.hpp
+ * :140-236 with base::kernel::Mass, base/kernel/Mass.hpp:80-110): the
+ * element mass matrix M_e = int density phi phi enters the matrix as
+ * mass_factor * M_e (mass_factor = a0 / (b0 dt)) with its Dirichlet
+ * lifting, and the RHS gets M_e * hist, where hist[node*dpn+c] =
+ * sum_{s>=1} -a_s/dt u^{n+1-s} (NULL = zero).  heat::Laplace only;
+ * density = 0 switches the terms off. */
+int ins_set_mass(ins_ctx* ctx, double density, double mass_factor, const double* hist);
+
 /* bodyForceComputation (base/asmb/BodyForce.hpp:67-82) with a built-in f */
 int ins_set_body_force(ins_ctx* ctx, int kind, const double* params);
 /* neumannForceComputation (base/asmb/NeumannForce.hpp:36-63) with a

@@ -34,7 +34,7 @@ SYMBOLS = [
     "ins_rhs_norm", "ins_set_csr", "ins_solve", "ins_get_solution", "ins_solution_norm",
     "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
-    "ins_server_running", "ins_set_field",
+    "ins_server_running", "ins_set_field", "ins_set_mass",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 
@@ -79,6 +79,7 @@ def load():
         "ins_sync": [vp],
         "ins_set_mesh": [vp, i32, i64, vp, i64, i32, vp],
         "ins_set_field": [vp, i32, i64, i32, vp],
+        "ins_set_mass": [vp, dbl, dbl, vp],
         "ins_set_dofs": [vp, i32, vp, i64],
         "ins_set_prescribed": [vp, vp],
         "ins_scale_prescribed": [vp, dbl],
@@ -171,6 +172,11 @@ class Context:
         fc = np.ascontiguousarray(fconn, np.int32)
         self._chk(self.L.ins_set_field(self.h, degree, n_field_nodes, fc.shape[1], _p(fc)))
         self.n_nodes = n_field_nodes
+
+    def set_mass(self, density, mass_factor=0.0, hist=None):
+        """time-step terms: matrix + mass_factor * M, RHS + M hist"""
+        h = None if hist is None else np.ascontiguousarray(hist, np.float64)
+        self._chk(self.L.ins_set_mass(self.h, density, mass_factor, _p(h)))
 
     def set_dofs(self, dpn, dof_id, n_active):
         d = np.ascontiguousarray(dof_id, np.int32)

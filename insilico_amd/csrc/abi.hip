@@ -62,7 +62,7 @@ static int download(Ctx* c, T* h, const DArr<T>& a, size_t n)
 
 static void release_all(Ctx* c)
 {
-    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->dof_id); release(c, c->rowmap);
+    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->mhist); release(c, c->stage_me); release(c, c->dof_id); release(c, c->rowmap);
     release(c, c->g); release(c, c->u); release(c, c->adj_ptr); release(c, c->adj);
     release(c, c->nb_ptr); release(c, c->nb); release(c, c->nboff); release(c, c->row_ptr);
     release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
@@ -497,6 +497,25 @@ int ins_set_material(ins_ctx* h, int kernel, int nq, int ncoef, const double* co
     c->ncoef = ncoef;
     INS_TRY(upload(c, c->coef, coef, (size_t)c->n_elem * ncoef));
     INS_TRY(make_tables(c));
+    c->have_matrix = false;
+    return INS_OK;
+}
+
+int ins_set_mass(ins_ctx* h, double density, double mass_factor, const double* hist)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (!(density >= 0.) || !std::isfinite(mass_factor))
+        return fail(c, INS_ERR_ARG, "ins_set_mass: density >= 0 and a finite factor");
+    if (density > 0. && !c->have_dofs)
+        return fail(c, INS_ERR_STATE, "ins_set_mass before ins_set_dofs");
+    c->density = density;
+    c->mass_factor = density > 0. ? mass_factor : 0.;
+    c->have_mhist = false;
+    if (density > 0. && hist) {
+        INS_TRY(upload(c, c->mhist, hist, (size_t)(c->n_nodes * c->dpn)));
+        c->have_mhist = true;
+    }
     c->have_matrix = false;
     return INS_OK;
 }

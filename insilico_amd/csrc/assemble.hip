@@ -132,11 +132,11 @@ __device__ inline T phys_grad(const T (&X)[NPE][DIM], const T* dphi, T (&gx)[NPE
 // ---------------------------------------------------------------------------
 // heat::Laplace element kernel (heat/Laplace.hpp:113-126 ->
 // base/kernel/Laplace.hpp:100-151): one element per lane.
-template <int DIM, int NQ1, typename T>
+template <int DIM, int NQ1, typename T, bool WANT_M = false>
 __global__ __launch_bounds__(BLK) void k_elem_laplace(
     int64_t E, const int32_t* __restrict__ conn, const double* __restrict__ xyz,
     const double* __restrict__ coef, int ncoef, const T* __restrict__ tab, Bf bf,
-    T* __restrict__ ke, double* __restrict__ fe)
+    T* __restrict__ ke, double* __restrict__ fe, T* __restrict__ me = nullptr, T rho = 0)
 {
     constexpr int NPE = 1 << DIM;
     constexpr int NQ = IPow<NQ1, DIM>::value;
@@ -155,9 +155,13 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
             for (int d = 0; d < DIM; d++) X[n][d] = (T)xyz[node * DIM + d];
         }
         const T kappa = (T)coef[e * ncoef];
-        T K[NK];
+        T K[NK], Mm[WANT_M ? NK : 1];
 #pragma unroll
         for (int k = 0; k < NK; k++) K[k] = (T)0;
+        if constexpr (WANT_M) {
+#pragma unroll
+            for (int k = 0; k < NK; k++) Mm[k] = (T)0;
+        }
         T fb[NPE];
 #pragma unroll
         for (int n = 0; n < NPE; n++) fb[n] = (T)0;
@@ -178,6 +182,17 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
                     entry *= scal;
                     K[k++] += entry;
                 }
+            if constexpr (WANT_M) {
+                // base/kernel/Mass.hpp:93-104: scalar = rho detJ w, entry =
+                // scalar phi_M phi_N
+                const T* phi = s_tab + TB::PHI + q * NPE;
+                const T sm_ = rho * det * w;
+                int km = 0;
+#pragma unroll
+                for (int M = 0; M < NPE; M++)
+#pragma unroll
+                    for (int N = M; N < NPE; N++) Mm[km++] += sm_ * phi[M] * phi[N];
+            }
             if (fe && bf.kind != INS_BF_NONE) {
                 const T* phi = s_tab + TB::PHI + q * NPE;
                 T x[3] = {0, 0, 0};
@@ -198,6 +213,11 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
         T2* dst = reinterpret_cast<T2*>(ke + e * NK);
 #pragma unroll
         for (int k = 0; k < NK / 2; k++) dst[k] = (T2){K[2 * k], K[2 * k + 1]};
+        if constexpr (WANT_M) {
+            T2* dm = reinterpret_cast<T2*>(me + e * NK);
+#pragma unroll
+            for (int k = 0; k < NK / 2; k++) dm[k] = (T2){Mm[2 * k], Mm[2 * k + 1]};
+        }
         if (fe) {
 #pragma unroll
             for (int n = 0; n < NPE; n++) fe[e * NPE + n] = (double)fb[n];
@@ -618,7 +638,7 @@ template <int DIM, int NQ1, typename T>
 __global__ __launch_bounds__(256) void k_elem_laplace_q2(
     int64_t E, const int32_t* __restrict__ gconn, const double* __restrict__ xyz,
     const double* __restrict__ coef, int ncoef, const T* __restrict__ tab, Bf bf,
-    T* __restrict__ ke, double* __restrict__ fe)
+    T* __restrict__ ke, double* __restrict__ fe, T* __restrict__ me, T rho)
 {
     using L = LapQ2<DIM, NQ1, T>;
     using TB = TabQ2<DIM, L::NQ>;
@@ -707,6 +727,47 @@ __global__ __launch_bounds__(256) void k_elem_laplace_q2(
                     }
                     t++;
                 }
+            if (me) {
+                // mass matrix Phi^T diag(rho detJ w) Phi over the points
+                // (base/kernel/Mass.hpp), same tiles
+                typename Mfma<T>::v4 accm[NT * (NT + 1) / 2];
+#pragma unroll
+                for (int t2 = 0; t2 < NT * (NT + 1) / 2; t2++) accm[t2] = (typename Mfma<T>::v4){0, 0, 0, 0};
+                for (int s0 = 0; s0 < NQ; s0 += 4) {
+                    const int q = s0 + k4;
+                    const bool qin = q < NQ;
+                    const T wq = qin ? rho * sm[L::DET + q] * tab[TB::W + q] : (T)0;
+                    T ph[NT];
+#pragma unroll
+                    for (int tt = 0; tt < NT; tt++) {
+                        const int M = tt * 16 + r16;
+                        ph[tt] = (qin && M < NF) ? tab[TB::FP + q * NF + M] : (T)0;
+                    }
+                    int t2 = 0;
+#pragma unroll
+                    for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                        for (int tj = ti; tj < NT; tj++) {
+                            accm[t2] = Mfma<T>::op(ph[ti], ph[tj] * wq, accm[t2]);
+                            t2++;
+                        }
+                }
+                T* dm = me + e * (int64_t)L::NK;
+                int t2 = 0;
+#pragma unroll
+                for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                    for (int tj = ti; tj < NT; tj++) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int M = ti * 16 + Mfma<T>::row(lane, r);
+                            const int N = tj * 16 + r16;
+                            if (M < NF && N < NF && M <= N)
+                                dm[M * NF - (M * (M - 1)) / 2 + (N - M)] = accm[t2][r];
+                        }
+                        t2++;
+                    }
+            }
             if (fe && lane < NF) {
                 T fb = (T)0;
                 if (bf.kind != INS_BF_NONE)
@@ -956,15 +1017,15 @@ struct KeElastic {
 
 // Gather: G lanes per node, lane j handles neighbour j of the node and the
 // S x S block of CSR entries coupling the two nodes.
-template <int NPE, int S, class KE, typename TA>
+template <int NPE, int S, class KE, typename TA, class ME>
 __global__ __launch_bounds__(BLK) void k_gather(
     int64_t n_nodes, int lgG, int lg_npe, const int32_t* __restrict__ adj_ptr,
     const int32_t* __restrict__ adj, const int32_t* __restrict__ nb_ptr,
     const int32_t* __restrict__ nb, const int32_t* __restrict__ nboff,
     const int32_t* __restrict__ dof_id, const int32_t* __restrict__ row_ptr,
     const int32_t* __restrict__ col, const int32_t* __restrict__ conn,
-    const typename PairWord<NPE>::type* __restrict__ pinfo, int monotone, KE ke,
-    double* __restrict__ v64, float* __restrict__ v32, int* __restrict__ err)
+    const typename PairWord<NPE>::type* __restrict__ pinfo, int monotone, KE ke, ME me,
+    double mfac, double* __restrict__ v64, float* __restrict__ v32, int* __restrict__ err)
 {
     constexpr int BITS = PairWord<NPE>::bits, FLAG = PairWord<NPE>::flag;
     const int G = 1 << lgG;
@@ -989,7 +1050,11 @@ __global__ __launch_bounds__(BLK) void k_gather(
         // the elements shared by n and m, in ascending element order: walk
         // n's (sorted) elements and find m among each element's vertices;
         // the loads are independent of each other and the same for all the
-        // lanes of node n
+        // lanes of node n.  A second pass adds the scaled mass matrices of
+        // a time step after all stiffness contributions, as the reference's
+        // separate element loop does (base/time/ReactionTerms.hpp:174-186).
+        const int npass = me.p ? 2 : 1;
+        for (int pass = 0; pass < npass; pass++)
         if (pinfo) {
             // precomputed at pattern set-up: one word per pair
             const auto w = pinfo[nbs + j];
@@ -1002,7 +1067,9 @@ __global__ __launch_bounds__(BLK) void k_gather(
 #pragma unroll
                 for (int cr = 0; cr < S; cr++)
 #pragma unroll
-                    for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(e, la, cr, lb, cc);
+                    for (int cc = 0; cc < S; cc++)
+                        v[cr][cc] += pass == 0 ? (TA)ke.get(e, la, cr, lb, cc)
+                                               : (TA)(mfac * me.get(e, la, cr, lb, cc));
             }
         } else {
             for (int32_t ia = as; ia < ae; ia++) {
@@ -1018,7 +1085,9 @@ __global__ __launch_bounds__(BLK) void k_gather(
 #pragma unroll
                 for (int cr = 0; cr < S; cr++)
 #pragma unroll
-                    for (int cc = 0; cc < S; cc++) v[cr][cc] += (TA)ke.get(e, la, cr, lb, cc);
+                    for (int cc = 0; cc < S; cc++)
+                        v[cr][cc] += pass == 0 ? (TA)ke.get(e, la, cr, lb, cc)
+                                               : (TA)(mfac * me.get(e, la, cr, lb, cc));
             }
         }
         const int32_t off = nboff[nbs + j];
@@ -1057,15 +1126,15 @@ __global__ __launch_bounds__(BLK) void k_gather(
 
 // RHS per node: lifting (per element: t = -sum_c g_c K_rc, then b += t),
 // element forces, face forces; each in ascending element / face order.
-template <int NPE, int S, class KE>
+template <int NPE, int S, class KE, class ME>
 __global__ __launch_bounds__(BLK) void k_rhs(
     int64_t n_nodes, int lg_npe, const int32_t* __restrict__ adj_ptr,
     const int32_t* __restrict__ adj, const int32_t* __restrict__ conn,
     const int32_t* __restrict__ dof_id, const double* __restrict__ g,
     const double* __restrict__ u, int incremental, int lift, KE ke,
     const double* __restrict__ fe, int lg_nfn, const int32_t* __restrict__ fadj_ptr,
-    const int32_t* __restrict__ fadj, const double* __restrict__ ff, int nfn,
-    double* __restrict__ b)
+    const int32_t* __restrict__ fadj, const double* __restrict__ ff, int nfn, ME me,
+    double mfac, const double* __restrict__ hist, double* __restrict__ b)
 {
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
          n += (int64_t)gridDim.x * blockDim.x) {
@@ -1106,6 +1175,28 @@ __global__ __launch_bounds__(BLK) void k_rhs(
                     const int64_t f = fadj[k] >> lg_nfn;
                     const int a = fadj[k] & fmask;
                     acc += ff[f * (nfn * S) + a * S + cr];
+                }
+            }
+            // time-step terms (base/time/ReactionTerms.hpp:174-230): per
+            // element the lifting of mfac * M_e with the prescribed values,
+            // then M_e times the history combination
+            if (me.p) {
+                for (int32_t k = as; k < ae; k++) {
+                    const int64_t e = adj[k] >> lg_npe;
+                    const int a = adj[k] & mask;
+                    double t = 0., f = 0.;
+                    for (int bnode = 0; bnode < NPE; bnode++) {
+                        const int64_t mnode = conn[e * NPE + bnode];
+#pragma unroll
+                        for (int cc = 0; cc < S; cc++) {
+                            const int64_t idx = mnode * S + cc;
+                            const double mv = me.get(e, a, cr, bnode, cc);
+                            if (lift && dof_id[idx] < 0) t -= g[idx] * (mfac * mv);
+                            if (hist) f += mv * hist[idx];
+                        }
+                    }
+                    acc += t;
+                    acc += f;
                 }
             }
             b[r] = acc;
@@ -1387,14 +1478,25 @@ void fill_tables(int dim, int nq1, std::vector<double>& t, int* face_off, int* n
     *nqf = nqfv;
 }
 
+template <int DIM, int NQ1, typename T>
+void launch_laplace_nq(Ctx* c, unsigned g, const T* tab, Bf bf, T* ke, double* fe, T* me)
+{
+    if (me)
+        k_elem_laplace<DIM, NQ1, T, true><<<g, BLK, 0, c->stream>>>(
+            c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe, me, (T)c->density);
+    else
+        k_elem_laplace<DIM, NQ1, T, false><<<g, BLK, 0, c->stream>>>(
+            c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe);
+}
+
 template <int DIM, typename T>
-int launch_laplace(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
+int launch_laplace(Ctx* c, const T* tab, Bf bf, T* ke, double* fe, T* me)
 {
     const unsigned g = grid_for(c->n_elem, BLK);
     switch (c->nq) {
-    case 1: k_elem_laplace<DIM, 1, T><<<g, BLK, 0, c->stream>>>(c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
-    case 2: k_elem_laplace<DIM, 2, T><<<g, BLK, 0, c->stream>>>(c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
-    case 3: k_elem_laplace<DIM, 3, T><<<g, BLK, 0, c->stream>>>(c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 1: launch_laplace_nq<DIM, 1, T>(c, g, tab, bf, ke, fe, me); break;
+    case 2: launch_laplace_nq<DIM, 2, T>(c, g, tab, bf, ke, fe, me); break;
+    case 3: launch_laplace_nq<DIM, 3, T>(c, g, tab, bf, ke, fe, me); break;
     default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1402,13 +1504,13 @@ int launch_laplace(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
 }
 
 template <int DIM, typename T>
-int launch_laplace_q2(Ctx* c, const T* tab, Bf bf, T* ke, double* fe)
+int launch_laplace_q2(Ctx* c, const T* tab, Bf bf, T* ke, double* fe, T* me)
 {
     const unsigned g = grid_for(c->n_elem, 4, 1u << 20);
     switch (c->nq) {
-    case 1: k_elem_laplace_q2<DIM, 1, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
-    case 2: k_elem_laplace_q2<DIM, 2, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
-    case 3: k_elem_laplace_q2<DIM, 3, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe); break;
+    case 1: k_elem_laplace_q2<DIM, 1, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe, me, (T)c->density); break;
+    case 2: k_elem_laplace_q2<DIM, 2, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe, me, (T)c->density); break;
+    case 3: k_elem_laplace_q2<DIM, 3, T><<<g, 256, 0, c->stream>>>(c->n_elem, c->gconn.p, c->xyz.p, c->coef.p, c->ncoef, tab, bf, ke, fe, me, (T)c->density); break;
     default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1489,9 +1591,10 @@ int lg2(int v)
     return l;
 }
 
-template <int NPE, int S, class KE>
-int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int incremental,
-                      const double* fe, const double* ff, int precision, int* derr)
+template <int NPE, int S, class KE, class ME>
+int launch_gather_rhs(Ctx* c, KE ke, ME me, int do_matrix, int do_rhs, int lift,
+                      int incremental, const double* fe, const double* ff, int precision,
+                      int* derr)
 {
     if (do_matrix) {
         int G = 1;
@@ -1507,22 +1610,22 @@ int launch_gather_rhs(Ctx* c, KE ke, int do_matrix, int do_rhs, int lift, int in
             else pw = c->pinfo64.p;
         }
         if (precision == INS_FP64)
-            k_gather<NPE, S, KE, double><<<g, BLK, 0, c->stream>>>(
+            k_gather<NPE, S, KE, double, ME><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
                 c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, pw,
-                c->monotone ? 1 : 0, ke, v64, v32, derr);
+                c->monotone ? 1 : 0, ke, me, c->mass_factor, v64, v32, derr);
         else
-            k_gather<NPE, S, KE, float><<<g, BLK, 0, c->stream>>>(
+            k_gather<NPE, S, KE, float, ME><<<g, BLK, 0, c->stream>>>(
                 c->n_nodes, lgG, c->lg_npe, c->adj_ptr.p, c->adj.p, c->nb_ptr.p, c->nb.p,
                 c->nboff.p, c->dof_id.p, c->row_ptr.p, c->col.p, c->conn.p, pw,
-                c->monotone ? 1 : 0, ke, nullptr, v32, derr);
+                c->monotone ? 1 : 0, ke, me, c->mass_factor, nullptr, v32, derr);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     if (do_rhs) {
-        k_rhs<NPE, S, KE><<<grid_for(c->n_nodes, BLK), BLK, 0, c->stream>>>(
+        k_rhs<NPE, S, KE, ME><<<grid_for(c->n_nodes, BLK), BLK, 0, c->stream>>>(
             c->n_nodes, c->lg_npe, c->adj_ptr.p, c->adj.p, c->conn.p, c->dof_id.p, c->g.p, c->u.p,
             incremental, lift, ke, fe, lg2(c->nfn > 0 ? c->nfn : 1), c->fadj_ptr.p, c->fadj.p, ff,
-            c->nfn, c->b.p);
+            c->nfn, me, c->mass_factor, c->have_mhist ? c->mhist.p : nullptr, c->b.p);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     return INS_OK;
@@ -1533,16 +1636,17 @@ int gather_q2_t(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int i
                 const double* fe, int precision, int* derr)
 {
     const T* st = (const T*)c->stage_ke.p;
+    const KeLaplace<T, NF> me{(!elastic && c->density > 0.) ? (const T*)c->stage_me.p : nullptr};
     const int s = c->dpn;
     constexpr int D = NF == 27 ? 3 : 2;
     if (elastic)
-        return launch_gather_rhs<NF, D>(c, KeElastic<T, NF, D>{st}, do_matrix, do_rhs, lift,
+        return launch_gather_rhs<NF, D>(c, KeElastic<T, NF, D>{st}, me, do_matrix, do_rhs, lift,
                                         incremental, fe, nullptr, precision, derr);
     if (s == 1)
-        return launch_gather_rhs<NF, 1>(c, KeLaplace<T, NF>{st}, do_matrix, do_rhs, lift,
+        return launch_gather_rhs<NF, 1>(c, KeLaplace<T, NF>{st}, me, do_matrix, do_rhs, lift,
                                         incremental, fe, nullptr, precision, derr);
     if (s == D)
-        return launch_gather_rhs<NF, D>(c, KeLaplace<T, NF>{st}, do_matrix, do_rhs, lift,
+        return launch_gather_rhs<NF, D>(c, KeLaplace<T, NF>{st}, me, do_matrix, do_rhs, lift,
                                         incremental, fe, nullptr, precision, derr);
     return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
 }
@@ -1630,6 +1734,12 @@ int assemble(Ctx* c, int precision, int flags)
     if (do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
     const bool need_fe = do_rhs && (c->bf_kind != INS_BF_NONE || want_r);
     if (need_fe) INS_TRY(ensure(c, c->stage_fe, (size_t)E * npe * s));
+    // time-step mass terms (heat::Laplace only)
+    const bool mass = c->density > 0.;
+    if (mass && elastic)
+        return fail(c, INS_ERR_UNSUPPORTED, "mass terms are provided for heat::Laplace");
+    if (mass) INS_TRY(ensure(c, c->stage_me, ke_elems * esize));
+    const void* mep = mass ? (const void*)c->stage_me.p : nullptr;
     if (do_matrix) {
         if (precision == INS_FP64) INS_TRY(ensure(c, c->val64, c->nnz));
         INS_TRY(ensure(c, c->val32, c->nnz));
@@ -1644,22 +1754,22 @@ int assemble(Ctx* c, int precision, int flags)
     // 1. element kernels
     if (c->profiling) INS_HIP_CHECK(c, hipEventRecord(c->ev_el[0], c->stream));
     if (!elastic) {
-        if (do_matrix || need_fe) {
+        if (do_matrix || need_fe || mass) {
             if (!do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
             if (c->deg == 2) {
                 if (precision == INS_FP64) {
-                    if (c->dim == 3) INS_TRY((launch_laplace_q2<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
-                    else INS_TRY((launch_laplace_q2<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
+                    if (c->dim == 3) INS_TRY((launch_laplace_q2<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe, (double*)mep)));
+                    else INS_TRY((launch_laplace_q2<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe, (double*)mep)));
                 } else {
-                    if (c->dim == 3) INS_TRY((launch_laplace_q2<3, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
-                    else INS_TRY((launch_laplace_q2<2, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
+                    if (c->dim == 3) INS_TRY((launch_laplace_q2<3, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe, (float*)mep)));
+                    else INS_TRY((launch_laplace_q2<2, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe, (float*)mep)));
                 }
             } else if (precision == INS_FP64) {
-                if (c->dim == 3) INS_TRY((launch_laplace<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
-                else INS_TRY((launch_laplace<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe)));
+                if (c->dim == 3) INS_TRY((launch_laplace<3, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe, (double*)mep)));
+                else INS_TRY((launch_laplace<2, double>(c, c->tab64.p, bf, (double*)c->stage_ke.p, fe, (double*)mep)));
             } else {
-                if (c->dim == 3) INS_TRY((launch_laplace<3, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
-                else INS_TRY((launch_laplace<2, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe)));
+                if (c->dim == 3) INS_TRY((launch_laplace<3, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe, (float*)mep)));
+                else INS_TRY((launch_laplace<2, float>(c, c->tab32.p, bf, (float*)c->stage_ke.p, fe, (float*)mep)));
             }
         }
     } else {
@@ -1699,25 +1809,25 @@ int assemble(Ctx* c, int precision, int flags)
         INS_TRY(gather_q2(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr));
     } else if (!elastic) {
         if (precision == INS_FP64) {
-            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
             else return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
         } else {
-            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
             else return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
         }
     } else {
         if (precision == INS_FP64) {
-            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<double, 8, 3>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<double, 4, 2>{(const double*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<double, 8, 3>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<double, 4, 2>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
         } else {
-            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<float, 8, 3>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<float, 4, 2>{(const float*)c->stage_ke.p}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<float, 8, 3>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<float, 4, 2>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
         }
     }
     if (do_matrix) {

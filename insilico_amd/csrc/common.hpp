@@ -156,6 +156,11 @@ struct Ctx {
     int nfn = 0;
     double traction[3] = {0, 0, 0};
     DArr<int32_t> faces, fadj_ptr, fadj;
+    // time-step terms: density of the mass matrix (0 = none), its factor in
+    // the matrix (a0 / (b0 dt)) and the history combination of the RHS
+    double density = 0., mass_factor = 0.;
+    DArr<double> mhist;
+    bool have_mhist = false;
 
     // assembly products
     bool have_matrix = false, have_rhs = false;
@@ -166,6 +171,7 @@ struct Ctx {
     DArr<double> dinv64;
     DArr<float> dinv32;
     DArr<uint8_t> stage_ke;  // element matrices (type depends on precision)
+    DArr<uint8_t> stage_me;  // element mass matrices (time steps)
     DArr<double> stage_fe;   // element force vectors (FP64)
     DArr<double> stage_ff;   // face force vectors
     Sell sell;

@@ -495,6 +495,58 @@ int orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe)
     return 0;
 }
 
+/* Element mass matrix: base/kernel/Mass.hpp:80-110 (scalar = factor * detJ
+ * * weight, entry = scalar * phi_M * phi_N on every component), integrated
+ * with the element's quadrature (base/Quadrature.hpp:133-141). */
+int orc_mass_element(const orc_problem* p, int64_t e, double* Me)
+{
+    const int dim = p->dim, gnpe = p->npe, s = p->dpn;
+    const int npe = field_npe(p), nee = npe * s, K = p->deg == 2 ? 2 : 1;
+    double X[8 * 3];
+    for (int n = 0; n < gnpe; n++) {
+        const int32_t node = p->conn[e * gnpe + n];
+        for (int d = 0; d < dim; d++) X[n * dim + d] = p->xyz[(int64_t)node * dim + d];
+    }
+    double wq[64], xq[64 * 3];
+    const int nq = orc_tensor_rule(dim, p->nq, wq, xq);
+    if (nq < 0) return -1;
+    for (int i = 0; i < nee * nee; i++) Me[i] = 0.;
+    for (int q = 0; q < nq; q++) {
+        double gphi[8], gdphi[8 * 3], gx[ORC_MAXF * 3];
+        double phi[ORC_MAXF], dphi[ORC_MAXF * 3];
+        orc_q1_shape(dim, xq + q * dim, gphi, gdphi);
+        double detJ;
+        if (K == 1) {
+            for (int n = 0; n < gnpe; n++) phi[n] = gphi[n];
+            detJ = orc_contravariant(dim, X, gdphi, gx);
+        } else {
+            orc_lagrange_shape(dim, K, xq + q * dim, phi, dphi);
+            detJ = contra_field(dim, X, gdphi, npe, dphi, gx);
+        }
+        const double scalar = p->density * detJ * wq[q];
+        for (int M = 0; M < npe; M++)
+            for (int N = 0; N < npe; N++) {
+                const double entry = scalar * phi[M] * phi[N];
+                for (int d = 0; d < s; d++) Me[(M * s + d) * nee + N * s + d] += entry;
+            }
+    }
+    return 0;
+}
+
+/* BDF weights (base/time/BDF.hpp:69-99, as written there) and the start-up
+ * rule of base/time/MultiStep.hpp:46-63: step + 2 < order + 1 falls back to
+ * the method of one order less. */
+int orc_bdf_weights(int order, int step, double* a)
+{
+    while (order > 1 && step + 2 < order + 1) order--;
+    static const double w[4][5] = {{1.0, -1.0}, {1.5, -2.0, 0.5},
+                                   {11. / 6., -3.0, 1.5, -1. / 3.},
+                                   {25. / 12., -4.0, 3.0, 4. / 3., 1. / 4.}};
+    if (order < 1 || order > 4) return -1;
+    for (int i = 0; i <= order; i++) a[i] = w[order - 1][i];
+    return order + 1;
+}
+
 /* Structured unit-cube mesh: tools/meshGeneration/unitCube/unitCube.hpp
  * :85-106 (nodes, i1 fastest, x = h*i with h = 1/e) and :217-257 (elements,
  * i1 fastest, lexicographic corner list reordered to hierarchic order by
@@ -757,6 +809,46 @@ int orc_assemble(const orc_problem* p, const int32_t* row_ptr,
                 const int32_t id = p->dof_id[(int64_t)ffc[f * ffn + nn] * s + c];
                 if (id >= 0) b[id] += ff[nn * s + c];
             }
+    }
+    /* time integration terms: base/time/ReactionTerms.hpp:174-230, one
+     * more serial element loop (lifting of mass_factor * M_e with the
+     * prescribed values, then M_e * mass_hist on the active rows) */
+    if (p->density > 0.) {
+        for (int64_t e = 0; e < p->n_elem; e++) {
+            double Me[ORC_MAXE * ORC_MAXE];
+            orc_mass_element(p, e, Me);
+            int32_t ids[ORC_MAXE];
+            int loc[ORC_MAXE];
+            int st[ORC_MAXE];
+            double gv[ORC_MAXE], hv[ORC_MAXE];
+            for (int nn = 0; nn < npe; nn++) {
+                const int64_t node = fc[e * npe + nn];
+                for (int c = 0; c < s; c++) {
+                    const int64_t k = node * s + c;
+                    st[nn * s + c] = p->dof_id[k] >= 0;
+                    gv[nn * s + c] = p->g ? p->g[k] : 0.;
+                    hv[nn * s + c] = p->mass_hist ? p->mass_hist[k] : 0.;
+                }
+            }
+            const int na = active_ids(p, e, ids, loc);
+            for (int r = 0; r < na; r++) {
+                const int lr = loc[r];
+                double sv = 0.;
+                for (int c = 0; c < nee; c++)
+                    if (!st[c]) sv -= gv[c] * (p->mass_factor * Me[lr * nee + c]);
+                for (int c = 0; c < na; c++) {
+                    const int64_t k = triplet_find(T, nnz, ids[r], ids[c]);
+                    if (k < 0) { err = -2; continue; }
+                    T[k].val += p->mass_factor * Me[lr * nee + loc[c]];
+                }
+                b[ids[r]] += sv;
+            }
+            for (int r = 0; r < na; r++) {
+                double f = 0.;
+                for (int c = 0; c < nee; c++) f += Me[loc[r] * nee + c] * hv[c];
+                b[ids[r]] += f;
+            }
+        }
     }
     for (int64_t k = 0; k < nnz; k++) val[k] = T[k].val;
     const double t2 = now_s();

@@ -82,6 +82,15 @@ typedef struct orc_problem {
     int tr_kind;
     int tr_axis;
     double tr_x0;
+    /* Time integration terms (base/time/ReactionTerms.hpp:140-236): with
+     * density > 0 the element mass matrix M_e (base/kernel/Mass.hpp, rho phi
+     * phi detJ w) enters as mass_factor * M_e (= a0/(b0 dt) M_e) into the
+     * matrix and its lifting, and M_e * mass_hist (mass_hist per node and
+     * component = sum_s -a_s/dt u^{n+1-s}, NULL = 0) into the RHS, in a
+     * second element loop after the stiffness and the forces. */
+    double density;
+    double mass_factor;
+    const double* mass_hist;
 } orc_problem;
 
 int  orc_gauss1d(int npts, double* w, double* x);
@@ -91,6 +100,11 @@ void orc_lagrange_shape(int dim, int deg, const double* xi, double* phi, double*
 double orc_contravariant(int dim, const double* X, const double* dphi,
                          double* gradx);
 int  orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe);
+/* element mass matrix rho phi_M phi_N detJ w (nee x nee, row-major) */
+int  orc_mass_element(const orc_problem* p, int64_t e, double* Me);
+/* BDF weights a_0..a_q with the start-up of base/time/MultiStep.hpp (order
+ * min(q, step+1)); returns the number of weights */
+int  orc_bdf_weights(int order, int step, double* a);
 int  orc_unit_cube(int dim, int e1, int e2, int e3, double* xyz, int32_t* conn);
 
 /* pattern as the reference's TripletContainer::registerFields builds it */

@@ -35,6 +35,7 @@ class Problem(C.Structure):
         ("deg", C.c_int), ("fnpe", C.c_int), ("fconn", C.c_void_p),
         ("ffn", C.c_int), ("ffaces", C.c_void_p),
         ("tr_kind", C.c_int), ("tr_axis", C.c_int), ("tr_x0", C.c_double),
+        ("density", C.c_double), ("mass_factor", C.c_double), ("mass_hist", C.c_void_p),
     ]
 
 
@@ -58,6 +59,8 @@ def lib():
         L.orc_contravariant.argtypes = [C.c_int, _f64p, _f64p, _f64p]
         L.orc_contravariant.restype = C.c_double
         L.orc_element.argtypes = [C.POINTER(Problem), C.c_int64, _f64p, _f64p]
+        L.orc_mass_element.argtypes = [C.POINTER(Problem), C.c_int64, _f64p]
+        L.orc_bdf_weights.argtypes = [C.c_int, C.c_int, _f64p]
         L.orc_unit_cube.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _i32p]
         L.orc_pattern.argtypes = [C.POINTER(Problem), C.POINTER(C.c_int64),
                                   C.c_void_p, C.c_void_p]
@@ -128,6 +131,14 @@ class OracleProblem:
                 p.tr_kind, p.tr_axis, p.tr_x0 = 1, int(lin[0]), float(lin[1])
         p.residual = int(residual)
         p.nthreads = nthreads
+        tr = getattr(prob, "meta", {}).get("transient")
+        if tr is not None:
+            p.density = float(tr["density"])
+            p.mass_factor = float(tr["mass_factor"])
+            if tr.get("hist") is not None:
+                h = np.ascontiguousarray(tr["hist"], np.float64)
+                self.keep.append(h)
+                p.mass_hist = _ptr(h)
         if getattr(prob, "deg", 1) == 2:
             fconn = np.ascontiguousarray(prob.fconn, np.int32)
             self.keep.append(fconn)
@@ -178,6 +189,21 @@ def element(prob, e, **kw):
     Fe = np.zeros(nee, np.float64)
     lib().orc_element(op.ref(), e, Ke, Fe)
     return Ke.reshape(nee, nee), Fe
+
+
+def mass_element(prob, e):
+    op = OracleProblem(prob)
+    fc = prob.fconn if getattr(prob, "deg", 1) == 2 else prob.conn
+    nee = fc.shape[1] * prob.dpn
+    Me = np.zeros(nee * nee, np.float64)
+    lib().orc_mass_element(op.ref(), e, Me)
+    return Me.reshape(nee, nee)
+
+
+def bdf_weights(order, step):
+    a = np.zeros(5)
+    k = lib().orc_bdf_weights(order, step, a)
+    return a[:k]
 
 
 def pcg(rp, ci, val, b, tol=1e-8, maxit=None, nthreads=1):
