@@ -236,6 +236,20 @@ int ins_solution_norm(ins_ctx* ctx, double* norm);
  * components get their prescribed value. */
 int ins_distribute(ins_ctx* ctx, int op);
 
+/* ---- post-processing -------------------------------------------------------- */
+#define INS_REF_ZERO     0  /* u = 0: the norm of u_h itself                   */
+#define INS_REF_SINPROD  1  /* u = params[0] * prod_d sin(params[1+d] * x_d)   */
+#define INS_REF_KELVIN   2  /* elastostatic fundamental solution
+                               (base/auxi/FundamentalSolution.hpp:160-200):
+                               params = lambda, mu, source[3], direction[3]   */
+/* base::post::errorComputation<order> (base/post/ErrorNorm.hpp:60-84,
+ * 200-224): |u - u_h| in the L2 norm (order 0) or the H1 semi-norm (order
+ * 1; not for INS_REF_KELVIN), integrated with the context's quadrature.
+ * u_h: values per field node and component (NULL = the context's current
+ * DoF values, as ins_distribute leaves them). */
+int ins_error_norm(ins_ctx* ctx, int order, int ref_kind, const double* ref_params,
+                   const double* u, double* out);
+
 /* ---- serving: many solves batched into shared launches --------------------
  * No single reference counterpart: the reference solves one system per
  * Eigen3::cgSolve call (base/solver/Eigen3.hpp:265-275).  A server runs the

@@ -34,9 +34,10 @@ SYMBOLS = [
     "ins_rhs_norm", "ins_set_csr", "ins_solve", "ins_get_solution", "ins_solution_norm",
     "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
-    "ins_server_running", "ins_set_field", "ins_set_mass",
+    "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
+REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
 
 
 class Stats(C.Structure):
@@ -80,6 +81,7 @@ def load():
         "ins_set_mesh": [vp, i32, i64, vp, i64, i32, vp],
         "ins_set_field": [vp, i32, i64, i32, vp],
         "ins_set_mass": [vp, dbl, dbl, vp],
+        "ins_error_norm": [vp, i32, i32, vp, vp, C.POINTER(dbl)],
         "ins_set_dofs": [vp, i32, vp, i64],
         "ins_set_prescribed": [vp, vp],
         "ins_scale_prescribed": [vp, dbl],
@@ -177,6 +179,17 @@ class Context:
         """time-step terms: matrix + mass_factor * M, RHS + M hist"""
         h = None if hist is None else np.ascontiguousarray(hist, np.float64)
         self._chk(self.L.ins_set_mass(self.h, density, mass_factor, _p(h)))
+
+    def error_norm(self, order, kind, params=None, u=None):
+        """|u - u_h| (order 0: L2, order 1: H1 semi-norm) on the device; u_h
+        from `u` (per field node and component) or the current DoF values"""
+        prm = np.zeros(8)
+        if params is not None:
+            prm[:len(params)] = params
+        uu = None if u is None else np.ascontiguousarray(u, np.float64)
+        out = C.c_double()
+        self._chk(self.L.ins_error_norm(self.h, order, kind, _p(prm), _p(uu), C.byref(out)))
+        return out.value
 
     def set_dofs(self, dpn, dof_id, n_active):
         d = np.ascontiguousarray(dof_id, np.int32)

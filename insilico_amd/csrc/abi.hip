@@ -761,4 +761,12 @@ int ins_server_running(const ins_server* h, int* n)
     return INS_OK;
 }
 
+int ins_error_norm(ins_ctx* h, int order, int ref_kind, const double* ref_params,
+                   const double* u, double* out)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    return error_norm(c, order, ref_kind, ref_params, u, out);
+}
+
 }  // extern "C"

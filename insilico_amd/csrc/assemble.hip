@@ -1250,6 +1250,148 @@ __global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Error norms (base/post/ErrorNorm.hpp:60-84,200-224): one element per lane;
+// per quadrature point x(xi) and detJ from the Q1 geometry, u_h or grad u_h
+// from the field functions, errorSquared += (ref - approx).(ref - approx)
+// detJ w.  Lanes sum their elements in order, blocks reduce in a fixed tree,
+// the host adds the block partials in block order.
+struct RefFn {
+    int kind;
+    double p[8];
+};
+
+template <int DIM>
+__device__ inline void ref_value(const RefFn& r, const double* x, int s, double* v, double* g)
+{
+    for (int c = 0; c < 3; c++) v[c] = 0.;
+    for (int i = 0; i < 9; i++) g[i] = 0.;
+    if (r.kind == INS_REF_SINPROD) {
+        double val = r.p[0];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) val *= sin(r.p[1 + d] * x[d]);
+        for (int c = 0; c < s; c++) v[c] = val;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            double gd = r.p[0] * r.p[1 + d] * cos(r.p[1 + d] * x[d]);
+#pragma unroll
+            for (int e = 0; e < DIM; e++)
+                if (e != d) gd *= sin(r.p[1 + e] * x[e]);
+            for (int c = 0; c < s; c++) g[d * s + c] = gd;
+        }
+    } else if (r.kind == INS_REF_KELVIN) {
+        // base/auxi/FundamentalSolution.hpp:160-200
+        const double lam = r.p[0], mu = r.p[1];
+        const double nu = lam / 2. / (lam + mu);
+        const double surf = 2. * (DIM - 1.) * M_PI;
+        const double fac1 = 1. / (4. * surf * mu * (1. - nu));
+        const double fac2 = 3. - 4. * nu;
+        double R[3], dist = 0.;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            R[d] = x[d] - r.p[2 + d];
+            dist += R[d] * R[d];
+        }
+        dist = sqrt(dist);
+        const double lk = (DIM == 2) ? log(1. / dist) : 1. / dist;
+        const double pw = (DIM == 2) ? dist * dist : dist * dist * dist;
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            double sum = 0.;
+#pragma unroll
+            for (int j = 0; j < DIM; j++) {
+                const double Uij = fac1 * ((i == j ? fac2 * lk : 0.) + (R[i] * R[j]) / pw);
+                sum += Uij * r.p[5 + j];
+            }
+            v[i] = sum;
+        }
+    }
+}
+
+template <int DIM, int NF, int S>
+__global__ __launch_bounds__(256) void k_error_norm(
+    int64_t E, const int32_t* __restrict__ gconn, const int32_t* __restrict__ fconn,
+    const double* __restrict__ xyz, const double* __restrict__ u,
+    const double* __restrict__ gd, const double* __restrict__ gp,
+    const double* __restrict__ fd, const double* __restrict__ fp,
+    const double* __restrict__ wq, int nq, int order, RefFn ref, double* __restrict__ part)
+{
+    constexpr int GN = 1 << DIM;
+    double acc = 0.;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        double X[GN * DIM];
+#pragma unroll
+        for (int n = 0; n < GN; n++) {
+            const int64_t node = gconn[e * GN + n];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) X[n * DIM + d] = xyz[node * DIM + d];
+        }
+        double err = 0.;
+        for (int q = 0; q < nq; q++) {
+            double inv[DIM][DIM];
+            const double det = geo_inverse<DIM, double>(X, gd + q * GN * DIM, inv);
+            double x[3] = {0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < GN; n++)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) x[d] += X[n * DIM + d] * gp[q * GN + n];
+            double rv[3], rg[9];
+            ref_value<DIM>(ref, x, S, rv, rg);
+            double d2 = 0.;
+            if (order == 0) {
+#pragma unroll
+                for (int c = 0; c < S; c++) {
+                    double v = 0.;
+                    for (int n = 0; n < NF; n++)
+                        v += u[(int64_t)fconn[e * NF + n] * S + c] * fp[q * NF + n];
+                    const double er = rv[c] - v;
+                    d2 += er * er;
+                }
+            } else {
+                double g[DIM][S];
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+#pragma unroll
+                    for (int c = 0; c < S; c++) g[d][c] = 0.;
+                for (int n = 0; n < NF; n++) {
+                    const double* f = fd + (q * NF + n) * DIM;
+                    double gx[DIM];
+#pragma unroll
+                    for (int i = 0; i < DIM; i++) {
+                        double t = 0.;
+#pragma unroll
+                        for (int a = 0; a < DIM; a++) t += inv[i][a] * f[a];
+                        gx[i] = t;
+                    }
+                    const int64_t node = fconn[e * NF + n];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+#pragma unroll
+                        for (int c = 0; c < S; c++) g[d][c] += gx[d] * u[node * S + c];
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+#pragma unroll
+                    for (int c = 0; c < S; c++) {
+                        const double er = rg[d * S + c] - g[d][c];
+                        d2 += er * er;
+                    }
+            }
+            err += d2 * det * wq[q];
+        }
+        acc += err;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // Jacobi: inverse diagonal of each row (Eigen DiagonalPreconditioner:
 // 1/a_ii, or 1 where a_ii = 0)
 __global__ void k_dinv(int64_t n, const int32_t* __restrict__ row_ptr,
@@ -1852,6 +1994,85 @@ int assemble(Ctx* c, int precision, int flags)
     if (do_matrix) c->have_matrix = true;
     if (do_rhs) c->have_rhs = true;
     c->have_solution = false;
+    return INS_OK;
+}
+
+template <int DIM, int NF, int S>
+static void launch_error_norm(Ctx* c, unsigned g, const double* u, int order, const RefFn& ref,
+                              double* part)
+{
+    const double* t = c->tab64.p;
+    int nq = 1;
+    for (int d = 0; d < DIM; d++) nq *= c->nq;
+    const int gn = 1 << DIM;
+    const double *gd, *gp, *fd, *fp, *w;
+    if (c->deg == 2) {
+        gd = t;
+        gp = gd + (size_t)nq * gn * DIM;
+        fd = gp + (size_t)nq * gn;
+        fp = fd + (size_t)nq * NF * DIM;
+        w = fp + (size_t)nq * NF;
+    } else {
+        gd = fd = t;
+        gp = fp = t + (size_t)nq * gn * DIM;
+        w = gp + (size_t)nq * gn;
+    }
+    k_error_norm<DIM, NF, S><<<g, 256, 0, c->stream>>>(c->n_elem, c->geo_conn(), c->conn.p,
+                                                        c->xyz.p, u, gd, gp, fd, fp, w, nq,
+                                                        order, ref, part);
+}
+
+int error_norm(Ctx* c, int order, int kind, const double* prm, const double* hu, double* out)
+{
+    if (c->kernel == 0) return fail(c, INS_ERR_STATE, "ins_error_norm before ins_set_material");
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_error_norm before ins_set_dofs");
+    if (order != 0 && order != 1) return fail(c, INS_ERR_ARG, "order must be 0 (L2) or 1 (H1)");
+    if (kind != INS_REF_ZERO && kind != INS_REF_SINPROD && kind != INS_REF_KELVIN)
+        return fail(c, INS_ERR_ARG, "unknown reference solution");
+    if (kind == INS_REF_KELVIN && (order != 0 || c->dpn != c->dim))
+        return fail(c, INS_ERR_UNSUPPORTED, "the fundamental solution is a vector L2 reference");
+    if ((kind != INS_REF_ZERO && !prm) || !out) return fail(c, INS_ERR_ARG, "ins_error_norm arguments");
+    if (c->tab_deg != c->deg) INS_TRY(make_tables(c));
+    RefFn ref;
+    ref.kind = kind;
+    for (int i = 0; i < 8; i++) ref.p[i] = (kind != INS_REF_ZERO) ? prm[i] : 0.;
+    const int64_t nd = c->n_nodes * c->dpn;
+    const double* u = c->u.p;
+    double* tmp = nullptr;
+    if (hu) {
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&tmp, nd * sizeof(double), c->stream));
+        INS_HIP_CHECK(c, hipMemcpyAsync(tmp, hu, nd * sizeof(double), hipMemcpyHostToDevice,
+                                        c->stream));
+        u = tmp;
+    }
+    const unsigned g = grid_for(c->n_elem, 256, 1024);
+    double* part = nullptr;
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&part, g * sizeof(double), c->stream));
+    const int key = c->dim * 1000 + c->npe * 10 + c->dpn;
+    switch (key) {
+    case 2041: launch_error_norm<2, 4, 1>(c, g, u, order, ref, part); break;
+    case 2042: launch_error_norm<2, 4, 2>(c, g, u, order, ref, part); break;
+    case 3081: launch_error_norm<3, 8, 1>(c, g, u, order, ref, part); break;
+    case 3083: launch_error_norm<3, 8, 3>(c, g, u, order, ref, part); break;
+    case 2091: launch_error_norm<2, 9, 1>(c, g, u, order, ref, part); break;
+    case 2092: launch_error_norm<2, 9, 2>(c, g, u, order, ref, part); break;
+    case 3271: launch_error_norm<3, 27, 1>(c, g, u, order, ref, part); break;
+    case 3273: launch_error_norm<3, 27, 3>(c, g, u, order, ref, part); break;
+    default:
+        (void)hipFreeAsync(part, c->stream);
+        if (tmp) (void)hipFreeAsync(tmp, c->stream);
+        return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
+    }
+    INS_HIP_CHECK(c, hipGetLastError());
+    std::vector<double> h(g);
+    INS_HIP_CHECK(c, hipMemcpyAsync(h.data(), part, g * sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    (void)hipFreeAsync(part, c->stream);
+    if (tmp) (void)hipFreeAsync(tmp, c->stream);
+    double sum = 0.;
+    for (unsigned i = 0; i < g; i++) sum += h[i];
+    *out = std::sqrt(sum);
     return INS_OK;
 }
 

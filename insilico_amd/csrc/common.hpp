@@ -283,6 +283,7 @@ int build_pattern(Ctx* c);
 int make_tables(Ctx* c);
 int assemble(Ctx* c, int precision, int flags);
 int compute_dinv(Ctx* c);
+int error_norm(Ctx* c, int order, int kind, const double* prm, const double* u, double* out);
 // solve.hip
 int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
           double* relres);

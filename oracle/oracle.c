@@ -998,3 +998,88 @@ double orc_l2_error_fundsol(const orc_problem* p, const double* uh,
     }
     return sqrt(err);
 }
+
+/* Reference solution and its gradient at x (value[c], grad[d*s+c]). */
+static void ref_solution(int dim, int s, int kind, const double* prm, const double* x,
+                         double* val, double* grad)
+{
+    for (int c = 0; c < s; c++) val[c] = 0.;
+    for (int i = 0; i < dim * s; i++) grad[i] = 0.;
+    if (kind == ORC_REF_SINPROD) {
+        double v = prm[0];
+        for (int d = 0; d < dim; d++) v *= sin(prm[1 + d] * x[d]);
+        for (int c = 0; c < s; c++) val[c] = v;
+        for (int d = 0; d < dim; d++) {
+            double g = prm[0] * prm[1 + d] * cos(prm[1 + d] * x[d]);
+            for (int e = 0; e < dim; e++)
+                if (e != d) g *= sin(prm[1 + e] * x[e]);
+            for (int c = 0; c < s; c++) grad[d * s + c] = g;
+        }
+    } else if (kind == ORC_REF_KELVIN) {
+        orc_fundsol_elasto(dim, prm[0], prm[1], x, prm + 2, prm + 5, val);
+    }
+}
+
+/* Error norm: base/post/ErrorNorm.hpp:60-84 (errorComputation: element loop,
+ * sqrt of the sum) and :200-224 (per point: x(xi), detJ, ref - approx,
+ * errorSquared += dot(error, error) * detJ * weight); approximation by
+ * base/post/evaluateField.hpp (value: sum_n phi_n u_n, gradient: sum_n
+ * grad_x phi_n u_n). */
+double orc_error_norm(const orc_problem* p, const double* uh, int order, int kind,
+                      const double* prm)
+{
+    const int dim = p->dim, gnpe = p->npe, s = p->dpn;
+    const int npe = field_npe(p), K = p->deg == 2 ? 2 : 1;
+    const int32_t* fc = field_conn(p);
+    double wq[64], xq[64 * 3];
+    const int nq = orc_tensor_rule(dim, p->nq, wq, xq);
+    if (nq < 0 || (order == 1 && kind == ORC_REF_KELVIN)) return -1.;
+    double err = 0.;
+    for (int64_t e = 0; e < p->n_elem; e++) {
+        double X[8 * 3], U[ORC_MAXE];
+        for (int n = 0; n < gnpe; n++) {
+            const int64_t node = p->conn[e * gnpe + n];
+            for (int d = 0; d < dim; d++) X[n * dim + d] = p->xyz[node * dim + d];
+        }
+        for (int n = 0; n < npe; n++) {
+            const int64_t node = fc[e * npe + n];
+            for (int c = 0; c < s; c++) U[n * s + c] = uh[node * s + c];
+        }
+        for (int q = 0; q < nq; q++) {
+            double gphi[8], gdphi[8 * 3], phi[ORC_MAXF], dphi[ORC_MAXF * 3], gx[ORC_MAXF * 3];
+            orc_q1_shape(dim, xq + q * dim, gphi, gdphi);
+            double detJ;
+            if (K == 1) {
+                for (int n = 0; n < gnpe; n++) phi[n] = gphi[n];
+                detJ = orc_contravariant(dim, X, gdphi, gx);
+            } else {
+                orc_lagrange_shape(dim, K, xq + q * dim, phi, dphi);
+                detJ = contra_field(dim, X, gdphi, npe, dphi, gx);
+            }
+            double x[3] = {0, 0, 0};
+            for (int n = 0; n < gnpe; n++)
+                for (int d = 0; d < dim; d++) x[d] += X[n * dim + d] * gphi[n];
+            double rv[3], rg[9];
+            ref_solution(dim, s, kind, prm, x, rv, rg);
+            double d2 = 0.;
+            if (order == 0) {
+                for (int c = 0; c < s; c++) {
+                    double v = 0.;
+                    for (int n = 0; n < npe; n++) v += U[n * s + c] * phi[n];
+                    const double er = rv[c] - v;
+                    d2 += er * er;
+                }
+            } else {
+                for (int d = 0; d < dim; d++)
+                    for (int c = 0; c < s; c++) {
+                        double g = 0.;
+                        for (int n = 0; n < npe; n++) g += gx[n * dim + d] * U[n * s + c];
+                        const double er = rg[d * s + c] - g;
+                        d2 += er * er;
+                    }
+            }
+            err += d2 * detJ * wq[q];
+        }
+    }
+    return sqrt(err);
+}

@@ -126,6 +126,14 @@ double orc_l2_error_fundsol(const orc_problem* p, const double* uh,
                             const double* dir);
 void orc_fundsol_elasto(int dim, double lambda, double mu, const double* y,
                         const double* x, const double* dir, double* out);
+/* reference solutions of the error norm */
+enum { ORC_REF_ZERO = 0, ORC_REF_SINPROD = 1, ORC_REF_KELVIN = 2 };
+/* |u - u_h| in the L2 norm (order 0) or the H1 semi-norm (order 1) with the
+ * problem's quadrature; uh per field node and component; prm: SINPROD amp,
+ * k_x, k_y, k_z (u = amp prod_d sin(k_d x_d)); KELVIN lambda, mu, source[3],
+ * direction[3] (order 0 only) */
+double orc_error_norm(const orc_problem* p, const double* uh, int order, int kind,
+                      const double* prm);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ def lib():
         L.orc_fundsol_elasto.argtypes = [C.c_int, C.c_double, C.c_double, _f64p,
                                          _f64p, _f64p, _f64p]
         L.orc_fundsol_elasto.restype = None
+        L.orc_error_norm.argtypes = [C.POINTER(Problem), _f64p, C.c_int, C.c_int, _f64p]
+        L.orc_error_norm.restype = C.c_double
         _lib = L
     return _lib
 
@@ -267,3 +269,14 @@ def fundsol_elasto(dim, lam, mu, y, x, direc):
     lib().orc_fundsol_elasto(dim, lam, mu, np.asarray(y, np.float64),
                              np.asarray(x, np.float64), np.asarray(direc, np.float64), out)
     return out
+
+
+REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
+
+
+def error_norm(prob, uh, order, kind, params):
+    """|u - u_h| (order 0: L2, order 1: H1 semi-norm) with the problem's quadrature."""
+    op = OracleProblem(prob)
+    prm = np.zeros(8)
+    prm[:len(params)] = params
+    return lib().orc_error_norm(op.ref(), np.ascontiguousarray(uh, np.float64), order, kind, prm)

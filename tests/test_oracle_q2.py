@@ -234,3 +234,42 @@ def test_q2_newton_golden_compref(oracle, golden, mode):
         assert (x is None) == (xr is None)
         if x is not None:
             assert abs(x - xr) <= (1e-5 if xr > 1e-10 else 1e-3) * xr, (s, i, x, xr)
+
+
+def test_error_norm_pinned(oracle, golden):
+    """orc_error_norm (base/post/ErrorNorm.hpp): zero for a reproduced field
+    (Q1: bilinear, Q2: biquadratic interpolant vs the same polynomial has
+    nonzero sin error only), equal to the pinned fundamental-solution L2
+    routine, and the closed-form L2/H1 norms of sin(pi x) sin(pi y) against
+    u_h = 0 (1/2 and pi/sqrt(2) on the unit square)."""
+    prob = P.poisson2d(16)
+    prob.nq = 3
+    z = np.zeros(prob.n_nodes)
+    prm = [1.0, np.pi, np.pi]
+    l2 = oracle.error_norm(prob, z, 0, oracle.REF_SINPROD, prm)
+    h1 = oracle.error_norm(prob, z, 1, oracle.REF_SINPROD, prm)
+    assert abs(l2 - 0.5) < 1e-4 and abs(h1 - np.pi / np.sqrt(2)) < 1e-3
+    # interpolant error decreases as h^2 (L2) / h (H1) for Q1, h^3 / h^2 for Q2
+    errs = {}
+    for deg, mk in ((1, P.poisson2d), (2, P.poisson2d_q2)):
+        for n in (8, 16):
+            pb = mk(n)
+            pb.nq = 3
+            xs = pb.fxyz if deg == 2 else pb.xyz
+            u = np.sin(np.pi * xs[:, 0]) * np.sin(np.pi * xs[:, 1])
+            errs[deg, n] = (oracle.error_norm(pb, u, 0, oracle.REF_SINPROD, prm),
+                            oracle.error_norm(pb, u, 1, oracle.REF_SINPROD, prm))
+    r1 = [errs[1, 8][i] / errs[1, 16][i] for i in (0, 1)]
+    r2 = [errs[2, 8][i] / errs[2, 16][i] for i in (0, 1)]
+    assert 3.7 < r1[0] < 4.3 and 1.8 < r1[1] < 2.2
+    assert 7.0 < r2[0] < 9.0 and 3.7 < r2[1] < 4.3
+    # agrees with the fundamental-solution routine pinned by linearElastic2D.ref.dat
+    xyz, conn = P.read_smf(os.path.join(golden, "quad.005.smf"), 2)
+    pe = P.linear_elastic_fundsol(xyz, conn, 2)
+    rng = np.random.default_rng(1)
+    uh = 1e-4 * rng.standard_normal(pe.n_nodes * 2)
+    m = pe.meta
+    a = oracle.l2_error_fundsol(pe, uh, m["lam"], m["mu"], m["src"], m["dir"])
+    b = oracle.error_norm(pe, uh, 0, oracle.REF_KELVIN,
+                          [m["lam"], m["mu"], m["src"][0], m["src"][1], 0.0, m["dir"][0], m["dir"][1], 0.0])
+    assert abs(a - b) <= 1e-13 * a
