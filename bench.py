@@ -230,6 +230,52 @@ def newton_main(args):
     ctx.close()
 
 
+def transient_main(args):
+    """heat::DiffusionDriver::advanceInTime steps (heat/DiffusionDriver.hpp
+    :88-118) on the 3-D diffusion problem: BDF2 with its BDF1 start-up, each
+    step = assembly of K + (a0/dt) M and of the RHS with the mass history
+    on the device, Jacobi-PCG to rtol, solution pushed into the history,
+    all on the device (ins_bdf_step)."""
+    from insilico_amd import abi, problems as P
+    n = args.n or 128
+    prob = P.diffusion3d(n, seed=args.seed)
+    dt, nsteps = 1e-3, max(args.steps, 2)
+    res = {}
+    for name, mode, fmt in (("mixed_sell", abi.SOLVE_MIXED, abi.FMT_SELL),
+                            ("fp64_csr", abi.SOLVE_FP64, abi.FMT_CSR)):
+        ctx = abi.Context(0)
+        ctx.load_problem(prob)
+        ctx.bdf_init(2, 1.0)
+        ctx.bdf_step(dt, abi.FP64, mode, fmt, args.rtol)   # warm-up step (BDF1 start-up)
+        ctx.sync()
+        t0 = time.perf_counter()
+        its, asm = [], []
+        for _ in range(nsteps):
+            it, _ = ctx.bdf_step(dt, abi.FP64, mode, fmt, args.rtol)
+            its.append(it)
+            asm.append(ctx.stats()["assemble_ms"])
+        ctx.sync()
+        sec = time.perf_counter() - t0
+        u = ctx.get_dof_values()
+        res[name] = {"ms_per_step": round(sec / nsteps * 1e3, 3), "iterations": its,
+                     "assemble_ms": round(statistics.mean(asm), 3),
+                     "max_u": float(np.abs(u).max())}
+        log("transient %s: %.1f ms per time step, iterations %s" % (name, sec / nsteps * 1e3, its))
+        ctx.close()
+    a, b = res["mixed_sell"], res["fp64_csr"]
+    out = {"metric": "time steps per second (BDF2 diffusion, assemble + solve per step)",
+           "value": round(1e3 / a["ms_per_step"], 3), "unit": "steps/s", "n_gpus": 1,
+           "steps": nsteps, "higher_is_better": True, "dtype": "f32 sweeps / f64 residual",
+           "data": "synthetic",
+           "config": {"workload": "3-D diffusion %d^3 Q1 hex, k=exp(N(0,1)), u=0 on x=0, f=1, "
+                                  "rho=1, dt=%g, BDF2 (BDF1 first step), u0=0, Jacobi-PCG to %g "
+                                  "per step" % (n, dt, args.rtol), "n_dof": prob.n_active},
+           "mixed_sell": a, "fp64_csr": b,
+           "speedup_vs_fp64": round(b["ms_per_step"] / a["ms_per_step"], 3),
+           "max_u_rel_diff": abs(a["max_u"] - b["max_u"]) / max(b["max_u"], 1e-300)}
+    print(json.dumps(out))
+
+
 def fp32_integration_run(ctx, prob, fmt, args):
     """FP32 element integration feeding the same mixed solve; its result is
     measured against the FP64-integrated system with a host SpMV.  Not the
@@ -288,7 +334,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="diffusion",
-                    choices=["diffusion", "elasticity", "newton", "q2"])
+                    choices=["diffusion", "elasticity", "newton", "q2", "transient"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--rtol", type=float, default=1e-8)
@@ -308,6 +354,9 @@ def main():
         return
     if args.config == "newton":
         newton_main(args)
+        return
+    if args.config == "transient":
+        transient_main(args)
         return
 
     from insilico_amd import abi, problems as P

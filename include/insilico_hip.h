@@ -183,6 +183,19 @@ int ins_set_material(ins_ctx* ctx, int kernel, int nq, int ncoef,
  * density = 0 switches the terms off. */
 int ins_set_mass(ins_ctx* ctx, double density, double mass_factor, const double* hist);
 
+/* heat::DiffusionDriver::advanceInTime (heat/DiffusionDriver.hpp:88-118)
+ * with BDF(order) (base/time/BDF.hpp, start-up of base/time/MultiStep.hpp)
+ * kept on the device: ins_bdf_init sets the order, the density and the
+ * initial DoF values u0 (per field node and component, NULL = 0); every
+ * ins_bdf_step(dt) forms the history combination,
+ * assembles K + (a0/dt) M and the RHS (ins_set_mass semantics), solves,
+ * sets the DoF values (setDoFsFromSolver) and pushes them into the history
+ * (pushHistory, base/BoundaryValueProblem.hpp:358-363).  ins_get_dof_values
+ * reads u^{n+1}. */
+int ins_bdf_init(ins_ctx* ctx, int order, double density, const double* u0);
+int ins_bdf_step(ins_ctx* ctx, double dt, int precision, int mode, int format, double rtol,
+                 int max_it, int* iters, double* relres);
+
 /* bodyForceComputation (base/asmb/BodyForce.hpp:67-82) with a built-in f */
 int ins_set_body_force(ins_ctx* ctx, int kind, const double* params);
 /* neumannForceComputation (base/asmb/NeumannForce.hpp:36-63) with a

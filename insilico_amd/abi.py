@@ -35,6 +35,7 @@ SYMBOLS = [
     "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
     "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
+    "ins_bdf_init", "ins_bdf_step",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
@@ -82,6 +83,8 @@ def load():
         "ins_set_field": [vp, i32, i64, i32, vp],
         "ins_set_mass": [vp, dbl, dbl, vp],
         "ins_error_norm": [vp, i32, i32, vp, vp, C.POINTER(dbl)],
+        "ins_bdf_init": [vp, i32, dbl, vp],
+        "ins_bdf_step": [vp, dbl, i32, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
         "ins_set_dofs": [vp, i32, vp, i64],
         "ins_set_prescribed": [vp, vp],
         "ins_scale_prescribed": [vp, dbl],
@@ -190,6 +193,20 @@ class Context:
         out = C.c_double()
         self._chk(self.L.ins_error_norm(self.h, order, kind, _p(prm), _p(uu), C.byref(out)))
         return out.value
+
+    def bdf_init(self, order, density=1.0, u0=None):
+        u = None if u0 is None else np.ascontiguousarray(u0, np.float64)
+        self._chk(self.L.ins_bdf_init(self.h, order, density, _p(u)))
+
+    def bdf_step(self, dt, precision=FP64, mode=SOLVE_MIXED, fmt=FMT_SELL, rtol=1e-8,
+                 max_it=None):
+        """one device-resident time step; returns (iterations, relres)"""
+        it = C.c_int()
+        rr = C.c_double()
+        self._chk(self.L.ins_bdf_step(self.h, dt, precision, mode, fmt, rtol,
+                                      max_it if max_it is not None else max(2 * self.n_active, 100),
+                                      C.byref(it), C.byref(rr)))
+        return it.value, rr.value
 
     def set_dofs(self, dpn, dof_id, n_active):
         d = np.ascontiguousarray(dof_id, np.int32)

@@ -62,7 +62,7 @@ static int download(Ctx* c, T* h, const DArr<T>& a, size_t n)
 
 static void release_all(Ctx* c)
 {
-    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->mhist); release(c, c->stage_me); release(c, c->dof_id); release(c, c->rowmap);
+    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->mhist); release(c, c->stage_me); for (auto& hh : c->bdf_hist) release(c, hh); release(c, c->dof_id); release(c, c->rowmap);
     release(c, c->g); release(c, c->u); release(c, c->adj_ptr); release(c, c->adj);
     release(c, c->nb_ptr); release(c, c->nb); release(c, c->nboff); release(c, c->row_ptr);
     release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
@@ -210,6 +210,7 @@ int ins_set_mesh(ins_ctx* h, int dim, int64_t n_nodes, const double* xyz, int64_
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if ((dim != 2 && dim != 3) || n_nodes <= 0 || n_elem <= 0 || !xyz || !conn)
         return fail(c, INS_ERR_ARG, "ins_set_mesh: bad arguments");
     if (npe != (1 << dim))
@@ -340,6 +341,7 @@ int ins_set_field(ins_ctx* h, int degree, int64_t n_field_nodes, int field_npe,
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (c->n_elem == 0) return fail(c, INS_ERR_STATE, "ins_set_field before ins_set_mesh");
     if (degree != 2) return fail(c, INS_ERR_UNSUPPORTED, "only fields of degree 2 on a Q1 mesh");
     const int want = (c->dim == 2) ? 9 : 27;
@@ -372,6 +374,7 @@ int ins_set_dofs(ins_ctx* h, int dpn, const int32_t* dof_id, int64_t n_active)
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (c->n_nodes == 0) return fail(c, INS_ERR_STATE, "ins_set_dofs before ins_set_mesh");
     if (dpn < 1 || dpn > 3 || !dof_id || n_active < 0)
         return fail(c, INS_ERR_ARG, "ins_set_dofs: bad arguments");
@@ -486,6 +489,7 @@ int ins_set_material(ins_ctx* h, int kernel, int nq, int ncoef, const double* co
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (c->n_elem == 0) return fail(c, INS_ERR_STATE, "ins_set_material before ins_set_mesh");
     if (kernel != INS_LAPLACE && kernel != INS_STVENANT && kernel != INS_NEOHOOKE)
         return fail(c, INS_ERR_UNSUPPORTED, "unknown kernel");
@@ -505,6 +509,7 @@ int ins_set_mass(ins_ctx* h, double density, double mass_factor, const double* h
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (!(density >= 0.) || !std::isfinite(mass_factor))
         return fail(c, INS_ERR_ARG, "ins_set_mass: density >= 0 and a finite factor");
     if (density > 0. && !c->have_dofs)
@@ -524,6 +529,7 @@ int ins_set_body_force(ins_ctx* h, int kind, const double* params)
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (kind != INS_BF_NONE && kind != INS_BF_CONST && kind != INS_BF_SINPROD)
         return fail(c, INS_ERR_ARG, "unknown body force kind");
     if (kind != INS_BF_NONE && !params) return fail(c, INS_ERR_ARG, "body force parameters");
@@ -537,6 +543,7 @@ int ins_set_traction(ins_ctx* h, int64_t n_faces, int nfn, const int32_t* faces,
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (c->n_nodes == 0) return fail(c, INS_ERR_STATE, "ins_set_traction before ins_set_mesh");
     if (n_faces == 0) {
         c->n_faces = 0;
@@ -561,6 +568,7 @@ int ins_assemble(ins_ctx* h, int precision, int flags)
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (precision != INS_FP64 && precision != INS_FP32)
         return fail(c, INS_ERR_ARG, "precision must be INS_FP64 or INS_FP32");
     if (!(flags & (INS_ASM_MATRIX | INS_ASM_RHS)))
@@ -628,6 +636,7 @@ int ins_set_csr(ins_ctx* h, int64_t n, const int32_t* row_ptr, const int32_t* co
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
+    c->bdf_staged = false;
     if (n <= 0 || !row_ptr || !col_idx || !val) return fail(c, INS_ERR_ARG, "ins_set_csr: bad arguments");
     if (row_ptr[0] != 0) return fail(c, INS_ERR_ARG, "row_ptr[0] must be 0");
     for (int64_t i = 0; i < n; i++)
@@ -767,6 +776,43 @@ int ins_error_norm(ins_ctx* h, int order, int ref_kind, const double* ref_params
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
     return error_norm(c, order, ref_kind, ref_params, u, out);
+}
+
+int ins_bdf_init(ins_ctx* h, int order, double density, const double* u0)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    c->bdf_staged = false;
+    if (order < 1 || order > 4 || !(density > 0.))
+        return fail(c, INS_ERR_ARG, "ins_bdf_init: order 1..4 and a positive density");
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_bdf_init before ins_set_dofs");
+    const int64_t nd = c->n_nodes * c->dpn;
+    if (u0) INS_TRY(upload(c, c->u, u0, (size_t)nd));
+    else INS_HIP_CHECK(c, hipMemsetAsync(c->u.p, 0, nd * sizeof(double), c->stream));
+    for (int s = 0; s < order; s++) {
+        INS_TRY(ensure(c, c->bdf_hist[s], (size_t)nd));
+        INS_HIP_CHECK(c, hipMemcpyAsync(c->bdf_hist[s].p, c->u.p, nd * sizeof(double),
+                                        hipMemcpyDeviceToDevice, c->stream));
+        c->bdf_ring[s] = s;
+    }
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->bdf_order = order;
+    c->bdf_step = 0;
+    c->bdf_density = density;
+    return INS_OK;
+}
+
+int ins_bdf_step(ins_ctx* h, double dt, int precision, int mode, int format, double rtol,
+                 int max_it, int* iters, double* relres)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (precision != INS_FP64 && precision != INS_FP32)
+        return fail(c, INS_ERR_ARG, "precision must be INS_FP64 or INS_FP32");
+    if (mode != INS_SOLVE_FP64 && mode != INS_SOLVE_MIXED) return fail(c, INS_ERR_ARG, "solve mode");
+    if (format != INS_FMT_CSR && format != INS_FMT_SELL) return fail(c, INS_ERR_ARG, "format");
+    if (!(rtol > 0.) || max_it < 0) return fail(c, INS_ERR_ARG, "rtol/max_it");
+    return bdf_step(c, dt, precision, mode, format, rtol, max_it, iters, relres);
 }
 
 }  // extern "C"

@@ -1849,8 +1849,12 @@ int assemble(Ctx* c, int precision, int flags)
         return fail(c, INS_ERR_UNSUPPORTED, "elasticity needs dofs_per_node == dim");
     if (!elastic && c->dpn != 1 && c->dpn != c->dim)
         return fail(c, INS_ERR_UNSUPPORTED, "Laplace supports 1 or dim components");
-    const int do_matrix = flags & INS_ASM_MATRIX;
-    const int do_rhs = flags & INS_ASM_RHS;
+    // ASM_STAGED: the RHS again from the element data staged by the last full
+    // assembly (the same matrix and loads, a new history or new prescribed
+    // values); no element kernels run
+    const bool staged = (flags & ASM_STAGED) != 0;
+    const int do_matrix = staged ? 0 : (flags & INS_ASM_MATRIX);
+    const int do_rhs = staged ? 1 : (flags & INS_ASM_RHS);
     const int want_r = (flags & INS_ASM_RESIDUAL) && elastic;
     const int incremental = (flags & INS_ASM_INCREMENTAL) ? 1 : 0;
     const int npe = c->npe, s = c->dpn;
@@ -1875,12 +1879,12 @@ int assemble(Ctx* c, int precision, int flags)
         ke_elems = (size_t)E * (npe * s) * (npe * s);
     if (do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
     const bool need_fe = do_rhs && (c->bf_kind != INS_BF_NONE || want_r);
-    if (need_fe) INS_TRY(ensure(c, c->stage_fe, (size_t)E * npe * s));
+    if (need_fe && !staged) INS_TRY(ensure(c, c->stage_fe, (size_t)E * npe * s));
     // time-step mass terms (heat::Laplace only)
     const bool mass = c->density > 0.;
     if (mass && elastic)
         return fail(c, INS_ERR_UNSUPPORTED, "mass terms are provided for heat::Laplace");
-    if (mass) INS_TRY(ensure(c, c->stage_me, ke_elems * esize));
+    if (mass && !staged) INS_TRY(ensure(c, c->stage_me, ke_elems * esize));
     const void* mep = mass ? (const void*)c->stage_me.p : nullptr;
     if (do_matrix) {
         if (precision == INS_FP64) INS_TRY(ensure(c, c->val64, c->nnz));
@@ -1895,7 +1899,8 @@ int assemble(Ctx* c, int precision, int flags)
 
     // 1. element kernels
     if (c->profiling) INS_HIP_CHECK(c, hipEventRecord(c->ev_el[0], c->stream));
-    if (!elastic) {
+    if (staged) {
+    } else if (!elastic) {
         if (do_matrix || need_fe || mass) {
             if (!do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
             if (c->deg == 2) {
@@ -1934,7 +1939,9 @@ int assemble(Ctx* c, int precision, int flags)
     if (c->profiling) INS_HIP_CHECK(c, hipEventRecord(c->ev_el[1], c->stream));
     // faces
     double* ff = nullptr;
-    if (do_rhs && c->n_faces > 0) {
+    if (staged && c->n_faces > 0) {
+        ff = c->stage_ff.p;
+    } else if (do_rhs && c->n_faces > 0) {
         INS_TRY(ensure(c, c->stage_ff, (size_t)c->n_faces * c->nfn * s));
         ff = c->stage_ff.p;
         const unsigned g = grid_for(c->n_faces, BLK);
@@ -1946,7 +1953,7 @@ int assemble(Ctx* c, int precision, int flags)
     }
 
     // 2./3. gather into CSR and RHS
-    const int lift = do_matrix;
+    const int lift = do_matrix || staged;
     if (c->deg == 2) {
         INS_TRY(gather_q2(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr));
     } else if (!elastic) {

@@ -161,6 +161,15 @@ struct Ctx {
     double density = 0., mass_factor = 0.;
     DArr<double> mhist;
     bool have_mhist = false;
+    // device-resident BDF stepping: hist[ring[s]] = u^{n-s}
+    int bdf_order = 0, bdf_step = 0, bdf_ring[4] = {0, 1, 2, 3};
+    double bdf_density = 0.;
+    // the staged element data and the CSR values of the last step's full
+    // assembly stay valid for the next step with the same factor a0/dt
+    bool bdf_staged = false;
+    double bdf_factor = 0.;
+    int bdf_precision = 0;
+    DArr<double> bdf_hist[4];
 
     // assembly products
     bool have_matrix = false, have_rhs = false;
@@ -281,6 +290,7 @@ inline int ceil_lg2(int v)
 int build_pattern(Ctx* c);
 // assemble.hip
 int make_tables(Ctx* c);
+constexpr int ASM_STAGED = 1 << 8;  // internal: RHS from the staged element data
 int assemble(Ctx* c, int precision, int flags);
 int compute_dinv(Ctx* c);
 int error_norm(Ctx* c, int order, int kind, const double* prm, const double* u, double* out);
@@ -290,6 +300,8 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
 int build_sell(Ctx* c, bool force);
 int prepare(Ctx* c, int mode, int format);
 int distribute(Ctx* c, int op);
+int bdf_step(Ctx* c, double dt, int precision, int mode, int format, double rtol, int max_it,
+             int* iters, double* relres);
 int vec_norm2(Ctx* c, const double* v, int64_t n, double* out);
 
 }  // namespace ins

@@ -1771,6 +1771,85 @@ int distribute(Ctx* c, int op)
     return INS_OK;
 }
 
+namespace {
+// out = sum_s w[s] * h_s over up to 4 history vectors
+__global__ void k_hist_comb(int64_t nd, int ns, const double* __restrict__ h0,
+                            const double* __restrict__ h1, const double* __restrict__ h2,
+                            const double* __restrict__ h3, double w0, double w1, double w2,
+                            double w3, double* __restrict__ out)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nd;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double v = w0 * h0[k];
+        if (ns > 1) v += w1 * h1[k];
+        if (ns > 2) v += w2 * h2[k];
+        if (ns > 3) v += w3 * h3[k];
+        out[k] = v;
+    }
+}
+}  // namespace
+
+// base/time/BDF.hpp:69-99 as written there, start-up of
+// base/time/MultiStep.hpp:46-63
+static int bdf_weights(int order, int step, double* a)
+{
+    while (order > 1 && step + 2 < order + 1) order--;
+    static const double w[4][5] = {{1.0, -1.0}, {1.5, -2.0, 0.5},
+                                   {11. / 6., -3.0, 1.5, -1. / 3.},
+                                   {25. / 12., -4.0, 3.0, 4. / 3., 1. / 4.}};
+    for (int i = 0; i <= order; i++) a[i] = w[order - 1][i];
+    return order + 1;
+}
+
+int bdf_step(Ctx* c, double dt, int precision, int mode, int format, double rtol, int max_it,
+             int* iters, double* relres)
+{
+    if (c->bdf_order < 1) return fail(c, INS_ERR_STATE, "ins_bdf_step before ins_bdf_init");
+    if (!(dt > 0.)) return fail(c, INS_ERR_ARG, "dt must be positive");
+    double a[5];
+    const int na = bdf_weights(c->bdf_order, c->bdf_step, a);
+    const int64_t nd = c->n_nodes * c->dpn;
+    INS_TRY(ensure(c, c->mhist, (size_t)nd));
+    const double* hp[4] = {nullptr, nullptr, nullptr, nullptr};
+    double w[4] = {0., 0., 0., 0.};
+    for (int s = 1; s < na; s++) {
+        hp[s - 1] = c->bdf_hist[c->bdf_ring[s - 1]].p;
+        w[s - 1] = -a[s] / dt;
+    }
+    k_hist_comb<<<grid_for(nd, 256), 256, 0, c->stream>>>(
+        nd, na - 1, hp[0], hp[1] ? hp[1] : hp[0], hp[2] ? hp[2] : hp[0], hp[3] ? hp[3] : hp[0],
+        w[0], w[1], w[2], w[3], c->mhist.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    c->density = c->bdf_density;
+    c->mass_factor = a[0] / dt;
+    c->have_mhist = true;
+    // the matrix K + (a0/dt) M changes only with a0/dt: a step with the same
+    // factor rebuilds the RHS from the staged element data (the values are
+    // the same bits a full re-assembly gives)
+    if (c->bdf_staged && c->have_matrix && c->bdf_factor == c->mass_factor &&
+        c->bdf_precision == precision) {
+        INS_TRY(assemble(c, precision, ASM_STAGED));
+    } else {
+        c->bdf_staged = false;
+        INS_TRY(assemble(c, precision, INS_ASM_MATRIX | INS_ASM_RHS));
+        c->bdf_staged = true;
+        c->bdf_factor = c->mass_factor;
+        c->bdf_precision = precision;
+    }
+    const int st = solve(c, mode, format, rtol, max_it, iters, relres);
+    if (st != INS_OK) return st;
+    INS_TRY(distribute(c, INS_DIST_SET));
+    // push: the oldest slot takes u^{n+1}
+    const int q = c->bdf_order;
+    const int last = c->bdf_ring[q - 1];
+    for (int s = q - 1; s > 0; s--) c->bdf_ring[s] = c->bdf_ring[s - 1];
+    c->bdf_ring[0] = last;
+    INS_HIP_CHECK(c, hipMemcpyAsync(c->bdf_hist[last].p, c->u.p, nd * sizeof(double),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    c->bdf_step++;
+    return INS_OK;
+}
+
 int vec_norm2(Ctx* c, const double* v, int64_t n, double* out)
 {
     INS_TRY(ensure_solver_state(c, INS_SOLVE_FP64));

@@ -69,3 +69,23 @@ def test_eigenmode_decay_on_device(order):
             u, _ = st.advance(dt, abi.SOLVE_FP64, abi.FMT_CSR, rtol=1e-13)
             y.append(-sum(a[s] * y[-s] for s in range(1, len(a))) / (a[0] + dt * lam))
             assert np.abs(u - y[-1] * u0).max() < 1e-11
+
+
+def test_device_stepping_equals_host_stepping():
+    """ins_bdf_init / ins_bdf_step (history kept on the device) give the same
+    iterates as the host-driven stepper (ins_set_mass with uploaded
+    history), step by step, for BDF2 with its start-up."""
+    prob = P.diffusion3d(8, seed=7)
+    rng = np.random.default_rng(3)
+    u0 = 0.01 * rng.standard_normal(prob.n_nodes)
+    with abi.Context(0) as c1, abi.Context(0) as c2:
+        c1.load_problem(prob)
+        c2.load_problem(prob)
+        host = DiffusionStepper(c1, prob, order=2, u0=u0)
+        c2.bdf_init(2, 1.0, u0)
+        for k in range(4):
+            u_h, it_h = host.advance(0.01, abi.SOLVE_FP64, abi.FMT_CSR, rtol=1e-12)
+            it_d, _ = c2.bdf_step(0.01, abi.FP64, abi.SOLVE_FP64, abi.FMT_CSR, 1e-12)
+            u_d = c2.get_dof_values()
+            assert it_h == it_d
+            assert np.abs(u_d - u_h).max() <= 1e-12 * np.abs(u_h).max()
