@@ -124,9 +124,13 @@ inline std::vector<std::vector<int32_t>> boundaryFaces(const Mesh& m)
 }
 
 // Field of DoF objects with `size` components per node (iso-parametric:
-// DoF = node, base/dof/copyConnectivity.hpp:34-60)
+// DoF = node, base/dof/copyConnectivity.hpp:34-60; degree 2: DoF objects of
+// the vertices, edges, faces and cells, see generateField)
 struct Field {
     int size = 1;
+    int degree = 1;
+    std::vector<int32_t> conn;      // degree 2: field nodes of every element, hierarchic
+    std::vector<double> support;    // degree 2: support point of every field node
     std::vector<int> status;        // per (node, comp)
     std::vector<double> value;      // current value (DegreeOfFreedom values_[0])
     std::vector<double> prescribed; // constraint value
@@ -144,7 +148,84 @@ struct Field {
         prescribed[node * size + comp] = v;
     }
     bool isActive(std::size_t node, int comp) const { return status[node * size + comp] == ACTIVE; }
+    std::size_t numNodes() const { return status.size() / size; }
 };
+
+// element n-faces in hierarchic vertex numbers (base/mesh/ElementFaces.hpp)
+namespace detail_ {
+static const int QUAD_EDGES[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+static const int HEX_EDGES[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+static const int HEX_FACES[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                    {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+}  // namespace detail_
+
+// base::dof::generate<base::fe::Basis<shape, degree>> (base/dof/generate.hpp
+// :66-115).  Degree 1: one DoF object per node.  Degree 2: the passes of
+// base/dof/IndexMap.hpp:222-280 over generateDoFIndicesFromFaces -- vertices,
+// edges, faces (HEX; the cell of a QUAD), cells (HEX) -- each walking the
+// elements in order and their n-faces in table order; an n-face seen before
+// (same sorted vertex ids) keeps its number.  Support points: the vertex,
+// the edge midpoint, the face and cell centres.
+inline Field generateField(const Mesh& m, int degree, int size)
+{
+    if (degree == 1) return Field(m.numNodes(), size);
+    const int dim = m.dim, nv = m.npe(), nf = dim == 2 ? 9 : 27;
+    const std::size_t E = m.numElements();
+    std::vector<int32_t> conn(E * nf, -1);
+    std::vector<std::vector<double>> pts;
+    std::map<std::vector<int32_t>, int32_t> seen;
+    int32_t next = 0;
+    auto point = [&](const int32_t* ge, const int* vs, int k) {
+        std::vector<double> x(dim, 0.);
+        for (int d = 0; d < dim; d++) {
+            double sum = 0.;
+            for (int v = 0; v < k; v++) sum += m.xyz[(std::size_t)ge[vs[v]] * dim + d];
+            x[d] = sum * (1.0 / k);
+        }
+        return x;
+    };
+    auto pass = [&](int col, int count, int k, const int* tab, bool shared) {
+        seen.clear();
+        for (std::size_t e = 0; e < E; e++) {
+            const int32_t* ge = &m.conn[e * nv];
+            for (int f = 0; f < count; f++) {
+                const int* vs = tab + f * k;
+                std::vector<int32_t> key(k);
+                for (int v = 0; v < k; v++) key[v] = ge[vs[v]];
+                std::sort(key.begin(), key.end());
+                int32_t id;
+                auto it = shared ? seen.find(key) : seen.end();
+                if (it != seen.end()) {
+                    id = it->second;
+                } else {
+                    id = next++;
+                    if (shared) seen.emplace(key, id);
+                    pts.push_back(point(ge, vs, k));
+                }
+                conn[e * nf + col + f] = id;
+            }
+        }
+    };
+    static const int VERT[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    static const int CELL[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    pass(0, nv, 1, VERT, true);
+    if (dim == 2) {
+        pass(4, 4, 2, &detail_::QUAD_EDGES[0][0], true);
+        pass(8, 1, 4, CELL, false);
+    } else {
+        pass(8, 12, 2, &detail_::HEX_EDGES[0][0], true);
+        pass(20, 6, 4, &detail_::HEX_FACES[0][0], true);
+        pass(26, 1, 8, CELL, false);
+    }
+    Field f((std::size_t)next, size);
+    f.degree = 2;
+    f.conn = std::move(conn);
+    f.support.resize((std::size_t)next * dim);
+    for (std::size_t i = 0; i < pts.size(); i++)
+        for (int d = 0; d < dim; d++) f.support[i * dim + d] = pts[i][d];
+    return f;
+}
 
 // base/dof/numbering.hpp:45-66
 inline std::size_t numberDoFsConsecutively(Field& f)
@@ -167,6 +248,55 @@ inline void scaleConstraints(Field& f, double factor)
 template <typename DIRIFUN>
 void constrainBoundary(const Mesh& m, Field& f, DIRIFUN diriFun)
 {
+    if (f.degree == 2) {
+        // the DoF objects of every boundary face: its vertices, edges and
+        // (3-D) its face centre, at their support points
+        const int nf = m.dim == 2 ? 9 : 27, nv = m.npe();
+        std::map<std::vector<int32_t>, std::pair<int, std::pair<std::size_t, int>>> cnt;
+        const int nfaces = m.dim == 3 ? 6 : 4;
+        for (std::size_t e = 0; e < m.numElements(); e++)
+            for (int fc = 0; fc < nfaces; fc++) {
+                std::vector<int32_t> key;
+                if (m.dim == 3)
+                    for (int k = 0; k < 4; k++) key.push_back(m.conn[e * nv + detail_::HEX_FACES[fc][k]]);
+                else
+                    for (int k = 0; k < 2; k++) key.push_back(m.conn[e * nv + detail_::QUAD_EDGES[fc][k]]);
+                std::sort(key.begin(), key.end());
+                auto it = cnt.find(key);
+                if (it == cnt.end()) cnt.emplace(key, std::make_pair(1, std::make_pair(e, fc)));
+                else it->second.first++;
+            }
+        std::vector<char> done(f.numNodes(), 0);
+        for (auto& kv : cnt) {
+            if (kv.second.first != 1) continue;
+            const std::size_t e = kv.second.second.first;
+            const int fc = kv.second.second.second;
+            std::vector<int> locs;
+            if (m.dim == 2) {
+                locs = {detail_::QUAD_EDGES[fc][0], detail_::QUAD_EDGES[fc][1], 4 + fc};
+            } else {
+                const int* w = detail_::HEX_FACES[fc];
+                for (int k = 0; k < 4; k++) locs.push_back(w[k]);
+                for (int k = 0; k < 4; k++) {
+                    const int a = w[k], b = w[(k + 1) % 4];
+                    for (int ed = 0; ed < 12; ed++)
+                        if ((detail_::HEX_EDGES[ed][0] == a && detail_::HEX_EDGES[ed][1] == b) ||
+                            (detail_::HEX_EDGES[ed][0] == b && detail_::HEX_EDGES[ed][1] == a))
+                            locs.push_back(8 + ed);
+                }
+                locs.push_back(20 + fc);
+            }
+            for (int loc : locs) {
+                const int32_t node = f.conn[e * nf + loc];
+                if (done[node]) continue;
+                done[node] = 1;
+                std::array<double, 3> x{0., 0., 0.};
+                for (int d = 0; d < m.dim; d++) x[d] = f.support[(std::size_t)node * m.dim + d];
+                diriFun(x, (std::size_t)node, f);
+            }
+        }
+        return;
+    }
     std::vector<char> done(m.numNodes(), 0);
     for (const auto& face : boundaryFaces(m))
         for (int32_t node : face) {
@@ -460,6 +590,9 @@ private:
         if (!meshSent_) {
             insilico::check(ins_set_mesh(ctx_, m.dim, (int64_t)m.numNodes(), m.xyz.data(),
                                          (int64_t)m.numElements(), m.npe(), m.conn.data()), ctx_);
+            if (f.degree == 2)
+                insilico::check(ins_set_field(ctx_, 2, (int64_t)f.numNodes(), m.dim == 2 ? 9 : 27,
+                                              f.conn.data()), ctx_);
             insilico::check(ins_set_dofs(ctx_, f.size, f.index.data(), (int64_t)n_), ctx_);
             int64_t nnz = 0;
             insilico::check(ins_build_pattern(ctx_, &nnz), ctx_);

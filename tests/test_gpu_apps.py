@@ -118,3 +118,34 @@ def test_compressible_newton_app(oracle, prec):
     for s, (c1, c2) in last.items():
         assert (c2 is not None and c2 < tol) or c1 < tol, (s, c1, c2)
     assert umax == pytest.approx(np.abs(u[1::3]).max(), rel=1e-6)
+
+
+def test_compressible_q2_app_golden(golden):
+    """reference/06-elastic/compressible.cpp on the drop-in C++ layer with a
+    degree-2 field (generateField, constrainBoundary at the Q2 support points,
+    device assembly of the Q2 residual and tangent, Jacobi-PCG): every |F| and
+    |x| of compRefOutD.dat down to the round-off floor."""
+    out = _run("compressible_q2", "2", "20", "1000", "0.3", "3.0", "5", "30", "1e-12", timeout=300)
+    lines = out.splitlines()
+    assert lines[0].strip() == "# Number of dofs 3239"
+    hist = []
+    for ln in lines:
+        if ln.startswith("#") or not ln.strip():
+            continue
+        v = ln.split()
+        hist.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]) if len(v) > 3 else None))
+    ref = []
+    with open(os.path.join(golden, "compRefOutD.dat")) as fh:
+        for ln in fh:
+            if ln.startswith("#") or not ln.strip():
+                continue
+            v = ln.split()
+            ref.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]) if len(v) > 3 else None))
+    assert [(s, i) for s, i, _, _ in hist] == [(s, i) for s, i, _, _ in ref]
+    for (s, i, f, x), (_, _, fr, xr) in zip(hist, ref):
+        if fr >= 1e-7:
+            assert abs(f - fr) <= 1e-5 * fr, (s, i, f, fr)
+        else:
+            assert f < 1e-6
+        if x is not None and xr > 1e-9:
+            assert abs(x - xr) <= 1e-4 * xr, (s, i, x, xr)
