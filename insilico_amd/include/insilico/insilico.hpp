@@ -401,6 +401,101 @@ struct HyperElastic {
 }  // namespace solid
 
 // ---------------------------------------------------------------------------
+// Time integration (base/time/BDF.hpp) and heat::DiffusionDriver
+// (heat/DiffusionDriver.hpp:88-118) on the device
+namespace base {
+namespace time {
+template <unsigned Q>
+struct BDF {
+    static constexpr unsigned order = Q;
+};
+}  // namespace time
+}  // namespace base
+
+namespace heat {
+// advanceInTime(numStep, stepSize): assemble K + (a0/dt) M and the RHS with
+// the mass history (base/time/ReactionTerms.hpp), solve (the reference's
+// choleskySolve: a PCG to 1e-13 here), set the DoFs and push the history --
+// all on the device (ins_bdf_step).  Steps must be called with numStep =
+// 0, 1, 2, ... (the BDF start-up follows the call count).  The field's DoFs
+// must be numbered and its constraints and initial values set.
+template <typename MSM = base::time::BDF<1>>
+class DiffusionDriver {
+public:
+    DiffusionDriver(const insilico::Mesh& m, insilico::Field& f, double conductivity = 1.0,
+                    double density = 1.0, int device = 0)
+        : mesh_(m), field_(f), kappa_(conductivity), rho_(density)
+    {
+        insilico::check(ins_create(&ctx_, device, 0), nullptr);
+    }
+    ~DiffusionDriver()
+    {
+        if (ctx_) ins_destroy(ctx_);
+    }
+    DiffusionDriver(const DiffusionDriver&) = delete;
+    DiffusionDriver& operator=(const DiffusionDriver&) = delete;
+
+    // body force f (bodyForceComputation): INS_BF_CONST or INS_BF_SINPROD
+    void setForce(int kind, const std::array<double, 4>& p)
+    {
+        bfKind_ = kind;
+        bf_ = p;
+    }
+    void advanceInTime(unsigned numStep, double stepSize, int mode = INS_SOLVE_FP64,
+                       int format = INS_FMT_SELL, double rtol = 1e-13)
+    {
+        if (!ready_) setup();
+        if (numStep != steps_) throw insilico::Error(INS_ERR_STATE, "time steps must be consecutive");
+        int it = 0;
+        double rel = 0.;
+        const int maxIt = (int)std::max<std::size_t>(2 * n_, 100);
+        insilico::check(ins_bdf_step(ctx_, stepSize, INS_FP64, mode, format, rtol, maxIt, &it, &rel), ctx_);
+        insilico::check(ins_get_dof_values(ctx_, field_.value.data()), ctx_);
+        steps_++;
+        lastIterations_ = it;
+    }
+    int lastIterations() const { return lastIterations_; }
+    ins_ctx* context() { return ctx_; }
+
+private:
+    void setup()
+    {
+        const auto& m = mesh_;
+        auto& f = field_;
+        n_ = 0;
+        for (auto id : f.index) n_ += id >= 0;
+        insilico::check(ins_set_mesh(ctx_, m.dim, (int64_t)m.numNodes(), m.xyz.data(),
+                                     (int64_t)m.numElements(), m.npe(), m.conn.data()), ctx_);
+        if (f.degree == 2)
+            insilico::check(ins_set_field(ctx_, 2, (int64_t)f.numNodes(), m.dim == 2 ? 9 : 27,
+                                          f.conn.data()), ctx_);
+        insilico::check(ins_set_dofs(ctx_, f.size, f.index.data(), (int64_t)n_), ctx_);
+        insilico::check(ins_set_prescribed(ctx_, f.prescribed.data()), ctx_);
+        // kernel degree 2 * fieldDeg (heat/DiffusionDriver.hpp:99)
+        const int nq = (2 * f.degree + 2) / 2;
+        std::vector<double> coef(m.numElements(), kappa_);
+        insilico::check(ins_set_material(ctx_, INS_LAPLACE, nq, 1, coef.data()), ctx_);
+        insilico::check(ins_set_body_force(ctx_, bfKind_, bf_.data()), ctx_);
+        int64_t nnz = 0;
+        insilico::check(ins_build_pattern(ctx_, &nnz), ctx_);
+        insilico::check(ins_bdf_init(ctx_, (int)MSM::order, rho_, f.value.data()), ctx_);
+        ready_ = true;
+    }
+
+    const insilico::Mesh& mesh_;
+    insilico::Field& field_;
+    double kappa_, rho_;
+    ins_ctx* ctx_ = nullptr;
+    std::size_t n_ = 0;
+    unsigned steps_ = 0;
+    bool ready_ = false;
+    int lastIterations_ = 0;
+    int bfKind_ = INS_BF_NONE;
+    std::array<double, 4> bf_{0., 0., 0., 0.};
+};
+}  // namespace heat
+
+// ---------------------------------------------------------------------------
 // SOLVER concept on the device (base/solver/Eigen3.hpp:64-342)
 namespace base {
 namespace solver {

@@ -132,3 +132,72 @@ double l2Error(const Mesh& m, const Field& f, REF ref)
 }
 
 }  // namespace insilico
+
+// ---------------------------------------------------------------------------
+// base::post::errorComputation<ORDER> (base/post/ErrorNorm.hpp:60-84) on the
+// device, for the reference solutions the library evaluates itself
+namespace insilico {
+// u = amp * prod_d sin(k_d x_d)
+struct SineProductSolution {
+    double amp;
+    std::array<double, 3> k;
+};
+// elastostatic fundamental solution (base/auxi/FundamentalSolution.hpp)
+struct FundamentalSolutionElastoStatic {
+    double lambda, mu;
+    std::array<double, 3> source, direction;
+};
+namespace detail_ {
+inline void refParams(const SineProductSolution& r, int& kind, double* p)
+{
+    kind = INS_REF_SINPROD;
+    p[0] = r.amp;
+    for (int d = 0; d < 3; d++) p[1 + d] = r.k[d];
+}
+inline void refParams(const FundamentalSolutionElastoStatic& r, int& kind, double* p)
+{
+    kind = INS_REF_KELVIN;
+    p[0] = r.lambda;
+    p[1] = r.mu;
+    for (int d = 0; d < 3; d++) {
+        p[2 + d] = r.source[d];
+        p[5 + d] = r.direction[d];
+    }
+}
+}  // namespace detail_
+}  // namespace insilico
+
+namespace base {
+namespace post {
+// |u - u_h| in the L2 norm (ORDER 0) or the H1 semi-norm (ORDER 1) of the
+// field's current values, with the given quadrature
+template <unsigned ORDER, unsigned DEG, Shape SHAPE, typename REF>
+double errorComputation(const Quadrature<DEG, SHAPE>&, const insilico::Mesh& m,
+                        const insilico::Field& f, const REF& ref, int device = 0)
+{
+    ins_ctx* c = nullptr;
+    insilico::check(ins_create(&c, device, 0), nullptr);
+    struct Guard {
+        ins_ctx* c;
+        ~Guard() { ins_destroy(c); }
+    } guard{c};
+    insilico::check(ins_set_mesh(c, m.dim, (int64_t)m.numNodes(), m.xyz.data(),
+                                 (int64_t)m.numElements(), m.npe(), m.conn.data()), c);
+    if (f.degree == 2)
+        insilico::check(ins_set_field(c, 2, (int64_t)f.numNodes(), m.dim == 2 ? 9 : 27, f.conn.data()), c);
+    // all components active: the norm reads the field's values only
+    std::vector<int32_t> ids(f.status.size());
+    for (std::size_t k = 0; k < ids.size(); k++) ids[k] = (int32_t)k;
+    insilico::check(ins_set_dofs(c, f.size, ids.data(), (int64_t)ids.size()), c);
+    std::vector<double> one(m.numElements(), 1.0);
+    insilico::check(ins_set_material(c, INS_LAPLACE, Quadrature<DEG, SHAPE>::numPointsPerDir, 1,
+                                     one.data()), c);
+    int kind = INS_REF_ZERO;
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    insilico::detail_::refParams(ref, kind, p);
+    double out = 0.;
+    insilico::check(ins_error_norm(c, (int)ORDER, kind, p, f.value.data(), &out), c);
+    return out;
+}
+}  // namespace post
+}  // namespace base

@@ -149,3 +149,25 @@ def test_compressible_q2_app_golden(golden):
             assert f < 1e-6
         if x is not None and xr > 1e-9:
             assert abs(x - xr) <= 1e-4 * xr, (s, i, x, xr)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_diffusion_driver_app(oracle, order):
+    """heat::DiffusionDriver on the drop-in layer (device time steps): the
+    nodal field follows the exact discrete decay of the sin mode, and the
+    device errorComputation<0> equals the oracle's error norm of that field
+    against exp(-2 pi^2 t) sin(pi x) sin(pi y)."""
+    import math
+    N, steps, dt = 16, 6, 0.004
+    out = _run("diffusion", N, order, steps, dt, timeout=120)
+    vals = {}
+    for ln in out.splitlines():
+        if ln.startswith("# "):
+            k, v = ln[2:].rsplit(" ", 1)
+            vals[k] = float(v)
+    assert vals["discrete deviation"] < 1e-11
+    prob = P.poisson2d(N)
+    u = vals["y_n"] * np.sin(np.pi * prob.xyz[:, 0]) * np.sin(np.pi * prob.xyz[:, 1])
+    t = steps * dt
+    ref = oracle.error_norm(prob, u, 0, oracle.REF_SINPROD, [math.exp(-2 * math.pi ** 2 * t), math.pi, math.pi])
+    assert abs(vals["L2 error"] - ref) <= 1e-7 * ref
