@@ -131,3 +131,50 @@ def test_sell_and_csr_agree():
             c.solve(abi.SOLVE_FP64, fmt, 1e-10)
             xs.append(c.get_solution())
         assert np.linalg.norm(xs[0] - xs[1]) <= 1e-7 * np.linalg.norm(xs[0])
+
+
+def test_new_entry_points_reject_bad_use():
+    """Degree-2 fields, time-step terms, BDF stepping and error norms report
+    misuse as status codes."""
+    prob = P.diffusion3d_q2(2, seed=1)
+    q1 = P.diffusion3d(3, seed=1)
+    with abi.Context(0) as c:
+        # field before mesh, wrong node count, out-of-range node, twice
+        assert _status(lambda: c.set_field(2, prob.n_fnodes, prob.fconn)) == ERR["STATE"]
+        c.set_mesh(prob.xyz, prob.conn)
+        assert _status(lambda: c.set_field(3, prob.n_fnodes, prob.fconn)) == ERR["UNSUPPORTED"]
+        assert _status(lambda: c.set_field(2, prob.n_fnodes, prob.fconn[:, :9])) == ERR["ARG"]
+        bad = prob.fconn.copy()
+        bad[0, 0] = prob.n_fnodes
+        assert _status(lambda: c.set_field(2, prob.n_fnodes, bad)) == ERR["RANGE"]
+        c.set_field(2, prob.n_fnodes, prob.fconn)
+        assert _status(lambda: c.set_field(2, prob.n_fnodes, prob.fconn)) == ERR["STATE"]
+        # Neumann faces on a degree-2 field
+        faces = np.array([[0, 1, 2, 3]], np.int32)
+        assert _status(lambda: c.set_traction(faces, (0.0, 0.0, 1.0))) == ERR["UNSUPPORTED"]
+        # mass terms and stepping before the DoFs
+        assert _status(lambda: c.set_mass(1.0, 10.0)) == ERR["STATE"]
+        assert _status(lambda: c.bdf_init(2, 1.0)) == ERR["STATE"]
+        c.set_dofs(1, prob.dof_id, prob.n_active)
+        assert _status(lambda: c.set_mass(-1.0, 1.0)) == ERR["ARG"]
+        assert _status(lambda: c.bdf_step(0.1)) == ERR["STATE"]            # no init
+        assert _status(lambda: c.bdf_init(5, 1.0)) == ERR["ARG"]
+        assert _status(lambda: c.bdf_init(2, 0.0)) == ERR["ARG"]
+        # error norm before a material, bad order / reference
+        assert _status(lambda: c.error_norm(0, abi.REF_ZERO)) == ERR["STATE"]
+        c.set_material(prob.kernel, prob.nq, prob.coef)
+        assert _status(lambda: c.error_norm(2, abi.REF_ZERO)) == ERR["ARG"]
+        assert _status(lambda: c.error_norm(0, 7)) == ERR["ARG"]
+        assert _status(lambda: c.error_norm(1, abi.REF_KELVIN, [1, 1, 0, 0, 0, 1, 0, 0])) == ERR["UNSUPPORTED"]
+        c.build_pattern()
+        c.bdf_init(2, 1.0)
+        assert _status(lambda: c.bdf_step(-0.1)) == ERR["ARG"]
+    with abi.Context(0) as c:
+        # mass terms are for heat::Laplace only
+        e = P.elasticity3d(2, seed=1)
+        c.load_problem(e)
+        c.set_mass(1.0, 10.0)
+        assert _status(lambda: c.assemble(abi.FP64)) == ERR["UNSUPPORTED"]
+        # a Q1 context after a Q2 one on the same handle works again
+        c.load_problem(q1)
+        c.assemble(abi.FP64)
