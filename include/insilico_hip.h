@@ -135,8 +135,9 @@ int ins_set_mesh(ins_ctx* ctx, int dim, int64_t n_nodes, const double* xyz,
  * (HEX27), numbered as base/dof/IndexMap.hpp:222-280 with
  * base/dof/generateDoFIndicesFromFaces.hpp does (vertices, edges, faces,
  * cells).  Call after ins_set_mesh; the DoFs, prescribed and current values
- * are then indexed by field node.  Neumann faces are not supported on a
- * degree-2 field. */
+ * are then indexed by field node, and Neumann faces (ins_set_traction,
+ * afterwards) list the 3 (LINE3) / 9 (QUAD9) field nodes of each face in its
+ * hierarchic order, vertices first. */
 int ins_set_field(ins_ctx* ctx, int degree, int64_t n_field_nodes, int field_npe,
                   const int32_t* field_conn);
 
@@ -199,10 +200,16 @@ int ins_bdf_step(ins_ctx* ctx, double dt, int precision, int mode, int format, d
 /* bodyForceComputation (base/asmb/BodyForce.hpp:67-82) with a built-in f */
 int ins_set_body_force(ins_ctx* ctx, int kind, const double* params);
 /* neumannForceComputation (base/asmb/NeumannForce.hpp:36-63) with a
- * constant traction over boundary faces given by their nodes (2 per edge in
- * 2-D, 4 per quadrilateral in 3-D, face-local Q1 order) */
+ * traction over boundary faces given by their nodes (2 per edge in 2-D, 4
+ * per quadrilateral in 3-D, face-local Q1 order; degree-2 fields: 3 / 9
+ * field nodes, vertices first) */
 int ins_set_traction(ins_ctx* ctx, int64_t n_faces, int nodes_per_face,
                      const int32_t* faces, const double* traction);
+/* A traction that varies along the faces, as a Neumann function of the
+ * position does (e.g. reference/06-elastic/PulledSheet.hpp neumannBC,
+ * value * (x_1 - 1/2) in component 0): the traction vector is scaled by
+ * (x[axis] - x0) at every surface point; axis = -1 = constant (default). */
+int ins_set_traction_profile(ins_ctx* ctx, int axis, double x0);
 
 /* ---- assembly ------------------------------------------------------------- */
 /* stiffnessMatrixComputation + computeResidualForces + bodyForceComputation

@@ -35,7 +35,7 @@ SYMBOLS = [
     "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
     "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
-    "ins_bdf_init", "ins_bdf_step",
+    "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
@@ -84,6 +84,7 @@ def load():
         "ins_set_mass": [vp, dbl, dbl, vp],
         "ins_error_norm": [vp, i32, i32, vp, vp, C.POINTER(dbl)],
         "ins_bdf_init": [vp, i32, dbl, vp],
+        "ins_set_traction_profile": [vp, i32, dbl],
         "ins_bdf_step": [vp, dbl, i32, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
         "ins_set_dofs": [vp, i32, vp, i64],
         "ins_set_prescribed": [vp, vp],
@@ -252,6 +253,9 @@ class Context:
         p[:len(params)] = params
         self._chk(self.L.ins_set_body_force(self.h, kind, _p(p)))
 
+    def set_traction_profile(self, axis=-1, x0=0.0):
+        self._chk(self.L.ins_set_traction_profile(self.h, axis, x0))
+
     def set_traction(self, faces, traction):
         if faces is None or len(faces) == 0:
             self._chk(self.L.ins_set_traction(self.h, 0, 0, None, None))
@@ -343,7 +347,12 @@ class Context:
         self.set_prescribed(prob.g)
         self.set_material(prob.kernel, prob.nq, prob.coef)
         self.set_body_force(prob.bf_kind, prob.bf)
-        self.set_traction(prob.faces, prob.traction)
+        if getattr(prob, "deg", 1) == 2:
+            self.set_traction(prob.ffaces, prob.traction)
+        else:
+            self.set_traction(prob.faces, prob.traction)
+        lin = prob.meta.get("traction_linear")
+        self.set_traction_profile(*(lin if lin is not None else (-1, 0.0)))
         return self.build_pattern()
 
 

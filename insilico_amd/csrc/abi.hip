@@ -62,7 +62,7 @@ static int download(Ctx* c, T* h, const DArr<T>& a, size_t n)
 
 static void release_all(Ctx* c)
 {
-    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->mhist); release(c, c->stage_me); for (auto& hh : c->bdf_hist) release(c, hh); release(c, c->dof_id); release(c, c->rowmap);
+    release(c, c->xyz); release(c, c->conn); release(c, c->gconn); release(c, c->pinfo64); release(c, c->node_order); release(c, c->fvert); release(c, c->mhist); release(c, c->stage_me); for (auto& hh : c->bdf_hist) release(c, hh); release(c, c->dof_id); release(c, c->rowmap);
     release(c, c->g); release(c, c->u); release(c, c->adj_ptr); release(c, c->adj);
     release(c, c->nb_ptr); release(c, c->nb); release(c, c->nboff); release(c, c->row_ptr);
     release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
@@ -264,6 +264,7 @@ static int order_field_nodes(Ctx* c, const int32_t* fconn)
     static const int FC3[6] = {4, 4, 5, 6, 5, 6};
     const int64_t nn = c->n_nodes;
     std::vector<int8_t> cls((size_t)nn, -1);
+    std::vector<int32_t> fvert((size_t)nn, -1);
     std::vector<double> pt((size_t)nn * 3, 0.);
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (size_t i = 0; i < (size_t)c->n_gnodes; i++)
@@ -275,6 +276,7 @@ static int order_field_nodes(Ctx* c, const int32_t* fconn)
         const int32_t* ge = &G[(size_t)e * gn];
         for (int k = 0; k < nf; k++) {
             const int32_t node = fconn[e * nf + k];
+            if (k < gn) fvert[node] = ge[k];
             if (cls[node] >= 0) continue;
             int vs[8], nv = 0, cl = 0;
             if (k < gn) {
@@ -332,6 +334,8 @@ static int order_field_nodes(Ctx* c, const int32_t* fconn)
     std::vector<int32_t> order((size_t)nn);
     for (int64_t i = 0; i < nn; i++) order[(size_t)i] = keys[(size_t)i].node;
     INS_TRY(upload(c, c->node_order, order.data(), order.size()));
+    INS_TRY(upload(c, c->fvert, fvert.data(), fvert.size()));
+    c->fvert_host.swap(fvert);
     c->have_order = true;
     return INS_OK;
 }
@@ -354,7 +358,7 @@ int ins_set_field(ins_ctx* h, int degree, int64_t n_field_nodes, int field_npe,
     for (int64_t i = 0; i < c->n_elem * field_npe; i++)
         if (field_conn[i] < 0 || field_conn[i] >= n_field_nodes)
             return fail(c, INS_ERR_RANGE, "field connectivity refers to a node out of range");
-    if (c->n_faces > 0) return fail(c, INS_ERR_UNSUPPORTED, "Neumann faces on a degree-2 field");
+    c->n_faces = 0;  // Neumann faces are given in field nodes: set them after the field
     // the Q1 connectivity becomes the geometry
     std::swap(c->gconn, c->conn);
     c->gnpe = c->npe;
@@ -549,18 +553,38 @@ int ins_set_traction(ins_ctx* h, int64_t n_faces, int nfn, const int32_t* faces,
         c->n_faces = 0;
         return INS_OK;
     }
-    if (n_faces < 0 || !faces || !traction || nfn != (1 << (c->dim - 1)))
+    const int want = c->deg == 2 ? (c->dim == 2 ? 3 : 9) : (1 << (c->dim - 1));
+    if (n_faces < 0 || !faces || !traction || nfn != want)
         return fail(c, INS_ERR_ARG, "ins_set_traction: bad arguments");
-    if (c->deg == 2) return fail(c, INS_ERR_UNSUPPORTED, "Neumann faces on a degree-2 field");
     for (int64_t i = 0; i < n_faces * nfn; i++)
         if (faces[i] < 0 || faces[i] >= c->n_nodes)
             return fail(c, INS_ERR_RANGE, "face refers to a node out of range");
+    if (c->deg == 2) {
+        // the first 2 (4) field nodes of a face are its vertices
+        const int gnf = 1 << (c->dim - 1);
+        for (int64_t f = 0; f < n_faces; f++)
+            for (int k = 0; k < gnf; k++)
+                if (c->fvert_host[(size_t)faces[f * nfn + k]] < 0)
+                    return fail(c, INS_ERR_ARG, "a face must list its vertices first");
+    }
     c->n_faces = n_faces;
     c->nfn = nfn;
     for (int i = 0; i < 3; i++) c->traction[i] = i < c->dim ? traction[i] : 0.;
     INS_TRY(upload(c, c->faces, faces, (size_t)n_faces * nfn));
     INS_TRY(build_adjacency(c, c->faces.p, n_faces, nfn, ceil_lg2(nfn), c->n_nodes,
                             c->fadj_ptr, c->fadj, nullptr));
+    return INS_OK;
+}
+
+int ins_set_traction_profile(ins_ctx* h, int axis, double x0)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    c->bdf_staged = false;
+    if (axis < -1 || axis >= 3 || (c->dim && axis >= c->dim))
+        return fail(c, INS_ERR_ARG, "ins_set_traction_profile: axis -1..dim-1");
+    c->tr_axis = axis;
+    c->tr_x0 = axis >= 0 ? x0 : 0.;
     return INS_OK;
 }
 

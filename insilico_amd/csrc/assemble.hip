@@ -1205,30 +1205,40 @@ __global__ __launch_bounds__(BLK) void k_rhs(
 }
 
 // Neumann traction on faces (base/asmb/NeumannForce.hpp:132-175 with the
-// surface metric of base/geometry.hpp:314-346): one face per lane.
-template <int DIM>
+// surface metric of base/geometry.hpp:314-346): one face per lane.  A face
+// lists its FFN field nodes in its own hierarchic order, the first GNF of
+// them its vertices (gmap maps them to geometry nodes; null = the field
+// nodes are the geometry nodes).  The traction is constant, or with
+// axis >= 0 scaled by (x[axis] - x0) at the surface point.
+template <int DIM, int FFN>
 __global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restrict__ faces,
+                                              const int32_t* __restrict__ gmap,
                                               const double* __restrict__ xyz,
-                                              const double* __restrict__ ftab, int nqf,
-                                              double t0, double t1, double t2,
-                                              double* __restrict__ ff)
+                                              const double* __restrict__ gdt,
+                                              const double* __restrict__ gpt,
+                                              const double* __restrict__ fpt,
+                                              const double* __restrict__ wt, int nqf,
+                                              double t0, double t1, double t2, int axis,
+                                              double x0, double* __restrict__ ff)
 {
-    constexpr int NFN = (DIM == 3) ? 4 : 2;
+    constexpr int GNF = (DIM == 3) ? 4 : 2;
     constexpr int LD = DIM - 1;
     const double tr[3] = {t0, t1, t2};
     for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
          f += (int64_t)gridDim.x * blockDim.x) {
-        double X[NFN][DIM];
-        for (int n = 0; n < NFN; n++) {
-            const int64_t node = faces[f * NFN + n];
+        double X[GNF][DIM];
+        for (int n = 0; n < GNF; n++) {
+            const int64_t fn = faces[f * FFN + n];
+            const int64_t node = gmap ? gmap[fn] : fn;
             for (int d = 0; d < DIM; d++) X[n][d] = xyz[node * DIM + d];
         }
-        double out[NFN * DIM];
-        for (int i = 0; i < NFN * DIM; i++) out[i] = 0.;
+        double out[FFN * DIM];
+        for (int i = 0; i < FFN * DIM; i++) out[i] = 0.;
         for (int q = 0; q < nqf; q++) {
-            const double* dphi = ftab + q * NFN * LD;
-            const double* phi = ftab + nqf * NFN * LD + q * NFN;
-            const double w = ftab[nqf * NFN * (LD + 1) + q];
+            const double* dphi = gdt + q * GNF * LD;
+            const double* gphi = gpt + q * GNF;
+            const double* phi = fpt + q * FFN;
+            const double w = wt[q];
             double detG;
             if constexpr (DIM == 3) {
                 double J[3][2] = {{0, 0}, {0, 0}, {0, 0}};
@@ -1243,10 +1253,16 @@ __global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restr
                 const double tx = X[1][0] - X[0][0], ty = X[1][1] - X[0][1];
                 detG = sqrt(tx * tx + ty * ty);
             }
-            for (int n = 0; n < NFN; n++)
-                for (int c = 0; c < DIM; c++) out[n * DIM + c] += tr[c] * phi[n] * w * detG;
+            double t[3] = {tr[0], tr[1], tr[2]};
+            if (axis >= 0) {
+                double xs = 0.;
+                for (int n = 0; n < GNF; n++) xs += X[n][axis] * gphi[n];
+                for (int c = 0; c < DIM; c++) t[c] = tr[c] * (xs - x0);
+            }
+            for (int n = 0; n < FFN; n++)
+                for (int c = 0; c < DIM; c++) out[n * DIM + c] += t[c] * phi[n] * w * detG;
         }
-        for (int i = 0; i < NFN * DIM; i++) ff[f * NFN * DIM + i] = out[i];
+        for (int i = 0; i < FFN * DIM; i++) ff[f * FFN * DIM + i] = out[i];
     }
 }
 
@@ -1560,8 +1576,11 @@ void lagrange2_shape(int dim, const double* xi, double* phi, double* dphi)
     }
 }
 
-// degree-2 field table: geometry dphi, geometry phi, field dphi, field phi, w
-void fill_tables_q2(int dim, int nq1, std::vector<double>& t)
+// degree-2 field table: geometry dphi, geometry phi, field dphi, field phi, w;
+// then the face table: geometry dphi, geometry phi, field phi (LINE3 ends
+// first then the midpoint, base/mesh/HierarchicOrder.hpp:131-150; QUAD9
+// hierarchic), w
+void fill_tables_q2(int dim, int nq1, std::vector<double>& t, int* face_off, int* nqf)
 {
     std::vector<double> w, xi;
     const int nq = tensor_rule(dim, nq1, w, xi);
@@ -1583,6 +1602,33 @@ void fill_tables_q2(int dim, int nq1, std::vector<double>& t)
         }
         t[W + q] = w[q];
     }
+    *face_off = (int)t.size();
+    std::vector<double> wf, xf;
+    const int ld = dim - 1, gnf = 1 << ld, ffn = dim == 2 ? 3 : 9;
+    const int nqfv = tensor_rule(ld, nq1, wf, xf);
+    const size_t b0 = t.size(), bGP = b0 + (size_t)nqfv * gnf * ld, bFP = bGP + (size_t)nqfv * gnf,
+                 bW = bFP + (size_t)nqfv * ffn;
+    t.resize(bW + nqfv, 0.);
+    for (int q = 0; q < nqfv; q++) {
+        double phi[4], dphi[8];
+        q1_shape(ld, &xf[q * ld], phi, dphi);
+        for (int n = 0; n < gnf; n++) {
+            for (int d = 0; d < ld; d++) t[b0 + (q * gnf + n) * ld + d] = dphi[n * ld + d];
+            t[bGP + q * gnf + n] = phi[n];
+        }
+        if (ld == 1) {
+            const double x = xf[q];
+            t[bFP + q * 3 + 0] = (1. - x) * (1. - 2. * x);
+            t[bFP + q * 3 + 1] = x * (2. * x - 1.);
+            t[bFP + q * 3 + 2] = 4. * x * (1. - x);
+        } else {
+            double fphi[9], fdphi[18];
+            lagrange2_shape(2, &xf[q * ld], fphi, fdphi);
+            for (int n = 0; n < 9; n++) t[bFP + q * 9 + n] = fphi[n];
+        }
+        t[bW + q] = wf[q];
+    }
+    *nqf = nqfv;
 }
 
 // volume table followed by the face table (dphi, phi, w on faces)
@@ -1775,7 +1821,7 @@ int launch_gather_rhs(Ctx* c, KE ke, ME me, int do_matrix, int do_rhs, int lift,
 
 template <typename T, int NF>
 int gather_q2_t(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int incremental,
-                const double* fe, int precision, int* derr)
+                const double* fe, const double* ff, int precision, int* derr)
 {
     const T* st = (const T*)c->stage_ke.p;
     const KeLaplace<T, NF> me{(!elastic && c->density > 0.) ? (const T*)c->stage_me.p : nullptr};
@@ -1783,26 +1829,26 @@ int gather_q2_t(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int i
     constexpr int D = NF == 27 ? 3 : 2;
     if (elastic)
         return launch_gather_rhs<NF, D>(c, KeElastic<T, NF, D>{st}, me, do_matrix, do_rhs, lift,
-                                        incremental, fe, nullptr, precision, derr);
+                                        incremental, fe, ff, precision, derr);
     if (s == 1)
         return launch_gather_rhs<NF, 1>(c, KeLaplace<T, NF>{st}, me, do_matrix, do_rhs, lift,
-                                        incremental, fe, nullptr, precision, derr);
+                                        incremental, fe, ff, precision, derr);
     if (s == D)
         return launch_gather_rhs<NF, D>(c, KeLaplace<T, NF>{st}, me, do_matrix, do_rhs, lift,
-                                        incremental, fe, nullptr, precision, derr);
+                                        incremental, fe, ff, precision, derr);
     return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
 }
 
 int gather_q2(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int incremental,
-              const double* fe, int precision, int* derr)
+              const double* fe, const double* ff, int precision, int* derr)
 {
     if (c->npe == 27)
         return precision == INS_FP64
-                   ? gather_q2_t<double, 27>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr)
-                   : gather_q2_t<float, 27>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr);
+                   ? gather_q2_t<double, 27>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)
+                   : gather_q2_t<float, 27>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr);
     return precision == INS_FP64
-               ? gather_q2_t<double, 9>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr)
-               : gather_q2_t<float, 9>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr);
+               ? gather_q2_t<double, 9>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)
+               : gather_q2_t<float, 9>(c, elastic, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr);
 }
 
 }  // namespace
@@ -1811,9 +1857,7 @@ int make_tables(Ctx* c)
 {
     std::vector<double> t;
     if (c->deg == 2) {
-        fill_tables_q2(c->dim, c->nq, t);
-        c->tab_face_off = 0;
-        c->tab_nqf = 0;
+        fill_tables_q2(c->dim, c->nq, t, &c->tab_face_off, &c->tab_nqf);
     } else {
         fill_tables(c->dim, c->nq, t, &c->tab_face_off, &c->tab_nqf);
     }
@@ -1945,17 +1989,33 @@ int assemble(Ctx* c, int precision, int flags)
         INS_TRY(ensure(c, c->stage_ff, (size_t)c->n_faces * c->nfn * s));
         ff = c->stage_ff.p;
         const unsigned g = grid_for(c->n_faces, BLK);
-        if (c->dim == 3)
-            k_face<3><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, c->xyz.p, c->tab64.p + c->tab_face_off, c->tab_nqf, c->traction[0], c->traction[1], c->traction[2], ff);
+        // face table: geometry dphi [nqf][gnf][dim-1], geometry phi [nqf][gnf],
+        // field phi [nqf][nfn], weights [nqf] (for Q1 the field phi is the
+        // geometry phi)
+        const int nqf = c->tab_nqf, ld = c->dim - 1, gnf = 1 << ld;
+        const double* gd = c->tab64.p + c->tab_face_off;
+        const double* gp = gd + (size_t)nqf * gnf * ld;
+        const double* fp = c->deg == 2 ? gp + (size_t)nqf * gnf : gp;
+        const double* w = fp + (size_t)nqf * c->nfn;
+        const int32_t* gmap = c->deg == 2 ? c->fvert.p : nullptr;
+        const double* T = c->traction;
+        const int ax = c->tr_axis;
+        const double x0 = c->tr_x0;
+        if (c->dim == 3 && c->nfn == 9)
+            k_face<3, 9><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
+        else if (c->dim == 3)
+            k_face<3, 4><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
+        else if (c->nfn == 3)
+            k_face<2, 3><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
         else
-            k_face<2><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, c->xyz.p, c->tab64.p + c->tab_face_off, c->tab_nqf, c->traction[0], c->traction[1], c->traction[2], ff);
+            k_face<2, 2><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
         INS_HIP_CHECK(c, hipGetLastError());
     }
 
     // 2./3. gather into CSR and RHS
     const int lift = do_matrix || staged;
     if (c->deg == 2) {
-        INS_TRY(gather_q2(c, elastic, do_matrix, do_rhs, lift, incremental, fe, precision, derr));
+        INS_TRY(gather_q2(c, elastic, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr));
     } else if (!elastic) {
         if (precision == INS_FP64) {
             if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));

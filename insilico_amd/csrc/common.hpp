@@ -115,6 +115,9 @@ struct Ctx {
     // then by support point; used to build the SELL slices
     DArr<int32_t> node_order;
     bool have_order = false;
+    // degree 2: geometry node of every vertex field node (-1 for the others)
+    DArr<int32_t> fvert;
+    std::vector<int32_t> fvert_host;
 
     // dofs
     int dpn = 0;
@@ -155,6 +158,8 @@ struct Ctx {
     int64_t n_faces = 0;
     int nfn = 0;
     double traction[3] = {0, 0, 0};
+    int tr_axis = -1;       // >= 0: traction scaled by (x[tr_axis] - tr_x0)
+    double tr_x0 = 0.;
     DArr<int32_t> faces, fadj_ptr, fadj;
     // time-step terms: density of the mass matrix (0 = none), its factor in
     // the matrix (a0 / (b0 dt)) and the history combination of the RHS

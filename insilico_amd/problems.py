@@ -391,21 +391,29 @@ def poisson2d_q2(n=16):
                    name="poisson2dQ2-%d" % n, deg=2, fconn=fconn, fxyz=fxyz)
 
 
-def elasticity3d_q2(n=8, seed=12345, nu=0.3, kernel=STVENANT):
+def elasticity3d_q2(n=8, seed=12345, nu=0.3, kernel=STVENANT, traction=None):
     """Elastic block with a degree-2 field: E_e ~ U[0.5, 2], clamped at z = 0,
-    body force (0, 0, -1), 2 Gauss points per direction."""
+    body force (0, 0, -1), optionally a traction on z = 1, 2 Gauss points per
+    direction."""
     xyz, conn = unit_cube(3, n)
     fconn, fxyz = q2_field(xyz, conn, 3)
-    el, fa, _ = boundary_faces(conn, 3)
-    bn = np.unique(q2_face_nodes(fconn, el, fa, 3))
+    el, fa, bf = boundary_faces(conn, 3)
+    fnodes = q2_face_nodes(fconn, el, fa, 3)
+    bn = np.unique(fnodes)
     con = _q2_constrained(fxyz, bn, lambda x: np.abs(x[:, 2]) < 1e-12, 3)
     dof, na = number_dofs(con)
     E = 0.5 + 1.5 * uniform01(seed, conn.shape[0])
     lam, mu = lame(E, nu)
-    return Problem(3, xyz, conn, 3, dof, na, np.zeros(fxyz.shape[0] * 3), kernel,
-                   np.stack([lam, mu], axis=1), 2, BF_CONST, (0.0, 0.0, -1.0, 0.0),
-                   name="elasticity3dQ2-%d" % n, meta={"seed": seed}, deg=2, fconn=fconn,
-                   fxyz=fxyz)
+    p = Problem(3, xyz, conn, 3, dof, na, np.zeros(fxyz.shape[0] * 3), kernel,
+                np.stack([lam, mu], axis=1), 2, BF_CONST, (0.0, 0.0, -1.0, 0.0),
+                name="elasticity3dQ2-%d" % n, meta={"seed": seed}, deg=2, fconn=fconn,
+                fxyz=fxyz)
+    if traction is not None:
+        top = np.all(np.abs(xyz[bf, 2] - 1.0) < 1e-12, axis=1)
+        p.faces = bf[top].astype(np.int32)
+        p.ffaces = fnodes[top]
+        p.traction = tuple(traction)
+    return p
 
 
 def pulled_sheet_q2(n=20, disp_controlled=True, E=1000.0, nu=0.3, dim=2):

@@ -149,9 +149,13 @@ def test_new_entry_points_reject_bad_use():
         assert _status(lambda: c.set_field(2, prob.n_fnodes, bad)) == ERR["RANGE"]
         c.set_field(2, prob.n_fnodes, prob.fconn)
         assert _status(lambda: c.set_field(2, prob.n_fnodes, prob.fconn)) == ERR["STATE"]
-        # Neumann faces on a degree-2 field
+        # Neumann faces on a degree-2 field: 9 field nodes, vertices first
         faces = np.array([[0, 1, 2, 3]], np.int32)
-        assert _status(lambda: c.set_traction(faces, (0.0, 0.0, 1.0))) == ERR["UNSUPPORTED"]
+        assert _status(lambda: c.set_traction(faces, (0.0, 0.0, 1.0))) == ERR["ARG"]
+        cell = prob.fconn[0, 26]
+        faces9 = np.array([[cell] * 9], np.int32)
+        assert _status(lambda: c.set_traction(faces9, (0.0, 0.0, 1.0))) == ERR["ARG"]
+        assert _status(lambda: c.set_traction_profile(3, 0.0)) == ERR["ARG"]
         # mass terms and stepping before the DoFs
         assert _status(lambda: c.set_mass(1.0, 10.0)) == ERR["STATE"]
         assert _status(lambda: c.bdf_init(2, 1.0)) == ERR["STATE"]

@@ -27,7 +27,15 @@ CASES = {
     "elasticity3dQ2-3": lambda: P.elasticity3d_q2(3, seed=4),
     "neohooke3dQ2-3": lambda: P.elasticity3d_q2(3, seed=6, kernel=P.NEOHOOKE),
     "sheet2dQ2-6": lambda: P.pulled_sheet_q2(6, True),
+    "sheet2dQ2N-6": lambda: _sheet_n(6),
+    "elasticity3dQ2T-3": lambda: P.elasticity3d_q2(3, seed=5, traction=(0.1, 0.0, -0.3)),
 }
+
+
+def _sheet_n(n):
+    p = P.pulled_sheet_q2(n, False)
+    p.traction = (120.0, 0.0, 0.0)
+    return p
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -126,13 +134,15 @@ def _read_compref(path):
     return ndof, rows
 
 
-def test_q2_compressible_newton_golden(golden):
-    """reference/06-elastic/compressible.cpp under displacement control on
-    the device: assembly of residual and tangent, Jacobi-PCG to 1e-12 (the
-    FP64 round-off floor of these systems is near 1e-13),
-    ||b||/N and ||dx||/N against compRefOutD.dat."""
-    ndof, ref = _read_compref(os.path.join(golden, "compRefOutD.dat"))
-    prob = P.pulled_sheet_q2(20, True)
+@pytest.mark.parametrize("mode", ["D", "N"])
+def test_q2_compressible_newton_golden(golden, mode):
+    """reference/06-elastic/compressible.cpp under displacement (D) and
+    traction (N: t_x = 300 (step+1)/5 (y - 1/2) on x = 1, Q2 faces and the
+    traction profile) control on the device: assembly of residual and
+    tangent, Jacobi-PCG to 1e-12 (the FP64 round-off floor of these systems
+    is near 1e-13), ||b||/N and ||dx||/N against compRefOut{D,N}.dat."""
+    ndof, ref = _read_compref(os.path.join(golden, "compRefOut%s.dat" % mode))
+    prob = P.pulled_sheet_q2(20, mode == "D")
     n, dim = prob.n_active, prob.dim
     assert n == ndof
     u = np.zeros(prob.n_fnodes * dim)
@@ -144,7 +154,10 @@ def test_q2_compressible_newton_golden(golden):
     with abi.Context(0) as ctx:
         ctx.load_problem(prob)
         for step in range(5):
-            g[prob.meta["right_x"] * dim] = 3.0 / 5 * (step + 1)
+            if mode == "D":
+                g[prob.meta["right_x"] * dim] = 3.0 / 5 * (step + 1)
+            else:
+                ctx.set_traction(prob.ffaces, (300.0 * (step + 1) / 5, 0.0, 0.0))
             ctx.set_prescribed(g)
             for it in range(30):
                 ctx.set_dof_values(u)
@@ -154,7 +167,10 @@ def test_q2_compressible_newton_golden(golden):
                 if c1 < 1e-9:
                     hist.append((step, it, c1, None))
                     break
-                _, _, st = ctx.solve(abi.SOLVE_FP64, abi.FMT_CSR, 1e-12, max_it=20000)
+                # the round-off floor of the traction-controlled systems is
+                # about 1.3e-12 (small loads against E = 1000)
+                rtol = 1e-12 if mode == "D" else 1e-11
+                _, _, st = ctx.solve(abi.SOLVE_FP64, abi.FMT_CSR, rtol, max_it=20000)
                 assert st == abi.INS_OK
                 x = ctx.get_solution()
                 u[act] += x[prob.dof_id[act]]
