@@ -270,6 +270,24 @@ int ins_distribute(ins_ctx* ctx, int op);
 int ins_error_norm(ins_ctx* ctx, int order, int ref_kind, const double* ref_params,
                    const double* u, double* out);
 
+#define INS_CELL_VALUE     0  /* u_h(xi): dpn values per element              */
+#define INS_CELL_GRADIENT  1  /* grad u_h(xi): dim*dpn, [d*dpn+c] = du_c/dx_d  */
+#define INS_CELL_PK2       2  /* 2nd Piola-Kirchhoff stress S: 9, row-major   */
+#define INS_CELL_CAUCHY    3  /* Cauchy stress F S F^T / det F: 9, row-major  */
+/* One local point xi of every element, as the reference's cell data are
+ * formed for base::io::vtk::writeCellData (base/io/vtk/LegacyWriter.hpp
+ * :314-328): base::post::evaluateField / evaluateFieldGradient
+ * (base/post/evaluateField.hpp:166-206, 231-273) and solid::secondPKStress /
+ * solid::cauchy (solid/Stress.hpp:38-81; F = I + grad u^T extended by the
+ * identity to 3x3, solid/Deformation.hpp:26-45) with the context's material
+ * (INS_STVENANT / INS_NEOHOOKE and their per-element lambda, mu; the stresses
+ * need dpn = dim).  xi: dim local coordinates in [0,1] (NULL = the centroid,
+ * base::ShapeCentroid); u: values per field node and component (NULL = the
+ * context's current DoF values); out: host array of n_elem * (values per
+ * element), element-major. */
+int ins_cell_eval(ins_ctx* ctx, int quantity, const double* xi, const double* u,
+                  double* out);
+
 /* ---- serving: many solves batched into shared launches --------------------
  * No single reference counterpart: the reference solves one system per
  * Eigen3::cgSolve call (base/solver/Eigen3.hpp:265-275).  A server runs the

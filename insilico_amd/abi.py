@@ -35,10 +35,11 @@ SYMBOLS = [
     "ins_distribute", "ins_prepare", "ins_server_create", "ins_server_destroy", "ins_server_last_error",
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
     "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
-    "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile",
+    "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile", "ins_cell_eval",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
+CELL_VALUE, CELL_GRADIENT, CELL_PK2, CELL_CAUCHY = 0, 1, 2, 3
 
 
 class Stats(C.Structure):
@@ -84,6 +85,7 @@ def load():
         "ins_set_mass": [vp, dbl, dbl, vp],
         "ins_error_norm": [vp, i32, i32, vp, vp, C.POINTER(dbl)],
         "ins_bdf_init": [vp, i32, dbl, vp],
+        "ins_cell_eval": [vp, i32, vp, vp, vp],
         "ins_set_traction_profile": [vp, i32, dbl],
         "ins_bdf_step": [vp, dbl, i32, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
         "ins_set_dofs": [vp, i32, vp, i64],
@@ -172,6 +174,7 @@ class Context:
         self._chk(self.L.ins_set_mesh(self.h, xyz.shape[1], xyz.shape[0], _p(xyz),
                                       conn.shape[0], conn.shape[1], _p(conn)))
         self.n_nodes, self.dim = xyz.shape
+        self.n_elem = conn.shape[0]
 
     def set_field(self, degree, n_field_nodes, fconn):
         """degree-2 field on the mesh just set (field nodes in hierarchic order)"""
@@ -194,6 +197,18 @@ class Context:
         out = C.c_double()
         self._chk(self.L.ins_error_norm(self.h, order, kind, _p(prm), _p(uu), C.byref(out)))
         return out.value
+
+    def cell_eval(self, quantity, xi=None, u=None):
+        """value, gradient, 2nd Piola-Kirchhoff or Cauchy stress at the local
+        point xi (None = centroid) of every element, on the device; rows per
+        element (value: dpn, gradient: dim*dpn, stresses: 9 = 3x3 row-major)"""
+        per = {CELL_VALUE: self.dpn, CELL_GRADIENT: self.dim * self.dpn,
+               CELL_PK2: 9, CELL_CAUCHY: 9}.get(quantity, 9)
+        x = None if xi is None else np.ascontiguousarray(xi, np.float64)
+        uu = None if u is None else np.ascontiguousarray(u, np.float64)
+        out = np.zeros(self.n_elem * per)
+        self._chk(self.L.ins_cell_eval(self.h, quantity, _p(x), _p(uu), _p(out)))
+        return out.reshape(self.n_elem, per)
 
     def bdf_init(self, order, density=1.0, u0=None):
         u = None if u0 is None else np.ascontiguousarray(u0, np.float64)

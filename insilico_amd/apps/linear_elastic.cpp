@@ -1,10 +1,15 @@
 // Linear elasticity with the elastostatic fundamental solution as boundary
 // data, as reference/06-elastic/linearElastic.cpp:80-216: Q1 StVenant,
 // E = 1000, nu = 0.25, all boundary DoFs constrained to U(x; y, e_1) with the
-// source y = (-0.1, ...), quadrature degree 3, CG solve; prints the L2 error.
+// source y = (-0.1, ...), quadrature degree 3, CG solve; writes base.vtk next
+// to the mesh (displacement at the nodes, Cauchy stress at the element
+// centroids, linearElastic.cpp:48-77) and prints the L2 error of the
+// displacement against the fundamental solution
+// (base::post::errorComputation<0>), both evaluated on the device.
 // Usage: linear_elastic mesh.smf dim
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <string>
 
 #include "insilico/insilico.hpp"
@@ -44,7 +49,20 @@ int main(int argc, char* argv[])
         solver.finishAssembly();
         solver.cgSolve(1e-13);
         base::dof::setDoFsFromSolver(solver, field);
-        std::printf("%g\n", l2Error(mesh, field, solFun));
+
+        const Material material(lambda, mu);
+        {
+            const std::string vtkFile = base::io::baseName(argv[1], ".smf") + ".vtk";
+            std::ofstream vtk(vtkFile.c_str());
+            if (!vtk) throw Error(INS_ERR_ARG, "cannot write " + vtkFile);
+            base::io::vtk::LegacyWriter vtkWriter(vtk);
+            vtkWriter.writeUnstructuredGrid(mesh);
+            base::io::vtk::writePointData(vtkWriter, mesh, field, "disp");
+            base::io::vtk::writeCellData(
+                vtkWriter, solid::cauchy(mesh, field, material, base::shapeCentroid()), 9, "sigma");
+        }
+        FundamentalSolutionElastoStatic ref{lambda, mu, {src[0], src[1], src[2]}, {dir[0], dir[1], dir[2]}};
+        std::printf("%g\n", base::post::errorComputation<0>(quadrature, mesh, field, ref));
     } catch (const Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
         return 1;

@@ -42,9 +42,8 @@ int main(int argc, char* argv[])
             const double x = mesh.xyz[2 * n], y = mesh.xyz[2 * n + 1];
             maxErr = std::max(maxErr, std::abs(field.value[n] - std::sin(M_PI * x) * std::sin(M_PI * y)));
         }
-        const double l2 = l2Error(mesh, field, [](const double* x) {
-            return std::array<double, 3>{std::sin(M_PI * x[0]) * std::sin(M_PI * x[1]), 0., 0.};
-        });
+        const double l2 = base::post::errorComputation<0>(
+            quadrature, mesh, field, SineProductSolution{1.0, {M_PI, M_PI, 0.}});
         std::printf("%zu %d %.6e %.6e %.6e\n", numDoFs, iters, solver.lastRelres(), maxErr, l2);
     } catch (const Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.status, e.what());

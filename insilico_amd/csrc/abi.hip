@@ -802,6 +802,13 @@ int ins_error_norm(ins_ctx* h, int order, int ref_kind, const double* ref_params
     return error_norm(c, order, ref_kind, ref_params, u, out);
 }
 
+int ins_cell_eval(ins_ctx* h, int quantity, const double* xi, const double* u, double* out)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    return cell_eval(c, quantity, xi, u, out);
+}
+
 int ins_bdf_init(ins_ctx* h, int order, double density, const double* u0)
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);

@@ -18,6 +18,7 @@
 //  3. an RHS kernel does the same per active row for the Dirichlet lifting
 //     (base/asmb/assembleMatrix.hpp:91-95), element forces and face tractions.
 #include <cmath>
+#include <cstring>
 
 #include "common.hpp"
 
@@ -2140,6 +2141,227 @@ int error_norm(Ctx* c, int order, int kind, const double* prm, const double* hu,
     double sum = 0.;
     for (unsigned i = 0; i < g; i++) sum += h[i];
     *out = std::sqrt(sum);
+    return INS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cell evaluation at one local point of every element (the cell data of the
+// reference's VTK output): field value, field gradient
+// (base/post/evaluateField.hpp:166-206, 231-273), second Piola-Kirchhoff and
+// Cauchy stress (solid/Stress.hpp:38-81 with F = I + grad u^T extended by
+// the identity, solid/Deformation.hpp:26-45).  One element per lane; the
+// shape functions at the point are uniform over the grid and come in as a
+// kernel argument; the per-element results go through LDS so that each
+// block writes one contiguous run of out.
+struct CellTab {
+    double gd[8 * 3];    // geometry (Q1) local gradients
+    double fp[27];       // field shape values
+    double fd[27 * 3];   // field local gradients
+};
+
+template <int DIM, int NF, int S, int QTY>
+struct CellPer {
+    static constexpr int value = QTY == INS_CELL_VALUE ? S : QTY == INS_CELL_GRADIENT ? DIM * S : 9;
+};
+
+template <int DIM, int NF, int S, int QTY, int MAT>
+__global__ __launch_bounds__(256) void k_cell_eval(
+    int64_t E, const int32_t* __restrict__ gconn, const int32_t* __restrict__ fconn,
+    const double* __restrict__ xyz, const double* __restrict__ u,
+    const double* __restrict__ coef, int ncoef, CellTab tab, double* __restrict__ out)
+{
+    constexpr int GN = 1 << DIM;
+    constexpr int PER = CellPer<DIM, NF, S, QTY>::value;
+    __shared__ double stage[256 * PER];
+    const int64_t first = (int64_t)blockIdx.x * 256;
+    const int64_t e = first + threadIdx.x;
+    if (e < E) {
+        double o[PER];
+        if constexpr (QTY == INS_CELL_VALUE) {
+#pragma unroll
+            for (int c = 0; c < S; c++) o[c] = 0.;
+            for (int n = 0; n < NF; n++) {
+                const int64_t node = fconn[e * NF + n];
+#pragma unroll
+                for (int c = 0; c < S; c++) o[c] += tab.fp[n] * u[node * S + c];
+            }
+        } else {
+            double X[GN * DIM];
+#pragma unroll
+            for (int n = 0; n < GN; n++) {
+                const int64_t node = gconn[e * GN + n];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) X[n * DIM + d] = xyz[node * DIM + d];
+            }
+            double inv[DIM][DIM];
+            (void)geo_inverse<DIM, double>(X, tab.gd, inv);
+            double G[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};   // du_c/dx_d
+            for (int n = 0; n < NF; n++) {
+                double gx[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; i++) {
+                    double t = 0.;
+#pragma unroll
+                    for (int a = 0; a < DIM; a++) t += inv[i][a] * tab.fd[n * DIM + a];
+                    gx[i] = t;
+                }
+                const int64_t node = fconn[e * NF + n];
+#pragma unroll
+                for (int c = 0; c < S; c++) {
+                    const double v = u[node * S + c];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) G[d][c] += gx[d] * v;
+                }
+            }
+            if constexpr (QTY == INS_CELL_GRADIENT) {
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+#pragma unroll
+                    for (int c = 0; c < S; c++) o[d * S + c] = G[d][c];
+            } else {
+                double F[3][3], Sg[3][3], Cd[6][6];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) F[i][j] = (i == j ? 1. : 0.) + G[j][i];
+                material<MAT, double>(coef[e * ncoef], coef[e * ncoef + 1], F, Sg, Cd, false);
+                if constexpr (QTY == INS_CELL_PK2) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) o[i * 3 + j] = Sg[i][j];
+                } else {
+                    const double J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                                     F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                                     F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+                    double FS[3][3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            double t = 0.;
+#pragma unroll
+                            for (int k = 0; k < 3; k++) t += F[i][k] * Sg[k][j];
+                            FS[i][j] = t;
+                        }
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            double t = 0.;
+#pragma unroll
+                            for (int k = 0; k < 3; k++) t += FS[i][k] * F[j][k];
+                            o[i * 3 + j] = t / J;
+                        }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) stage[threadIdx.x * PER + k] = o[k];
+    }
+    __syncthreads();
+    const int64_t cnt = (E - first < 256 ? E - first : 256) * PER;
+    for (int64_t i = threadIdx.x; i < cnt; i += 256) out[first * PER + i] = stage[i];
+}
+
+template <int DIM, int NF, int S, int QTY, int MAT>
+static void launch_cell(Ctx* c, const CellTab& tab, const double* u, double* out)
+{
+    const unsigned g = (unsigned)((c->n_elem + 255) / 256);
+    k_cell_eval<DIM, NF, S, QTY, MAT><<<g, 256, 0, c->stream>>>(
+        c->n_elem, c->geo_conn(), c->conn.p, c->xyz.p, u, c->coef.p, c->ncoef, tab, out);
+}
+
+template <int DIM, int NF, int S>
+static bool dispatch_cell(Ctx* c, int qty, const CellTab& tab, const double* u, double* out)
+{
+    switch (qty) {
+    case INS_CELL_VALUE: launch_cell<DIM, NF, S, INS_CELL_VALUE, 0>(c, tab, u, out); return true;
+    case INS_CELL_GRADIENT:
+        launch_cell<DIM, NF, S, INS_CELL_GRADIENT, 0>(c, tab, u, out);
+        return true;
+    default: break;
+    }
+    if constexpr (S == DIM) {
+        const bool sv = c->kernel == INS_STVENANT;
+        if (qty == INS_CELL_PK2) {
+            if (sv) launch_cell<DIM, NF, S, INS_CELL_PK2, INS_STVENANT>(c, tab, u, out);
+            else launch_cell<DIM, NF, S, INS_CELL_PK2, INS_NEOHOOKE>(c, tab, u, out);
+        } else {
+            if (sv) launch_cell<DIM, NF, S, INS_CELL_CAUCHY, INS_STVENANT>(c, tab, u, out);
+            else launch_cell<DIM, NF, S, INS_CELL_CAUCHY, INS_NEOHOOKE>(c, tab, u, out);
+        }
+        return true;
+    }
+    return false;
+}
+
+int cell_eval(Ctx* c, int qty, const double* hxi, const double* hu, double* hout)
+{
+    if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_cell_eval before ins_set_dofs");
+    if (qty < INS_CELL_VALUE || qty > INS_CELL_CAUCHY || !hout)
+        return fail(c, INS_ERR_ARG, "ins_cell_eval arguments");
+    const int dim = c->dim, nf = c->npe, s = c->dpn;
+    if (qty >= INS_CELL_PK2) {
+        if (c->kernel != INS_STVENANT && c->kernel != INS_NEOHOOKE)
+            return fail(c, INS_ERR_STATE, "stresses need a solid material (ins_set_material)");
+        if (s != dim || c->ncoef < 2 || !c->coef.p)
+            return fail(c, INS_ERR_ARG, "stresses need dpn = dim and lambda, mu per element");
+    }
+    double xi[3] = {0.5, 0.5, 0.5};
+    if (hxi)
+        for (int d = 0; d < dim; d++) {
+            if (!(hxi[d] >= 0. && hxi[d] <= 1.)) return fail(c, INS_ERR_ARG, "xi outside [0,1]");
+            xi[d] = hxi[d];
+        }
+    CellTab tab;
+    std::memset(&tab, 0, sizeof(tab));
+    {
+        double phi[8], dphi[24];
+        q1_shape(dim, xi, phi, dphi);
+        for (int i = 0; i < (1 << dim) * dim; i++) tab.gd[i] = dphi[i];
+        if (c->deg == 2) {
+            lagrange2_shape(dim, xi, tab.fp, tab.fd);
+        } else {
+            for (int n = 0; n < (1 << dim); n++) tab.fp[n] = phi[n];
+            for (int i = 0; i < (1 << dim) * dim; i++) tab.fd[i] = dphi[i];
+        }
+    }
+    const int per = qty == INS_CELL_VALUE ? s : qty == INS_CELL_GRADIENT ? dim * s : 9;
+    const int64_t nd = c->n_nodes * s;
+    const double* u = c->u.p;
+    double* tmp = nullptr;
+    double* dout = nullptr;
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&dout, (size_t)c->n_elem * per * sizeof(double),
+                                    c->stream));
+    if (hu) {
+        INS_HIP_CHECK(c, hipMallocAsync((void**)&tmp, nd * sizeof(double), c->stream));
+        INS_HIP_CHECK(c, hipMemcpyAsync(tmp, hu, nd * sizeof(double), hipMemcpyHostToDevice,
+                                        c->stream));
+        u = tmp;
+    }
+    bool ok = false;
+    const int key = dim * 1000 + nf * 10 + s;
+    switch (key) {
+    case 2041: ok = dispatch_cell<2, 4, 1>(c, qty, tab, u, dout); break;
+    case 2042: ok = dispatch_cell<2, 4, 2>(c, qty, tab, u, dout); break;
+    case 3081: ok = dispatch_cell<3, 8, 1>(c, qty, tab, u, dout); break;
+    case 3083: ok = dispatch_cell<3, 8, 3>(c, qty, tab, u, dout); break;
+    case 2091: ok = dispatch_cell<2, 9, 1>(c, qty, tab, u, dout); break;
+    case 2092: ok = dispatch_cell<2, 9, 2>(c, qty, tab, u, dout); break;
+    case 3271: ok = dispatch_cell<3, 27, 1>(c, qty, tab, u, dout); break;
+    case 3273: ok = dispatch_cell<3, 27, 3>(c, qty, tab, u, dout); break;
+    default: break;
+    }
+    if (ok) {
+        INS_HIP_CHECK(c, hipGetLastError());
+        INS_HIP_CHECK(c, hipMemcpyAsync(hout, dout, (size_t)c->n_elem * per * sizeof(double),
+                                        hipMemcpyDeviceToHost, c->stream));
+    }
+    (void)hipFreeAsync(dout, c->stream);
+    if (tmp) (void)hipFreeAsync(tmp, c->stream);
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (!ok) return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
     return INS_OK;
 }
 

@@ -299,6 +299,7 @@ constexpr int ASM_STAGED = 1 << 8;  // internal: RHS from the staged element dat
 int assemble(Ctx* c, int precision, int flags);
 int compute_dinv(Ctx* c);
 int error_norm(Ctx* c, int order, int kind, const double* prm, const double* u, double* out);
+int cell_eval(Ctx* c, int quantity, const double* xi, const double* u, double* out);
 // solve.hip
 int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
           double* relres);

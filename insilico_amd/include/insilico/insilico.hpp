@@ -245,49 +245,66 @@ inline void scaleConstraints(Field& f, double factor)
 
 // base::dof::constrainBoundary with a functor diriFun(x, node, field)
 // applied at the nodes of the boundary faces (base/dof/constrainBoundary.hpp:52-120)
+// The DoF objects of every boundary face of a degree-2 field in the face's
+// own hierarchic order: LINE3 = (v0, v1, mid) for a QUAD edge; QUAD9 = the 4
+// vertices in face-table order, the edges (w0,w1), (w1,w2), (w2,w3),
+// (w3,w0), the centre (base/mesh/HierarchicOrder.hpp:131-150, 201-270).
+// Faces come in the order of their sorted vertex ids (the boundary is the
+// set of faces met by one element, base/mesh/MeshBoundary.hpp).
+inline std::vector<std::vector<int32_t>> boundaryFieldFaces(const Mesh& m, const Field& f)
+{
+    if (f.degree != 2) return boundaryFaces(m);
+    const int nf = m.dim == 2 ? 9 : 27, nv = m.npe();
+    std::map<std::vector<int32_t>, std::pair<int, std::pair<std::size_t, int>>> cnt;
+    const int nfaces = m.dim == 3 ? 6 : 4;
+    for (std::size_t e = 0; e < m.numElements(); e++)
+        for (int fc = 0; fc < nfaces; fc++) {
+            std::vector<int32_t> key;
+            if (m.dim == 3)
+                for (int k = 0; k < 4; k++) key.push_back(m.conn[e * nv + detail_::HEX_FACES[fc][k]]);
+            else
+                for (int k = 0; k < 2; k++) key.push_back(m.conn[e * nv + detail_::QUAD_EDGES[fc][k]]);
+            std::sort(key.begin(), key.end());
+            auto it = cnt.find(key);
+            if (it == cnt.end()) cnt.emplace(key, std::make_pair(1, std::make_pair(e, fc)));
+            else it->second.first++;
+        }
+    std::vector<std::vector<int32_t>> out;
+    for (auto& kv : cnt) {
+        if (kv.second.first != 1) continue;
+        const std::size_t e = kv.second.second.first;
+        const int fc = kv.second.second.second;
+        std::vector<int> locs;
+        if (m.dim == 2) {
+            locs = {detail_::QUAD_EDGES[fc][0], detail_::QUAD_EDGES[fc][1], 4 + fc};
+        } else {
+            const int* w = detail_::HEX_FACES[fc];
+            for (int k = 0; k < 4; k++) locs.push_back(w[k]);
+            for (int k = 0; k < 4; k++) {
+                const int a = w[k], b = w[(k + 1) % 4];
+                for (int ed = 0; ed < 12; ed++)
+                    if ((detail_::HEX_EDGES[ed][0] == a && detail_::HEX_EDGES[ed][1] == b) ||
+                        (detail_::HEX_EDGES[ed][0] == b && detail_::HEX_EDGES[ed][1] == a))
+                        locs.push_back(8 + ed);
+            }
+            locs.push_back(20 + fc);
+        }
+        std::vector<int32_t> nodes;
+        for (int loc : locs) nodes.push_back(f.conn[e * nf + loc]);
+        out.push_back(std::move(nodes));
+    }
+    return out;
+}
+
 template <typename DIRIFUN>
 void constrainBoundary(const Mesh& m, Field& f, DIRIFUN diriFun)
 {
     if (f.degree == 2) {
         // the DoF objects of every boundary face: its vertices, edges and
         // (3-D) its face centre, at their support points
-        const int nf = m.dim == 2 ? 9 : 27, nv = m.npe();
-        std::map<std::vector<int32_t>, std::pair<int, std::pair<std::size_t, int>>> cnt;
-        const int nfaces = m.dim == 3 ? 6 : 4;
-        for (std::size_t e = 0; e < m.numElements(); e++)
-            for (int fc = 0; fc < nfaces; fc++) {
-                std::vector<int32_t> key;
-                if (m.dim == 3)
-                    for (int k = 0; k < 4; k++) key.push_back(m.conn[e * nv + detail_::HEX_FACES[fc][k]]);
-                else
-                    for (int k = 0; k < 2; k++) key.push_back(m.conn[e * nv + detail_::QUAD_EDGES[fc][k]]);
-                std::sort(key.begin(), key.end());
-                auto it = cnt.find(key);
-                if (it == cnt.end()) cnt.emplace(key, std::make_pair(1, std::make_pair(e, fc)));
-                else it->second.first++;
-            }
         std::vector<char> done(f.numNodes(), 0);
-        for (auto& kv : cnt) {
-            if (kv.second.first != 1) continue;
-            const std::size_t e = kv.second.second.first;
-            const int fc = kv.second.second.second;
-            std::vector<int> locs;
-            if (m.dim == 2) {
-                locs = {detail_::QUAD_EDGES[fc][0], detail_::QUAD_EDGES[fc][1], 4 + fc};
-            } else {
-                const int* w = detail_::HEX_FACES[fc];
-                for (int k = 0; k < 4; k++) locs.push_back(w[k]);
-                for (int k = 0; k < 4; k++) {
-                    const int a = w[k], b = w[(k + 1) % 4];
-                    for (int ed = 0; ed < 12; ed++)
-                        if ((detail_::HEX_EDGES[ed][0] == a && detail_::HEX_EDGES[ed][1] == b) ||
-                            (detail_::HEX_EDGES[ed][0] == b && detail_::HEX_EDGES[ed][1] == a))
-                            locs.push_back(8 + ed);
-                }
-                locs.push_back(20 + fc);
-            }
-            for (int loc : locs) {
-                const int32_t node = f.conn[e * nf + loc];
+        for (const auto& face : boundaryFieldFaces(m, f)) {
+            for (const int32_t node : face) {
                 if (done[node]) continue;
                 done[node] = 1;
                 std::array<double, 3> x{0., 0., 0.};
@@ -569,10 +586,15 @@ public:
         bf_ = p;
         flags_ |= INS_ASM_RHS;
     }
-    void requestTraction(std::vector<std::vector<int32_t>> faces, const std::array<double, 3>& t)
+    // traction t on the faces; with axis >= 0 it is scaled by (x[axis] - x0)
+    // at every surface quadrature point
+    void requestTraction(std::vector<std::vector<int32_t>> faces, const std::array<double, 3>& t,
+                         int axis = -1, double x0 = 0.)
     {
         faces_ = std::move(faces);
         traction_ = t;
+        trAxis_ = axis;
+        trX0_ = x0;
         flags_ |= INS_ASM_RHS;
     }
     void setPrecision(int p) { precision_ = p; }
@@ -580,7 +602,13 @@ public:
     // iteration re-assembles into the same device context, pattern kept)
     void reset()
     {
+        // a fresh solver: no loads requested
         flags_ = 0;
+        bfKind_ = INS_BF_NONE;
+        faces_.clear();
+        traction_ = {0., 0., 0.};
+        trAxis_ = -1;
+        trX0_ = 0.;
         solved_ = false;
         assembled_ = false;
         hostRhs_.clear();
@@ -613,12 +641,17 @@ public:
     }
 
     // Eigen3::cgSolve (base/solver/Eigen3.hpp:265-275): Jacobi-PCG
+    // Eigen's ConjugateGradient stops after 2n iterations and returns its
+    // last iterate without an error (base/solver/Eigen3.hpp:265-275): so
+    // does this call when rtol is not reached (converged() tells)
     int cgSolve(double rtol = 1e-12, int mode = INS_SOLVE_FP64, int format = INS_FMT_SELL)
     {
         int it = 0;
         double rel = 0.;
         const int maxIt = (int)std::max<std::size_t>(2 * n_, 100);
-        insilico::check(ins_solve(ctx_, mode, format, rtol, maxIt, &it, &rel), ctx_);
+        const int st = ins_solve(ctx_, mode, format, rtol, maxIt, &it, &rel);
+        converged_ = st == INS_OK;
+        if (st != INS_ERR_NOCONV) insilico::check(st, ctx_);
         x_.resize(n_);
         insilico::check(ins_get_solution(ctx_, x_.data()), ctx_);
         solved_ = true;
@@ -632,6 +665,7 @@ public:
 
     double getValue(std::size_t i) const { return x_.at(i); }  // Eigen3.hpp:293-296
     double lastRelres() const { return lastRelres_; }
+    bool converged() const { return converged_; }
     std::size_t size() const { return n_; }
 
     // Eigen3::debugLHS format (row col value), base/solver/Eigen3.hpp:311-317
@@ -702,6 +736,7 @@ private:
         const int nfn = faces_.empty() ? 0 : (int)faces_[0].size();
         insilico::check(ins_set_traction(ctx_, (int64_t)faces_.size(), nfn, faces_.empty() ? nullptr : flat.data(),
                                          traction_.data()), ctx_);
+        insilico::check(ins_set_traction_profile(ctx_, faces_.empty() ? -1 : trAxis_, trX0_), ctx_);
         insilico::check(ins_assemble(ctx_, precision_, flags_), ctx_);
     }
 
@@ -718,8 +753,11 @@ private:
     std::array<double, 4> bf_{0., 0., 0., 0.};
     std::vector<std::vector<int32_t>> faces_;
     std::array<double, 3> traction_{0., 0., 0.};
+    int trAxis_ = -1;
+    double trX0_ = 0.;
     std::vector<double> x_;
     double lastRelres_ = 0.;
+    bool converged_ = false;
 };
 
 }  // namespace solver
@@ -817,6 +855,21 @@ void neumannForceComputation(solver::Hip& solver, FB& fieldBinder,
 {
     solver.bind(fieldBinder.mesh, fieldBinder.field);
     solver.requestTraction(std::move(faces), t);
+}
+// a traction growing linearly across the face, t(x) = t * (x[axis] - x0),
+// as the Neumann function of reference/06-elastic/PulledSheet.hpp:63-77
+// (t_x = value * (y - 1/2) on x = 1)
+struct LinearTraction {
+    std::array<double, 3> t;
+    int axis;
+    double x0;
+};
+template <typename FB>
+void neumannForceComputation(solver::Hip& solver, FB& fieldBinder,
+                             std::vector<std::vector<int32_t>> faces, const LinearTraction& lt)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestTraction(std::move(faces), lt.t, lt.axis, lt.x0);
 }
 
 }  // namespace asmb

@@ -474,6 +474,31 @@ def fundsol_elasto(dim, lam, mu, y, x, direc):
     return out
 
 
+def distorted_block(dim, n, deg=1, kernel=STVENANT, lam=1.5, mu=0.75, amp=0.2, seed=7):
+    """Unit square/cube of n^dim elements with the interior nodes moved by up
+    to amp * h per coordinate (elements stay convex for amp < 0.25), one
+    material for all elements, no constraints: a mesh on which the point
+    evaluations of a field are checked (a degree-2 field takes its nodes at
+    the Lagrange points of the distorted Q1 geometry)."""
+    xyz, conn = unit_cube(dim, n)
+    inner = np.all((xyz > 1e-12) & (xyz < 1.0 - 1e-12), axis=1)
+    shift = (uniform01(seed, xyz.size).reshape(xyz.shape) - 0.5) * (2.0 * amp / n)
+    xyz = np.where(inner[:, None], xyz + shift, xyz)
+    coef = np.tile(np.array([[lam, mu]]), (conn.shape[0], 1))
+    if kernel == LAPLACE:
+        coef = np.ones((conn.shape[0], 1))
+    dpn = 1 if kernel == LAPLACE else dim
+    p = Problem(dim, np.ascontiguousarray(xyz), conn, dpn, None, 0, None, kernel, coef, 2,
+                name="distorted%dD-%d" % (dim, n))
+    if deg == 2:
+        p.deg = 2
+        p.fconn, p.fxyz = q2_field(xyz, conn, dim)
+    nf = p.n_fnodes
+    p.dof_id, p.n_active = number_dofs(np.zeros((nf, dpn), bool))
+    p.g = np.zeros(nf * dpn)
+    return p
+
+
 def linear_elastic_fundsol(xyz, conn, dim, E=1000.0, nu=0.25, nq=2):
     """reference/06-elastic/linearElastic.cpp:80-216: Q1 StVenant, the whole
     boundary constrained to the elastostatic fundamental solution with source

@@ -1083,3 +1083,97 @@ double orc_error_norm(const orc_problem* p, const double* uh, int order, int kin
     }
     return sqrt(err);
 }
+
+/* Cell evaluation at one local point xi of every element.
+ * ORC_CELL_VALUE: u_h(xi) (base/post/evaluateField.hpp:166-206), s values;
+ * ORC_CELL_GRADIENT: grad u_h, out[d*s+c] = d u_c / d x_d
+ *   (base/post/evaluateField.hpp:231-273), dim*s values;
+ * ORC_CELL_PK2 / ORC_CELL_CAUCHY: F = I + (grad u)^T extended by the
+ *   identity to 3x3 (solid/Deformation.hpp:26-45), S of the material law
+ *   (mat/hypel/StVenant.hpp, NeoHookeanCompressible.hpp) and, for the
+ *   Cauchy stress, sigma = F S F^T / det F (solid/Stress.hpp:38-58,
+ *   secondPKStress :64-81); 9 values per element, row-major 3x3.
+ * Returns the values per element, or -1 for an unknown quantity. */
+int orc_cell_eval(const orc_problem* p, const double* uh, int quantity, const double* xi,
+                  double* out)
+{
+    const int dim = p->dim, gnpe = p->npe, s = p->dpn;
+    const int npe = field_npe(p), K = p->deg == 2 ? 2 : 1;
+    const int32_t* fc = field_conn(p);
+    int per;
+    switch (quantity) {
+    case ORC_CELL_VALUE: per = s; break;
+    case ORC_CELL_GRADIENT: per = dim * s; break;
+    case ORC_CELL_PK2:
+    case ORC_CELL_CAUCHY:
+        if (p->kernel == ORC_LAPLACE || s != dim) return -1;
+        per = 9;
+        break;
+    default: return -1;
+    }
+    double gphi[8], gdphi[8 * 3], phi[ORC_MAXF], dphi[ORC_MAXF * 3];
+    orc_q1_shape(dim, xi, gphi, gdphi);
+    if (K == 2) orc_lagrange_shape(dim, K, xi, phi, dphi);
+    for (int64_t e = 0; e < p->n_elem; e++) {
+        double X[8 * 3], U[ORC_MAXE], gx[ORC_MAXF * 3];
+        for (int n = 0; n < gnpe; n++) {
+            const int64_t node = p->conn[e * gnpe + n];
+            for (int d = 0; d < dim; d++) X[n * dim + d] = p->xyz[node * dim + d];
+        }
+        for (int n = 0; n < npe; n++) {
+            const int64_t node = fc[e * npe + n];
+            for (int c = 0; c < s; c++) U[n * s + c] = uh[node * s + c];
+        }
+        if (K == 1) {
+            for (int n = 0; n < gnpe; n++) phi[n] = gphi[n];
+            (void)orc_contravariant(dim, X, gdphi, gx);
+        } else {
+            (void)contra_field(dim, X, gdphi, npe, dphi, gx);
+        }
+        double* o = out + e * per;
+        if (quantity == ORC_CELL_VALUE) {
+            for (int c = 0; c < s; c++) {
+                double v = 0.;
+                for (int n = 0; n < npe; n++) v += phi[n] * U[n * s + c];
+                o[c] = v;
+            }
+            continue;
+        }
+        double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  /* G[d][c] = du_c/dx_d */
+        for (int d = 0; d < dim; d++)
+            for (int c = 0; c < s; c++) {
+                double g = 0.;
+                for (int n = 0; n < npe; n++) g += gx[n * dim + d] * U[n * s + c];
+                G[d][c] = g;
+            }
+        if (quantity == ORC_CELL_GRADIENT) {
+            for (int d = 0; d < dim; d++)
+                for (int c = 0; c < s; c++) o[d * s + c] = G[d][c];
+            continue;
+        }
+        double F[3][3], S[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) F[i][j] = (i == j ? 1. : 0.) + G[j][i];
+        material(p->kernel, p->coef[e * p->ncoef + 0], p->coef[e * p->ncoef + 1], F, S, NULL);
+        if (quantity == ORC_CELL_PK2) {
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) o[i * 3 + j] = S[i][j];
+            continue;
+        }
+        const double J = det3(F);
+        double FS[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0.;
+                for (int k = 0; k < 3; k++) t += F[i][k] * S[k][j];
+                FS[i][j] = t;
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0.;
+                for (int k = 0; k < 3; k++) t += FS[i][k] * F[j][k];
+                o[i * 3 + j] = t / J;
+            }
+    }
+    return per;
+}

@@ -134,6 +134,14 @@ enum { ORC_REF_ZERO = 0, ORC_REF_SINPROD = 1, ORC_REF_KELVIN = 2 };
  * direction[3] (order 0 only) */
 double orc_error_norm(const orc_problem* p, const double* uh, int order, int kind,
                       const double* prm);
+/* quantities of orc_cell_eval */
+enum { ORC_CELL_VALUE = 0, ORC_CELL_GRADIENT = 1, ORC_CELL_PK2 = 2, ORC_CELL_CAUCHY = 3 };
+/* one local point xi (dim values) of every element: the field value (dpn
+ * per element), its gradient (dim*dpn, [d*dpn+c] = du_c/dx_d), or the
+ * second Piola-Kirchhoff / Cauchy stress of the solid kernels (9 per
+ * element, row-major 3x3); returns the count per element, -1 if invalid */
+int orc_cell_eval(const orc_problem* p, const double* uh, int quantity, const double* xi,
+                  double* out);
 
 #ifdef __cplusplus
 }

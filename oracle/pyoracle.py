@@ -78,6 +78,7 @@ def lib():
         L.orc_fundsol_elasto.restype = None
         L.orc_error_norm.argtypes = [C.POINTER(Problem), _f64p, C.c_int, C.c_int, _f64p]
         L.orc_error_norm.restype = C.c_double
+        L.orc_cell_eval.argtypes = [C.POINTER(Problem), _f64p, C.c_int, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -280,3 +281,22 @@ def error_norm(prob, uh, order, kind, params):
     prm = np.zeros(8)
     prm[:len(params)] = params
     return lib().orc_error_norm(op.ref(), np.ascontiguousarray(uh, np.float64), order, kind, prm)
+
+
+CELL_VALUE, CELL_GRADIENT, CELL_PK2, CELL_CAUCHY = 0, 1, 2, 3
+
+
+def cell_eval(prob, uh, quantity, xi=None):
+    """Value, gradient, 2nd Piola-Kirchhoff or Cauchy stress at the local point
+    xi (default: the element centroid) of every element; rows per element."""
+    op = OracleProblem(prob)
+    if xi is None:
+        xi = np.full(prob.dim, 0.5)
+    per = {CELL_VALUE: prob.dpn, CELL_GRADIENT: prob.dim * prob.dpn,
+           CELL_PK2: 9, CELL_CAUCHY: 9}[quantity]
+    out = np.zeros(prob.n_elem * per)
+    got = lib().orc_cell_eval(op.ref(), np.ascontiguousarray(uh, np.float64), quantity,
+                              np.ascontiguousarray(xi, np.float64), out)
+    if got != per:
+        raise ValueError("orc_cell_eval: quantity not defined for this problem")
+    return out.reshape(prob.n_elem, per)

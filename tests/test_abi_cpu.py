@@ -85,3 +85,88 @@ int main(int argc, char** argv) {
     assert np.array_equal(c, fconn)
     sp = np.array([float(v) for v in vals[1 + fconn.size:]]).reshape(fxyz.shape)
     assert np.array_equal(sp, fxyz)
+
+
+def test_cpp_vtk_writer_reproduces_reference_layout(tmp_path, golden):
+    """base::io::vtk::LegacyWriter of the drop-in layer: the grid of
+    square_20.smf and the disp / sigma data of
+    reference/06-elastic/quad.020.Dref.vtk, fed back as values, come out as
+    the reference's file line for line (only the free-text title line
+    differs); the Q2 boundary faces and the node values of a degree-2 field
+    follow the host setup.  Compiled with g++, run without a device."""
+    import subprocess
+    import numpy as np
+    ref_path = os.path.join(golden, "quad.020.Dref.vtk")
+    with open(ref_path) as fh:
+        ref = fh.read().splitlines()
+    disp = [float(v) for v in "\n".join(ref).split("VECTORS disp float")[1].split("CELL_DATA")[0].split()]
+    sig = [float(v) for v in "\n".join(ref).split("TENSORS sigma float")[1].split()]
+    vals = tmp_path / "vals.txt"
+    vals.write_text(" ".join(repr(v) for v in disp) + "\n" + " ".join(repr(v) for v in sig) + "\n")
+    src = tmp_path / "vtkw.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include "insilico/insilico.hpp"
+#include "insilico/post.hpp"
+int main(int argc, char** argv) {
+    insilico::Mesh m = insilico::readSMF(argv[1], insilico::smfDimension(argv[1]));
+    std::ifstream in(argv[2]);
+    std::string l1, l2;
+    std::getline(in, l1);
+    std::getline(in, l2);
+    // disp: 3 values per node in the file, the field holds dim = 2
+    insilico::Field f = insilico::generateField(m, 1, 2);
+    std::istringstream s1(l1), s2(l2);
+    for (std::size_t n = 0; n < m.numNodes(); n++) {
+        double a, b, c;
+        s1 >> a >> b >> c;
+        f.value[2 * n] = a;
+        f.value[2 * n + 1] = b;
+    }
+    std::vector<double> sigma;
+    double v;
+    while (s2 >> v) sigma.push_back(v);
+    std::ofstream out(argv[3]);
+    base::io::vtk::LegacyWriter w(out);
+    w.writeUnstructuredGrid(m);
+    base::io::vtk::writePointData(w, m, f, "disp");
+    base::io::vtk::writeCellData(w, sigma, 9, "sigma");
+    // Q2 boundary faces: count, then the nodes of the first face
+    insilico::Field q2 = insilico::generateField(m, 2, 2);
+    auto faces = insilico::boundaryFieldFaces(m, q2);
+    std::printf("%zu %zu\n", faces.size(), faces[0].size());
+    for (std::size_t i = 0; i < q2.value.size(); i++) q2.value[i] = (double)i;
+    auto nv = base::post::evaluateFieldAtNodes(m, q2);
+    for (double x : nv) std::printf("%.17g\n", x);
+    std::printf("%s\n", base::io::baseName("dir/quad.020.smf", ".smf").c_str());
+    std::printf("%s\n", base::io::leadingZeros(5).c_str());
+    return 0;
+}
+''')
+    exe = tmp_path / "vtkw"
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "insilico_amd", "include")]
+    subprocess.run(["g++", "-O1", "-std=c++17"] + inc + [str(src), "-o", str(exe),
+                    "-L" + os.path.join(ROOT, "insilico_amd"), "-linsilico_hip",
+                    "-Wl,-rpath," + os.path.join(ROOT, "insilico_amd")], check=True, timeout=120)
+    smf = os.path.join(golden, "square_20.smf")
+    outp = tmp_path / "out.vtk"
+    res = subprocess.run([str(exe), smf, str(vals), str(outp)], capture_output=True, text=True,
+                         timeout=60, check=True)
+    got = outp.read_text().splitlines()
+    assert len(got) == len(ref)
+    diff = [(i, a, b) for i, (a, b) in enumerate(zip(got, ref)) if a != b and i != 1]
+    assert not diff, diff[:5]
+    lines = res.stdout.split("\n")
+    nf, fn = map(int, lines[0].split())
+    assert (nf, fn) == (80, 3)
+    from insilico_amd import problems as P
+    xyz, conn = P.read_smf(smf, 2)
+    fconn, _ = P.q2_field(xyz, conn, 2)
+    nodev = np.array([float(x) for x in lines[1:1 + 2 * xyz.shape[0]]]).reshape(-1, 2)
+    want = np.zeros_like(nodev)
+    want[conn[:, :4].ravel()] = np.stack([2 * fconn[:, :4].ravel(), 2 * fconn[:, :4].ravel() + 1], 1)
+    assert np.array_equal(nodev, want)
+    assert lines[1 + 2 * xyz.shape[0]] == "dir/quad.020"
+    assert lines[2 + 2 * xyz.shape[0]] == "0005"

@@ -43,11 +43,32 @@ def test_poisson_app(oracle):
 @pytest.mark.parametrize("dim,mesh,N", [(2, "quad.002.smf", 2), (2, "quad.005.smf", 5),
                                         (2, "quad.010.smf", 10), (3, "cube.002.smf", 2),
                                         (3, "cube.004.smf", 4), (3, "cube.008.smf", 8)])
-def test_linear_elastic_app_golden(golden, dim, mesh, N):
-    """reference/06-elastic/linearElastic{2D,3D}.ref.dat to the printed 6 digits."""
+def test_linear_elastic_app_golden(oracle, golden, read_vtk, tmp_path, dim, mesh, N):
+    """reference/06-elastic/linearElastic{2D,3D}.ref.dat to the printed 6
+    digits (L2 error on the device); the VTK file it writes holds the grid,
+    the displacement and the centroid Cauchy stress of the oracle's solve of
+    the same problem to the printed digits."""
+    import shutil
     ref = dict(np.loadtxt(os.path.join(golden, "linearElastic%dD.ref.dat" % dim)))
-    err = float(_run("linear_elastic", os.path.join(golden, mesh), dim))
+    shutil.copy(os.path.join(golden, mesh), tmp_path / mesh)
+    err = float(_run("linear_elastic", tmp_path / mesh, dim))
     assert err == pytest.approx(ref[N], rel=3e-6), (N, err, ref[N])
+    vtk = read_vtk(str(tmp_path / mesh.replace(".smf", ".vtk")))
+    xyz, conn = P.read_smf(os.path.join(golden, mesh), dim)
+    prob = P.linear_elastic_fundsol(xyz, conn, dim)
+    assert vtk["cells"] == prob.conn.tolist()
+    rp, ci, val, b, _ = oracle.assemble(prob)
+    x, _, _ = oracle.pcg(rp, ci, val, b, tol=1e-13)
+    u = prob.g.copy()
+    act = prob.dof_id >= 0
+    u[act] = x[prob.dof_id[act]]
+    sig = oracle.cell_eval(prob, u, oracle.CELL_CAUCHY)
+    w = vtk["cell_data"]["sigma"]
+    assert np.all(np.abs(sig - w) <= 1e-5 * np.abs(sig).max() + 1e-5 * np.abs(w)), np.abs(sig - w).max()
+    disp = np.zeros((prob.n_nodes, 3))
+    disp[:, :dim] = u.reshape(-1, dim)
+    wd = vtk["point_data"]["disp"]
+    assert np.all(np.abs(disp - wd) <= 1e-5 * np.abs(disp).max() + 1e-5 * np.abs(wd))
 
 
 def newton_oracle(oracle, n, steps, pull, maxit, tol, mu, lam):
@@ -120,27 +141,44 @@ def test_compressible_newton_app(oracle, prec):
     assert umax == pytest.approx(np.abs(u[1::3]).max(), rel=1e-6)
 
 
-def test_compressible_q2_app_golden(golden):
-    """reference/06-elastic/compressible.cpp on the drop-in C++ layer with a
-    degree-2 field (generateField, constrainBoundary at the Q2 support points,
-    device assembly of the Q2 residual and tangent, Jacobi-PCG): every |F| and
-    |x| of compRefOutD.dat down to the round-off floor."""
-    out = _run("compressible_q2", "2", "20", "1000", "0.3", "3.0", "5", "30", "1e-12", timeout=300)
-    lines = out.splitlines()
-    assert lines[0].strip() == "# Number of dofs 3239"
-    hist = []
-    for ln in lines:
+INPUTS = {
+    # reference/06-elastic/inputCompRef{D,N}.dat
+    "D": "E 1000.\nnu 0.3\nmaxIter 30\nloadSteps 5\npull 3.0\ntraction 1000.0\n"
+         "dispControlled 1\ntolerance 1.e-12\nisIncompressible 0\n",
+    "N": "E 1000.\nnu 0.3\nmaxIter 30\nloadSteps 5\npull 3.0\ntraction 300.0\n"
+         "dispControlled 0\ntolerance 1.e-12\nisIncompressible 0\n",
+}
+
+
+def _table(text):
+    rows = []
+    for ln in text.splitlines():
         if ln.startswith("#") or not ln.strip():
             continue
         v = ln.split()
-        hist.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]) if len(v) > 3 else None))
-    ref = []
-    with open(os.path.join(golden, "compRefOutD.dat")) as fh:
-        for ln in fh:
-            if ln.startswith("#") or not ln.strip():
-                continue
-            v = ln.split()
-            ref.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]) if len(v) > 3 else None))
+        rows.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]) if len(v) > 3 else None))
+    return rows
+
+
+@pytest.mark.parametrize("mode", ["D", "N"])
+def test_compressible_q2_app_golden(golden, read_vtk, tmp_path, mode):
+    """reference/06-elastic/compressible.cpp run as its test harness runs it
+    (compressibleTest.mk: quad.020.smf with inputCompRef{D,N}.dat) on the
+    drop-in C++ layer with a degree-2 field (generateField, constrainBoundary
+    at the Q2 support points, the linear traction profile on the Q2 faces of
+    x = 1, device assembly of the Q2 residual and tangent, Jacobi-PCG): every
+    |F| and |x| of compRefOut{D,N}.dat down to the round-off floor, and the
+    last VTK file (displacement at the nodes, Cauchy stress at the centroids,
+    evaluated on the device) against quad.020.{D,N}ref.vtk."""
+    import shutil
+    shutil.copy(os.path.join(golden, "square_20.smf"), tmp_path / "quad.020.smf")
+    (tmp_path / "input.dat").write_text(INPUTS[mode])
+    out = _run("compressible_q2", tmp_path / "quad.020.smf", tmp_path / "input.dat", timeout=300)
+    with open(os.path.join(golden, "compRefOut%s.dat" % mode)) as fh:
+        head = fh.readline()
+        ref = _table(fh.read())
+    assert out.splitlines()[0].strip() == head.strip()
+    hist = _table(out)
     assert [(s, i) for s, i, _, _ in hist] == [(s, i) for s, i, _, _ in ref]
     for (s, i, f, x), (_, _, fr, xr) in zip(hist, ref):
         if fr >= 1e-7:
@@ -149,6 +187,18 @@ def test_compressible_q2_app_golden(golden):
             assert f < 1e-6
         if x is not None and xr > 1e-9:
             assert abs(x - xr) <= 1e-4 * xr, (s, i, x, xr)
+    for step in range(6):
+        assert (tmp_path / ("quad.020.%04d.vtk" % step)).exists()
+    got = read_vtk(str(tmp_path / "quad.020.0005.vtk"))
+    want = read_vtk(os.path.join(golden, "quad.020.%sref.vtk" % mode))
+    np.testing.assert_array_equal(got["points"], want["points"])
+    assert got["cells"] == want["cells"] and got["cell_types"] == want["cell_types"]
+    for sec, name in (("point_data", "disp"), ("cell_data", "sigma")):
+        g, w = got[sec][name], want[sec][name]
+        assert g.shape == w.shape
+        tol = 1e-5 * np.maximum(np.abs(w), np.abs(w).max(axis=1, keepdims=True) * 1e-3) + 1e-12
+        bad = np.argwhere(np.abs(g - w) > tol)
+        assert bad.size == 0, (name, [(tuple(b), g[tuple(b)], w[tuple(b)]) for b in bad[:5]])
 
 
 @pytest.mark.parametrize("order", [1, 2])

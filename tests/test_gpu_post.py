@@ -54,3 +54,77 @@ def test_linear_elastic_l2_on_device(golden, dim, mesh, N):
         ctx.distribute(abi.DIST_SET)
         err = ctx.error_norm(0, abi.REF_KELVIN, [m["lam"], m["mu"]] + src + dr)
     assert err == pytest.approx(ref[N], rel=3e-6), (N, err, ref[N])
+
+
+def _cell_ctx(ctx, prob):
+    ctx.set_mesh(prob.xyz, prob.conn)
+    if prob.deg == 2:
+        ctx.set_field(2, prob.n_fnodes, prob.fconn)
+    ctx.set_dofs(prob.dpn, prob.dof_id, prob.n_active)
+    ctx.set_material(prob.kernel, prob.nq, prob.coef)
+
+
+def _smooth_field(prob, amp, seed):
+    """a non-affine displacement with |grad u| = O(amp): every element keeps
+    det F > 0"""
+    X = prob.fxyz if prob.deg == 2 else prob.xyz
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(1.0, 3.0, size=(prob.dpn, prob.dim))
+    ph = rng.uniform(0.0, 2 * np.pi, size=prob.dpn)
+    u = np.stack([np.sin(X @ k[c] + ph[c]) for c in range(prob.dpn)], axis=1)
+    return (amp * u).ravel()
+
+
+@pytest.mark.parametrize("dim,deg,kernel", [(2, 1, P.STVENANT), (2, 2, P.NEOHOOKE),
+                                            (3, 1, P.NEOHOOKE), (3, 2, P.STVENANT),
+                                            (3, 2, P.NEOHOOKE)])
+def test_cell_eval_matches_oracle(oracle, dim, deg, kernel):
+    """ins_cell_eval (value, gradient, PK2, Cauchy) against the oracle on a
+    distorted mesh, a non-affine field, at the centroid and at an off-centre
+    point; the element count is not a multiple of the 256-element block."""
+    n = {2: 23, 3: 7}[dim]
+    prob = P.distorted_block(dim, n, deg=deg, kernel=kernel, lam=3.0, mu=1.2)
+    u = _smooth_field(prob, 0.05, 17 + dim + deg)
+    xis = [None, np.linspace(0.15, 0.8, dim)]
+    qs = [oracle.CELL_VALUE, oracle.CELL_GRADIENT, oracle.CELL_PK2, oracle.CELL_CAUCHY]
+    with abi.Context(0) as ctx:
+        _cell_ctx(ctx, prob)
+        got = {(q, i): ctx.cell_eval(q, xi, u) for q in qs for i, xi in enumerate(xis)}
+        ctx.set_dof_values(u)
+        cur = ctx.cell_eval(abi.CELL_CAUCHY)
+    for (q, i), g in got.items():
+        ref = oracle.cell_eval(prob, u, q, xis[i])
+        scale = np.abs(ref).max()
+        np.testing.assert_allclose(g, ref, rtol=0, atol=1e-12 * scale, err_msg=str((q, i)))
+    np.testing.assert_array_equal(cur, got[(oracle.CELL_CAUCHY, 0)])
+
+
+def test_cell_stress_homogeneous_stretch():
+    """x -> diag(1.3, 1, 1) x on a distorted HEX27 mesh: every element holds
+    the closed-form neo-Hookean uniaxial Cauchy stress."""
+    lam, mu, s = 2.0, 0.8, 1.3
+    prob = P.distorted_block(3, 6, deg=2, kernel=P.NEOHOOKE, lam=lam, mu=mu)
+    u = (prob.fxyz * np.array([s - 1.0, 0.0, 0.0])).ravel()
+    with abi.Context(0) as ctx:
+        _cell_ctx(ctx, prob)
+        sig = ctx.cell_eval(abi.CELL_CAUCHY, u=u).reshape(-1, 3, 3)
+    L = np.log(s)
+    ref = np.diag([(lam * L - mu) / s + mu * s, lam * L / s, lam * L / s])
+    np.testing.assert_allclose(sig, np.broadcast_to(ref, sig.shape), rtol=0, atol=1e-12)
+
+
+def test_cell_eval_errors():
+    prob = P.distorted_block(2, 4, kernel=P.LAPLACE)
+    with abi.Context(0) as ctx:
+        _cell_ctx(ctx, prob)
+        u = np.zeros(prob.n_nodes)
+        with pytest.raises(abi.InsError) as ei:
+            ctx.cell_eval(abi.CELL_CAUCHY, u=u)
+        assert ei.value.status == abi.ERRORS_INV["STATE"]
+        with pytest.raises(abi.InsError) as ei:
+            ctx.cell_eval(abi.CELL_VALUE, xi=[0.5, 1.5], u=u)
+        assert ei.value.status == abi.ERRORS_INV["ARG"]
+        with pytest.raises(abi.InsError) as ei:
+            ctx.cell_eval(7, u=u)
+        assert ei.value.status == abi.ERRORS_INV["ARG"]
+        assert ctx.cell_eval(abi.CELL_GRADIENT, u=u).shape == (16, 2)

@@ -167,7 +167,7 @@ def test_q2_elastic_element_rigid_modes_and_fd(oracle, kernel):
         assert np.allclose(fd, Ku[:, j], atol=1e-6 * abs(Ku).max())
 
 
-def _newton_oracle(oracle, disp_controlled):
+def _newton_oracle(oracle, disp_controlled, want_state=False):
     """reference/06-elastic/compressible.cpp:245-335 driven by the oracle:
     5 load steps, ramp (step+1)/step of the prescribed u_x or of the
     traction, full Newton with incremental lifting, direct solves,
@@ -199,6 +199,8 @@ def _newton_oracle(oracle, disp_controlled):
             hist.append((step, it, c1, c2))
             if c2 < tol:
                 break
+    if want_state:
+        return n, hist, prob, u
     return n, hist
 
 
@@ -273,3 +275,43 @@ def test_error_norm_pinned(oracle, golden):
     b = oracle.error_norm(pe, uh, 0, oracle.REF_KELVIN,
                           [m["lam"], m["mu"], m["src"][0], m["src"][1], 0.0, m["dir"][0], m["dir"][1], 0.0])
     assert abs(a - b) <= 1e-13 * a
+
+
+def vtk_compare(got, ref, rel=1e-5):
+    """values printed with 6 significant digits: each within a relative 1e-5
+    of the printed one, or within 1e-5 of the largest entry of its row"""
+    got, ref = np.asarray(got, float), np.asarray(ref, float)
+    assert got.shape == ref.shape
+    tol = rel * np.maximum(np.abs(ref), np.abs(ref).max(axis=1, keepdims=True) * 1e-3) + 1e-12
+    bad = np.argwhere(np.abs(got - ref) > tol)
+    assert bad.size == 0, [(tuple(b), got[tuple(b)], ref[tuple(b)]) for b in bad[:5]]
+
+
+def vertex_values(prob, u):
+    """field values at the geometry nodes (base::post::evaluateFieldAtNodes:
+    the degree-2 field at a vertex is the DoF value of that vertex)"""
+    dim, nv = prob.dim, 2 ** prob.dim
+    out = np.zeros((prob.n_nodes, 3))
+    uf = u.reshape(-1, dim)
+    out[prob.conn[:, :nv].ravel(), :dim] = uf[prob.fconn[:, :nv].ravel()]
+    return out
+
+
+@pytest.mark.parametrize("mode", ["D", "N"])
+def test_q2_vtk_golden(oracle, golden, read_vtk, mode):
+    """reference/06-elastic/quad.020.{D,N}ref.vtk (the last of the 5 load
+    steps of compressible.cpp, ref06::writeVTKFile :60-89): grid, the
+    displacement at the nodes and the Cauchy stress at the element centroid
+    (solid::cauchy) of the oracle's Newton solution."""
+    ref = read_vtk(os.path.join(golden, "quad.020.%sref.vtk" % mode))
+    _, _, prob, u = _newton_oracle(oracle, mode == "D", want_state=True)
+    np.testing.assert_allclose(ref["points"][:, :2], prob.xyz, atol=1e-12)
+    assert ref["cells"] == prob.conn.tolist()
+    assert ref["cell_types"] == [9] * prob.n_elem
+    vtk_compare(vertex_values(prob, u), ref["point_data"]["disp"])
+    sig = oracle.cell_eval(prob, u, oracle.CELL_CAUCHY)
+    vtk_compare(sig, ref["cell_data"]["sigma"])
+    # all 3600 stress entries print to the same 6 significant digits
+    with open(os.path.join(golden, "quad.020.%sref.vtk" % mode)) as fh:
+        printed = fh.read().split("TENSORS sigma float")[1].split()
+    assert ["%g" % v for v in sig.ravel()] == printed
