@@ -5,6 +5,8 @@
 #pragma once
 
 #include <cmath>
+#include <algorithm>
+#include <array>
 #include <fstream>
 #include <map>
 #include <ostream>
@@ -47,6 +49,38 @@ inline Mesh readSMF(const std::string& path, int dim)
             m.conn.push_back(v);
         }
     }
+    return m;
+}
+
+// Structured grid file (base/io/sgf/Reader.hpp:155-310): '#' comment lines,
+// the element counts per direction (three numbers, trailing zeros for fewer
+// dimensions), the node coordinates in lexicographic order (first index
+// fastest), one node per line.  The grid cells in lexicographic order become
+// the Q1 elements, vertices in hierarchic order; sizes receives the counts.
+inline Mesh readSGF(const std::string& path, std::array<int, 3>* sizes = nullptr)
+{
+    std::ifstream in(path);
+    if (!in) throw Error(INS_ERR_ARG, "cannot open " + path);
+    std::string line;
+    int n[3] = {0, 0, 0};
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        ss >> n[0] >> n[1] >> n[2];
+        break;
+    }
+    const int dim = n[2] > 0 ? 3 : 2;
+    if (n[0] < 1 || n[1] < 1) throw Error(INS_ERR_ARG, "bad grid sizes in " + path);
+    Mesh m = unitCube(dim, n[0], n[1], dim == 3 ? n[2] : 1);
+    std::size_t k = 0;
+    while (k < m.numNodes() && std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        for (int d = 0; d < dim; d++) ss >> m.xyz[k * dim + d];
+        k++;
+    }
+    if (k != m.numNodes()) throw Error(INS_ERR_ARG, "too few nodes in " + path);
+    if (sizes) *sizes = {n[0], n[1], dim == 3 ? n[2] : 0};
     return m;
 }
 
@@ -361,6 +395,53 @@ public:
         }
         vtk_ << "CELL_TYPES " << E << "\n";
         for (std::size_t e = 0; e < E; e++) vtk_ << (dim == 2 ? 9 : 12) << "\n";
+    }
+
+    // The grid as a structured grid sampled `resolution` times per cell and
+    // direction (LegacyWriter::writeStructuredGrid with
+    // base::mesh::sampleGridGeometry, base/mesh/sampleStructured.hpp): the
+    // sample points in lexicographic order, each the Q1 geometry of the
+    // cell it falls in (the last cell for the points on the upper faces).
+    // The mesh must be a grid of sizes[] cells in lexicographic order.
+    void writeStructuredGrid(const insilico::Mesh& m, const std::array<int, 3>& sizes,
+                             int resolution = 1)
+    {
+        const int dim = m.dim, npe = m.npe(), r = resolution < 1 ? 1 : resolution;
+        int cnt[3] = {1, 1, 1};
+        vtk_ << "DATASET STRUCTURED_GRID\nDIMENSIONS ";
+        for (int d = 0; d < dim; d++) {
+            cnt[d] = r * sizes[d] + 1;
+            vtk_ << cnt[d] << " ";
+        }
+        for (int d = dim; d < 3; d++) vtk_ << "1 ";
+        vtk_ << "\n";
+        vtk_ << "POINTS " << (std::size_t)cnt[0] * cnt[1] * cnt[2] << " float\n";
+        static const int V[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                    {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        for (int k = 0; k < cnt[2]; k++)
+            for (int j = 0; j < cnt[1]; j++)
+                for (int i = 0; i < cnt[0]; i++) {
+                    const int g[3] = {i, j, k};
+                    int cell[3] = {0, 0, 0};
+                    double xi[3] = {0., 0., 0.};
+                    for (int d = 0; d < dim; d++) {
+                        cell[d] = std::min(g[d] / r, sizes[d] - 1);
+                        xi[d] = (double)(g[d] - r * cell[d]) * (1. / r);
+                    }
+                    const std::size_t e = (std::size_t)cell[0] +
+                        (std::size_t)sizes[0] * ((std::size_t)cell[1] +
+                                                 (std::size_t)sizes[1] * (std::size_t)cell[2]);
+                    double x[3] = {0., 0., 0.};
+                    for (int v = 0; v < npe; v++) {
+                        double w = 1.;
+                        for (int d = 0; d < dim; d++) w *= V[v][d] ? xi[d] : 1. - xi[d];
+                        const int32_t node = m.conn[e * npe + v];
+                        for (int d = 0; d < dim; d++) x[d] += w * m.xyz[(std::size_t)node * dim + d];
+                    }
+                    for (int d = 0; d < dim; d++) vtk_ << x[d] << " ";
+                    for (int d = dim; d < 3; d++) vtk_ << "0 ";
+                    vtk_ << "\n";
+                }
     }
 
     // values: one datum of ncomp values per point / cell

@@ -137,6 +137,24 @@ def read_smf(path, dim):
     return xyz, conn
 
 
+def read_sgf(path):
+    """Structured grid file (base/io/sgf/Reader.hpp:155-310): '#' comment
+    lines, the element counts per direction (three numbers, trailing zeros
+    for fewer dimensions), then the node coordinates in lexicographic order
+    (first index fastest), one node per line.  The elements are the grid
+    cells in lexicographic order; returned as the Q1 mesh of those cells
+    with their vertices in hierarchic order: (xyz, conn, sizes)."""
+    with open(path) as fh:
+        lines = [ln.split() for ln in fh if ln.strip() and not ln.startswith("#")]
+    sizes = [int(v) for v in lines[0][:3]]
+    dim = sum(1 for v in sizes if v > 0)
+    sizes = sizes[:dim]
+    nn = int(np.prod([v + 1 for v in sizes]))
+    xyz = np.array([[float(v) for v in ln[:dim]] for ln in lines[1:1 + nn]])
+    _, conn = unit_cube(dim, *sizes)
+    return xyz, conn, tuple(sizes)
+
+
 def boundary_faces(conn, dim):
     """(element, face) pairs on the boundary and their face node lists."""
     table = HEX_FACES if dim == 3 else QUAD_EDGES
