@@ -70,6 +70,7 @@ extern "C" {
 #define INS_BF_NONE     0
 #define INS_BF_CONST    1  /* params[c] = f_c                                */
 #define INS_BF_SINPROD  2  /* f = params[0] * prod_d sin(params[1+d] * x_d)  */
+#define INS_BF_COSPROD  3  /* f = params[0] * prod_d cos(params[1+d] * x_d)  */
 
 /* solve modes */
 #define INS_SOLVE_FP64   0  /* FP64 Jacobi-PCG on the exported CSR           */
@@ -210,6 +211,16 @@ int ins_set_traction(ins_ctx* ctx, int64_t n_faces, int nodes_per_face,
  * value * (x_1 - 1/2) in component 0): the traction vector is scaled by
  * (x[axis] - x0) at every surface point; axis = -1 = constant (default). */
 int ins_set_traction_profile(ins_ctx* ctx, int axis, double x0);
+/* Neumann data given as the co-normal derivative of a built-in function u
+ * (ref_kind INS_REF_SINPROD / INS_REF_COSPROD with its params, see the
+ * post-processing section): t_c = sum_d n_d du_c/dx_d at every surface
+ * point, as reference/05-mixedPoisson/GivenData.hpp neumannBC computes it,
+ * with n the unit normal of base::SurfaceNormal (base/geometry.hpp:314-345:
+ * 2-D (dy, -dx) of the face tangent, 3-D the cross product of the surface
+ * Jacobian's columns; outward for faces in the element face-table
+ * orientation).  Applies to the faces of ins_set_traction in place of its
+ * traction vector; ref_kind INS_REF_ZERO switches back to the traction. */
+int ins_set_neumann_flux(ins_ctx* ctx, int ref_kind, const double* ref_params);
 
 /* ---- assembly ------------------------------------------------------------- */
 /* stiffnessMatrixComputation + computeResidualForces + bodyForceComputation
@@ -262,6 +273,9 @@ int ins_distribute(ins_ctx* ctx, int op);
 #define INS_REF_KELVIN   2  /* elastostatic fundamental solution
                                (base/auxi/FundamentalSolution.hpp:160-200):
                                params = lambda, mu, source[3], direction[3]   */
+#define INS_REF_COSPROD  3  /* u = params[0] * prod_d cos(params[1+d] * x_d)
+                               (reference/05-mixedPoisson/
+                               ReferenceSolution.hpp)                          */
 /* base::post::errorComputation<order> (base/post/ErrorNorm.hpp:60-84,
  * 200-224): |u - u_h| in the L2 norm (order 0) or the H1 semi-norm (order
  * 1; not for INS_REF_KELVIN), integrated with the context's quadrature.

@@ -36,9 +36,10 @@ SYMBOLS = [
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
     "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
     "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile", "ins_cell_eval",
+    "ins_set_neumann_flux",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
-REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
+REF_ZERO, REF_SINPROD, REF_KELVIN, REF_COSPROD = 0, 1, 2, 3
 CELL_VALUE, CELL_GRADIENT, CELL_PK2, CELL_CAUCHY = 0, 1, 2, 3
 
 
@@ -86,6 +87,7 @@ def load():
         "ins_error_norm": [vp, i32, i32, vp, vp, C.POINTER(dbl)],
         "ins_bdf_init": [vp, i32, dbl, vp],
         "ins_cell_eval": [vp, i32, vp, vp, vp],
+        "ins_set_neumann_flux": [vp, i32, vp],
         "ins_set_traction_profile": [vp, i32, dbl],
         "ins_bdf_step": [vp, dbl, i32, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
         "ins_set_dofs": [vp, i32, vp, i64],
@@ -197,6 +199,14 @@ class Context:
         out = C.c_double()
         self._chk(self.L.ins_error_norm(self.h, order, kind, _p(prm), _p(uu), C.byref(out)))
         return out.value
+
+    def set_neumann_flux(self, kind, params=None):
+        """Neumann data grad u . n of a built-in function (REF_SINPROD /
+        REF_COSPROD) on the traction faces; REF_ZERO = the traction vector"""
+        prm = np.zeros(8)
+        if params is not None:
+            prm[:len(params)] = params
+        self._chk(self.L.ins_set_neumann_flux(self.h, kind, _p(prm)))
 
     def cell_eval(self, quantity, xi=None, u=None):
         """value, gradient, 2nd Piola-Kirchhoff or Cauchy stress at the local
@@ -368,6 +378,8 @@ class Context:
             self.set_traction(prob.faces, prob.traction)
         lin = prob.meta.get("traction_linear")
         self.set_traction_profile(*(lin if lin is not None else (-1, 0.0)))
+        flux = prob.meta.get("neumann_flux")
+        self.set_neumann_flux(*(flux if flux is not None else (REF_ZERO, None)))
         return self.build_pattern()
 
 

@@ -534,7 +534,8 @@ int ins_set_body_force(ins_ctx* h, int kind, const double* params)
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
     c->bdf_staged = false;
-    if (kind != INS_BF_NONE && kind != INS_BF_CONST && kind != INS_BF_SINPROD)
+    if (kind != INS_BF_NONE && kind != INS_BF_CONST && kind != INS_BF_SINPROD &&
+        kind != INS_BF_COSPROD)
         return fail(c, INS_ERR_ARG, "unknown body force kind");
     if (kind != INS_BF_NONE && !params) return fail(c, INS_ERR_ARG, "body force parameters");
     c->bf_kind = kind;
@@ -585,6 +586,20 @@ int ins_set_traction_profile(ins_ctx* h, int axis, double x0)
         return fail(c, INS_ERR_ARG, "ins_set_traction_profile: axis -1..dim-1");
     c->tr_axis = axis;
     c->tr_x0 = axis >= 0 ? x0 : 0.;
+    return INS_OK;
+}
+
+int ins_set_neumann_flux(ins_ctx* h, int ref_kind, const double* ref_params)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    c->bdf_staged = false;
+    if (ref_kind != INS_REF_ZERO && ref_kind != INS_REF_SINPROD && ref_kind != INS_REF_COSPROD)
+        return fail(c, INS_ERR_ARG, "ins_set_neumann_flux: INS_REF_ZERO, _SINPROD or _COSPROD");
+    if (ref_kind != INS_REF_ZERO && !ref_params)
+        return fail(c, INS_ERR_ARG, "ins_set_neumann_flux: params");
+    c->flux_kind = ref_kind;
+    for (int i = 0; i < 8; i++) c->flux_p[i] = ref_kind != INS_REF_ZERO ? ref_params[i] : 0.;
     return INS_OK;
 }
 

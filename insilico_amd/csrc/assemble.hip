@@ -61,6 +61,11 @@ __device__ inline T body_force(const Bf& bf, const T* x, int dim, int comp)
         for (int d = 0; d < dim; d++) f *= sin((T)bf.p[1 + d] * x[d]);
         return f;
     }
+    if (bf.kind == INS_BF_COSPROD) {
+        T f = (T)bf.p[0];
+        for (int d = 0; d < dim; d++) f *= cos((T)bf.p[1 + d] * x[d]);
+        return f;
+    }
     return (T)0;
 }
 
@@ -1205,74 +1210,10 @@ __global__ __launch_bounds__(BLK) void k_rhs(
     }
 }
 
-// Neumann traction on faces (base/asmb/NeumannForce.hpp:132-175 with the
-// surface metric of base/geometry.hpp:314-346): one face per lane.  A face
-// lists its FFN field nodes in its own hierarchic order, the first GNF of
-// them its vertices (gmap maps them to geometry nodes; null = the field
-// nodes are the geometry nodes).  The traction is constant, or with
-// axis >= 0 scaled by (x[axis] - x0) at the surface point.
-template <int DIM, int FFN>
-__global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restrict__ faces,
-                                              const int32_t* __restrict__ gmap,
-                                              const double* __restrict__ xyz,
-                                              const double* __restrict__ gdt,
-                                              const double* __restrict__ gpt,
-                                              const double* __restrict__ fpt,
-                                              const double* __restrict__ wt, int nqf,
-                                              double t0, double t1, double t2, int axis,
-                                              double x0, double* __restrict__ ff)
-{
-    constexpr int GNF = (DIM == 3) ? 4 : 2;
-    constexpr int LD = DIM - 1;
-    const double tr[3] = {t0, t1, t2};
-    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
-         f += (int64_t)gridDim.x * blockDim.x) {
-        double X[GNF][DIM];
-        for (int n = 0; n < GNF; n++) {
-            const int64_t fn = faces[f * FFN + n];
-            const int64_t node = gmap ? gmap[fn] : fn;
-            for (int d = 0; d < DIM; d++) X[n][d] = xyz[node * DIM + d];
-        }
-        double out[FFN * DIM];
-        for (int i = 0; i < FFN * DIM; i++) out[i] = 0.;
-        for (int q = 0; q < nqf; q++) {
-            const double* dphi = gdt + q * GNF * LD;
-            const double* gphi = gpt + q * GNF;
-            const double* phi = fpt + q * FFN;
-            const double w = wt[q];
-            double detG;
-            if constexpr (DIM == 3) {
-                double J[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-                for (int n = 0; n < 4; n++)
-                    for (int i = 0; i < 3; i++)
-                        for (int a = 0; a < 2; a++) J[i][a] += X[n][i] * dphi[n * 2 + a];
-                const double cx = J[1][0] * J[2][1] - J[2][0] * J[1][1];
-                const double cy = J[2][0] * J[0][1] - J[0][0] * J[2][1];
-                const double cz = J[0][0] * J[1][1] - J[1][0] * J[0][1];
-                detG = sqrt(cx * cx + cy * cy + cz * cz);
-            } else {
-                const double tx = X[1][0] - X[0][0], ty = X[1][1] - X[0][1];
-                detG = sqrt(tx * tx + ty * ty);
-            }
-            double t[3] = {tr[0], tr[1], tr[2]};
-            if (axis >= 0) {
-                double xs = 0.;
-                for (int n = 0; n < GNF; n++) xs += X[n][axis] * gphi[n];
-                for (int c = 0; c < DIM; c++) t[c] = tr[c] * (xs - x0);
-            }
-            for (int n = 0; n < FFN; n++)
-                for (int c = 0; c < DIM; c++) out[n * DIM + c] += t[c] * phi[n] * w * detG;
-        }
-        for (int i = 0; i < FFN * DIM; i++) ff[f * FFN * DIM + i] = out[i];
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Error norms (base/post/ErrorNorm.hpp:60-84,200-224): one element per lane;
-// per quadrature point x(xi) and detJ from the Q1 geometry, u_h or grad u_h
-// from the field functions, errorSquared += (ref - approx).(ref - approx)
-// detJ w.  Lanes sum their elements in order, blocks reduce in a fixed tree,
-// the host adds the block partials in block order.
+// Built-in functions of the position: reference solutions of the error
+// norms and the Neumann data given as their co-normal derivative.
+// v[c] = u_c(x), g[d*s+c] = du_c/dx_d.
 struct RefFn {
     int kind;
     double p[8];
@@ -1294,6 +1235,19 @@ __device__ inline void ref_value(const RefFn& r, const double* x, int s, double*
 #pragma unroll
             for (int e = 0; e < DIM; e++)
                 if (e != d) gd *= sin(r.p[1 + e] * x[e]);
+            for (int c = 0; c < s; c++) g[d * s + c] = gd;
+        }
+    } else if (r.kind == INS_REF_COSPROD) {
+        double val = r.p[0];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) val *= cos(r.p[1 + d] * x[d]);
+        for (int c = 0; c < s; c++) v[c] = val;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            double gd = -r.p[0] * r.p[1 + d] * sin(r.p[1 + d] * x[d]);
+#pragma unroll
+            for (int e = 0; e < DIM; e++)
+                if (e != d) gd *= cos(r.p[1 + e] * x[e]);
             for (int c = 0; c < s; c++) g[d * s + c] = gd;
         }
     } else if (r.kind == INS_REF_KELVIN) {
@@ -1325,6 +1279,91 @@ __device__ inline void ref_value(const RefFn& r, const double* x, int s, double*
     }
 }
 
+
+// Neumann data on faces (base/asmb/NeumannForce.hpp:132-175 with the
+// surface normal and metric of base/geometry.hpp:314-346): one face per
+// lane, S field components.  A face lists its FFN field nodes in its own
+// hierarchic order, the first GNF of them its vertices (gmap maps them to
+// geometry nodes; null = the field nodes are the geometry nodes).  The data
+// t_c is the traction (constant, or with axis >= 0 scaled by
+// (x[axis] - x0) at the surface point), or with flux.kind != 0 the
+// co-normal derivative sum_d n_d du_c/dx_d of a built-in function.
+template <int DIM, int FFN, int S>
+__global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restrict__ faces,
+                                              const int32_t* __restrict__ gmap,
+                                              const double* __restrict__ xyz,
+                                              const double* __restrict__ gdt,
+                                              const double* __restrict__ gpt,
+                                              const double* __restrict__ fpt,
+                                              const double* __restrict__ wt, int nqf,
+                                              double t0, double t1, double t2, int axis,
+                                              double x0, RefFn flux, double* __restrict__ ff)
+{
+    constexpr int GNF = (DIM == 3) ? 4 : 2;
+    constexpr int LD = DIM - 1;
+    const double tr[3] = {t0, t1, t2};
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        double X[GNF][DIM];
+        for (int n = 0; n < GNF; n++) {
+            const int64_t fn = faces[f * FFN + n];
+            const int64_t node = gmap ? gmap[fn] : fn;
+            for (int d = 0; d < DIM; d++) X[n][d] = xyz[node * DIM + d];
+        }
+        double out[FFN * S];
+        for (int i = 0; i < FFN * S; i++) out[i] = 0.;
+        for (int q = 0; q < nqf; q++) {
+            const double* dphi = gdt + q * GNF * LD;
+            const double* gphi = gpt + q * GNF;
+            const double* phi = fpt + q * FFN;
+            const double w = wt[q];
+            double detG, nrm[3];
+            if constexpr (DIM == 3) {
+                double J[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+                for (int n = 0; n < 4; n++)
+                    for (int i = 0; i < 3; i++)
+                        for (int a = 0; a < 2; a++) J[i][a] += X[n][i] * dphi[n * 2 + a];
+                nrm[0] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+                nrm[1] = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                nrm[2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+                detG = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            } else {
+                const double tx = X[1][0] - X[0][0], ty = X[1][1] - X[0][1];
+                nrm[0] = ty;
+                nrm[1] = -tx;
+                nrm[2] = 0.;
+                detG = sqrt(tx * tx + ty * ty);
+            }
+            double t[3] = {tr[0], tr[1], tr[2]};
+            if (flux.kind != 0) {
+                double xs[3] = {0., 0., 0.};
+                for (int n = 0; n < GNF; n++)
+                    for (int d = 0; d < DIM; d++) xs[d] += X[n][d] * gphi[n];
+                double rv[3], rg[9];
+                ref_value<DIM>(flux, xs, S, rv, rg);
+                for (int c = 0; c < S; c++) {
+                    double v = 0.;
+                    for (int d = 0; d < DIM; d++) v += rg[d * S + c] * (nrm[d] / detG);
+                    t[c] = v;
+                }
+            } else if (axis >= 0) {
+                double xs = 0.;
+                for (int n = 0; n < GNF; n++) xs += X[n][axis] * gphi[n];
+                for (int c = 0; c < S; c++) t[c] = tr[c] * (xs - x0);
+            }
+            for (int n = 0; n < FFN; n++)
+                for (int c = 0; c < S; c++) out[n * S + c] += t[c] * phi[n] * w * detG;
+        }
+        for (int i = 0; i < FFN * S; i++) ff[f * FFN * S + i] = out[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Error norms (base/post/ErrorNorm.hpp:60-84,200-224): one element per lane;
+// per quadrature point x(xi) and detJ from the Q1 geometry, u_h or grad u_h
+// from the field functions, errorSquared += (ref - approx).(ref - approx)
+// detJ w.  Lanes sum their elements in order, blocks reduce in a fixed tree,
+// the host adds the block partials in block order.
 template <int DIM, int NF, int S>
 __global__ __launch_bounds__(256) void k_error_norm(
     int64_t E, const int32_t* __restrict__ gconn, const int32_t* __restrict__ fconn,
@@ -2002,14 +2041,26 @@ int assemble(Ctx* c, int precision, int flags)
         const double* T = c->traction;
         const int ax = c->tr_axis;
         const double x0 = c->tr_x0;
-        if (c->dim == 3 && c->nfn == 9)
-            k_face<3, 9><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
-        else if (c->dim == 3)
-            k_face<3, 4><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
-        else if (c->nfn == 3)
-            k_face<2, 3><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
-        else
-            k_face<2, 2><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, fp, w, nqf, T[0], T[1], T[2], ax, x0, ff);
+        RefFn flux;
+        flux.kind = c->flux_kind;
+        for (int i = 0; i < 8; i++) flux.p[i] = c->flux_p[i];
+        // ff holds nfn * s values per face (the gather reads that stride)
+        const int key = c->dim * 100 + c->nfn * 10 + s;
+#define INS_FACE(D, N, S_)                                                                      \
+    k_face<D, N, S_><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, \
+                                               fp, w, nqf, T[0], T[1], T[2], ax, x0, flux, ff)
+        switch (key) {
+        case 393: INS_FACE(3, 9, 3); break;
+        case 391: INS_FACE(3, 9, 1); break;
+        case 343: INS_FACE(3, 4, 3); break;
+        case 341: INS_FACE(3, 4, 1); break;
+        case 232: INS_FACE(2, 3, 2); break;
+        case 231: INS_FACE(2, 3, 1); break;
+        case 222: INS_FACE(2, 2, 2); break;
+        case 221: INS_FACE(2, 2, 1); break;
+        default: return fail(c, INS_ERR_UNSUPPORTED, "Neumann faces: dofs per node must be 1 or dim");
+        }
+#undef INS_FACE
         INS_HIP_CHECK(c, hipGetLastError());
     }
 
@@ -2095,7 +2146,8 @@ int error_norm(Ctx* c, int order, int kind, const double* prm, const double* hu,
     if (c->kernel == 0) return fail(c, INS_ERR_STATE, "ins_error_norm before ins_set_material");
     if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_error_norm before ins_set_dofs");
     if (order != 0 && order != 1) return fail(c, INS_ERR_ARG, "order must be 0 (L2) or 1 (H1)");
-    if (kind != INS_REF_ZERO && kind != INS_REF_SINPROD && kind != INS_REF_KELVIN)
+    if (kind != INS_REF_ZERO && kind != INS_REF_SINPROD && kind != INS_REF_KELVIN &&
+        kind != INS_REF_COSPROD)
         return fail(c, INS_ERR_ARG, "unknown reference solution");
     if (kind == INS_REF_KELVIN && (order != 0 || c->dpn != c->dim))
         return fail(c, INS_ERR_UNSUPPORTED, "the fundamental solution is a vector L2 reference");

@@ -160,6 +160,10 @@ struct Ctx {
     double traction[3] = {0, 0, 0};
     int tr_axis = -1;       // >= 0: traction scaled by (x[tr_axis] - tr_x0)
     double tr_x0 = 0.;
+    // Neumann data as the co-normal derivative of a built-in function
+    // (INS_REF_* kind, 0 = the traction vector above)
+    int flux_kind = 0;
+    double flux_p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DArr<int32_t> faces, fadj_ptr, fadj;
     // time-step terms: density of the mass matrix (0 = none), its factor in
     // the matrix (a0 / (b0 dt)) and the history combination of the RHS

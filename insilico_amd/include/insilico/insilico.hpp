@@ -597,6 +597,14 @@ public:
         trX0_ = x0;
         flags_ |= INS_ASM_RHS;
     }
+    // Neumann data as the co-normal derivative of a built-in function on the
+    // faces of requestTraction
+    void requestFlux(int refKind, const std::array<double, 8>& p)
+    {
+        fluxKind_ = refKind;
+        fluxP_ = p;
+        flags_ |= INS_ASM_RHS;
+    }
     void setPrecision(int p) { precision_ = p; }
     // forget the recorded assembly requests and the solution (a new Newton
     // iteration re-assembles into the same device context, pattern kept)
@@ -609,6 +617,7 @@ public:
         traction_ = {0., 0., 0.};
         trAxis_ = -1;
         trX0_ = 0.;
+        fluxKind_ = INS_REF_ZERO;
         solved_ = false;
         assembled_ = false;
         hostRhs_.clear();
@@ -737,6 +746,8 @@ private:
         insilico::check(ins_set_traction(ctx_, (int64_t)faces_.size(), nfn, faces_.empty() ? nullptr : flat.data(),
                                          traction_.data()), ctx_);
         insilico::check(ins_set_traction_profile(ctx_, faces_.empty() ? -1 : trAxis_, trX0_), ctx_);
+        insilico::check(ins_set_neumann_flux(ctx_, faces_.empty() ? INS_REF_ZERO : fluxKind_,
+                                             fluxP_.data()), ctx_);
         insilico::check(ins_assemble(ctx_, precision_, flags_), ctx_);
     }
 
@@ -755,6 +766,8 @@ private:
     std::array<double, 3> traction_{0., 0., 0.};
     int trAxis_ = -1;
     double trX0_ = 0.;
+    int fluxKind_ = INS_REF_ZERO;
+    std::array<double, 8> fluxP_{0., 0., 0., 0., 0., 0., 0., 0.};
     std::vector<double> x_;
     double lastRelres_ = 0.;
     bool converged_ = false;
@@ -848,6 +861,20 @@ void bodyForceComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& f
     solver.requestBodyForce(INS_BF_SINPROD, {f.amplitude, f.k[0], f.k[1], f.k[2]});
 }
 
+// A * prod_d cos(k_d x_d): -Lap u of u = prod_d cos(k_d x_d) with
+// A = sum_d k_d^2 (reference/05-mixedPoisson/GivenData.hpp forceFun)
+struct CosineProductForce {
+    double amplitude;
+    std::array<double, 3> k;
+};
+template <typename FTB, unsigned DEG, Shape SH, typename FB>
+void bodyForceComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                          const CosineProductForce& f)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestBodyForce(INS_BF_COSPROD, {f.amplitude, f.k[0], f.k[1], f.k[2]});
+}
+
 // constant traction on the given boundary faces
 template <typename FB>
 void neumannForceComputation(solver::Hip& solver, FB& fieldBinder,
@@ -870,6 +897,20 @@ void neumannForceComputation(solver::Hip& solver, FB& fieldBinder,
 {
     solver.bind(fieldBinder.mesh, fieldBinder.field);
     solver.requestTraction(std::move(faces), lt.t, lt.axis, lt.x0);
+}
+// Neumann data grad u . n of u = A * prod_d cos(k_d x_d), n the outward unit
+// normal of the face (reference/05-mixedPoisson/GivenData.hpp neumannBC)
+struct CosineProductFlux {
+    double amplitude;
+    std::array<double, 3> k;
+};
+template <typename FB>
+void neumannForceComputation(solver::Hip& solver, FB& fieldBinder,
+                             std::vector<std::vector<int32_t>> faces, const CosineProductFlux& fl)
+{
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestTraction(std::move(faces), {0., 0., 0.});
+    solver.requestFlux(INS_REF_COSPROD, {fl.amplitude, fl.k[0], fl.k[1], fl.k[2], 0., 0., 0., 0.});
 }
 
 }  // namespace asmb

@@ -163,6 +163,13 @@ struct SineProductSolution {
     double amp;
     std::array<double, 3> k;
 };
+// u = 0: the norm of the field itself
+struct ZeroSolution {};
+// u = amp * prod_d cos(k_d x_d) (reference/05-mixedPoisson/ReferenceSolution.hpp)
+struct CosineProductSolution {
+    double amp;
+    std::array<double, 3> k;
+};
 // elastostatic fundamental solution (base/auxi/FundamentalSolution.hpp)
 struct FundamentalSolutionElastoStatic {
     double lambda, mu;
@@ -172,6 +179,13 @@ namespace detail_ {
 inline void refParams(const SineProductSolution& r, int& kind, double* p)
 {
     kind = INS_REF_SINPROD;
+    p[0] = r.amp;
+    for (int d = 0; d < 3; d++) p[1 + d] = r.k[d];
+}
+inline void refParams(const ZeroSolution&, int& kind, double*) { kind = INS_REF_ZERO; }
+inline void refParams(const CosineProductSolution& r, int& kind, double* p)
+{
+    kind = INS_REF_COSPROD;
     p[0] = r.amp;
     for (int d = 0; d < 3; d++) p[1 + d] = r.k[d];
 }
@@ -221,6 +235,30 @@ double errorComputation(const Quadrature<DEG, SHAPE>&, const insilico::Mesh& m,
     return out;
 }
 }  // namespace post
+
+namespace kernel {
+// base::kernel::Measure (base/kernel/Measure.hpp): the integrand 1
+struct Measure {};
+}  // namespace kernel
+
+namespace asmb {
+// base::asmb::simplyIntegrate with base::kernel::Measure: adds the mesh
+// volume sum_e sum_q detJ w, integrated on the device as the squared L2
+// norm of the constant field 1 (one component)
+template <typename FTB, unsigned DEG, Shape SHAPE, typename FB>
+void simplyIntegrate(const Quadrature<DEG, SHAPE>& q, double& result, FB& fieldBinder,
+                     const kernel::Measure&)
+{
+    const insilico::Mesh& m = fieldBinder.mesh;
+    const std::size_t nn = fieldBinder.field.numNodes();
+    insilico::Field one = fieldBinder.field;
+    one.size = 1;
+    one.status.assign(nn, insilico::ACTIVE);
+    one.value.assign(nn, 1.0);
+    const double n = post::errorComputation<0>(q, m, one, insilico::ZeroSolution{});
+    result += n * n;
+}
+}  // namespace asmb
 }  // namespace base
 
 // ---------------------------------------------------------------------------

@@ -28,7 +28,8 @@ import math
 import numpy as np
 
 LAPLACE, STVENANT, NEOHOOKE = 1, 2, 3
-BF_NONE, BF_CONST, BF_SINPROD = 0, 1, 2
+BF_NONE, BF_CONST, BF_SINPROD, BF_COSPROD = 0, 1, 2, 3
+REF_COSPROD = 3
 
 # lexicographic corner -> hierarchic position (HierarchicOrder.hpp, K = 1)
 _H2 = np.array([0, 1, 3, 2])
@@ -466,6 +467,47 @@ def pulled_sheet_q2(n=20, disp_controlled=True, E=1000.0, nu=0.3, dim=2):
         p.traction = (0.0, 0.0, 0.0)
         p.meta["traction_linear"] = (1, 0.5)
     p.meta.update(right_x=right, E=E, lam=lam, mu=mu)
+    return p
+
+
+def mixed_poisson_q2(xyz, conn, alpha=(3.0, 5.0, 4.0), deg=2):
+    """reference/05-mixedPoisson/mixedPoisson.cpp:85-204 with GivenData.hpp
+    and ReferenceSolution.hpp: -Lap u = f on the mesh with a degree-2 field
+    (deg=1: the iso-parametric Q1 field), u = prod_d cos(alpha_d x_d)
+    prescribed at the boundary support points with some x_d = 0, the
+    co-normal derivative grad u . n as Neumann data on every boundary face
+    (its contributions to constrained rows drop out), f = -Lap u =
+    (sum_d alpha_d^2) prod_d cos(alpha_d x_d), kappa = 1, Gauss rule of
+    degree 3 (2 points per direction, volume and surface)."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    conn = np.ascontiguousarray(conn, np.int32)
+    dim = xyz.shape[1]
+    al = np.array(alpha[:dim], np.float64)
+    el, fa, bf = boundary_faces(conn, dim)
+    if deg == 2:
+        fconn, fxyz = q2_field(xyz, conn, dim)
+        fnodes = q2_face_nodes(fconn, el, fa, dim)
+    else:
+        fconn, fxyz, fnodes = None, xyz, bf
+    bn = np.unique(fnodes)
+    con = np.zeros((fxyz.shape[0], 1), bool)
+    dir_nodes = bn[np.any(np.abs(fxyz[bn]) < 1e-5, axis=1)]
+    con[dir_nodes, 0] = True
+    dof, na = number_dofs(con)
+    g = np.zeros(fxyz.shape[0])
+    val = np.ones(dir_nodes.shape[0])
+    for d in range(dim):
+        val = val * np.cos(al[d] * fxyz[dir_nodes, d])
+    g[dir_nodes] = val
+    amp = float(np.sum(al * al))
+    bfp = (amp,) + tuple(al) + (0.0,) * (3 - dim)
+    p = Problem(dim, xyz, conn, 1, dof, na, g, LAPLACE, np.ones((conn.shape[0], 1)), 2,
+                BF_COSPROD, bfp, faces=bf.astype(np.int32), traction=(0.0, 0.0, 0.0),
+                name="mixedPoissonQ%d" % deg)
+    if deg == 2:
+        p.deg, p.fconn, p.fxyz, p.ffaces = 2, fconn, fxyz, fnodes
+    p.meta["neumann_flux"] = (REF_COSPROD, [1.0] + list(al) + [0.0] * (7 - dim))
+    p.meta["alpha"] = al
     return p
 
 

@@ -372,8 +372,18 @@ static double body_force(const orc_problem* p, const double* x, int comp)
         for (int d = 0; d < p->dim; d++) f *= sin(p->bf[1 + d] * x[d]);
         return f;
     }
+    if (p->bf_kind == ORC_BF_COSPROD) {
+        /* f = -Laplace(u) of reference/05-mixedPoisson/GivenData.hpp
+         * forceFun, amp = sum_d alpha_d^2 */
+        double f = p->bf[0];
+        for (int d = 0; d < p->dim; d++) f *= cos(p->bf[1 + d] * x[d]);
+        return f;
+    }
     return 0.;
 }
+
+static void ref_solution(int dim, int s, int kind, const double* prm, const double* x,
+                         double* val, double* grad);
 
 /* Element matrix Ke (nee x nee, row-major) and element force vector Fe (nee):
  *  - Laplace: heat/Laplace.hpp:113-126 -> base/kernel/Laplace.hpp:100-151
@@ -713,7 +723,39 @@ static void face_force(const orc_problem* p, int64_t f, double* out)
             detG = sqrt(tx * tx + ty * ty);
         }
         double t[3] = {p->traction[0], p->traction[1], p->traction[2]};
-        if (p->tr_kind == 1) {
+        if (p->flux_kind != 0) {
+            /* surface point and unit normal of base::SurfaceNormal
+             * (base/geometry.hpp:255-345; 2-D: (J10, -J00) from
+             * base/linearAlgebra.hpp:93-99; 3-D: J.col(0) x J.col(1)) */
+            double gp[4], gd[4 * 2], xs[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
+            if (dim == 3) {
+                orc_q1_shape(2, xq + q * 2, gp, gd);
+                double J[3][2] = {{0}};
+                for (int n = 0; n < 4; n++)
+                    for (int i = 0; i < 3; i++)
+                        for (int a = 0; a < 2; a++) J[i][a] += X[n * 3 + i] * gd[n * 2 + a];
+                nrm[0] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+                nrm[1] = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+                nrm[2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+            } else {
+                gp[0] = 1. - xq[q];
+                gp[1] = xq[q];
+                nrm[0] = X[3] - X[1];
+                nrm[1] = -(X[2] - X[0]);
+            }
+            double len = 0.;
+            for (int d = 0; d < dim; d++) len += nrm[d] * nrm[d];
+            len = sqrt(len);
+            for (int n = 0; n < nfn; n++)
+                for (int d = 0; d < dim; d++) xs[d] += X[n * dim + d] * gp[n];
+            double rv[3], rg[9];
+            ref_solution(dim, s, p->flux_kind, p->flux_p, xs, rv, rg);
+            for (int c = 0; c < s; c++) {
+                double v = 0.;
+                for (int d = 0; d < dim; d++) v += rg[d * s + c] * (nrm[d] / len);
+                t[c] = v;
+            }
+        } else if (p->tr_kind == 1) {
             /* surface point x(eta) of the Q1 face geometry */
             double gp[4], gd[4 * 2], xs = 0.;
             if (dim == 3) orc_q1_shape(2, xq + q * 2, gp, gd);
@@ -1013,6 +1055,18 @@ static void ref_solution(int dim, int s, int kind, const double* prm, const doub
             double g = prm[0] * prm[1 + d] * cos(prm[1 + d] * x[d]);
             for (int e = 0; e < dim; e++)
                 if (e != d) g *= sin(prm[1 + e] * x[e]);
+            for (int c = 0; c < s; c++) grad[d * s + c] = g;
+        }
+    } else if (kind == ORC_REF_COSPROD) {
+        /* reference/05-mixedPoisson/ReferenceSolution.hpp evaluate and
+         * evaluateGradient (with an amplitude) */
+        double v = prm[0];
+        for (int d = 0; d < dim; d++) v *= cos(prm[1 + d] * x[d]);
+        for (int c = 0; c < s; c++) val[c] = v;
+        for (int d = 0; d < dim; d++) {
+            double g = -prm[0] * prm[1 + d] * sin(prm[1 + d] * x[d]);
+            for (int e = 0; e < dim; e++)
+                if (e != d) g *= cos(prm[1 + e] * x[e]);
             for (int c = 0; c < s; c++) grad[d * s + c] = g;
         }
     } else if (kind == ORC_REF_KELVIN) {

@@ -42,7 +42,7 @@ extern "C" {
 /* kernel ids (same numbering as the product C-ABI) */
 enum { ORC_LAPLACE = 1, ORC_STVENANT = 2, ORC_NEOHOOKE = 3 };
 /* body force kinds */
-enum { ORC_BF_NONE = 0, ORC_BF_CONST = 1, ORC_BF_SINPROD = 2 };
+enum { ORC_BF_NONE = 0, ORC_BF_CONST = 1, ORC_BF_SINPROD = 2, ORC_BF_COSPROD = 3 };
 
 typedef struct orc_problem {
     int dim;                 /* 2 or 3 */
@@ -91,6 +91,12 @@ typedef struct orc_problem {
     double density;
     double mass_factor;
     const double* mass_hist;
+    /* Neumann data as the co-normal derivative t_c = sum_d n_d du_c/dx_d of
+     * a built-in function (ORC_REF_SINPROD / ORC_REF_COSPROD with its
+     * params, reference/05-mixedPoisson/GivenData.hpp neumannBC) in place of
+     * the traction when flux_kind != 0 */
+    int flux_kind;
+    double flux_p[8];
 } orc_problem;
 
 int  orc_gauss1d(int npts, double* w, double* x);
@@ -127,10 +133,11 @@ double orc_l2_error_fundsol(const orc_problem* p, const double* uh,
 void orc_fundsol_elasto(int dim, double lambda, double mu, const double* y,
                         const double* x, const double* dir, double* out);
 /* reference solutions of the error norm */
-enum { ORC_REF_ZERO = 0, ORC_REF_SINPROD = 1, ORC_REF_KELVIN = 2 };
+enum { ORC_REF_ZERO = 0, ORC_REF_SINPROD = 1, ORC_REF_KELVIN = 2, ORC_REF_COSPROD = 3 };
 /* |u - u_h| in the L2 norm (order 0) or the H1 semi-norm (order 1) with the
  * problem's quadrature; uh per field node and component; prm: SINPROD amp,
- * k_x, k_y, k_z (u = amp prod_d sin(k_d x_d)); KELVIN lambda, mu, source[3],
+ * k_x, k_y, k_z (u = amp prod_d sin(k_d x_d)), COSPROD likewise with cos;
+ * KELVIN lambda, mu, source[3],
  * direction[3] (order 0 only) */
 double orc_error_norm(const orc_problem* p, const double* uh, int order, int kind,
                       const double* prm);

@@ -36,6 +36,7 @@ class Problem(C.Structure):
         ("ffn", C.c_int), ("ffaces", C.c_void_p),
         ("tr_kind", C.c_int), ("tr_axis", C.c_int), ("tr_x0", C.c_double),
         ("density", C.c_double), ("mass_factor", C.c_double), ("mass_hist", C.c_void_p),
+        ("flux_kind", C.c_int), ("flux_p", C.c_double * 8),
     ]
 
 
@@ -132,6 +133,11 @@ class OracleProblem:
             lin = getattr(prob, "meta", {}).get("traction_linear")
             if lin is not None:
                 p.tr_kind, p.tr_axis, p.tr_x0 = 1, int(lin[0]), float(lin[1])
+            flux = getattr(prob, "meta", {}).get("neumann_flux")
+            if flux is not None:
+                p.flux_kind = int(flux[0])
+                for i, v in enumerate(flux[1]):
+                    p.flux_p[i] = v
         p.residual = int(residual)
         p.nthreads = nthreads
         tr = getattr(prob, "meta", {}).get("transient")
@@ -272,7 +278,7 @@ def fundsol_elasto(dim, lam, mu, y, x, direc):
     return out
 
 
-REF_ZERO, REF_SINPROD, REF_KELVIN = 0, 1, 2
+REF_ZERO, REF_SINPROD, REF_KELVIN, REF_COSPROD = 0, 1, 2, 3
 
 
 def error_norm(prob, uh, order, kind, params):

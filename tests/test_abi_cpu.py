@@ -39,10 +39,11 @@ def test_apps_built_and_print_usage():
     arguments they print their usage and touch no device."""
     import subprocess
     apps = os.path.join(ROOT, "insilico_amd", "apps")
-    for name in ("poisson", "linear_elastic", "compressible", "compressible_q2", "diffusion", "serve"):
+    for name in ("poisson", "linear_elastic", "compressible", "compressible_q2", "mixed_poisson",
+                 "diffusion", "serve"):
         exe = os.path.join(apps, name)
         assert os.path.exists(exe), exe
-    for name in ("linear_elastic", "compressible", "compressible_q2", "diffusion"):
+    for name in ("linear_elastic", "compressible", "compressible_q2", "mixed_poisson", "diffusion"):
         out = subprocess.run([os.path.join(apps, name)], capture_output=True, text=True, timeout=60)
         assert out.returncode == 0 and "Usage" in out.stdout
 

@@ -201,6 +201,37 @@ def test_compressible_q2_app_golden(golden, read_vtk, tmp_path, mode):
         assert bad.size == 0, (name, [(tuple(b), g[tuple(b)], w[tuple(b)]) for b in bad[:5]])
 
 
+def test_mixed_poisson_app_golden(golden, read_vtk, tmp_path):
+    """reference/05-mixedPoisson/mixedPoisson.cpp as its test harness runs it
+    (appTest.mk: square_020.smf) on the drop-in C++ layer: Q2 field,
+    cosine-product body force, co-normal-derivative Neumann data on the
+    boundary, device assembly and solve, device L2 / H1 errors and mesh
+    volume; stdout against the reference's `ref` and square_020.vtk against
+    its ref.vtk (temperature at the nodes, flux at the centroids) to the
+    printed digits."""
+    import shutil
+    shutil.copy(os.path.join(golden, "square_20.smf"), tmp_path / "square_020.smf")
+    out = _run("mixed_poisson", tmp_path / "square_020.smf").splitlines()
+    with open(os.path.join(golden, "mixedPoisson.ref")) as fh:
+        ref = fh.read().splitlines()
+    assert len(out) == len(ref)
+    assert out[0] == ref[0]
+    for a, b in zip(out[1:], ref[1:]):
+        ka, va = a.rsplit(" ", 1)
+        kb, vb = b.rsplit(" ", 1)
+        assert ka == kb and abs(float(va) - float(vb)) <= 1e-5 * abs(float(vb)), (a, b)
+    got = read_vtk(str(tmp_path / "square_020.vtk"))
+    want = read_vtk(os.path.join(golden, "mixedPoisson.ref.vtk"))
+    np.testing.assert_array_equal(got["points"], want["points"])
+    assert got["cells"] == want["cells"] and got["cell_types"] == want["cell_types"]
+    for sec, name in (("point_data", "temperature"), ("cell_data", "flux")):
+        g, w = got[sec][name], want[sec][name]
+        assert g.shape == w.shape
+        tol = 1e-5 * np.maximum(np.abs(w), np.abs(w).max(axis=1, keepdims=True) * 1e-3) + 1e-9
+        bad = np.argwhere(np.abs(g - w) > tol)
+        assert bad.size == 0, (name, [(tuple(b), g[tuple(b)], w[tuple(b)]) for b in bad[:5]])
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_diffusion_driver_app(oracle, order):
     """heat::DiffusionDriver on the drop-in layer (device time steps): the

@@ -29,7 +29,21 @@ CASES = {
     "sheet2dQ2-6": lambda: P.pulled_sheet_q2(6, True),
     "sheet2dQ2N-6": lambda: _sheet_n(6),
     "elasticity3dQ2T-3": lambda: P.elasticity3d_q2(3, seed=5, traction=(0.1, 0.0, -0.3)),
+    # scalar Neumann data grad u . n on every boundary face (2-D and 3-D
+    # normals), cosine-product body force
+    "mixedPoissonQ2-20": lambda: P.mixed_poisson_q2(*P.unit_cube(2, 20)),
+    "mixedPoissonQ1-24": lambda: P.mixed_poisson_q2(*P.unit_cube(2, 24), deg=1),
+    "mixedPoisson3dQ2-3": lambda: _distorted_mp(3, 2),
+    "mixedPoisson3dQ1-5": lambda: _distorted_mp(5, 1),
 }
+
+
+def _distorted_mp(n, deg):
+    """mixed Poisson on a sheared cube with moved interior nodes: the face
+    normals are not axis aligned"""
+    b = P.distorted_block(3, n, kernel=P.LAPLACE, amp=0.2, seed=11)
+    A = np.array([[1.0, 0.3, 0.1], [0.0, 1.0, 0.2], [0.0, 0.0, 1.0]])
+    return P.mixed_poisson_q2(b.xyz @ A.T, b.conn, deg=deg)
 
 
 def _sheet_n(n):

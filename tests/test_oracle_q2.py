@@ -315,3 +315,43 @@ def test_q2_vtk_golden(oracle, golden, read_vtk, mode):
     with open(os.path.join(golden, "quad.020.%sref.vtk" % mode)) as fh:
         printed = fh.read().split("TENSORS sigma float")[1].split()
     assert ["%g" % v for v in sig.ravel()] == printed
+
+
+def _mixed_poisson_oracle(oracle, golden):
+    """reference/05-mixedPoisson/mixedPoisson.cpp on the oracle: Q2 field on
+    square_020.smf, assembly, direct solve (choleskySolve), setDoFsFromSolver"""
+    xyz, conn = P.read_smf(os.path.join(golden, "square_20.smf"), 2)
+    prob = P.mixed_poisson_q2(xyz, conn)
+    rp, ci, val, b, _ = oracle.assemble(prob)
+    n = prob.n_active
+    x = spla.spsolve(sp.csr_matrix((val, ci, rp), shape=(n, n)).tocsc(), b)
+    u = prob.g.copy()
+    act = prob.dof_id >= 0
+    u[act] = x[prob.dof_id[act]]
+    return prob, u
+
+
+def test_mixed_poisson_golden(oracle, golden, read_vtk):
+    """reference/05-mixedPoisson/ref (number of DoFs, L2 and H1 errors of the
+    solution against u = cos(3x) cos(5y), mesh volume) and ref.vtk (the
+    temperature at the nodes and the flux = grad u_h at the element
+    centroids) to the printed digits: pins the cosine-product body force,
+    the co-normal-derivative Neumann data with the surface normal, and the
+    Q2 Laplace path end to end."""
+    prob, u = _mixed_poisson_oracle(oracle, golden)
+    with open(os.path.join(golden, "mixedPoisson.ref")) as fh:
+        ref = fh.read().splitlines()
+    assert ref[0] == "Number of dofs %d" % prob.n_active
+    prm = [1.0, 3.0, 5.0]
+    l2 = oracle.error_norm(prob, u, 0, oracle.REF_COSPROD, prm)
+    h1 = oracle.error_norm(prob, u, 1, oracle.REF_COSPROD, prm)
+    vol = oracle.error_norm(prob, np.ones_like(u), 0, oracle.REF_ZERO, prm) ** 2
+    assert ref[1] == "L2-error = %g" % l2, (ref[1], l2)
+    assert ref[2] == "H1-error = %g" % h1, (ref[2], h1)
+    assert ref[3] == "Volume of mesh: %g" % vol
+    vtk = read_vtk(os.path.join(golden, "mixedPoisson.ref.vtk"))
+    temp = np.zeros((prob.n_nodes, 1))
+    temp[prob.conn[:, :4].ravel(), 0] = u[prob.fconn[:, :4].ravel()]
+    vtk_compare(temp, vtk["point_data"]["temperature"])
+    flux = oracle.cell_eval(prob, u, oracle.CELL_GRADIENT)
+    vtk_compare(np.hstack([flux, np.zeros((prob.n_elem, 1))]), vtk["cell_data"]["flux"])
