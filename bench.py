@@ -32,6 +32,10 @@ METRIC = "solved DOF/s at 1e-8 rel. residual (mixed vs FP64 PCG); p50/p99 solve 
 
 def spmv_bytes(n, nnz, nslice_vals, fmt, vbytes, xbytes):
     """Algorithmic bytes of one SpMV launch of the given storage."""
+    if fmt == "sym":
+        # upper-triangle values of the symmetric slices (each read once from
+        # HBM; its mirror read hits the caches) + x read + q write
+        return nslice_vals * vbytes + 2 * n * xbytes
     if fmt == "sell":
         # values (padded slices) + pattern id per row + x read + q write
         return nslice_vals * vbytes + n * 4 + 2 * n * xbytes
@@ -411,17 +415,24 @@ def main():
     element_ms = statistics.mean(run_path.element_ms) if run_path.element_ms else None
     fp32_integration = fp32_integration_run(ctx, prob, fmt, args)
     n = prob.n_active
-    sell_vals = ctx.stats()["sell_values"]
-    # the library falls back to CSR when the SELL padding would not pay
+    stn = ctx.stats()
+    sell_vals = stn["sell_values"]
+    # the library falls back to CSR when the SELL padding would not pay, and
+    # runs the FP32 sweeps of scalar problems on the symmetric slices
     fmt_name = "sell" if (fmt == abi.FMT_SELL and sell_vals) else "csr"
+    if fmt_name == "sell" and stn["sell_sym_offsets"]:
+        fmt_name = "sym"
+        sell_vals = (n + 63) // 64 * 64 * stn["sell_sym_offsets"]
     alg = spmv_bytes(n, nnz, sell_vals, fmt_name, 4, 4)
     achieved = round(alg / (spmv_ms / 1e3) / 1e9, 1) if spmv_calls else None
     default_n = {"diffusion": 128, "elasticity": 96, "q2": 64}[args.config]
     traffic = read_traffic("%s%d" % (args.config, args.n or default_n), "mixed_" + fmt_name)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
-            "kernel": ("k_spmv_sell<float, float, float, 4>" if fmt_name == "sell" else
-                         "k_spmv_csr<LPR, float, float, float>") + " (q = A p with p.q partials)",
+            "kernel": {"sym": "k_spmv_sell<float, float, float, 4> on the symmetric slices",
+                       "sell": "k_spmv_sell<float, float, float, 4>",
+                       "csr": "k_spmv_csr<LPR, float, float, float>"}[fmt_name]
+                      + " (q = A p with p.q partials)",
             "bytes_per_launch": alg, "avg_launch_ms": round(spmv_ms, 5),
             "launches_timed": spmv_calls,
             "replacement_launch_ms": round(rep_ms, 5), "replacement_launches": rep_calls}

@@ -101,7 +101,8 @@ typedef struct ins_stats {
     int64_t spmv_calls;     /* profiled: launches timed                      */
     int64_t sell_values;    /* padded value count of the SELL storage (0 = none) */
     int     sell_patterns;  /* distinct column-offset lists of the SELL storage  */
-    int     reserved;
+    int     sell_sym_offsets; /* > 0: the FP32 inner sweeps use the symmetric
+                                 slices (upper offsets shared by all rows)       */
     double  rep_ms;         /* profiled: total time of the residual-replacement
                                launches of the dominant kernel (FP64 residual) */
     int64_t rep_calls;

@@ -50,7 +50,7 @@ class Stats(C.Structure):
                 ("n_active", C.c_int64), ("nnz", C.c_int64),
                 ("device_bytes", C.c_int64), ("spmv_ms", C.c_double),
                 ("spmv_calls", C.c_int64), ("sell_values", C.c_int64),
-                ("sell_patterns", C.c_int), ("reserved", C.c_int),
+                ("sell_patterns", C.c_int), ("sell_sym_offsets", C.c_int),
                 ("rep_ms", C.c_double), ("rep_calls", C.c_int64),
                 ("element_ms", C.c_double)]
 

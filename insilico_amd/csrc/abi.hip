@@ -186,6 +186,7 @@ int ins_get_stats(const ins_ctx* h, ins_stats* out)
     out->device_bytes = c->held;
     out->sell_values = (c->sell.built && c->sell.ok) ? c->sell.nvals : 0;
     out->sell_patterns = (c->sell.built && c->sell.ok) ? c->sell.npat : 0;
+    out->sell_sym_offsets = (c->sell.built && c->sell.ok && c->sell.sym_on) ? c->sell.nu : 0;
     return INS_OK;
 }
 
@@ -468,6 +469,7 @@ int ins_build_pattern(ins_ctx* h, int64_t* nnz_out)
     if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_build_pattern before ins_set_dofs");
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     INS_TRY(build_pattern(c));
+    c->csr_from_host = false;
     INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     INS_HIP_CHECK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -699,6 +701,7 @@ int ins_set_csr(ins_ctx* h, int64_t n, const int32_t* row_ptr, const int32_t* co
     c->have_pattern = true;
     c->have_matrix = true;
     c->have_solution = false;
+    c->csr_from_host = true;
     c->sell.built = false;
     c->sell.stale = true;
     INS_TRY(compute_dinv(c));

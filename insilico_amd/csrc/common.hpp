@@ -88,6 +88,19 @@ struct Sell {
     bool stale = true;       // values older than the last assembly
     bool zero64 = false;     // v64 / v32 sized and zeroed for the current slices
     bool zero32 = false;
+    // Symmetric FP32 slices for the inner sweeps of scalar problems in the
+    // identity row order: only the entries (r, r + u_k) for the offsets
+    // u_k >= 0 of one list shared by all rows, stored at
+    // (r / 64) * 64 * nu + k * 64 + r % 64; an entry below the diagonal,
+    // (r, r - u_k), is read at its mirror (r - u_k, r).  Half the values of
+    // the full slices.
+    bool sym_on = false;
+    bool zeros = false;      // vs sized and zeroed
+    int nu = 0;
+    DArr<int32_t> su;        // nu offsets, ascending
+    DArr<float> vs;
+    DArr<int32_t> smap;      // nnz: vs position of every CSR entry (-1 below the diagonal)
+    int64_t nvs = 0;
 };
 
 struct Ctx {
@@ -131,6 +144,9 @@ struct Ctx {
 
     // pattern
     bool have_pattern = false;
+    // the CSR came from ins_set_csr (values of unknown symmetry) rather than
+    // from the element assembly (symmetric pattern and values)
+    bool csr_from_host = false;
     int64_t nnz = 0;
     int max_nb = 0;         // max neighbour nodes of a node
     int max_adj = 0;        // max elements of a node

@@ -124,7 +124,7 @@ struct Sweep {
 };
 
 // items are visited as first + j*stride (+ per-block offsets < step)
-__device__ int g_xcd_mode = 0;  // 1: per-XCD ranges (set from INS_XCD)
+__device__ int g_xcd_mode = 0;  // 1: per-XCD ranges (INS_XCD, default 1)
 __device__ int g_nt = 0;        // 1: streaming (non-temporal) vector stores and value loads
 
 template <typename T>
@@ -138,6 +138,7 @@ __device__ inline Sweep xcd_sweep(int64_t total, int64_t step, Blk b = grid_blk(
 {
     const int g = b.n;
     Sweep w;
+
     if ((g % NXCD) || !g_xcd_mode) {
         w.first = (int64_t)b.id * step;
         w.end = total;
@@ -231,6 +232,59 @@ __device__ inline TA sell_row(int64_t row, bool live, int64_t n, int lane, int64
     return s;
 }
 
+// Symmetric FP32 slices (scalar problems, identity row order): the values
+// of the offsets u_k >= 0 shared by all rows, k-major within a slice (lane
+// contiguous, 256 bytes per wave and offset); the entry (r, r - u_k) below
+// the diagonal is the mirror (r - u_k, r), one k-row of the slice holding
+// row r - u_k.  Every stored value is read twice per SpMV, once for its own
+// row and once for its mirror, the second time from the caches.
+struct SymView {
+    const float* v;     // null: the full slices
+    const int32_t* u;   // nu offsets, ascending (wave-uniform, scalar loads)
+    int nu;
+};
+
+template <typename TX, typename TA, int G = 8>
+__device__ inline TA sym_row(int64_t row, bool live, int64_t n, int lane, int64_t slice,
+                             const SymView& Y, const TX* __restrict__ x)
+{
+    TA s = (TA)0;
+    if (!live) return s;
+    const int nu = Y.nu;
+    const int64_t ld = (int64_t)64 * nu;   // values per slice
+    const float* __restrict__ own = Y.v + slice * ld + lane;
+    const int64_t last = n - 1;
+    for (int k0 = 0; k0 < nu; k0 += G) {
+        float vu[G], vl[G];
+        TX xu[G], xl[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            vu[j] = 0.f;
+            vl[j] = 0.f;
+            xu[j] = (TX)0;
+            xl[j] = (TX)0;
+            const int k = k0 + j;
+            if (k < nu) {
+                const int d = Y.u[k];
+                int64_t iu = row + d;
+                iu = iu > last ? last : iu;   // padded entries: value 0
+                vu[j] = own[(int64_t)k * 64];
+                xu[j] = x[iu];
+                const int64_t jl = row - d;
+                if (d > 0 && jl >= 0) {
+                    vl[j] = Y.v[(jl >> 6) * ld + (int64_t)k * 64 + (jl & 63)];
+                    xl[j] = x[jl];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) s += (TA)vu[j] * (TA)xu[j];
+#pragma unroll
+        for (int j = 0; j < G; j++) s += (TA)vl[j] * (TA)xl[j];
+    }
+    return s;
+}
+
 // CSR row with LPR lanes; each lane keeps 4 entries in flight
 template <int LPR, typename TV, typename TX, typename TA>
 __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ rp,
@@ -275,11 +329,11 @@ struct Rep {
 // K1: q = A p with per-block partials of p.q; at the replacement iteration
 // (it == rep_at) the FP64 true residual instead, with partials of t.t and
 // of r32.z
-template <typename TV, typename TX, typename TA, int SGK, typename TVR>
+template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYMG = 8>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
                                       TX* __restrict__ q, double* part, const Scal* sc, int it,
-                                      const Rep<TVR>& R)
+                                      const Rep<TVR>& R, const SymView& Y)
 {
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
@@ -311,7 +365,13 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
         const int64_t rho = slice * 64 + lane;
         const bool live = rho < n;
         const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
-        const TA s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
+        TA s;
+        if constexpr (sizeof(TV) == 4) {
+            if (Y.v) s = sym_row<TX, TA, SYMG>(row, live, n, lane, slice, Y, p);
+            else s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
+        } else {
+            s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
+        }
         if (live) {
             const TX qs = (TX)s;
             st(&q[row], qs, g_nt);
@@ -323,12 +383,13 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     block_partials<1>(v, part, sl, b);
 }
 
-template <typename TV, typename TX, typename TA, int SGK, typename TVR>
+template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYMG = 8>
 __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
                                                   const TX* __restrict__ p, TX* __restrict__ q,
-                                                  double* part, const Scal* sc, int it, Rep<TVR> R)
+                                                  double* part, const Scal* sc, int it, Rep<TVR> R,
+                                                  SymView Y)
 {
-    spmv_sell_body<TV, TX, TA, SGK, TVR>(grid_blk(), n, S, val, p, q, part, sc, it, R);
+    spmv_sell_body<TV, TX, TA, SGK, TVR, SYMG>(grid_blk(), n, S, val, p, q, part, sc, it, R, Y);
 }
 
 template <int LPR, typename TV, typename TX, typename TA, typename TVR>
@@ -1085,6 +1146,51 @@ __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
 }
 
 // copy CSR values to their slice positions (padding stays zero)
+// position in the symmetric slices of every CSR entry (row, col >= row):
+// offset col - row looked up in the shared ascending list; -1 below the
+// diagonal.  err: an offset missing from the list.
+__global__ void k_sym_map(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                          const int32_t* __restrict__ su, int nu, int32_t* __restrict__ smap,
+                          int* err)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        for (int32_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int64_t d = (int64_t)col[e] - r;
+            if (d < 0) {
+                smap[e] = -1;
+                continue;
+            }
+            int lo = 0, hi = nu - 1, k = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (su[mid] == d) {
+                    k = mid;
+                    break;
+                }
+                if (su[mid] < d) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            if (k < 0) {
+                smap[e] = -1;
+                atomicOr(err, 1);
+                continue;
+            }
+            smap[e] = (int32_t)((r >> 6) * 64 * (int64_t)nu + (int64_t)k * 64 + (r & 63));
+        }
+    }
+}
+
+__global__ void k_sym_scatter(int64_t nnz, const int32_t* __restrict__ smap,
+                              const float* __restrict__ v32, float* __restrict__ vs)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t o = smap[k];
+        if (o >= 0) vs[o] = v32[k];
+    }
+}
+
 __global__ void k_sell_scatter(int64_t nnz, const int32_t* __restrict__ map,
                                const double* __restrict__ v64, const float* __restrict__ v32,
                                double* __restrict__ s64, float* __restrict__ s32)
@@ -1162,7 +1268,7 @@ int ensure_solver_state(Ctx* c, int mode)
     if (!c->h_scal) INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_scal, sizeof(Scal), 0));
     if (!c->h_snap) {
         INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_snap, 2 * sizeof(Scal), 0));
-        const int mode = env_int("INS_XCD", 0);
+        const int mode = env_int("INS_XCD", 1);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_mode), &mode, sizeof(int)));
         const int nt = env_int("INS_NT", 0);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_nt), &nt, sizeof(int)));
@@ -1272,8 +1378,11 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
     const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32};
     if (format == INS_FMT_SELL) {
         const unsigned g = sell_grid(n);
-        INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR>, g, SB, n, sell_view(c), sell_val,
-                             p, q, c->partial.p, sc, it, R));
+        SymView Y{nullptr, nullptr, 0};
+        if (sizeof(TV) == 4 && c->sell.sym_on && !c->sperm)
+            Y = SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu};
+        INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR>, g, SB, n, sell_view(c),
+                             sell_val, p, q, c->partial.p, sc, it, R, Y));
         *g_out = (int)g;
     } else {
         const int lpr = pick_lpr(c);
@@ -1399,6 +1508,70 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
         (*it_launched)++;
     }
     INS_HIP_CHECK(c, hipGetLastError());
+    return INS_OK;
+}
+
+// Symmetric FP32 slices (see SymView): built when the matrix is scalar in
+// the identity row order, the upper column offsets of all slice lists
+// together number at most SYM_MAXU, and the shared list pads the upper
+// triangle by at most 1/8.  INS_SYM=0 switches them off.
+constexpr int SYM_MAXU = 64;
+
+int build_sym(Ctx* c)
+{
+    Sell& S = c->sell;
+    S.sym_on = false;
+    S.zeros = false;
+    static const int enabled = env_int("INS_SYM", 1);
+    if (!enabled || !S.ok || S.perm_on || c->dpn != 1 || S.npat <= 0 || c->csr_from_host)
+        return INS_OK;
+    std::vector<int32_t> plen(S.npat), pptr(S.npat + 1);
+    INS_HIP_CHECK(c, hipMemcpyAsync(plen.data(), S.pat_len.p, S.npat * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipMemcpyAsync(pptr.data(), S.pat_ptr.p, (S.npat + 1) * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const int64_t ndel = (int64_t)pptr[S.npat - 1] + plen[S.npat - 1];
+    std::vector<int32_t> del(ndel > 0 ? ndel : 1);
+    INS_HIP_CHECK(c, hipMemcpyAsync(del.data(), S.pat_delta.p, ndel * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> up, lowmag;
+    for (int64_t i = 0; i < ndel; i++) {
+        if (del[i] >= 0) up.push_back(del[i]);
+        else lowmag.push_back(-del[i]);
+    }
+    std::sort(up.begin(), up.end());
+    up.erase(std::unique(up.begin(), up.end()), up.end());
+    std::sort(lowmag.begin(), lowmag.end());
+    lowmag.erase(std::unique(lowmag.begin(), lowmag.end()), lowmag.end());
+    // a symmetric pattern mirrors every lower offset into the upper list
+    for (int32_t d : lowmag)
+        if (!std::binary_search(up.begin(), up.end(), d)) return INS_OK;
+    const int nu = (int)up.size();
+    if (nu == 0 || nu > SYM_MAXU) return INS_OK;
+    const int64_t nsl = S.nslices;
+    const int64_t nvs = nsl * 64 * (int64_t)nu;
+    const int64_t upper = (c->nnz + c->n_active) / 2;
+    if (nvs > upper + upper / 8 + 64 * 64 || nvs >= INT32_MAX) return INS_OK;
+    INS_TRY(ensure(c, S.su, nu));
+    INS_HIP_CHECK(c, hipMemcpyAsync(S.su.p, up.data(), nu * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, c->stream));
+    INS_TRY(ensure(c, S.smap, c->nnz));
+    int* err = nullptr;
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&err, sizeof(int), c->stream));
+    INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    k_sym_map<<<grid_for(c->n_active, 256), 256, 0, c->stream>>>(
+        c->n_active, c->row_ptr.p, c->col.p, S.su.p, nu, S.smap.p, err);
+    INS_HIP_CHECK(c, hipGetLastError());
+    int herr = 0;
+    INS_HIP_CHECK(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    (void)hipFreeAsync(err, c->stream);
+    if (herr) return INS_OK;
+    S.nu = nu;
+    S.nvs = nvs;
+    S.sym_on = true;
     return INS_OK;
 }
 
@@ -1580,6 +1753,7 @@ int build_sell(Ctx* c, bool force)
     S.stale = true;
     S.zero64 = false;
     S.zero32 = false;
+    INS_TRY(build_sym(c));
     return INS_OK;
 }
 
@@ -1604,6 +1778,16 @@ int fill_sell(Ctx* c)
         c->nnz, S.map.p, f64 ? c->val64.p : nullptr, c->val32.p, f64 ? S.v64.p : nullptr,
         S.v32.p);
     INS_HIP_CHECK(c, hipGetLastError());
+    if (S.sym_on) {
+        if (!S.zeros) {
+            INS_TRY(ensure(c, S.vs, S.nvs));
+            INS_HIP_CHECK(c, hipMemsetAsync(S.vs.p, 0, S.nvs * sizeof(float), c->stream));
+            S.zeros = true;
+        }
+        k_sym_scatter<<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(c->nnz, S.smap.p,
+                                                                     c->val32.p, S.vs.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+    }
     S.stale = false;
     return INS_OK;
 }
@@ -1909,6 +2093,9 @@ struct SlotDesc {
     const double INS_GLOBAL* v64;  // FP64 slice values (residual replacement)
     double INS_GLOBAL* x64;
     const double INS_GLOBAL* b;
+    const float INS_GLOBAL* vs;     // symmetric slices of the FP32 sweeps (null: none)
+    const int32_t INS_GLOBAL* su;
+    int nu;
     int64_t join;  // batched iteration of the request's first launch
     int g_spmv, g_vec;
 };
@@ -1953,7 +2140,9 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
     spmv_sell_body<float, float, float, 4, double>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv},
                                                    d.n, sell_of(d.S), (const float*)d.val,
                                                    (const float*)d.p, (float*)d.q, (double*)d.part,
-                                                   (const Scal*)d.sc, (int)(it - d.join), R);
+                                                   (const Scal*)d.sc, (int)(it - d.join), R,
+                                                   SymView{(const float*)d.vs,
+                                                           (const int32_t*)d.su, d.nu});
 }
 
 __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
@@ -2116,6 +2305,10 @@ int upload_table(Server* sv)
         INS_AS(d.v64, c->sell.v64.p);
         INS_AS(d.x64, c->sx64);
         INS_AS(d.b, c->sb);
+        const bool sym = c->sell.sym_on && !c->sperm;
+        INS_AS(d.vs, sym ? c->sell.vs.p : nullptr);
+        INS_AS(d.su, sym ? c->sell.su.p : nullptr);
+        d.nu = sym ? c->sell.nu : 0;
 #undef INS_AS
         d.g_spmv = (int)sell_grid(d.n);
         d.g_vec = (int)vec_grid(d.n);

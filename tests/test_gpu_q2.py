@@ -181,9 +181,11 @@ def test_q2_compressible_newton_golden(golden, mode):
                 if c1 < 1e-9:
                     hist.append((step, it, c1, None))
                     break
-                # the round-off floor of the traction-controlled systems is
-                # about 1.3e-12 (small loads against E = 1000)
-                rtol = 1e-12 if mode == "D" else 1e-11
+                # FP64 round-off floors of these systems: about 1e-12 under
+                # displacement control (the exact value moves with the
+                # order of the dot-product partial sums) and 1.3e-12 under
+                # traction control (small loads against E = 1000)
+                rtol = 3e-12 if mode == "D" else 1e-11
                 _, _, st = ctx.solve(abi.SOLVE_FP64, abi.FMT_CSR, rtol, max_it=20000)
                 assert st == abi.INS_OK
                 x = ctx.get_solution()
