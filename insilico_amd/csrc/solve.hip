@@ -124,7 +124,7 @@ struct Sweep {
 };
 
 // items are visited as first + j*stride (+ per-block offsets < step)
-__device__ int g_xcd_mode = 0;  // 1: per-XCD ranges (INS_XCD, default 1)
+__device__ int g_xcd_mode = 0;  // 1: per-XCD ranges for every kernel (INS_XCD)
 __device__ int g_nt = 0;        // 1: streaming (non-temporal) vector stores and value loads
 
 template <typename T>
@@ -134,12 +134,12 @@ __device__ inline void st(T* p, T v, bool nt)
     else *p = v;
 }
 
-__device__ inline Sweep xcd_sweep(int64_t total, int64_t step, Blk b = grid_blk())
+__device__ inline Sweep xcd_sweep(int64_t total, int64_t step, Blk b = grid_blk(),
+                                  bool per_xcd = false)
 {
     const int g = b.n;
     Sweep w;
-
-    if ((g % NXCD) || !g_xcd_mode) {
+    if ((g % NXCD) || !(g_xcd_mode || per_xcd)) {
         w.first = (int64_t)b.id * step;
         w.end = total;
         w.stride = (int64_t)g * step;
@@ -337,7 +337,9 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
 {
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
-    const Sweep w = xcd_sweep(nsl, SB / 64, b);
+    // the symmetric slices sweep per-XCD ranges: a mirror read then finds
+    // the slice it needs in the same XCD's L2
+    const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
     if (it == sc->rep_at) {
         double tt = 0., rz = 0.;
         for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
@@ -1268,7 +1270,7 @@ int ensure_solver_state(Ctx* c, int mode)
     if (!c->h_scal) INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_scal, sizeof(Scal), 0));
     if (!c->h_snap) {
         INS_HIP_CHECK(c, hipHostMalloc((void**)&c->h_snap, 2 * sizeof(Scal), 0));
-        const int mode = env_int("INS_XCD", 1);
+        const int mode = env_int("INS_XCD", 0);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_mode), &mode, sizeof(int)));
         const int nt = env_int("INS_NT", 0);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_nt), &nt, sizeof(int)));
