@@ -113,6 +113,12 @@ int ins_create(ins_ctx** out, int device, uint64_t hbm_budget_bytes)
     Ctx* c = new (std::nothrow) Ctx();
     if (!c) return INS_ERR_OOM;
     c->device = device;
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
+                hipSuccess && cus > 0)
+            c->n_cu = cus;
+    }
     c->budget = hbm_budget_bytes;
     // temporaries of the setup steps are stream-ordered allocations on the
     // context stream (no device-wide synchronisation); keep their memory in

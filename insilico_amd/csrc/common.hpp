@@ -105,6 +105,7 @@ struct Sell {
 
 struct Ctx {
     int device = 0;
+    int n_cu = 256;         // compute units of the device
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     int64_t held = 0;

@@ -1237,6 +1237,36 @@ unsigned sell_grid(int64_t n)
     return round8(std::max<int64_t>(1, std::min<int64_t>(g, MAXG)));
 }
 
+// Blocks of the symmetric SpMV.  Each XCD sweeps its eighth of the slices
+// with its blocks' 4 waves per step.  Measured at 128^3 (n = 2.13M): the
+// blocks per XCD should be a whole multiple m of the XCD's CUs (5 per CU:
+// 30.1 us; 4.75 or 5.25 per CU: 32.3-33.0 us) and the last step of the
+// sweep nearly full (8 per CU, 4.07 steps: 34.8 us).  m runs over 4..8;
+// the fullest last step wins, the smaller m on a tie within 2%.
+unsigned sym_grid(int64_t n, int cus)
+{
+    const int64_t nsl = (n + 63) / 64;
+    const int wpb = SB / 64;
+    const int cpx = std::max(1, cus / NXCD);   // CUs per XCD
+    if (nsl < (int64_t)4 * cpx * NXCD * wpb) return sell_grid(n);
+    int64_t per = (nsl + NXCD - 1) / NXCD;
+    per = (per + wpb - 1) / wpb * wpb;
+    int best = 0;
+    double best_eff = -1.;
+    for (int m = 4; m <= 8; m++) {
+        const int64_t nloc = (int64_t)m * cpx;
+        if (nloc * NXCD > MAXG) break;
+        const int64_t wv = nloc * wpb;
+        const int64_t steps = (per + wv - 1) / wv;
+        const double eff = (double)per / (double)(steps * wv);
+        if (eff > best_eff + 0.02) {
+            best_eff = eff;
+            best = m;
+        }
+    }
+    return (unsigned)(best * cpx * NXCD);
+}
+
 unsigned csr_grid(int64_t n, int lpr)
 {
     int64_t g = (n * lpr + SB - 1) / SB;
@@ -1379,10 +1409,13 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
     const Scal* sc = c->scal.p;
     const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32};
     if (format == INS_FMT_SELL) {
-        const unsigned g = sell_grid(n);
+        unsigned g = sell_grid(n);
         SymView Y{nullptr, nullptr, 0};
-        if (sizeof(TV) == 4 && c->sell.sym_on && !c->sperm)
+        if (sizeof(TV) == 4 && c->sell.sym_on && !c->sperm) {
             Y = SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu};
+            static const int forced = env_int("INS_SYM_GRID", 0);
+            g = forced >= 8 ? round8(std::min(forced, MAXG)) : sym_grid(n, c->n_cu);
+        }
         INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR>, g, SB, n, sell_view(c),
                              sell_val, p, q, c->partial.p, sc, it, R, Y));
         *g_out = (int)g;
