@@ -337,10 +337,9 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
 {
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
-    // the symmetric slices sweep per-XCD ranges: a mirror read then finds
-    // the slice it needs in the same XCD's L2
-    const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
     if (it == sc->rep_at) {
+        // FP64 true residual on the full slices, round-robin placement
+        const Sweep w = xcd_sweep(nsl, SB / 64, b);
         double tt = 0., rz = 0.;
         for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
             const int64_t rho = slice * 64 + lane;
@@ -362,6 +361,9 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
         return;
     }
     if (sc->done) return;
+    // the symmetric slices sweep per-XCD ranges: a mirror read then finds
+    // the slice it needs in the same XCD's L2
+    const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
     double pq = 0.;
     for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
         const int64_t rho = slice * 64 + lane;
