@@ -1149,7 +1149,6 @@ __global__ __launch_bounds__(256) void k_sell_map(int64_t n, SellView S,
     }
 }
 
-// copy CSR values to their slice positions (padding stays zero)
 // position in the symmetric slices of every CSR entry (row, col >= row):
 // offset col - row looked up in the shared ascending list; -1 below the
 // diagonal.  err: an offset missing from the list.
@@ -1195,6 +1194,7 @@ __global__ void k_sym_scatter(int64_t nnz, const int32_t* __restrict__ smap,
     }
 }
 
+// copy CSR values to their slice positions (padding stays zero)
 __global__ void k_sell_scatter(int64_t nnz, const int32_t* __restrict__ map,
                                const double* __restrict__ v64, const float* __restrict__ v32,
                                double* __restrict__ s64, float* __restrict__ s32)
