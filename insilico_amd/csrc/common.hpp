@@ -64,6 +64,10 @@ struct Scal {
     int nrep;        // mixed: residual replacements so far
     double rref;     // mixed: |b - A x|^2 at the last replacement
     double delta2;   // mixed: replace when |r|^2 < delta2 * rref
+    int stop;        // FP64: converged in K3 of the last iteration; the next
+                     // K1 turns it into done (K3 itself never reads it, so
+                     // every block of that K3 still does its x update)
+    int pad_;
 };
 
 // SELL-64 matrix: 64-row slices, values column-major inside a slice; every

@@ -360,7 +360,7 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
         block_partials<2>(v, part, sl, b);
         return;
     }
-    if (sc->done) return;
+    if (sc->done || sc->stop) return;
     // the symmetric slices sweep per-XCD ranges: a mirror read then finds
     // the slice it needs in the same XCD's L2
     const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __res
         block_partials<2>(v, part, sl);
         return;
     }
-    if (sc->done) return;
+    if (sc->done || sc->stop) return;
     double pq = 0.;
     for (int64_t row = w.first + threadIdx.x / LPR; row < w.end; row += w.stride) {
         const TA s = csr_row<LPR, TV, TX, TA>(row, sub, rp, col, val, p);
@@ -451,7 +451,8 @@ struct V16<double> {
     static constexpr int W = 2;
 };
 
-// K2: alpha from the p.q partials; x += alpha p; r -= alpha q; r.r, r.z partials
+// K2: alpha from the p.q partials; r -= alpha q; r.r, r.z partials (the
+// x update waits for K3)
 template <typename TX, typename TD>
 __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* __restrict__ r,
                                       const TX* __restrict__ p, const TX* __restrict__ q,
@@ -488,6 +489,11 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
         return;
     }
     if (sc->done) return;
+    if (sc->stop) {
+        // converged in the last K3: done from here on
+        if (leader(b)) sc->done = 1;
+        return;
+    }
     const int par = it & 1;
     double t[1];
     const int slq[1] = {P_PQ};
@@ -515,14 +521,13 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
     // 16-byte accesses over the full vectors, then the n % W tail
     const int64_t nv = n / W;
     const Sweep w = xcd_sweep(nv, VB, b);
-    VT* __restrict__ xv = reinterpret_cast<VT*>(x);
+    (void)x;  // x += alpha p is done by K3, which reads p anyway
+    (void)p;
     VT* __restrict__ rv = reinterpret_cast<VT*>(r);
-    const VT* __restrict__ pv = reinterpret_cast<const VT*>(p);
     const VT* __restrict__ qv = reinterpret_cast<const VT*>(q);
     const VT* __restrict__ dv = reinterpret_cast<const VT*>(dinv);
     for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
         const VT ri = rv[i] - alpha * qv[i];
-        xv[i] = xv[i] + alpha * pv[i];
         rv[i] = ri;
         const VT zi = dv[i] * ri;
 #pragma unroll
@@ -532,7 +537,6 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
         }
     }
     for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB) {
-        x[i] = x[i] + alpha * p[i];
         const TX ri = r[i] - alpha * q[i];
         r[i] = ri;
         const TX zi = (TX)dinv[i] * ri;
@@ -554,7 +558,7 @@ __global__ __launch_bounds__(VB) void k_update_xr(int64_t n, TX* __restrict__ x,
     update_xr_body<TX, TD>(grid_blk(), n, x, r, p, q, dinv, part, g_pq, sc, it);
 }
 
-// K3: stop test; beta; p = D^-1 r + beta p.  In mixed precision, once the
+// K3: stop test; x += alpha p; beta; p = D^-1 r + beta p.  In mixed precision, once the
 // recursive residual fell below delta^2 of the last true one, or below the
 // tolerance, the next iteration is a replacement: every block folds its rows
 // of x32 into x64 here and the leader sets rep_at.  At the replacement
@@ -563,13 +567,14 @@ template <typename TX, typename TD>
 __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
                                      const TX* __restrict__ r, const TD* __restrict__ dinv,
                                      const double* part, int g, Scal* sc, int it,
-                                     double* __restrict__ x64, TX* __restrict__ x32)
+                                     double* __restrict__ x64, TX* __restrict__ x)
 {
     typedef typename V16<TX>::type VT;
     constexpr int W = V16<TX>::W;
     constexpr bool mixed = sizeof(TX) == 4;
     const int64_t nv = n / W;
     const Sweep w = xcd_sweep(nv, VB, b);
+    VT* __restrict__ xv = reinterpret_cast<VT*>(x);
     VT* __restrict__ pv = reinterpret_cast<VT*>(p);
     const VT* __restrict__ rv = reinterpret_cast<const VT*>(r);
     const VT* __restrict__ dv = reinterpret_cast<const VT*>(dinv);
@@ -596,7 +601,7 @@ __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
             sc->breakdown = 1;
             sc->done = 1;
         } else if (conv && !mixed) {
-            sc->done = 1;
+            sc->stop = 1;
         } else if (conv || pause) {
             sc->conv32 = conv ? 1 : 0;
             sc->rep_at = it + 1;
@@ -607,26 +612,57 @@ __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
         }
     }
     if (bad) return;
+    // x += alpha p with the direction of this iteration's SpMV (alpha from
+    // K2), done here where p is read anyway; then fold x32 into x64 before
+    // a replacement, and p = D^-1 r + beta p
+    const TX alpha = (TX)sc->alpha;
     if (mixed && (conv || pause)) {
-        for (int64_t i = (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB) {
-            x64[i] += (double)x32[i];
-            x32[i] = (TX)0;
+        // every item is read and written by one thread in one visit (the p
+        // update of another thread's item must not precede its x update)
+        const TX beta = conv ? (TX)0 : (TX)(rz / rho);
+        for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
+            const VT pi = pv[i];
+            const VT xi = xv[i] + alpha * pi;
+#pragma unroll
+            for (int k = 0; k < W; k++) x64[i * W + k] += (double)xi[k];
+            xv[i] = VT{};
+            if (!conv) pv[i] = dv[i] * rv[i] + beta * pi;
+        }
+        for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB) {
+            const TX pi = p[i];
+            const TX xi = x[i] + alpha * pi;
+            x64[i] += (double)xi;
+            x[i] = (TX)0;
+            if (!conv) p[i] = (TX)dinv[i] * r[i] + beta * pi;
+        }
+        return;
+    } else if (conv) {
+        for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) xv[i] = xv[i] + alpha * pv[i];
+        for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB)
+            x[i] = x[i] + alpha * p[i];
+        return;
+    } else {
+        const TX beta = (TX)(rz / rho);
+        for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) {
+            const VT pi = pv[i];
+            xv[i] = xv[i] + alpha * pi;
+            pv[i] = dv[i] * rv[i] + beta * pi;
+        }
+        for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB) {
+            const TX pi = p[i];
+            x[i] = x[i] + alpha * pi;
+            p[i] = (TX)dinv[i] * r[i] + beta * pi;
         }
     }
-    if (conv) return;
-    const TX beta = (TX)(rz / rho);
-    for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) pv[i] = dv[i] * rv[i] + beta * pv[i];
-    for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB)
-        p[i] = (TX)dinv[i] * r[i] + beta * p[i];
 }
 
 template <typename TX, typename TD>
 __global__ __launch_bounds__(VB) void k_update_p(int64_t n, TX* __restrict__ p,
                                                  const TX* __restrict__ r,
                                                  const TD* __restrict__ dinv, const double* part,
-                                                 int g, Scal* sc, int it, double* x64, TX* x32)
+                                                 int g, Scal* sc, int it, double* x64, TX* x)
 {
-    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, it, x64, x32);
+    update_p_body<TX, TD>(grid_blk(), n, p, r, dinv, part, g, sc, it, x64, x);
 }
 
 // start: x = 0, r = b, p = D^-1 b; partials of b.b and r.z
@@ -669,6 +705,7 @@ __global__ __launch_bounds__(VB) void k_init_fin(const double* part, int g, doub
         sc->breakdown = 0;
         sc->delta2 = delta2;
         sc->done = (t[0] == 0.) ? 1 : 0;
+        sc->stop = 0;
         sc->fin = sc->done;
         sc->restart = 0;
         sc->rep_at = INT_MAX;
@@ -1540,8 +1577,7 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
         k_update_xr<TX, TX><<<gv, VB, 0, c->stream>>>(n, x, r, p, q, dinv, c->partial.p, g1,
                                                       c->scal.p, it);
         k_update_p<TX, TX><<<gv, VB, 0, c->stream>>>(n, p, r, dinv, c->partial.p, (int)gv,
-                                                     c->scal.p, it, c->sx64,
-                                                     sizeof(TX) == 4 ? x : nullptr);
+                                                     c->scal.p, it, c->sx64, x);
         (*it_launched)++;
     }
     INS_HIP_CHECK(c, hipGetLastError());
@@ -1931,7 +1967,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
         Scal S;
         INS_TRY(pipe.pop(&S));
         reps_seen = S.nrep;
-        if (S.done) stop = true;
+        if (S.done || S.stop) stop = true;
     }
     INS_TRY(pipe.drain());
     if (mixed) {
