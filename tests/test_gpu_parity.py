@@ -173,3 +173,27 @@ def test_symmetric_slices_mixed_solve(oracle, name, make):
     _, rel32, st32 = ctx32.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
     assert st32 == abi.INS_OK and rel32 <= 1e-8
     ctx32.close()
+
+
+def test_host_csr_keeps_full_slices(oracle):
+    """A CSR set from the host (ins_set_csr) is of unknown symmetry: the
+    mixed solve keeps the full slices for it (no symmetric slices) and
+    still reaches the requested residual of the uploaded system."""
+    prob = P.diffusion3d(28, seed=7)
+    with abi.Context(0) as c:
+        c.load_problem(prob)
+        c.assemble(abi.FP64, abi.ASM_MATRIX | abi.ASM_RHS)
+        c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+        assert c.stats()["sell_sym_offsets"] == 14
+        rp, ci = c.get_pattern()
+        val = c.get_csr_values()
+        b = c.get_rhs()
+        # the same symmetric matrix from the host: full slices, same solution
+        c.set_csr(rp, ci, val)
+        c.set_rhs(b)
+        _, rel, st = c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+        assert st == abi.INS_OK and rel <= 1e-8
+        assert c.stats()["sell_sym_offsets"] == 0
+        x = c.get_solution()
+    r = b - oracle.spmv(rp, ci, val, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-8 * 1.001
