@@ -295,7 +295,8 @@ def test_pcg_against_direct(oracle):
 
 
 def test_poisson_manufactured_convergence(oracle):
-    """cfg1: error against sin(pi x) sin(pi y) decreases as O(h^2)."""
+    """2-D Q1 Poisson on the unit square: the error against sin(pi x)
+    sin(pi y) decreases as O(h^2)."""
     errs = []
     for n in (8, 16, 32):
         p = P.poisson2d(n)
