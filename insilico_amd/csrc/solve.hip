@@ -329,7 +329,10 @@ struct Rep {
 // K1: q = A p with per-block partials of p.q; at the replacement iteration
 // (it == rep_at) the FP64 true residual instead, with partials of t.t and
 // of r32.z
-template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYMG = 8>
+// SYM: 0 = full slices only, 1 = symmetric slices only, 2 = either, chosen
+// per call by Y.v (the batched serving kernel).  Separate instantiations keep
+// the symmetric row's registers out of the full-slice kernel.
+template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM = 0>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
                                       TX* __restrict__ q, double* part, const Scal* sc, int it,
@@ -370,8 +373,10 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
         const bool live = rho < n;
         const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
         TA s;
-        if constexpr (sizeof(TV) == 4) {
-            if (Y.v) s = sym_row<TX, TA, SYMG>(row, live, n, lane, slice, Y, p);
+        if constexpr (sizeof(TV) == 4 && SYM == 1) {
+            s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
+        } else if constexpr (sizeof(TV) == 4 && SYM == 2) {
+            if (Y.v) s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
             else s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
         } else {
             s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
@@ -387,13 +392,13 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     block_partials<1>(v, part, sl, b);
 }
 
-template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYMG = 8>
+template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM = 0>
 __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const TV* __restrict__ val,
                                                   const TX* __restrict__ p, TX* __restrict__ q,
                                                   double* part, const Scal* sc, int it, Rep<TVR> R,
                                                   SymView Y)
 {
-    spmv_sell_body<TV, TX, TA, SGK, TVR, SYMG>(grid_blk(), n, S, val, p, q, part, sc, it, R, Y);
+    spmv_sell_body<TV, TX, TA, SGK, TVR, SYM>(grid_blk(), n, S, val, p, q, part, sc, it, R, Y);
 }
 
 template <int LPR, typename TV, typename TX, typename TA, typename TVR>
@@ -1455,8 +1460,12 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
             static const int forced = env_int("INS_SYM_GRID", 0);
             g = forced >= 8 ? round8(std::min(forced, MAXG)) : sym_grid(n, c->n_cu);
         }
-        INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR>, g, SB, n, sell_view(c),
-                             sell_val, p, q, c->partial.p, sc, it, R, Y));
+        if (Y.v)
+            INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 1>, g, SB, n, sell_view(c),
+                                 sell_val, p, q, c->partial.p, sc, it, R, Y));
+        else
+            INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 0>, g, SB, n, sell_view(c),
+                                 sell_val, p, q, c->partial.p, sc, it, R, Y));
         *g_out = (int)g;
     } else {
         const int lpr = pick_lpr(c);
@@ -2210,7 +2219,7 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
     const SlotDesc& d = bt->d[s];
     const Rep<double> R{(const double*)d.v64, (const double*)d.x64, (const double*)d.b,
                         (float*)d.r, (const float*)d.dinv};
-    spmv_sell_body<float, float, float, 4, double>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv},
+    spmv_sell_body<float, float, float, 4, double, 2>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv},
                                                    d.n, sell_of(d.S), (const float*)d.val,
                                                    (const float*)d.p, (float*)d.q, (double*)d.part,
                                                    (const Scal*)d.sc, (int)(it - d.join), R,
