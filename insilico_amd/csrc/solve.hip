@@ -126,6 +126,9 @@ struct Sweep {
 // items are visited as first + j*stride (+ per-block offsets < step)
 __device__ int g_xcd_mode = 0;  // 1: per-XCD ranges for every kernel (INS_XCD)
 __device__ int g_nt = 0;        // 1: streaming (non-temporal) vector stores and value loads
+// mixed: a replacement restarts the direction when |b - A x|^2 exceeds this
+// multiple of the recursive |r|^2 (INS_RESTART_RATIO)
+__device__ double g_restart_ratio = 64.;
 
 template <typename T>
 __device__ inline void st(T* p, T v, bool nt)
@@ -484,7 +487,7 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
                 // keep the direction while the recursion still tracks the
                 // true residual; once the FP32 recursion has converged or
                 // drifted, restart from p = D^-1 r
-                sc->restart = (sc->conv32 || t[0] > 64. * sc->rr) ? 1 : 0;
+                sc->restart = (sc->conv32 || t[0] > g_restart_ratio * sc->rr) ? 1 : 0;
                 sc->rho[(it + 1) & 1] = t[1];
                 sc->rr = t[0];
                 sc->rref = t[0];
@@ -1325,6 +1328,20 @@ int env_int(const char* name, int dflt)
     return v ? atoi(v) : dflt;
 }
 
+double env_double(const char* name, double dflt)
+{
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atof(v) : dflt;
+}
+
+// mixed: replace the residual once |r|^2 fell below delta^2 of the last
+// true one (INS_DELTA2, 0.01: every 10x reduction of |r|)
+double mixed_delta2()
+{
+    static const double d = env_double("INS_DELTA2", 0.01);
+    return d;
+}
+
 int ensure_solver_state(Ctx* c, int mode)
 {
     const int64_t n = c->n_active;
@@ -1348,6 +1365,8 @@ int ensure_solver_state(Ctx* c, int mode)
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_mode), &mode, sizeof(int)));
         const int nt = env_int("INS_NT", 0);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_nt), &nt, sizeof(int)));
+        const double ratio = env_double("INS_RESTART_RATIO", 64.);
+        INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_restart_ratio), &ratio, sizeof(double)));
     }
     for (int i = 0; i < 2; i++)
         if (!c->evs[i]) INS_HIP_CHECK(c, hipEventCreateWithFlags(&c->evs[i], hipEventDisableTiming));
@@ -1934,8 +1953,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     INS_TRY(solve_vectors(c, format, mode, c->stream));
     int64_t launched = 0;
-    // replace the residual after every 10x reduction (delta = 0.1)
-    const double delta2 = 0.01;
+    const double delta2 = mixed_delta2();
     if (!mixed) {
         k_init<double, double><<<gv, VB, 0, c->stream>>>(n, c->sb, c->sx64, c->r.p, c->p.p,
                                                          c->sd64, c->partial.p);
@@ -2506,7 +2524,8 @@ int server_attach(Server* sv, Ctx* c, double rtol, int max_it, int* slot)
     SRV_HIP(sv, hipMemsetAsync(c->sx64, 0, n * sizeof(double), sv->stream));
     k_init<float, float><<<gv, VB, 0, sv->stream>>>(n, c->sb, c->x32.p, c->r32.p, c->p32.p,
                                                     c->sd32, c->partial.p);
-    k_init_fin<<<1, VB, 0, sv->stream>>>(c->partial.p, (int)gv, rtol * rtol, 0.01, sv->scal + s);
+    k_init_fin<<<1, VB, 0, sv->stream>>>(c->partial.p, (int)gv, rtol * rtol, mixed_delta2(),
+                                 sv->scal + s);
     SRV_HIP(sv, hipGetLastError());
     SlotH& h = sv->slots[s];
     h = SlotH();
