@@ -107,22 +107,27 @@ def cpu_baseline(prob, ctx, iters_fp64, repeats=3):
     rp, ci = ctx.get_pattern()
     val = ctx.get_csr_values()
     b = ctx.get_rhs()
-    alls = [_cpu_rate(prob, O, rp, ci, val, b, iters_fp64, ncores, 131072, 20) for _ in range(repeats)]
-    ones = [_cpu_rate(prob, O, rp, ci, val, b, iters_fp64, 1, 16384, 3) for _ in range(repeats)]
+    # about 10-15 s of host work in all: ~3 s per all-core sample, ~0.8 s per
+    # one-thread sample at 128^3
+    k_all, k_one = 200, 10
+    alls = [_cpu_rate(prob, O, rp, ci, val, b, iters_fp64, ncores, 1048576, k_all)
+            for _ in range(repeats)]
+    ones = [_cpu_rate(prob, O, rp, ci, val, b, iters_fp64, 1, 65536, k_one)
+            for _ in range(repeats)]
     t_all, t_asm_s, E_s, t_it = sorted(alls)[repeats // 2]
     t_one, t_asm1, E_1, t_it1 = sorted(ones)[repeats // 2]
     return {
         "value": prob.n_active / t_all, "unit": "DOF/s", "cores": ncores, "kind": "port",
         "sample": "oracle (reference algorithm in C, -O3 -fopenmp, %d threads): assembly of "
-                  "%d of %d elements (%.2f s) and 20 Jacobi-CG iterations (OpenMP CSR SpMV) on "
+                  "%d of %d elements (%.2f s) and %d Jacobi-CG iterations (OpenMP CSR SpMV) on "
                   "the full matrix (%.3f s/iter), extrapolated to %d elements + %d iterations; "
-                  "median of %d" % (ncores, E_s, prob.n_elem, t_asm_s, t_it, prob.n_elem,
+                  "median of %d" % (ncores, E_s, prob.n_elem, t_asm_s, k_all, t_it, prob.n_elem,
                                     iters_fp64, repeats),
         "t_assemble_s": prob.n_elem * t_asm_s / E_s, "t_iter_s": t_it,
         "value_1thread": prob.n_active / t_one,
-        "sample_1thread": "1 thread: assembly of %d elements (%.2f s) and 3 CG iterations "
+        "sample_1thread": "1 thread: assembly of %d elements (%.2f s) and %d CG iterations "
                           "(%.3f s/iter), extrapolated likewise; median of %d"
-                          % (E_1, t_asm1, t_it1, repeats),
+                          % (E_1, t_asm1, k_one, t_it1, repeats),
     }
 
 
