@@ -238,6 +238,7 @@ class Context:
         d = np.ascontiguousarray(dof_id, np.int32)
         self._chk(self.L.ins_set_dofs(self.h, dpn, _p(d), n_active))
         self.dpn, self.n_active = dpn, n_active
+        self.dof_n_active = n_active
 
     def set_prescribed(self, g):
         g = None if g is None else np.ascontiguousarray(g, np.float64)
@@ -259,6 +260,7 @@ class Context:
         nnz = C.c_int64()
         self._chk(self.L.ins_build_pattern(self.h, C.byref(nnz)))
         self.nnz = nnz.value
+        self.n_active = self.dof_n_active
         return nnz.value
 
     def get_pattern(self):

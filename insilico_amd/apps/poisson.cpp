@@ -2,6 +2,8 @@
 // u = 0 on the boundary, Q1 on an N x N mesh, written against the drop-in API
 // in the order of the reference's heat::PoissonDriver (heat/PoissonDriver.hpp
 // :185-210) and reference/04-heat/dirichlet.cpp:120-173.
+// Usage: poisson [N [rtol]] (rtol of the PCG, default 1e-10; one below
+// FP64 round-off makes cgSolve report INS_ERR_NOCONV, exit status 2).
 // Prints: numDoFs iterations relres max|u_h - u| L2(u_h - u)
 #include <cstdio>
 #include <cstdlib>
@@ -13,6 +15,7 @@ int main(int argc, char* argv[])
 {
     using namespace insilico;
     const int N = argc > 1 ? std::atoi(argv[1]) : 64;
+    const double rtol = argc > 2 ? std::atof(argv[2]) : 1e-10;
     try {
         Mesh mesh = unitCube(2, N, N);
         Field field(mesh.numNodes(), 1);
@@ -34,7 +37,7 @@ int main(int argc, char* argv[])
             quadrature, solver, fieldBinder,
             base::asmb::SineProductForce{2. * M_PI * M_PI, {M_PI, M_PI, 0.}});
         solver.finishAssembly();
-        const int iters = solver.cgSolve(1e-10);
+        const int iters = solver.cgSolve(rtol);
         base::dof::setDoFsFromSolver(solver, field);
 
         double maxErr = 0.;
@@ -47,7 +50,7 @@ int main(int argc, char* argv[])
         std::printf("%zu %d %.6e %.6e %.6e\n", numDoFs, iters, solver.lastRelres(), maxErr, l2);
     } catch (const Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.status, e.what());
-        return 1;
+        return e.status == INS_ERR_NOCONV ? 2 : 1;
     }
     return 0;
 }

@@ -408,6 +408,8 @@ int ins_set_dofs(ins_ctx* h, int dpn, const int32_t* dof_id, int64_t n_active)
         if (!seen[i]) return fail(c, INS_ERR_RANGE, "DoF id missing: " + std::to_string(i));
     c->dpn = dpn;
     c->n_active = n_active;
+    c->dof_n_active = n_active;
+    c->csr_from_host = false;
     c->monotone = mono;
     INS_TRY(upload(c, c->dof_id, dof_id, (size_t)nd));
     INS_TRY(ensure(c, c->g, (size_t)nd));
@@ -473,6 +475,10 @@ int ins_build_pattern(ins_ctx* h, int64_t* nnz_out)
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
     if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_build_pattern before ins_set_dofs");
+    // a CSR installed by ins_set_csr may have had another size: the pattern
+    // is rebuilt for the DoFs
+    c->n_active = c->dof_n_active;
+    invalidate_pattern(c);
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
     INS_TRY(build_pattern(c));
     c->csr_from_host = false;
@@ -696,6 +702,12 @@ int ins_set_csr(ins_ctx* h, int64_t n, const int32_t* row_ptr, const int32_t* co
             if (k > row_ptr[i] && col_idx[k] <= col_idx[k - 1])
                 return fail(c, INS_ERR_ARG, "columns must be strictly ascending per row");
         }
+    // the element assembly, the RHS kernels and ins_distribute work on the
+    // DoF maps of ins_set_dofs; they refuse (INS_ERR_STATE) until
+    // ins_build_pattern rebuilds the pattern, or, for ins_distribute, unless
+    // the uploaded system has as many rows as there are active DoFs.  The
+    // right-hand side must be set again for the uploaded system.
+    invalidate_pattern(c);
     c->n_active = n;
     c->nnz = nnz;
     INS_TRY(upload(c, c->row_ptr, row_ptr, (size_t)n + 1));

@@ -1927,6 +1927,10 @@ int compute_dinv(Ctx* c)
 int assemble(Ctx* c, int precision, int flags)
 {
     if (!c->have_pattern) return fail(c, INS_ERR_STATE, "ins_assemble before ins_build_pattern");
+    if (c->csr_from_host)
+        return fail(c, INS_ERR_STATE,
+                    "ins_assemble after ins_set_csr: call ins_build_pattern to assemble into the "
+                    "DoF pattern again");
     if (c->kernel == 0) return fail(c, INS_ERR_STATE, "ins_assemble before ins_set_material");
     const bool elastic = (c->kernel == INS_STVENANT || c->kernel == INS_NEOHOOKE);
     if (elastic && c->dpn != c->dim)

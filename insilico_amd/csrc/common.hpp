@@ -139,13 +139,19 @@ struct Ctx {
 
     // dofs
     int dpn = 0;
-    int64_t n_active = 0;
+    int64_t n_active = 0;      // rows of the system being solved
+    int64_t dof_n_active = 0;  // active DoFs of ins_set_dofs (ins_set_csr may
+                               // install a system of another size)
     bool monotone = true;
     DArr<int32_t> dof_id;   // n_nodes*dpn
     DArr<int32_t> rowmap;   // n_active: node*dpn + comp of each row
     DArr<double> g;         // prescribed
     DArr<double> u;         // current values
     bool have_dofs = false;
+
+    // the DoF-derived maps (dof_id, rowmap, the component order of the SELL
+    // slices) describe the system being solved
+    bool dofs_match() const { return have_dofs && n_active == dof_n_active; }
 
     // pattern
     bool have_pattern = false;

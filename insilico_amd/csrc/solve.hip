@@ -1713,8 +1713,10 @@ int build_sell(Ctx* c, bool force)
     // nodes (grouped by n-face class, then by support point), in which the
     // rows of a slice are neighbours of one kind with shared column offsets;
     // the face-based DoF numbering itself scatters them.
-    const bool ordered = c->have_order;
-    S.perm_on = c->dpn > 1 || ordered;
+    // (a CSR uploaded by ins_set_csr with another row count has no DoF
+    // structure: its slices keep the identity row order)
+    const bool ordered = c->have_order && c->dofs_match();
+    S.perm_on = c->dofs_match() && (c->dpn > 1 || ordered);
     const int32_t* brp = c->row_ptr.p;   // CSR the slices are built from
     const int32_t* bcol = c->col.p;
     int32_t *prp = nullptr, *pcol = nullptr, *pent = nullptr;
@@ -2047,6 +2049,10 @@ int prepare(Ctx* c, int mode, int format)
 int distribute(Ctx* c, int op)
 {
     if (!c->have_solution) return fail(c, INS_ERR_STATE, "ins_distribute before ins_solve");
+    if (!c->dofs_match())
+        return fail(c, INS_ERR_STATE,
+                    "ins_distribute: the solved system (ins_set_csr) does not have one row per "
+                    "active DoF");
     const int64_t nd = c->n_nodes * c->dpn;
     k_distribute<<<grid_for(nd, 256), 256, 0, c->stream>>>(nd, c->dof_id.p, c->x.p, c->g.p,
                                                             c->u.p, op == INS_DIST_ADD);

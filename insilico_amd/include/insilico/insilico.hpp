@@ -649,10 +649,12 @@ public:
         return v;
     }
 
-    // Eigen3::cgSolve (base/solver/Eigen3.hpp:265-275): Jacobi-PCG
-    // Eigen's ConjugateGradient stops after 2n iterations and returns its
-    // last iterate without an error (base/solver/Eigen3.hpp:265-275): so
-    // does this call when rtol is not reached (converged() tells)
+    // Eigen3::cgSolve (base/solver/Eigen3.hpp:265-275): Jacobi-PCG from
+    // x0 = 0 to |b - A x| <= rtol |b|, at most 2n iterations.  Eigen's
+    // ConjugateGradient hands back its last iterate silently when the limit
+    // is reached; this call throws insilico::Error(INS_ERR_NOCONV) instead,
+    // after storing that iterate (getValue, lastRelres and converged() stay
+    // readable), so that an unconverged solve is never taken for a solution.
     int cgSolve(double rtol = 1e-12, int mode = INS_SOLVE_FP64, int format = INS_FMT_SELL)
     {
         int it = 0;
@@ -660,20 +662,29 @@ public:
         const int maxIt = (int)std::max<std::size_t>(2 * n_, 100);
         const int st = ins_solve(ctx_, mode, format, rtol, maxIt, &it, &rel);
         converged_ = st == INS_OK;
+        lastRelres_ = rel;
+        lastIterations_ = it;
         if (st != INS_ERR_NOCONV) insilico::check(st, ctx_);
         x_.resize(n_);
         insilico::check(ins_get_solution(ctx_, x_.data()), ctx_);
         solved_ = true;
-        lastRelres_ = rel;
+        if (st == INS_ERR_NOCONV)
+            throw insilico::Error(st, std::string("cgSolve: ") + ins_last_error(ctx_) +
+                                          " (relres " + std::to_string(rel) + " > rtol " +
+                                          std::to_string(rtol) + ")");
         return it;
     }
-    // direct solves of the reference (choleskySolve/luSolve) have no device
-    // factorisation here: they run the same PCG to a tight tolerance
+    // Direct solves (Eigen3::choleskySolve / luSolve / superLUSolve,
+    // base/solver/Eigen3.hpp:157-230) return the solution to round-off or
+    // abort.  Here: the FP64 PCG to 1e-13 relative residual, which either
+    // meets that bound or throws INS_ERR_NOCONV (as above).
     void choleskySolve() { cgSolve(1e-13); }
     void luSolve() { cgSolve(1e-13); }
+    void superLUSolve() { cgSolve(1e-13); }
 
     double getValue(std::size_t i) const { return x_.at(i); }  // Eigen3.hpp:293-296
     double lastRelres() const { return lastRelres_; }
+    int lastIterations() const { return lastIterations_; }
     bool converged() const { return converged_; }
     std::size_t size() const { return n_; }
 
@@ -770,6 +781,7 @@ private:
     std::array<double, 8> fluxP_{0., 0., 0., 0., 0., 0., 0., 0.};
     std::vector<double> x_;
     double lastRelres_ = 0.;
+    int lastIterations_ = 0;
     bool converged_ = false;
 };
 

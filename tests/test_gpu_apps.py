@@ -40,6 +40,18 @@ def test_poisson_app(oracle):
     assert float(l2) < 1e-3
 
 
+def test_cg_solve_reports_no_convergence():
+    """A tolerance below what FP64 round-off reaches: the drop-in cgSolve
+    stops at its 2n iteration limit and reports INS_ERR_NOCONV (an exception
+    the driver turns into exit status 2) instead of handing back the last
+    iterate as a solution (Eigen's ConjugateGradient would return silently,
+    base/solver/Eigen3.hpp:265-275)."""
+    exe = os.path.join(APPS, "poisson")
+    out = subprocess.run([exe, "16", "1e-30"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 2, (out.returncode, out.stderr)
+    assert "error -7" in out.stderr and "relres" in out.stderr
+
+
 @pytest.mark.parametrize("dim,mesh,N", [(2, "quad.002.smf", 2), (2, "quad.005.smf", 5),
                                         (2, "quad.010.smf", 10), (3, "cube.002.smf", 2),
                                         (3, "cube.004.smf", 4), (3, "cube.008.smf", 8)])

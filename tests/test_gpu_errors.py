@@ -182,3 +182,37 @@ def test_new_entry_points_reject_bad_use():
         # a Q1 context after a Q2 one on the same handle works again
         c.load_problem(q1)
         c.assemble(abi.FP64)
+
+
+def test_host_csr_of_another_size_blocks_the_dof_paths(oracle):
+    """ins_set_csr installs a system that need not have the DoFs' size: the
+    element assembly (and with it the RHS and time-step kernels) and
+    ins_distribute then refuse with INS_ERR_STATE before any launch, the
+    uploaded system still solves, a vector problem's host CSR keeps the
+    identity row order of its slices, and ins_build_pattern brings the DoF
+    pattern back, after which the assembly equals the oracle's again."""
+    prob = P.elasticity3d(4, seed=5)
+    small = P.diffusion3d(3, seed=2)
+    srp, sci, sval, sb, _ = oracle.assemble(small)
+    with abi.Context(0) as c:
+        c.load_problem(prob)
+        assert small.n_active < prob.n_active
+        c.set_csr(srp, sci, sval)
+        assert _status(lambda: c.assemble(abi.FP64)) == ERR["STATE"]
+        assert _status(lambda: c.solve(abi.SOLVE_FP64, abi.FMT_CSR, 1e-8)) == ERR["STATE"]  # no rhs
+        c.set_rhs(sb)
+        for mode, fmt in ((abi.SOLVE_FP64, abi.FMT_CSR), (abi.SOLVE_MIXED, abi.FMT_SELL)):
+            _, rel, st = c.solve(mode, fmt, 1e-8)
+            assert st == abi.INS_OK and rel <= 1e-8
+            x = c.get_solution()
+            r = sb - oracle.spmv(srp, sci, sval, x)
+            assert np.linalg.norm(r) <= 1.001e-8 * np.linalg.norm(sb)
+        assert _status(lambda: c.distribute(abi.DIST_SET)) == ERR["STATE"]
+        c.build_pattern()
+        c.assemble(abi.FP64)
+        rp, ci = c.get_pattern()
+        val, b = c.get_csr_values(), c.get_rhs()
+    orp, oci, oval, ob, _ = oracle.assemble(prob)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert np.max(np.abs(val - oval)) <= 1e-12 * np.max(np.abs(oval))
+    assert np.max(np.abs(b - ob)) <= 1e-12 * np.max(np.abs(ob))
