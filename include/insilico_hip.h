@@ -118,6 +118,9 @@ int ins_destroy(ins_ctx* ctx);
 const char* ins_strerror(int status);
 const char* ins_last_error(const ins_ctx* ctx);
 int ins_get_stats(const ins_ctx* ctx, ins_stats* out);
+/* Free and total device memory of GPU `device` (hipMemGetInfo), for
+ * admission by free HBM.  No reference counterpart. */
+int ins_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 /* time every dominant-kernel launch with HIP events (adds small overhead) */
 int ins_set_profiling(ins_ctx* ctx, int on);
 int ins_sync(ins_ctx* ctx);
@@ -335,7 +338,11 @@ int ins_server_step(ins_server* srv, int k, int* n_finished);
 int ins_server_poll(ins_server* srv, int slot, int* state, int* status,
                     int* iters, double* relres);
 /* Free the slot.  A finished request's context then holds its solution
- * (ins_get_solution); detaching a running request cancels it. */
+ * (ins_get_solution); detaching a running request cancels it.  The chunk
+ * still queued on the server stream may name the context: its next call
+ * (any ins_* on it, ins_destroy included) first waits for the server
+ * stream's work queued up to the detach, so it can be reused or destroyed
+ * at once. */
 int ins_server_detach(ins_server* srv, int slot);
 int ins_server_running(const ins_server* srv, int* n_running);
 

@@ -2,6 +2,10 @@
 // uneven sizes, admitted by free device memory, solved by continuous batching
 // on one GPU (ins_server_*), with arrival-to-result latency per request.
 //
+// Admission is first come first served while the device reports enough free
+// HBM (hipMemGetInfo through ins_device_memory, less a margin) for the
+// request's estimated footprint, within an optional budget.
+//
 // Requests: 50/50 diffusion (-div(k grad u) = 1, k_e = exp(g_e), g_e ~ N(0,1),
 // u = 0 on x = 0) and linear elasticity (E_e ~ U[0.5, 2], nu = 0.3, clamped
 // z = 0, traction (0, 0, -0.01) on z = 1) on n^3 Q1 hex meshes with
@@ -294,9 +298,12 @@ struct Opts {
 };
 
 // Runs the specs (arrival times set) through the server; returns the makespan.
+// device memory left to the runtime and the server when admitting
+const double FREE_MARGIN = 4e9;
+
 double run_stream(const Opts& o, std::vector<Spec>& specs, std::vector<Result>& res,
                   const std::vector<char>& keep, uint64_t budget, const char* phase,
-                  int* max_resident, double* max_bytes)
+                  int* max_resident, double* max_bytes, double* min_free)
 {
     ins_server* srv = nullptr;
     {
@@ -346,7 +353,17 @@ double run_stream(const Opts& o, std::vector<Spec>& specs, std::vector<Result>& 
                 if (!ctxs[i].busy && (k < 0 || ctxs[i].held > ctxs[k].held)) k = i;
             if (k < 0) break;
             freeable = ctxs[k].held;
-            if (held_total() - freeable + reserved + s.est_bytes > (double)budget) {
+            // free HBM as the device reports it (the context about to be
+            // reused gives its buffers back), less the estimates of the
+            // requests still in setup and a margin for the runtime
+            uint64_t fr = 0, tot = 0;
+            {
+                const int st = ins_device_memory(0, &fr, &tot);
+                if (st != INS_OK) die("ins_device_memory", st, ins_strerror(st));
+            }
+            *min_free = std::min(*min_free, (double)fr);
+            const bool no_room = (double)fr + freeable - reserved - s.est_bytes < FREE_MARGIN;
+            if (no_room || held_total() - freeable + reserved + s.est_bytes > (double)budget) {
                 // give memory of other idle contexts back before waiting
                 bool freed = false;
                 for (int i = 0; i < o.slots; i++)
@@ -502,8 +519,9 @@ int main(int argc, char** argv)
     for (auto& s : cal) s.t_arrival = 0.;
     std::vector<Result> rc;
     int mr = 0;
-    double mb = 0.;
-    const double t_cal = run_stream(o, cal, rc, std::vector<char>(nc, 0), budget, "capacity", &mr, &mb);
+    double mb = 0., mf = 1e300;
+    const double t_cal = run_stream(o, cal, rc, std::vector<char>(nc, 0), budget, "capacity", &mr,
+                                    &mb, &mf);
     const double capacity = nc / t_cal;
     const double rate = o.load * capacity;
 
@@ -523,10 +541,11 @@ int main(int argc, char** argv)
     }
     std::vector<Result> res;
     int max_res = 0;
-    double max_bytes = 0.;
+    double max_bytes = 0., min_free = 1e300;
     std::fprintf(stderr, "[serve] capacity %.2f req/s from %d requests; stream of %d at %.2f req/s\n",
                  capacity, nc, o.requests, rate);
-    const double makespan = run_stream(o, specs, res, keep, budget, "stream", &max_res, &max_bytes);
+    const double makespan = run_stream(o, specs, res, keep, budget, "stream", &max_res, &max_bytes,
+                                       &min_free);
 
     // (3) the sampled requests alone through ins_solve: bit-identical?
     int equal = 0;
@@ -595,14 +614,17 @@ int main(int argc, char** argv)
                 "\"makespan_s\": %.3f, \"throughput_req_per_s\": %.3f, \"throughput_dof_per_s\": %.1f, "
                 "\"p50_latency_ms\": %.2f, \"p99_latency_ms\": %.2f, \"mean_latency_ms\": %.2f, "
                 "\"p50_queue_wait_ms\": %.2f, \"p99_queue_wait_ms\": %.2f, \"p50_setup_ms\": %.2f, "
-                "\"max_resident\": %d, \"max_device_gb\": %.2f, \"budget_gb\": %.1f, \"slots\": %d, "
+                "\"max_resident\": %d, \"max_device_gb\": %.2f, \"budget_gb\": %.1f, "
+                "\"admission\": \"free HBM (hipMemGetInfo) less %.0f GB margin, and the budget\", "
+                "\"min_free_gb\": %.2f, \"slots\": %d, "
                 "\"converged\": %d, \"worst_status\": %d, \"max_relres\": %.3e, "
                 "\"alone_vs_mix_checked\": %zu, \"alone_vs_mix_bitwise_equal\": %d}\n",
                 o.requests, n_diff, o.requests - n_diff, o.nmin, o.nmax, o.rtol, capacity, o.load,
                 rate, makespan, o.requests / makespan, dofs / makespan, 1e3 * pct(lat, 50),
                 1e3 * pct(lat, 99), 1e3 * (lat.empty() ? 0. : [&] { double s = 0; for (double v : lat) s += v; return s / lat.size(); }()),
                 1e3 * pct(wait, 50), 1e3 * pct(wait, 99), 1e3 * pct(setup_t, 50), max_res,
-                max_bytes / 1e9, budget / 1e9, o.slots, ok, worst_status, worst_rel, check.size(),
+                max_bytes / 1e9, budget / 1e9, FREE_MARGIN / 1e9, min_free / 1e9, o.slots, ok,
+                worst_status, worst_rel, check.size(),
                 equal);
     return 0;
 }

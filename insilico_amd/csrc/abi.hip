@@ -79,6 +79,17 @@ static void release_all(Ctx* c)
     release(c, c->pinfo);
 }
 
+// A context detached from a server may still be named by the batched work
+// queued on the server stream before the detach: wait for that work before
+// the context's buffers are reused, freed or read
+int wait_server_release(Ctx* c)
+{
+    if (!c->srv_pending) return INS_OK;
+    c->srv_pending = false;
+    INS_HIP_CHECK(c, hipEventSynchronize(c->srv_ev));
+    return INS_OK;
+}
+
 static void invalidate_pattern(Ctx* c)
 {
     c->have_pattern = false;
@@ -98,6 +109,7 @@ using namespace ins;
         hipError_t de_ = hipSetDevice((ctx)->device);             \
         if (de_ != hipSuccess)                                    \
             return fail((ctx), INS_ERR_HIP, "hipSetDevice failed"); \
+        INS_TRY(::ins::wait_server_release(ctx));                 \
     } while (0)
 
 extern "C" {
@@ -143,8 +155,10 @@ int ins_destroy(ins_ctx* h)
     Ctx* c = reinterpret_cast<Ctx*>(h);
     if (!c) return INS_ERR_ARG;
     hipSetDevice(c->device);
+    (void)wait_server_release(c);
     hipStreamSynchronize(c->stream);
     release_all(c);
+    if (c->srv_ev) hipEventDestroy(c->srv_ev);
     if (c->h_scal) hipHostFree(c->h_scal);
     if (c->h_snap) hipHostFree(c->h_snap);
     for (int i = 0; i < 2; i++)
@@ -180,6 +194,19 @@ const char* ins_last_error(const ins_ctx* h)
 {
     const Ctx* c = reinterpret_cast<const Ctx*>(h);
     return c ? c->err.c_str() : "null context";
+}
+
+int ins_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return INS_ERR_HIP;
+    if (device < 0 || device >= ndev || !free_bytes || !total_bytes) return INS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return INS_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return INS_ERR_HIP;
+    *free_bytes = f;
+    *total_bytes = t;
+    return INS_OK;
 }
 
 int ins_get_stats(const ins_ctx* h, ins_stats* out)

@@ -19,6 +19,7 @@ namespace ins {
 // status helpers
 struct Ctx;
 int fail(Ctx* c, int code, const std::string& msg);
+int wait_server_release(Ctx* c);
 
 #define INS_HIP_CHECK(ctx, call)                                              \
     do {                                                                      \
@@ -244,6 +245,11 @@ struct Ctx {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_el[2] = {nullptr, nullptr};  // profiled element kernels
+    // set by ins_server_detach: the server stream's work queued up to the
+    // detach, which may still name this context's buffers; every later call
+    // on the context waits for it before touching device memory
+    hipEvent_t srv_ev = nullptr;
+    bool srv_pending = false;
     // profiling: one event pair per SpMV launch of a solve (grown on demand)
     std::vector<hipEvent_t> prof_ev;
     size_t prof_n = 0;

@@ -1431,15 +1431,17 @@ int solve_vectors_out(Ctx* c, hipStream_t st)
 }
 
 // lanes per row of the CSR kernels, from the mean row length
-int pick_lpr(Ctx* c)
+int lpr_for(int64_t n, int64_t nnz)
 {
     static const int forced = env_int("INS_CSR_LPR", 0);
     if (forced == 4 || forced == 8 || forced == 16 || forced == 32) return forced;
-    const double avg = c->n_active ? (double)c->nnz / (double)c->n_active : 1.;
+    const double avg = n ? (double)nnz / (double)n : 1.;
     if (avg > 48) return 16;
     if (avg > 12) return 8;
     return 4;
 }
+
+int pick_lpr(Ctx* c) { return lpr_for(c->n_active, c->nnz); }
 
 // K1; returns the number of partials written
 // Launch of the dominant kernel on the context stream.  With profiling on,
@@ -2605,6 +2607,13 @@ int server_detach(Server* sv, int s)
     if (!sv || s < 0 || s >= sv->max_slots) return INS_ERR_ARG;
     SlotH& h = sv->slots[s];
     if (h.state == INS_SLOT_FREE) return sfail(sv, INS_ERR_STATE, "slot not in use");
+    // the chunk queued behind the one that showed the request finished still
+    // has the request in its table: the context's next call waits for the
+    // server stream's work up to here before its buffers change hands
+    if (!h.c->srv_ev)
+        SRV_HIP(sv, hipEventCreateWithFlags(&h.c->srv_ev, hipEventDisableTiming));
+    SRV_HIP(sv, hipEventRecord(h.c->srv_ev, sv->stream));
+    h.c->srv_pending = true;
     if (h.state == INS_SLOT_RUNNING) {
         // cancel: nothing of this request may still be queued when it leaves
         h.state = INS_SLOT_FREE;
@@ -2621,5 +2630,6 @@ int server_detach(Server* sv, int s)
 }
 
 int server_running(Server* sv) { return sv ? sv->nrun : 0; }
+
 
 }  // namespace ins

@@ -161,9 +161,18 @@ def boundary_faces(conn, dim):
     table = HEX_FACES if dim == 3 else QUAD_EDGES
     nf = table.shape[0]
     fn = conn[:, table].reshape(-1, table.shape[1])          # (E*nf, nfn)
-    key = np.sort(fn, axis=1)
-    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
-    on = cnt[inv.ravel()] == 1
+    # a face is on the boundary when its sorted vertex ids occur once; the
+    # sorted ids packed two per 64-bit key, the keys sorted together
+    key = np.sort(fn, axis=1).astype(np.uint64)
+    k1 = (key[:, 0] << np.uint64(32)) | key[:, 1]
+    k2 = ((key[:, 2] << np.uint64(32)) | key[:, 3]) if key.shape[1] == 4 else np.zeros_like(k1)
+    order = np.lexsort((k2, k1))
+    s1, s2 = k1[order], k2[order]
+    new = np.ones(order.shape[0], bool)
+    new[1:] = (s1[1:] != s1[:-1]) | (s2[1:] != s2[:-1])
+    grp = np.cumsum(new) - 1
+    on = np.empty(order.shape[0], bool)
+    on[order] = np.bincount(grp)[grp] == 1
     idx = np.nonzero(on)[0]
     return idx // nf, idx % nf, fn[idx].astype(np.int32)
 
