@@ -306,6 +306,29 @@ int ins_error_norm(ins_ctx* ctx, int order, int ref_kind, const double* ref_para
 int ins_cell_eval(ins_ctx* ctx, int quantity, const double* xi, const double* u,
                   double* out);
 
+/* ---- allocation-free solve ------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The FP64 Jacobi-PCG of ins_solve(INS_SOLVE_FP64,
+ * INS_FMT_CSR) -- same kernels, grids and reduction order, so the same
+ * iterates -- on a CSR system held in caller-owned DEVICE arrays of GPU
+ * `device`: row_ptr (n+1, row_ptr[n] = nnz), col_idx (nnz, ascending per
+ * row), val (nnz), b (n) are read, x (n) receives the solution; stream is a
+ * hipStream_t (NULL = the default stream) and the call returns after the
+ * stream has finished the solve.  Its scratch (scalars, block partials,
+ * Jacobi diagonal, iterates r, p, q) is a static device array of
+ * 32 MiB + 80 KiB that each GPU the call runs on holds until the process
+ * ends; the call allocates nothing.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race.  n > INS_NOALLOC_MAX_ROWS,
+ * null pointers, rtol <= 0 or row_ptr[n] != nnz give INS_ERR_ARG; otherwise
+ * the statuses of ins_solve (INS_ERR_NOCONV with x the last iterate,
+ * INS_ERR_BREAKDOWN). */
+#define INS_NOALLOC_MAX_ROWS (1 << 20)
+int ins_solve_csr_noalloc(int device, void* stream, int64_t n, int64_t nnz,
+                          const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                          const double* b, double* x, double rtol, int max_it, int* iters,
+                          double* relres);
+
 /* ---- serving: many solves batched into shared launches --------------------
  * No single reference counterpart: the reference solves one system per
  * Eigen3::cgSolve call (base/solver/Eigen3.hpp:265-275).  A server runs the

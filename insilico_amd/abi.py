@@ -36,7 +36,7 @@ SYMBOLS = [
     "ins_server_attach", "ins_server_step", "ins_server_poll", "ins_server_detach",
     "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
     "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile", "ins_cell_eval",
-    "ins_set_neumann_flux", "ins_device_memory",
+    "ins_set_neumann_flux", "ins_device_memory", "ins_solve_csr_noalloc",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 REF_ZERO, REF_SINPROD, REF_KELVIN, REF_COSPROD = 0, 1, 2, 3

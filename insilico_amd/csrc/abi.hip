@@ -872,6 +872,15 @@ int ins_cell_eval(ins_ctx* h, int quantity, const double* xi, const double* u, d
     return cell_eval(c, quantity, xi, u, out);
 }
 
+int ins_solve_csr_noalloc(int device, void* stream, int64_t n, int64_t nnz,
+                          const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                          const double* b, double* x, double rtol, int max_it, int* iters,
+                          double* relres)
+{
+    return solve_csr_noalloc(device, stream, n, nnz, row_ptr, col_idx, val, b, x, rtol, max_it,
+                             iters, relres);
+}
+
 int ins_bdf_init(ins_ctx* h, int order, double density, const double* u0)
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);

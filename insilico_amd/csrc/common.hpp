@@ -346,5 +346,8 @@ int distribute(Ctx* c, int op);
 int bdf_step(Ctx* c, double dt, int precision, int mode, int format, double rtol, int max_it,
              int* iters, double* relres);
 int vec_norm2(Ctx* c, const double* v, int64_t n, double* out);
+int solve_csr_noalloc(int device, void* stream, int64_t n, int64_t nnz, const int32_t* rp,
+                      const int32_t* col, const double* val, const double* b, double* x,
+                      double rtol, int max_it, int* iters, double* relres);
 
 }  // namespace ins

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <climits>
 #include <deque>
+#include <mutex>
 #include <cmath>
 #include <cstdlib>
 
@@ -2631,5 +2632,154 @@ int server_detach(Server* sv, int s)
 
 int server_running(Server* sv) { return sv ? sv->nrun : 0; }
 
+// ===========================================================================
+// Allocation-free FP64 CSR solve (ins_solve_csr_noalloc).
+//
+// The Jacobi-PCG of ins_solve(INS_SOLVE_FP64, INS_FMT_CSR) on a CSR system in
+// caller-owned device arrays, with the same kernels, grids and reduction
+// order (the iterates are the bits ins_solve computes for that system).  Its
+// scratch -- the device scalars, the block partials, the Jacobi diagonal and
+// the iterates r, p, q -- lives in `scratch`, a static device array of fixed
+// size: the call allocates nothing (no hipMalloc, hipFree, pinned host
+// memory or events).  Every GPU holds its own copy of the array from the
+// first call on it until the process ends, and every call on that GPU uses
+// it: calls that overlap on one GPU race.  Nothing in it has to start at
+// zero (each quantity is written before it is read).
+constexpr size_t NA_SCAL = 256;                                  // Scal, padded
+constexpr size_t NA_PART = (size_t)NSLOT * MAXG * sizeof(double);
+constexpr size_t NA_HEAD = NA_SCAL + NA_PART;
+constexpr size_t NA_BYTES = NA_HEAD + (size_t)4 * INS_NOALLOC_MAX_ROWS * sizeof(double);
+static_assert(sizeof(Scal) <= NA_SCAL, "scalars fit their slot");
+static_assert(NA_PART % 256 == 0, "diagonal and iterates stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+
+constexpr int NA_MAXDEV = 64;
+std::once_flag na_once[NA_MAXDEV];
+void* na_ptr[NA_MAXDEV];
+hipError_t na_rc[NA_MAXDEV];
+
+// Jacobi diagonal 1 / a_rr of a CSR with ascending columns (1 where the
+// diagonal is absent or zero), as the context's assembly computes it
+__global__ void k_na_dinv(int64_t n, const int32_t* __restrict__ row_ptr,
+                          const int32_t* __restrict__ col, const double* __restrict__ val,
+                          double* __restrict__ d)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = row_ptr[r], hi = row_ptr[r + 1];
+        const int32_t end = hi;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (col[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        const double a = (lo < end && col[lo] == r) ? val[lo] : 0.;
+        d[r] = a != 0. ? 1. / a : 1.;
+    }
+}
+
+template <int LPR>
+void na_spmv(hipStream_t st, unsigned g, int64_t n, const int32_t* rp, const int32_t* col,
+             const double* val, const double* p, double* q, double* part, const Scal* sc, int it,
+             const Rep<double>& R)
+{
+    k_spmv_csr<LPR, double, double, double, double><<<g, SB, 0, st>>>(n, rp, col, val, p, q, part,
+                                                                       sc, it, R);
+}
+
+template <int LPR>
+void na_resid(hipStream_t st, unsigned g, int64_t n, const int32_t* rp, const int32_t* col,
+              const double* val, const double* x, const double* b, double* part)
+{
+    k_resid_csr<LPR, double><<<g, SB, 0, st>>>(n, rp, col, val, x, b, part);
+}
+
+int na_read(hipStream_t st, const Scal* dsc, Scal* h)
+{
+    if (hipMemcpyAsync(h, dsc, sizeof(Scal), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return INS_ERR_HIP;
+    return INS_OK;
+}
+
+}  // namespace
+
+int solve_csr_noalloc(int device, void* stream, int64_t n, int64_t nnz, const int32_t* rp,
+                      const int32_t* col, const double* val, const double* b, double* x,
+                      double rtol, int max_it, int* iters, double* relres)
+{
+    if (n <= 0 || n > INS_NOALLOC_MAX_ROWS || nnz < n || nnz > INT32_MAX || !rp || !col ||
+        !val || !b || !x || !(rtol > 0.) || max_it < 0 || device < 0 || device >= NA_MAXDEV)
+        return INS_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev) return INS_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return INS_ERR_HIP;
+    // the device address of this GPU's copy of the scratch array, resolved
+    // once per device with that device current
+    std::call_once(na_once[device], [device] {
+        na_rc[device] = hipGetSymbolAddress(&na_ptr[device], HIP_SYMBOL(scratch));
+    });
+    if (na_rc[device] != hipSuccess || !na_ptr[device]) return INS_ERR_HIP;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the CSR must end at nnz (a 4-byte read of row_ptr[n])
+    int32_t last = -1;
+    if (hipMemcpyAsync(&last, rp + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return INS_ERR_HIP;
+    if ((int64_t)last != nnz) return INS_ERR_ARG;
+
+    unsigned char* base = static_cast<unsigned char*>(na_ptr[device]);
+    Scal* sc = reinterpret_cast<Scal*>(base);
+    double* part = reinterpret_cast<double*>(base + NA_SCAL);
+    double* dinv = reinterpret_cast<double*>(base + NA_HEAD);
+    double* r = dinv + n;
+    double* p = r + n;
+    double* q = p + n;
+
+    const unsigned gv = vec_grid(n);
+    const int lpr = lpr_for(n, nnz);
+    const unsigned g1 = csr_grid(n, lpr);
+    const Rep<double> R{val, x, b, nullptr, nullptr};   // FP64: no replacement
+    k_na_dinv<<<grid_for(n, 256), 256, 0, st>>>(n, rp, col, val, dinv);
+    k_init<double, double><<<gv, VB, 0, st>>>(n, b, x, r, p, dinv, part);
+    k_init_fin<<<1, VB, 0, st>>>(part, (int)gv, rtol * rtol, 0., sc);
+    if (hipGetLastError() != hipSuccess) return INS_ERR_HIP;
+    Scal S;
+    INS_TRY(na_read(st, sc, &S));
+    int enq = 0;
+    while (!S.done && !S.stop && enq < max_it) {
+        const int k = std::min(CHUNK, max_it - enq);
+        for (int i = 0; i < k; i++) {
+            const int it = enq + i;
+            if (lpr == 32) na_spmv<32>(st, g1, n, rp, col, val, p, q, part, sc, it, R);
+            else if (lpr == 16) na_spmv<16>(st, g1, n, rp, col, val, p, q, part, sc, it, R);
+            else if (lpr == 8) na_spmv<8>(st, g1, n, rp, col, val, p, q, part, sc, it, R);
+            else na_spmv<4>(st, g1, n, rp, col, val, p, q, part, sc, it, R);
+            k_update_xr<double, double><<<gv, VB, 0, st>>>(n, x, r, p, q, dinv, part, (int)g1,
+                                                            sc, it);
+            k_update_p<double, double><<<gv, VB, 0, st>>>(n, p, r, dinv, part, (int)gv, sc, it, x,
+                                                           x);
+        }
+        if (hipGetLastError() != hipSuccess) return INS_ERR_HIP;
+        enq += k;
+        INS_TRY(na_read(st, sc, &S));
+    }
+    // |b - A x| in FP64 over the same rows
+    if (lpr == 32) na_resid<32>(st, g1, n, rp, col, val, x, b, part);
+    else if (lpr == 16) na_resid<16>(st, g1, n, rp, col, val, x, b, part);
+    else if (lpr == 8) na_resid<8>(st, g1, n, rp, col, val, x, b, part);
+    else na_resid<4>(st, g1, n, rp, col, val, x, b, part);
+    k_resid_fin<<<1, VB, 0, st>>>(part, (int)g1, sc);
+    if (hipGetLastError() != hipSuccess) return INS_ERR_HIP;
+    INS_TRY(na_read(st, sc, &S));
+    const double rel = S.bb > 0. ? std::sqrt(S.tt / S.bb) : 0.;
+    if (iters) *iters = S.iters;
+    if (relres) *relres = rel;
+    if (S.breakdown) return INS_ERR_BREAKDOWN;
+    if (!(rel <= rtol)) return INS_ERR_NOCONV;
+    return INS_OK;
+}
 
 }  // namespace ins
