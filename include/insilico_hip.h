@@ -108,6 +108,10 @@ typedef struct ins_stats {
     int64_t rep_calls;
     double  element_ms;     /* profiled: element kernels of the last
                                ins_assemble (integration only, no gather) */
+    int64_t sym_values;     /* values of the symmetric slices (0 = none)     */
+    int     sym_block;      /* their block size: 1 = scalar rows, 2 or 3 =
+                               node blocks of a vector problem (offsets in
+                               sell_sym_offsets are then node offsets)        */
 } ins_stats;
 
 /* ---- context ------------------------------------------------------------ */

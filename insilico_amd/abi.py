@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("spmv_calls", C.c_int64), ("sell_values", C.c_int64),
                 ("sell_patterns", C.c_int), ("sell_sym_offsets", C.c_int),
                 ("rep_ms", C.c_double), ("rep_calls", C.c_int64),
-                ("element_ms", C.c_double)]
+                ("element_ms", C.c_double), ("sym_values", C.c_int64),
+                ("sym_block", C.c_int)]
 
 
 class InsError(RuntimeError):

@@ -219,7 +219,10 @@ int ins_get_stats(const ins_ctx* h, ins_stats* out)
     out->device_bytes = c->held;
     out->sell_values = (c->sell.built && c->sell.ok) ? c->sell.nvals : 0;
     out->sell_patterns = (c->sell.built && c->sell.ok) ? c->sell.npat : 0;
-    out->sell_sym_offsets = (c->sell.built && c->sell.ok && c->sell.sym_on) ? c->sell.nu : 0;
+    const bool sym = c->sell.built && c->sell.ok && c->sell.sym_on;
+    out->sell_sym_offsets = sym ? c->sell.nu : 0;
+    out->sym_values = sym ? c->sell.nvs : 0;
+    out->sym_block = sym ? c->sell.sdpn : 0;
     return INS_OK;
 }
 

@@ -106,6 +106,17 @@ struct Sell {
     DArr<float> vs;
     DArr<int32_t> smap;      // nnz: vs position of every CSR entry (-1 below the diagonal)
     int64_t nvs = 0;
+    // Vector problems whose nodes carry all or none of their dpn components,
+    // rows interleaved per node: the same idea on dpn x dpn node blocks
+    // (sdpn = dpn).  su lists the node offsets u_k >= 0 shared by all
+    // nodes; node a stores its blocks (a, a + u_k), the diagonal block as its
+    // upper triangle, at (a / 64) * 64 * nbv + slot * 64 + a % 64 with
+    // nbv = dpn (dpn + 1) / 2 + (nu - 1) dpn^2 values per node; the block
+    // (a, a - u_k) is read transposed at node a - u_k.  The FP32 sweeps then
+    // run on the component-major vectors of the full slices (component c of
+    // node a at c * snodes + a).  sdpn = 1: the scalar slices above.
+    int sdpn = 1;
+    int64_t snodes = 0;
 };
 
 struct Ctx {

@@ -246,6 +246,8 @@ struct SymView {
     const float* v;     // null: the full slices
     const int32_t* u;   // nu offsets, ascending (wave-uniform, scalar loads)
     int nu;
+    int dpn;            // 1: scalar rows; 2, 3: node blocks (see bsym_node)
+    int64_t nn;         // node blocks: nodes per component
 };
 
 template <typename TX, typename TA, int G = 8>
@@ -317,6 +319,123 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
     return s;
 }
 
+// Symmetric node-block slices (vector problems, one lane per node of a
+// 64-node slice): out[i] = sum over the node's D x D blocks of the rows of
+// its D components.  Blocks (a, a + u_k), u_k >= 0, are stored at node a,
+// k-major and lane contiguous: slot 0 the upper triangle of the diagonal
+// block (i <= j, row-major), slot T + (k - 1) D^2 + i D + j the entry (i, j)
+// of block k >= 1.  Block (a, a - u_k) is block (a - u_k, a) transposed, read
+// at node a - u_k.  Vectors are component-major: component c of node b at
+// c * nn + b.  Offsets are taken G at a time so that G blocks of loads are
+// in flight per wave.
+template <int D>
+__device__ inline constexpr int tri_slot(int i, int j)
+{
+    return i * D - i * (i - 1) / 2 + (j - i);
+}
+
+template <int D, typename TX, typename TA, int G = 3>
+__device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, const SymView& Y,
+                                 const TX* __restrict__ x, TA (&out)[D])
+{
+    constexpr int T = D * (D + 1) / 2, D2 = D * D;
+    const int nu = Y.nu;
+    const int64_t nn = Y.nn, last = nn - 1;
+    const int64_t ld = (int64_t)64 * (T + (nu - 1) * D2);
+#pragma unroll
+    for (int i = 0; i < D; i++) out[i] = (TA)0;
+    if (!live) return;
+    const float* __restrict__ own = Y.v + slice * ld + lane;
+    {
+        float bv[T];
+        TX xa[D];
+#pragma unroll
+        for (int e = 0; e < T; e++) bv[e] = own[(int64_t)e * 64];
+#pragma unroll
+        for (int j = 0; j < D; j++) xa[j] = x[j * nn + a];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++)
+                out[i] += (TA)bv[i <= j ? tri_slot<D>(i, j) : tri_slot<D>(j, i)] * (TA)xa[j];
+    }
+    for (int k0 = 1; k0 < nu; k0 += G) {
+        float bu[G][D2], bl[G][D2];
+        TX xu[G][D], xl[G][D];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int k = k0 + g;
+#pragma unroll
+            for (int e = 0; e < D2; e++) {
+                bu[g][e] = 0.f;
+                bl[g][e] = 0.f;
+            }
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                xu[g][j] = (TX)0;
+                xl[g][j] = (TX)0;
+            }
+            if (k < nu) {
+                const int d = Y.u[k];
+                const int64_t slot = T + (int64_t)(k - 1) * D2;
+                int64_t bn = a + d;
+                bn = bn > last ? last : bn;   // padded blocks hold zeros
+#pragma unroll
+                for (int e = 0; e < D2; e++) bu[g][e] = own[(slot + e) * 64];
+#pragma unroll
+                for (int j = 0; j < D; j++) xu[g][j] = x[j * nn + bn];
+                const int64_t m = a - d;
+                if (m >= 0) {
+                    const float* __restrict__ mp = Y.v + (m >> 6) * ld + slot * 64 + (m & 63);
+#pragma unroll
+                    for (int e = 0; e < D2; e++) bl[g][e] = mp[(int64_t)e * 64];
+#pragma unroll
+                    for (int j = 0; j < D; j++) xl[g][j] = x[j * nn + m];
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) out[i] += (TA)bu[g][i * D + j] * (TA)xu[g][j];
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) out[i] += (TA)bl[g][j * D + i] * (TA)xl[g][j];
+        }
+    }
+}
+
+// q = A p on the node blocks: every lane writes the D rows of its node and
+// adds their p.q to the block partial
+template <int D, typename TX, typename TA>
+__device__ inline double bsym_sweep(Blk b, const SymView& Y, const TX* __restrict__ p,
+                                    TX* __restrict__ q)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nn = Y.nn;
+    const int64_t nsl = (nn + 63) / 64;
+    const Sweep w = xcd_sweep(nsl, SB / 64, b, true);
+    double pq = 0.;
+    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+        const int64_t a = slice * 64 + lane;
+        const bool live = a < nn;
+        TA out[D];
+        bsym_node<D, TX, TA>(live ? a : 0, live, lane, slice, Y, p, out);
+        if (live) {
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                const TX qs = (TX)out[i];
+                st(&q[i * nn + a], qs, g_nt);
+                pq += (double)p[i * nn + a] * (double)qs;
+            }
+        }
+    }
+    return pq;
+}
+
 // ---------------------------------------------------------------------------
 // Operands of the residual replacement of mixed precision, carried by K1:
 // t = b - A x64 with the FP64 (or, after an FP32 assembly, FP32) values in
@@ -368,6 +487,15 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
         return;
     }
     if (sc->done || sc->stop) return;
+    if constexpr (sizeof(TV) == 4 && SYM != 0) {
+        if (Y.v && Y.dpn > 1) {
+            double v[1] = {Y.dpn == 3 ? bsym_sweep<3, TX, TA>(b, Y, p, q)
+                                      : bsym_sweep<2, TX, TA>(b, Y, p, q)};
+            const int sl[1] = {P_PQ};
+            block_partials<1>(v, part, sl, b);
+            return;
+        }
+    }
     // the symmetric slices sweep per-XCD ranges: a mirror read then finds
     // the slice it needs in the same XCD's L2
     const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
@@ -1230,6 +1358,47 @@ __global__ void k_sym_map(int64_t n, const int32_t* __restrict__ rp, const int32
     }
 }
 
+// slice position of every CSR entry in the node-block slices (-1: below
+// the diagonal, read at its mirror); rows interleaved, r = a * D + i
+__global__ void k_bsym_map(int64_t n, int D, const int32_t* __restrict__ rp,
+                           const int32_t* __restrict__ col, const int32_t* __restrict__ su, int nu,
+                           int64_t nbv, int32_t* __restrict__ smap, int* err)
+{
+    const int T = D * (D + 1) / 2;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = r / D;
+        const int i = (int)(r - a * D);
+        for (int32_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int64_t bn = col[e] / D;
+            const int j = (int)(col[e] - bn * D);
+            const int64_t d = bn - a;
+            if (d < 0 || (d == 0 && j < i)) {
+                smap[e] = -1;
+                continue;
+            }
+            int lo = 0, hi = nu - 1, k = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (su[mid] == d) {
+                    k = mid;
+                    break;
+                }
+                if (su[mid] < d) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            if (k < 0) {
+                smap[e] = -1;
+                atomicOr(err, 1);
+                continue;
+            }
+            const int64_t slot = k == 0 ? (int64_t)(i * D - i * (i - 1) / 2 + (j - i))
+                                        : (int64_t)T + (int64_t)(k - 1) * D * D + i * D + j;
+            smap[e] = (int32_t)((a >> 6) * 64 * nbv + slot * 64 + (a & 63));
+        }
+    }
+}
+
 __global__ void k_sym_scatter(int64_t nnz, const int32_t* __restrict__ smap,
                               const float* __restrict__ v32, float* __restrict__ vs)
 {
@@ -1467,6 +1636,32 @@ int launch_timed(Ctx* c, F kernel, unsigned grid, unsigned block, Args... args)
     return INS_OK;
 }
 
+// The FP32 sweeps of a SELL solve use the symmetric slices when they were
+// built for the vectors in use: the scalar slices in the identity order, the
+// node-block slices on the component-major vectors
+bool use_sym(const Ctx* c, bool permuted)
+{
+    return c->sell.sym_on && (c->sell.sdpn > 1) == permuted;
+}
+
+SymView sym_view(const Ctx* c)
+{
+    return SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu, c->sell.sdpn, c->sell.snodes};
+}
+
+// Blocks of the SpMV on the full slices (sym = false) or on the symmetric
+// slices, the same for a solve alone and inside a server batch, so that a
+// request's partial sums, and with them its iterates, do not depend on where
+// it runs
+unsigned spmv_grid(const Ctx* c, bool sym)
+{
+    const int64_t n = c->n_active;
+    if (!sym) return sell_grid(n);
+    static const int forced = env_int("INS_SYM_GRID", 0);
+    if (forced >= 8) return round8(std::min(forced, MAXG));
+    return sym_grid(c->sell.sdpn > 1 ? c->sell.snodes : n, c->n_cu);
+}
+
 template <typename TV, typename TX, typename TA, typename TVR>
 int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
                   int* g_out, int it, const TVR* rep_val)
@@ -1475,13 +1670,10 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
     const Scal* sc = c->scal.p;
     const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32};
     if (format == INS_FMT_SELL) {
-        unsigned g = sell_grid(n);
-        SymView Y{nullptr, nullptr, 0};
-        if (sizeof(TV) == 4 && c->sell.sym_on && !c->sperm) {
-            Y = SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu};
-            static const int forced = env_int("INS_SYM_GRID", 0);
-            g = forced >= 8 ? round8(std::min(forced, MAXG)) : sym_grid(n, c->n_cu);
-        }
+        const bool sym = sizeof(TV) == 4 && use_sym(c, c->sperm);
+        const unsigned g = spmv_grid(c, sym);
+        SymView Y{nullptr, nullptr, 0, 1, 0};
+        if (sym) Y = sym_view(c);
         if (Y.v)
             INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 1>, g, SB, n, sell_view(c),
                                  sell_val, p, q, c->partial.p, sc, it, R, Y));
@@ -1621,14 +1813,105 @@ int run_chunk(Ctx* c, int format, const TV* csr_val, const TV* sell_val, TX* x, 
 // triangle by at most 1/8.  INS_SYM=0 switches them off.
 constexpr int SYM_MAXU = 64;
 
+// Node-block symmetric slices of a vector problem (see Sell::sdpn): built
+// when every node carries all or none of its dpn components, the active rows
+// are interleaved per node in node order (row a * dpn + c), the node offsets
+// of the pattern are one list of at most SYM_MAXU offsets shared by all
+// nodes whose lower offsets mirror upper ones, and the stored blocks pad the
+// upper triangle by at most 1/8.  The offsets are read off the full slices'
+// column-offset lists, which hold (c' - c) N + delta in the component-major
+// numbering; every CSR entry is then placed by k_bsym_map, and an entry that
+// finds no place leaves the full slices in use.
+int build_bsym(Ctx* c)
+{
+    Sell& S = c->sell;
+    const int D = c->dpn;
+    if (D < 2 || D > 3 || c->have_order || !c->dofs_match() || !S.perm_on) return INS_OK;
+    const int64_t n = c->n_active;
+    if (n % D) return INS_OK;
+    const int64_t N = n / D;
+    {
+        std::vector<int32_t> dof((size_t)(c->n_nodes * D));
+        INS_HIP_CHECK(c, hipMemcpyAsync(dof.data(), c->dof_id.p, dof.size() * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, c->stream));
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        int64_t a = 0;
+        for (int64_t node = 0; node < c->n_nodes; node++) {
+            const int32_t* id = &dof[(size_t)(node * D)];
+            int act = 0;
+            for (int k = 0; k < D; k++) act += id[k] >= 0;
+            if (act == 0) continue;
+            if (act != D) return INS_OK;
+            for (int k = 0; k < D; k++)
+                if (id[k] != a * D + k) return INS_OK;
+            a++;
+        }
+        if (a != N) return INS_OK;
+    }
+    std::vector<int32_t> plen(S.npat), pptr(S.npat + 1);
+    INS_HIP_CHECK(c, hipMemcpyAsync(plen.data(), S.pat_len.p, S.npat * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipMemcpyAsync(pptr.data(), S.pat_ptr.p, (S.npat + 1) * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const int64_t ndel = (int64_t)pptr[S.npat - 1] + plen[S.npat - 1];
+    std::vector<int32_t> del(ndel > 0 ? ndel : 1);
+    INS_HIP_CHECK(c, hipMemcpyAsync(del.data(), S.pat_delta.p, ndel * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> up, lowmag;
+    for (int64_t i = 0; i < ndel; i++) {
+        int64_t dd = ((int64_t)del[i] % N + N) % N;   // the node offset, in (-N/2, N/2]
+        if (dd > N / 2) dd -= N;
+        if (dd >= 0) up.push_back((int32_t)dd);
+        else lowmag.push_back((int32_t)-dd);
+    }
+    std::sort(up.begin(), up.end());
+    up.erase(std::unique(up.begin(), up.end()), up.end());
+    std::sort(lowmag.begin(), lowmag.end());
+    lowmag.erase(std::unique(lowmag.begin(), lowmag.end()), lowmag.end());
+    for (int32_t d : lowmag)
+        if (!std::binary_search(up.begin(), up.end(), d)) return INS_OK;
+    const int nu = (int)up.size();
+    if (nu == 0 || nu > SYM_MAXU || up[0] != 0) return INS_OK;
+    const int64_t nbv = (int64_t)D * (D + 1) / 2 + (int64_t)(nu - 1) * D * D;
+    const int64_t nvs = (N + 63) / 64 * 64 * nbv;
+    const int64_t upper = (c->nnz + c->n_active) / 2;
+    if (nvs > upper + upper / 8 + 64 * 64 || nvs >= INT32_MAX) return INS_OK;
+    INS_TRY(ensure(c, S.su, nu));
+    INS_HIP_CHECK(c, hipMemcpyAsync(S.su.p, up.data(), nu * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, c->stream));
+    INS_TRY(ensure(c, S.smap, c->nnz));
+    int* err = nullptr;
+    INS_HIP_CHECK(c, hipMallocAsync((void**)&err, sizeof(int), c->stream));
+    INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+    k_bsym_map<<<grid_for(n, 256), 256, 0, c->stream>>>(n, D, c->row_ptr.p, c->col.p, S.su.p, nu,
+                                                       nbv, S.smap.p, err);
+    INS_HIP_CHECK(c, hipGetLastError());
+    int herr = 0;
+    INS_HIP_CHECK(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    (void)hipFreeAsync(err, c->stream);
+    if (herr) return INS_OK;
+    S.nu = nu;
+    S.nvs = nvs;
+    S.sdpn = D;
+    S.snodes = N;
+    S.sym_on = true;
+    return INS_OK;
+}
+
 int build_sym(Ctx* c)
 {
     Sell& S = c->sell;
     S.sym_on = false;
     S.zeros = false;
+    S.sdpn = 1;
+    S.snodes = 0;
     static const int enabled = env_int("INS_SYM", 1);
-    if (!enabled || !S.ok || S.perm_on || c->dpn != 1 || S.npat <= 0 || c->csr_from_host)
-        return INS_OK;
+    if (!enabled || !S.ok || S.npat <= 0 || c->csr_from_host) return INS_OK;
+    if (c->dpn > 1) return build_bsym(c);
+    if (S.perm_on) return INS_OK;
     std::vector<int32_t> plen(S.npat), pptr(S.npat + 1);
     INS_HIP_CHECK(c, hipMemcpyAsync(plen.data(), S.pat_len.p, S.npat * sizeof(int32_t),
                                     hipMemcpyDeviceToHost, c->stream));
@@ -2205,6 +2488,8 @@ struct SlotDesc {
     const float INS_GLOBAL* vs;     // symmetric slices of the FP32 sweeps (null: none)
     const int32_t INS_GLOBAL* su;
     int nu;
+    int sdpn;
+    int64_t snodes;
     int64_t join;  // batched iteration of the request's first launch
     int g_spmv, g_vec;
 };
@@ -2251,7 +2536,8 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
                                                    (const float*)d.p, (float*)d.q, (double*)d.part,
                                                    (const Scal*)d.sc, (int)(it - d.join), R,
                                                    SymView{(const float*)d.vs,
-                                                           (const int32_t*)d.su, d.nu});
+                                                           (const int32_t*)d.su, d.nu, d.sdpn,
+                                                           d.snodes});
 }
 
 __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
@@ -2414,12 +2700,14 @@ int upload_table(Server* sv)
         INS_AS(d.v64, c->sell.v64.p);
         INS_AS(d.x64, c->sx64);
         INS_AS(d.b, c->sb);
-        const bool sym = c->sell.sym_on && !c->sperm;
+        const bool sym = use_sym(c, c->sperm);
         INS_AS(d.vs, sym ? c->sell.vs.p : nullptr);
         INS_AS(d.su, sym ? c->sell.su.p : nullptr);
         d.nu = sym ? c->sell.nu : 0;
+        d.sdpn = sym ? c->sell.sdpn : 1;
+        d.snodes = sym ? c->sell.snodes : 0;
 #undef INS_AS
-        d.g_spmv = (int)sell_grid(d.n);
+        d.g_spmv = (int)spmv_grid(c, sym);
         d.g_vec = (int)vec_grid(d.n);
         d.join = h.join_it;
         T.pre_spmv[ns] = gs;

@@ -6,6 +6,7 @@
   (symmetric FP32 slices) to 1e-8 relative residual measured by the oracle's
   FP64 SpMV on the oracle's matrix.
 * Elasticity 96^3 (884,736 HEX8, 2,709,792 rows, 214,983,054 entries): the
+  mixed solve on the symmetric 3 x 3 node-block slices, and the
   rows of the top three node layers -- traction-loaded -- against the oracle
   assembled over the element layers that hold all their elements (the oracle
   sums each entry over ascending element ids, so a slab of elements in global
@@ -98,6 +99,7 @@ def test_elasticity96_top_layers_and_solve(oracle):
         val, b = c.get_csr_values(), c.get_rhs()
         _, rel, st = c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
         assert st == abi.INS_OK and rel <= 1e-8
+        assert c.stats()["sym_block"] == 3           # the FP32 sweeps on 3 x 3 node blocks
         x = c.get_solution()
     assert nnz == 214983054 and prob.n_active == 2709792
     r = b - oracle.spmv(rp, ci, val, x, nthreads=NTH)

@@ -176,3 +176,32 @@ def test_serving_large_batch_equals_alone():
         assert it == im and np.array_equal(c.get_solution(), xm)
     for c in ctxs:
         c.close()
+
+
+def test_serving_large_requests_and_node_blocks_equal_alone(oracle):
+    """Requests large enough for the symmetric SpMV's own grid (diffusion
+    72^3: 383,688 rows; elasticity 30^3 on node-block slices) batched with
+    small ones: every request takes the grid and the partial-sum order it has
+    in ins_solve, so the batch gives the same bits, and the node-block
+    slices (3 x 3 blocks of the elasticity requests) give a solution the
+    oracle's FP64 SpMV confirms."""
+    probs = [P.diffusion3d(72, seed=70), P.elasticity3d(30, seed=71), P.diffusion3d(10, seed=72),
+             P.elasticity3d(6, seed=73)]
+    ctxs = [_ctx(p) for p in probs]
+    for c, p in zip(ctxs, probs):
+        c.prepare(abi.SOLVE_MIXED, abi.FMT_SELL)
+        st = c.stats()
+        if p.n_active > 100000:   # small meshes pad too much and keep the full slices
+            assert st["sym_block"] == p.dpn and st["sym_values"] > 0
+    with abi.Server(0, 8) as srv:
+        mix = _serve(srv, ctxs, stagger=2, k=8)
+    for c, p, (xm, im, rm, sm) in zip(ctxs, probs, mix):
+        assert sm == abi.INS_OK and rm <= 1e-8
+        it, rr, st = c.solve(abi.SOLVE_MIXED, abi.FMT_SELL, 1e-8)
+        assert st == abi.INS_OK and it == im and np.array_equal(c.get_solution(), xm)
+        rp, ci = c.get_pattern()
+        val, b = c.get_csr_values(), c.get_rhs()
+        r = b - oracle.spmv(rp, ci, val, xm)
+        assert np.linalg.norm(r) <= 1.001e-8 * np.linalg.norm(b)
+    for c in ctxs:
+        c.close()
