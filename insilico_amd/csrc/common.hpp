@@ -117,6 +117,9 @@ struct Sell {
     // node a at c * snodes + a).  sdpn = 1: the scalar slices above.
     int sdpn = 1;
     int64_t snodes = 0;
+    // scalar slices whose offsets form stencil lines (c - 1, c, c + 1): the
+    // SpMV loads p once per line and shifts it across the wave
+    bool slines = false;
 };
 
 struct Ctx {

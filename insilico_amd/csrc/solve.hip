@@ -248,6 +248,7 @@ struct SymView {
     int nu;
     int dpn;            // 1: scalar rows; 2, 3: node blocks (see bsym_node)
     int64_t nn;         // node blocks: nodes per component
+    int lines;          // scalar rows of stencil lines (see sym_lines_sweep)
 };
 
 template <typename TX, typename TA, int G = 8>
@@ -289,6 +290,108 @@ __device__ inline TA sym_row(int64_t row, bool live, int64_t n, int lane, int64_
         for (int j = 0; j < G; j++) s += (TA)vl[j] * (TA)xl[j];
     }
     return s;
+}
+
+// Wave-wide shifts by one lane (the GFX9 wavefront-shift DPP controls):
+// lane i takes lane i - 1's value (shr) or lane i + 1's (shl); the lane
+// without a source keeps `edge`.
+__device__ inline float wave_shr1(float v, float edge)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                 __builtin_bit_cast(int, v), 0x138,
+                                                                 0xf, 0xf, false));
+}
+__device__ inline float wave_shl1(float v, float edge)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                 __builtin_bit_cast(int, v), 0x130,
+                                                                 0xf, 0xf, false));
+}
+
+// Symmetric slices whose offsets are stencil lines: u = {0, 1, c1 - 1, c1,
+// c1 + 1, c2 - 1, ...} (NU = 14: the 27-point rows of a 3-D grid; NU = 5:
+// the 9-point rows of a 2-D grid), so that every row reads p on the lines
+// c - 1, c, c + 1 around the centres 0, +-c_m.  A wave loads each line of p
+// once (64 consecutive values, coalesced) and takes its two neighbours by
+// wave shifts, with the two values past the wave's ends as edge values: 9
+// p loads per row instead of 27.  Every address is a wave-uniform base plus
+// a lane offset fixed for the whole sweep (the mirror offsets are computed
+// once).  The slices whose lines reach past row 0 or row n - 1 take
+// sym_row; the sums run in sym_row's order, so the results are the same
+// bits.
+template <int NU>
+__device__ inline double sym_lines_sweep(Blk b, int64_t n, const SymView& Y,
+                                         const float* __restrict__ p, float* __restrict__ q)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nsl = (n + 63) / 64;
+    const Sweep w = xcd_sweep(nsl, SB / 64, b, true);
+    int du[NU];
+#pragma unroll
+    for (int k = 0; k < NU; k++) du[k] = __builtin_amdgcn_readfirstlane(Y.u[k]);
+    const int dmax = du[NU - 1];
+    const int64_t ld = (int64_t)64 * NU;
+    const int back = 1 + (dmax >> 6);        // slices a mirror entry lies back, at most
+    uint32_t mo[NU];                          // mirror of (row, row - u_k) from slice - back
+#pragma unroll
+    for (int k = 0; k < NU; k++) {
+        const int t = lane - du[k];
+        mo[k] = (uint32_t)((int64_t)(back + (t >> 6)) * ld + k * 64 + (t & 63));
+    }
+    const int64_t s_lo = back, s_hi = (n - 1 - dmax - 64) / 64;
+    double pq = 0.;
+    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+        const int64_t row = slice * 64 + lane;
+        const bool live = row < n;
+        float s;
+        if (slice < s_lo || slice > s_hi) {
+            s = sym_row<float, float, 8>(live ? row : 0, live, n, lane, slice, Y, p);
+        } else {
+            const float* __restrict__ own = Y.v + slice * ld;
+            const float* __restrict__ mb = Y.v + (slice - back) * ld;
+            const float* __restrict__ pb = p + slice * 64;
+            float vu[NU], vl[NU], xu[NU], xl[NU];
+#pragma unroll
+            for (int k = 0; k < NU; k++) {
+                vu[k] = own[k * 64 + lane];
+                vl[k] = k > 0 ? mb[mo[k]] : 0.f;
+            }
+            {
+                const float a = pb[lane];
+                xu[0] = a;
+                xu[1] = wave_shl1(a, pb[64]);
+                xl[0] = 0.f;
+                xl[1] = wave_shr1(a, pb[-1]);
+            }
+#pragma unroll
+            for (int m = 0; m < (NU - 2) / 3; m++) {
+                const int c = du[3 + 3 * m];
+                const float au = pb[c + lane];
+                xu[2 + 3 * m] = wave_shr1(au, pb[c - 1]);
+                xu[3 + 3 * m] = au;
+                xu[4 + 3 * m] = wave_shl1(au, pb[c + 64]);
+                const float al = pb[lane - c];
+                xl[2 + 3 * m] = wave_shl1(al, pb[64 - c]);
+                xl[3 + 3 * m] = al;
+                xl[4 + 3 * m] = wave_shr1(al, pb[-c - 1]);
+            }
+            s = 0.f;
+#pragma unroll
+            for (int k0 = 0; k0 < NU; k0 += 8) {
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (k0 + j < NU) s += vu[k0 + j] * xu[k0 + j];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (k0 + j < NU) s += vl[k0 + j] * xl[k0 + j];
+            }
+        }
+        if (live) {
+            st(&q[row], s, g_nt);
+            pq += (double)p[row] * (double)s;
+        }
+    }
+    return pq;
 }
 
 // CSR row with LPR lanes; each lane keeps 4 entries in flight
@@ -452,9 +555,11 @@ struct Rep {
 // K1: q = A p with per-block partials of p.q; at the replacement iteration
 // (it == rep_at) the FP64 true residual instead, with partials of t.t and
 // of r32.z
-// SYM: 0 = full slices only, 1 = symmetric slices only, 2 = either, chosen
-// per call by Y.v (the batched serving kernel).  Separate instantiations keep
-// the symmetric row's registers out of the full-slice kernel.
+// SYM: 0 = full slices only, 1 = scalar symmetric slices (generic rows),
+// 3 = scalar symmetric slices of stencil lines, 4 = node-block symmetric
+// slices, 2 = any of them, chosen per call by Y (the batched serving
+// kernel).  Separate instantiations keep each row form's registers and
+// schedule out of the others' kernels (all share the replacement path).
 template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM = 0>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
@@ -487,8 +592,8 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
         return;
     }
     if (sc->done || sc->stop) return;
-    if constexpr (sizeof(TV) == 4 && SYM != 0) {
-        if (Y.v && Y.dpn > 1) {
+    if constexpr (sizeof(TV) == 4 && (SYM == 4 || SYM == 2)) {
+        if (SYM == 4 || (Y.v && Y.dpn > 1)) {
             double v[1] = {Y.dpn == 3 ? bsym_sweep<3, TX, TA>(b, Y, p, q)
                                       : bsym_sweep<2, TX, TA>(b, Y, p, q)};
             const int sl[1] = {P_PQ};
@@ -496,32 +601,44 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
             return;
         }
     }
-    // the symmetric slices sweep per-XCD ranges: a mirror read then finds
-    // the slice it needs in the same XCD's L2
-    const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
-    double pq = 0.;
-    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
-        const int64_t rho = slice * 64 + lane;
-        const bool live = rho < n;
-        const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
-        TA s;
-        if constexpr (sizeof(TV) == 4 && SYM == 1) {
-            s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
-        } else if constexpr (sizeof(TV) == 4 && SYM == 2) {
-            if (Y.v) s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
-            else s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
-        } else {
-            s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
-        }
-        if (live) {
-            const TX qs = (TX)s;
-            st(&q[row], qs, g_nt);
-            pq += (double)p[row] * (double)qs;
+    if constexpr (sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TA) == 4 &&
+                  (SYM == 3 || SYM == 2)) {
+        if (SYM == 3 || (Y.v && Y.lines)) {
+            double v[1] = {Y.nu == 14 ? sym_lines_sweep<14>(b, n, Y, p, q)
+                                      : sym_lines_sweep<5>(b, n, Y, p, q)};
+            const int sl[1] = {P_PQ};
+            block_partials<1>(v, part, sl, b);
+            return;
         }
     }
-    double v[1] = {pq};
-    const int sl[1] = {P_PQ};
-    block_partials<1>(v, part, sl, b);
+    if constexpr (SYM != 3 && SYM != 4) {
+        // the symmetric slices sweep per-XCD ranges: a mirror read then finds
+        // the slice it needs in the same XCD's L2
+        const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
+        double pq = 0.;
+        for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+            const int64_t rho = slice * 64 + lane;
+            const bool live = rho < n;
+            const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
+            TA s;
+            if constexpr (sizeof(TV) == 4 && SYM == 1) {
+                s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
+            } else if constexpr (sizeof(TV) == 4 && SYM == 2) {
+                if (Y.v) s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
+                else s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
+            } else {
+                s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
+            }
+            if (live) {
+                const TX qs = (TX)s;
+                st(&q[row], qs, g_nt);
+                pq += (double)p[row] * (double)qs;
+            }
+        }
+        double v[1] = {pq};
+        const int sl[1] = {P_PQ};
+        block_partials<1>(v, part, sl, b);
+    }
 }
 
 template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM = 0>
@@ -1454,34 +1571,19 @@ unsigned sell_grid(int64_t n)
     return round8(std::max<int64_t>(1, std::min<int64_t>(g, MAXG)));
 }
 
-// Blocks of the symmetric SpMV.  Each XCD sweeps its eighth of the slices
-// with its blocks' 4 waves per step.  Measured at 128^3 (n = 2.13M): the
-// blocks per XCD should be a whole multiple m of the XCD's CUs (5 per CU:
-// 30.1 us; 4.75 or 5.25 per CU: 32.3-33.0 us) and the last step of the
-// sweep nearly full (8 per CU, 4.07 steps: 34.8 us).  m runs over 4..8;
-// the fullest last step wins, the smaller m on a tie within 2%.
+// Blocks of the symmetric SpMV: each XCD sweeps its eighth of the slices
+// with its blocks' 4 waves per step, 4 blocks on every CU of the XCD.
+// Measured at 128^3 (n = 2.13M, 1024 vs 1280 blocks on one box): the
+// line-sharing rows 24.2 vs 31.5 us, the generic rows 30.9 vs 40.7 us; 768,
+// 1152, 1536 and 2048 blocks were all slower than 1024.  Small problems
+// (fewer than 4 slices per wave of such a grid) take the SELL grid.
 unsigned sym_grid(int64_t n, int cus)
 {
     const int64_t nsl = (n + 63) / 64;
     const int wpb = SB / 64;
     const int cpx = std::max(1, cus / NXCD);   // CUs per XCD
     if (nsl < (int64_t)4 * cpx * NXCD * wpb) return sell_grid(n);
-    int64_t per = (nsl + NXCD - 1) / NXCD;
-    per = (per + wpb - 1) / wpb * wpb;
-    int best = 0;
-    double best_eff = -1.;
-    for (int m = 4; m <= 8; m++) {
-        const int64_t nloc = (int64_t)m * cpx;
-        if (nloc * NXCD > MAXG) break;
-        const int64_t wv = nloc * wpb;
-        const int64_t steps = (per + wv - 1) / wv;
-        const double eff = (double)per / (double)(steps * wv);
-        if (eff > best_eff + 0.02) {
-            best_eff = eff;
-            best = m;
-        }
-    }
-    return (unsigned)(best * cpx * NXCD);
+    return (unsigned)std::min(4 * cpx * NXCD, MAXG);
 }
 
 unsigned csr_grid(int64_t n, int lpr)
@@ -1646,7 +1748,8 @@ bool use_sym(const Ctx* c, bool permuted)
 
 SymView sym_view(const Ctx* c)
 {
-    return SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu, c->sell.sdpn, c->sell.snodes};
+    return SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu, c->sell.sdpn, c->sell.snodes,
+                   c->sell.slines ? 1 : 0};
 }
 
 // Blocks of the SpMV on the full slices (sym = false) or on the symmetric
@@ -1672,9 +1775,15 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
     if (format == INS_FMT_SELL) {
         const bool sym = sizeof(TV) == 4 && use_sym(c, c->sperm);
         const unsigned g = spmv_grid(c, sym);
-        SymView Y{nullptr, nullptr, 0, 1, 0};
+        SymView Y{nullptr, nullptr, 0, 1, 0, 0};
         if (sym) Y = sym_view(c);
-        if (Y.v)
+        if (Y.v && Y.dpn > 1)
+            INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 4>, g, SB, n, sell_view(c),
+                                 sell_val, p, q, c->partial.p, sc, it, R, Y));
+        else if (Y.v && Y.lines)
+            INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 3>, g, SB, n, sell_view(c),
+                                 sell_val, p, q, c->partial.p, sc, it, R, Y));
+        else if (Y.v)
             INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 1>, g, SB, n, sell_view(c),
                                  sell_val, p, q, c->partial.p, sc, it, R, Y));
         else
@@ -1908,6 +2017,7 @@ int build_sym(Ctx* c)
     S.zeros = false;
     S.sdpn = 1;
     S.snodes = 0;
+    S.slines = false;
     static const int enabled = env_int("INS_SYM", 1);
     if (!enabled || !S.ok || S.npat <= 0 || c->csr_from_host) return INS_OK;
     if (c->dpn > 1) return build_bsym(c);
@@ -1958,6 +2068,15 @@ int build_sym(Ctx* c)
     if (herr) return INS_OK;
     S.nu = nu;
     S.nvs = nvs;
+    // stencil lines: {0, 1, c1 - 1, c1, c1 + 1, ...} with 1 < c1 - 1, the
+    // centres ascending and every line at least one wave (64 rows) apart
+    S.slines = (nu == 14 || nu == 5) && up[0] == 0 && up[1] == 1;
+    for (int m = 0; S.slines && m < (nu - 2) / 3; m++) {
+        const int32_t c0 = up[3 + 3 * m];
+        S.slines = up[2 + 3 * m] == c0 - 1 && up[4 + 3 * m] == c0 + 1;
+    }
+    static const int lines_on = env_int("INS_SYM_LINES", 1);
+    S.slines = S.slines && lines_on;
     S.sym_on = true;
     return INS_OK;
 }
@@ -2490,6 +2609,7 @@ struct SlotDesc {
     int nu;
     int sdpn;
     int64_t snodes;
+    int slines;
     int64_t join;  // batched iteration of the request's first launch
     int g_spmv, g_vec;
 };
@@ -2537,7 +2657,7 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
                                                    (const Scal*)d.sc, (int)(it - d.join), R,
                                                    SymView{(const float*)d.vs,
                                                            (const int32_t*)d.su, d.nu, d.sdpn,
-                                                           d.snodes});
+                                                           d.snodes, d.slines});
 }
 
 __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
@@ -2706,6 +2826,7 @@ int upload_table(Server* sv)
         d.nu = sym ? c->sell.nu : 0;
         d.sdpn = sym ? c->sell.sdpn : 1;
         d.snodes = sym ? c->sell.snodes : 0;
+        d.slines = sym && c->sell.slines ? 1 : 0;
 #undef INS_AS
         d.g_spmv = (int)spmv_grid(c, sym);
         d.g_vec = (int)vec_grid(d.n);
