@@ -71,6 +71,8 @@ extern "C" {
 #define INS_BF_CONST    1  /* params[c] = f_c                                */
 #define INS_BF_SINPROD  2  /* f = params[0] * prod_d sin(params[1+d] * x_d)  */
 #define INS_BF_COSPROD  3  /* f = params[0] * prod_d cos(params[1+d] * x_d)  */
+#define INS_BF_TABLE    4  /* values of a caller's f(x) at the quadrature
+                              points (ins_set_body_force_values)            */
 
 /* solve modes */
 #define INS_SOLVE_FP64   0  /* FP64 Jacobi-PCG on the exported CSR           */
@@ -176,7 +178,9 @@ int ins_get_pattern(ins_ctx* ctx, int32_t* row_ptr, int32_t* col_idx);
 
 /* ---- kernels and loads ----------------------------------------------------- */
 /* KERNEL object + quadrature: kernel = INS_LAPLACE (ncoef = 1: kappa_e, the
- * conductivity function of heat/Laplace.hpp:94-97 evaluated per element) or
+ * conductivity function of heat/Laplace.hpp:94-97 evaluated per element;
+ * ncoef = nq^dim: kappa(element, xi_q) at every quadrature point, in the
+ * order of ins_quadrature_rule, as heat/Laplace.hpp:113-126 calls it) or
  * INS_STVENANT / INS_NEOHOOKE (ncoef = 2: lambda_e, mu_e, mat/Lame.hpp:28-40);
  * nq = Gauss points per direction of base::Quadrature
  * (base/quad/GaussLegendre.hpp:33-36: nq = (DEG+2)/2). */
@@ -208,6 +212,23 @@ int ins_bdf_step(ins_ctx* ctx, double dt, int precision, int mode, int format, d
 
 /* bodyForceComputation (base/asmb/BodyForce.hpp:67-82) with a built-in f */
 int ins_set_body_force(ins_ctx* ctx, int kind, const double* params);
+
+/* The quadrature rule of base::Quadrature (base/quad/GaussLegendre.hpp:33-36,
+ * 114-134; base/quad/TensorProduct.hpp:123-175): npts (1..3) Gauss points
+ * per direction on [0,1]^dim, the first coordinate fastest; w[q] and
+ * xi[q*dim + d] for q < npts^dim (dim = 1, 2, 3; dim - 1 gives the surface
+ * rule of base::SurfaceQuadrature).  Returns npts^dim, or INS_ERR_ARG.  No
+ * context: the points at which ins_set_body_force_values,
+ * ins_set_traction_values and a per-point conductivity are given. */
+int ins_quadrature_rule(int dim, int npts, double* w, double* xi);
+
+/* bodyForceComputation with a caller's function f(x) (BodyForce.hpp:67-82,
+ * 172-201: f is evaluated at x(xi_q), the Q1 geometry at each quadrature
+ * point): values f[(e * nqp + q) * dpn + c], nqp = nq^dim of
+ * ins_set_material, q in the order of ins_quadrature_rule.  Sets the kind
+ * INS_BF_TABLE; NULL drops the values (kind INS_BF_NONE).  Call after
+ * ins_set_material and ins_set_dofs. */
+int ins_set_body_force_values(ins_ctx* ctx, const double* f);
 /* neumannForceComputation (base/asmb/NeumannForce.hpp:36-63) with a
  * traction over boundary faces given by their nodes (2 per edge in 2-D, 4
  * per quadrilateral in 3-D, face-local Q1 order; degree-2 fields: 3 / 9
@@ -229,6 +250,14 @@ int ins_set_traction_profile(ins_ctx* ctx, int axis, double x0);
  * orientation).  Applies to the faces of ins_set_traction in place of its
  * traction vector; ref_kind INS_REF_ZERO switches back to the traction. */
 int ins_set_neumann_flux(ins_ctx* ctx, int ref_kind, const double* ref_params);
+/* neumannForceComputation with a caller's function f(x, n)
+ * (NeumannForce.hpp:36-63,132-175: f at the surface point x(eta_q) and the
+ * unit normal n(eta_q) of base::SurfaceNormal, base/geometry.hpp:314-346):
+ * values t[(f * nqf + q) * dpn + c] for the faces of ins_set_traction, nqf =
+ * nq^(dim-1) surface points in the order of ins_quadrature_rule(dim - 1,
+ * nq).  They replace the traction vector and the co-normal flux; NULL
+ * drops them.  ins_set_traction with new faces drops them too. */
+int ins_set_traction_values(ins_ctx* ctx, const double* t);
 
 /* ---- assembly ------------------------------------------------------------- */
 /* stiffnessMatrixComputation + computeResidualForces + bodyForceComputation

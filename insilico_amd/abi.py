@@ -19,7 +19,7 @@ ERRORS_INV = {v: k for k, v in ERRORS.items()}
 LAPLACE, STVENANT, NEOHOOKE = 1, 2, 3
 FP64, FP32 = 0, 1
 ASM_MATRIX, ASM_RHS, ASM_RESIDUAL, ASM_INCREMENTAL = 1, 2, 4, 8
-BF_NONE, BF_CONST, BF_SINPROD = 0, 1, 2
+BF_NONE, BF_CONST, BF_SINPROD, BF_COSPROD, BF_TABLE = 0, 1, 2, 3, 4
 SOLVE_FP64, SOLVE_MIXED = 0, 1
 FMT_CSR, FMT_SELL = 0, 1
 DIST_SET, DIST_ADD = 0, 1
@@ -37,6 +37,7 @@ SYMBOLS = [
     "ins_server_running", "ins_set_field", "ins_set_mass", "ins_error_norm",
     "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile", "ins_cell_eval",
     "ins_set_neumann_flux", "ins_device_memory", "ins_solve_csr_noalloc",
+    "ins_quadrature_rule", "ins_set_body_force_values", "ins_set_traction_values",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 REF_ZERO, REF_SINPROD, REF_KELVIN, REF_COSPROD = 0, 1, 2, 3
@@ -89,6 +90,9 @@ def load():
         "ins_bdf_init": [vp, i32, dbl, vp],
         "ins_cell_eval": [vp, i32, vp, vp, vp],
         "ins_set_neumann_flux": [vp, i32, vp],
+        "ins_quadrature_rule": [i32, i32, vp, vp],
+        "ins_set_body_force_values": [vp, vp],
+        "ins_set_traction_values": [vp, vp],
         "ins_set_traction_profile": [vp, i32, dbl],
         "ins_bdf_step": [vp, dbl, i32, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
         "ins_set_dofs": [vp, i32, vp, i64],
@@ -134,6 +138,19 @@ def load():
 
 def _p(a):
     return None if a is None else a.ctypes.data
+
+
+def quadrature_rule(dim, npts):
+    """(w, xi): the library's tensor Gauss rule, npts per direction on
+    [0, 1]^dim, first coordinate fastest (no device needed)"""
+    L = load()
+    nq = npts ** dim
+    w = np.zeros(nq)
+    xi = np.zeros(nq * dim)
+    got = L.ins_quadrature_rule(dim, npts, _p(w), _p(xi))
+    if got != nq:
+        raise InsError(got, "ins_quadrature_rule(%d, %d)" % (dim, npts))
+    return w, xi.reshape(nq, dim)
 
 
 class Context:
@@ -281,6 +298,17 @@ class Context:
         p[:len(params)] = params
         self._chk(self.L.ins_set_body_force(self.h, kind, _p(p)))
 
+    def set_body_force_values(self, f):
+        """values of a body force at the quadrature points, (E, nqp, dpn)"""
+        v = None if f is None else np.ascontiguousarray(f, np.float64)
+        self._chk(self.L.ins_set_body_force_values(self.h, _p(v)))
+
+    def set_traction_values(self, t):
+        """values of a Neumann function at the faces' surface points,
+        (n_faces, nqf, dpn)"""
+        v = None if t is None else np.ascontiguousarray(t, np.float64)
+        self._chk(self.L.ins_set_traction_values(self.h, _p(v)))
+
     def set_traction_profile(self, axis=-1, x0=0.0):
         self._chk(self.L.ins_set_traction_profile(self.h, axis, x0))
 
@@ -383,6 +411,10 @@ class Context:
         self.set_traction_profile(*(lin if lin is not None else (-1, 0.0)))
         flux = prob.meta.get("neumann_flux")
         self.set_neumann_flux(*(flux if flux is not None else (REF_ZERO, None)))
+        if prob.meta.get("bf_table") is not None:
+            self.set_body_force_values(prob.meta["bf_table"])
+        if prob.meta.get("traction_table") is not None:
+            self.set_traction_values(prob.meta["traction_table"])
         return self.build_pattern()
 
 

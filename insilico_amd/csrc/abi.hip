@@ -77,6 +77,7 @@ static void release_all(Ctx* c)
     release(c, c->partial); release(c, c->scal); release(c, c->dflag);
     release(c, c->xp); release(c, c->bp); release(c, c->d64p); release(c, c->d32p);
     release(c, c->pinfo);
+    release(c, c->bftab); release(c, c->ttab);
 }
 
 // A context detached from a server may still be named by the batched work
@@ -544,6 +545,11 @@ int ins_set_material(ins_ctx* h, int kernel, int nq, int ncoef, const double* co
     if (nq < 1 || nq > 3) return fail(c, INS_ERR_UNSUPPORTED, "1..3 Gauss points per direction");
     const int need = (kernel == INS_LAPLACE) ? 1 : 2;
     if (ncoef < need || !coef) return fail(c, INS_ERR_ARG, "too few element coefficients");
+    int nqp = 1;
+    for (int d = 0; d < c->dim; d++) nqp *= nq;
+    if (kernel == INS_LAPLACE && ncoef != 1 && ncoef != nqp)
+        return fail(c, INS_ERR_ARG,
+                    "heat::Laplace takes 1 conductivity per element or 1 per quadrature point");
     c->kernel = kernel;
     c->nq = nq;
     c->ncoef = ncoef;
@@ -579,11 +585,62 @@ int ins_set_body_force(ins_ctx* h, int kind, const double* params)
     CTX_CHECK(c);
     c->bdf_staged = false;
     if (kind != INS_BF_NONE && kind != INS_BF_CONST && kind != INS_BF_SINPROD &&
-        kind != INS_BF_COSPROD)
+        kind != INS_BF_COSPROD && kind != INS_BF_TABLE)
         return fail(c, INS_ERR_ARG, "unknown body force kind");
+    if (kind == INS_BF_TABLE) {
+        if (!c->have_bftab) return fail(c, INS_ERR_STATE, "no body force values set");
+        c->bf_kind = kind;
+        return INS_OK;
+    }
     if (kind != INS_BF_NONE && !params) return fail(c, INS_ERR_ARG, "body force parameters");
     c->bf_kind = kind;
     for (int i = 0; i < 4; i++) c->bf[i] = (kind != INS_BF_NONE) ? params[i] : 0.;
+    return INS_OK;
+}
+
+int ins_quadrature_rule(int dim, int npts, double* w, double* xi)
+{
+    if (dim < 1 || dim > 3 || npts < 1 || npts > 3 || !w || !xi) return INS_ERR_ARG;
+    const int nq = quadrature_rule(dim, npts, w, xi);
+    return nq > 0 ? nq : INS_ERR_ARG;
+}
+
+int ins_set_body_force_values(ins_ctx* h, const double* f)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    c->bdf_staged = false;
+    if (!f) {
+        c->have_bftab = false;
+        if (c->bf_kind == INS_BF_TABLE) c->bf_kind = INS_BF_NONE;
+        return INS_OK;
+    }
+    if (c->kernel == 0 || !c->have_dofs)
+        return fail(c, INS_ERR_STATE, "ins_set_body_force_values before ins_set_material / ins_set_dofs");
+    int64_t nqp = 1;
+    for (int d = 0; d < c->dim; d++) nqp *= c->nq;
+    INS_TRY(upload(c, c->bftab, f, (size_t)(c->n_elem * nqp * c->dpn)));
+    c->have_bftab = true;
+    c->bf_kind = INS_BF_TABLE;
+    return INS_OK;
+}
+
+int ins_set_traction_values(ins_ctx* h, const double* t)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    c->bdf_staged = false;
+    if (!t) {
+        c->have_ttab = false;
+        return INS_OK;
+    }
+    if (c->n_faces <= 0 || c->kernel == 0 || !c->have_dofs)
+        return fail(c, INS_ERR_STATE,
+                    "ins_set_traction_values before ins_set_traction / ins_set_material");
+    int64_t nqf = 1;
+    for (int d = 0; d + 1 < c->dim; d++) nqf *= c->nq;
+    INS_TRY(upload(c, c->ttab, t, (size_t)(c->n_faces * nqf * c->dpn)));
+    c->have_ttab = true;
     return INS_OK;
 }
 
@@ -593,6 +650,7 @@ int ins_set_traction(ins_ctx* h, int64_t n_faces, int nfn, const int32_t* faces,
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
     c->bdf_staged = false;
+    c->have_ttab = false;   // values of the faces set before
     if (c->n_nodes == 0) return fail(c, INS_ERR_STATE, "ins_set_traction before ins_set_mesh");
     if (n_faces == 0) {
         c->n_faces = 0;

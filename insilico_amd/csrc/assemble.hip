@@ -28,9 +28,13 @@ namespace {
 
 constexpr int BLK = 256;
 
+// body force: a built-in function of x, or (INS_BF_TABLE) the values of a
+// caller's function at the quadrature points, tab[(e * nqp + q) * s + c]
 struct Bf {
     int kind;
     double p[4];
+    const double* tab;
+    int nqp, s;
 };
 
 template <int B, int E>
@@ -53,8 +57,9 @@ struct Tab {
 };
 
 template <typename T>
-__device__ inline T body_force(const Bf& bf, const T* x, int dim, int comp)
+__device__ inline T body_force(const Bf& bf, const T* x, int dim, int comp, int64_t e, int q)
 {
+    if (bf.kind == INS_BF_TABLE) return (T)bf.tab[(e * bf.nqp + q) * bf.s + comp];
     if (bf.kind == INS_BF_CONST) return (T)bf.p[comp];
     if (bf.kind == INS_BF_SINPROD) {
         T f = (T)bf.p[0];
@@ -176,7 +181,8 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
             T gx[NPE][DIM];
             const T det = phys_grad<DIM, NPE, T>(X, dphi, gx);
             const T w = s_tab[TB::W + q];
-            const T scal = kappa * det * w;
+            // kappa(element, xi): one value per element, or one per point
+            const T scal = (ncoef == NQ ? (T)coef[e * ncoef + q] : kappa) * det * w;
             int k = 0;
 #pragma unroll
             for (int M = 0; M < NPE; M++)
@@ -206,7 +212,7 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
                 for (int n = 0; n < NPE; n++)
 #pragma unroll
                     for (int d = 0; d < DIM; d++) x[d] += X[n][d] * phi[n];
-                const T f = body_force<T>(bf, x, DIM, 0);
+                const T f = body_force<T>(bf, x, DIM, 0, e, q);
 #pragma unroll
                 for (int n = 0; n < NPE; n++) fb[n] += f * phi[n] * w * det;
             }
@@ -464,7 +470,7 @@ __global__ __launch_bounds__(256) void k_elem_elastic(
                 for (int n = 0; n < NPE; n++)
 #pragma unroll
                     for (int c = 0; c < S; c++)
-                        sm[L::FB + q * NEE + n * S + c] = body_force<T>(bf, x, DIM, c) * phi[n] * w * det;
+                        sm[L::FB + q * NEE + n * S + c] = body_force<T>(bf, x, DIM, c, e, q) * phi[n] * w * det;
             }
         }
         __syncthreads();
@@ -673,7 +679,7 @@ __global__ __launch_bounds__(256) void k_elem_laplace_q2(
 #pragma unroll
                 for (int a = 0; a < DIM; a++) sm[L::INV + (q * DIM + i) * DIM + a] = inv[i][a];
             const T w = tab[TB::W + q];
-            sm[L::SC + q] = (T)coef[e * ncoef] * det * w;
+            sm[L::SC + q] = (T)coef[e * ncoef + (ncoef == NQ ? q : 0)] * det * w;
             sm[L::DET + q] = det;
             if (fe && bf.kind != INS_BF_NONE) {
                 T x[3] = {0, 0, 0};
@@ -681,7 +687,7 @@ __global__ __launch_bounds__(256) void k_elem_laplace_q2(
                 for (int n = 0; n < GN; n++)
 #pragma unroll
                     for (int d = 0; d < DIM; d++) x[d] += sm[L::X + n * DIM + d] * tab[TB::GP + q * GN + n];
-                sm[L::FQ + q] = body_force<T>(bf, x, DIM, 0);
+                sm[L::FQ + q] = body_force<T>(bf, x, DIM, 0, e, q);
             }
         }
         __syncthreads();
@@ -909,7 +915,7 @@ __global__ __launch_bounds__(64) void k_elem_elastic_q2(
 #pragma unroll
                     for (int d = 0; d < DIM; d++) x[d] += Xg[n * DIM + d] * s_tab[TB::GP + q * GN + n];
 #pragma unroll
-                for (int c = 0; c < S; c++) sm[L::FQ + q * 3 + c] = body_force<T>(bf, x, DIM, c);
+                for (int c = 0; c < S; c++) sm[L::FQ + q * 3 + c] = body_force<T>(bf, x, DIM, c, e, q);
             }
         }
         __syncthreads();
@@ -1297,7 +1303,9 @@ __global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restr
                                               const double* __restrict__ fpt,
                                               const double* __restrict__ wt, int nqf,
                                               double t0, double t1, double t2, int axis,
-                                              double x0, RefFn flux, double* __restrict__ ff)
+                                              double x0, RefFn flux,
+                                              const double* __restrict__ ttab,
+                                              double* __restrict__ ff)
 {
     constexpr int GNF = (DIM == 3) ? 4 : 2;
     constexpr int LD = DIM - 1;
@@ -1335,7 +1343,10 @@ __global__ __launch_bounds__(BLK) void k_face(int64_t nf, const int32_t* __restr
                 detG = sqrt(tx * tx + ty * ty);
             }
             double t[3] = {tr[0], tr[1], tr[2]};
-            if (flux.kind != 0) {
+            if (ttab) {
+                // a caller's Neumann function, evaluated at this point
+                for (int c = 0; c < S; c++) t[c] = ttab[(f * nqf + q) * S + c];
+            } else if (flux.kind != 0) {
                 double xs[3] = {0., 0., 0.};
                 for (int n = 0; n < GNF; n++)
                     for (int d = 0; d < DIM; d++) xs[d] += X[n][d] * gphi[n];
@@ -1893,6 +1904,18 @@ int gather_q2(Ctx* c, bool elastic, int do_matrix, int do_rhs, int lift, int inc
 
 }  // namespace
 
+int quadrature_rule(int dim, int npts, double* w, double* xi)
+{
+    std::vector<double> ww, xx;
+    const int nq = tensor_rule(dim, npts, ww, xx);
+    if (nq < 0) return -1;
+    for (int q = 0; q < nq; q++) {
+        w[q] = ww[q];
+        for (int d = 0; d < dim; d++) xi[q * dim + d] = xx[(size_t)q * dim + d];
+    }
+    return nq;
+}
+
 int make_tables(Ctx* c)
 {
     std::vector<double> t;
@@ -1950,6 +1973,12 @@ int assemble(Ctx* c, int precision, int flags)
     Bf bf;
     bf.kind = c->bf_kind;
     for (int i = 0; i < 4; i++) bf.p[i] = c->bf[i];
+    bf.tab = c->bftab.p;
+    bf.nqp = 1;
+    for (int d = 0; d < c->dim; d++) bf.nqp *= c->nq;
+    bf.s = s;
+    if (bf.kind == INS_BF_TABLE && !c->have_bftab)
+        return fail(c, INS_ERR_STATE, "body force values not set");
     if (c->tab_deg != c->deg) INS_TRY(make_tables(c));
 
     INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
@@ -2052,7 +2081,8 @@ int assemble(Ctx* c, int precision, int flags)
         const int key = c->dim * 100 + c->nfn * 10 + s;
 #define INS_FACE(D, N, S_)                                                                      \
     k_face<D, N, S_><<<g, BLK, 0, c->stream>>>(c->n_faces, c->faces.p, gmap, c->xyz.p, gd, gp, \
-                                               fp, w, nqf, T[0], T[1], T[2], ax, x0, flux, ff)
+                                               fp, w, nqf, T[0], T[1], T[2], ax, x0, flux,    \
+                                               c->have_ttab ? c->ttab.p : nullptr, ff)
         switch (key) {
         case 393: INS_FACE(3, 9, 3); break;
         case 391: INS_FACE(3, 9, 1); break;

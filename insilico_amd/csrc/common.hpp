@@ -207,6 +207,11 @@ struct Ctx {
     int flux_kind = 0;
     double flux_p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DArr<int32_t> faces, fadj_ptr, fadj;
+    // values of caller functions at the quadrature points: the body force
+    // (INS_BF_TABLE) per element, point and component, and the Neumann
+    // function per face, surface point and component
+    DArr<double> bftab, ttab;
+    bool have_bftab = false, have_ttab = false;
     // time-step terms: density of the mass matrix (0 = none), its factor in
     // the matrix (a0 / (b0 dt)) and the history combination of the RHS
     double density = 0., mass_factor = 0.;
@@ -346,6 +351,8 @@ inline int ceil_lg2(int v)
 int build_pattern(Ctx* c);
 // assemble.hip
 int make_tables(Ctx* c);
+// the tensor Gauss rule of the element kernels: returns npts^dim, or -1
+int quadrature_rule(int dim, int npts, double* w, double* xi);
 constexpr int ASM_STAGED = 1 << 8;  // internal: RHS from the staged element data
 int assemble(Ctx* c, int precision, int flags);
 int compute_dinv(Ctx* c);

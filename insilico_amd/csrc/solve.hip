@@ -31,6 +31,13 @@
 
 #include "common.hpp"
 
+// Multiply-adds fuse only within one source expression (s += a * b is one
+// FMA everywhere).  With contraction left to the backend, whether a product
+// fuses depended on its other uses inside each kernel, so that the same
+// inline row sum rounded differently in the solve's kernel and in the
+// serving batch's kernel; a request must give the same bits in both.
+#pragma clang fp contract(on)
+
 namespace ins {
 
 namespace {
@@ -2434,8 +2441,18 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     if (S.breakdown)
         status = fail(c, INS_ERR_BREAKDOWN, "CG breakdown (p.Ap <= 0 or non-finite) after " +
                                                 std::to_string(S.iters) + " iterations");
-    else if (!(rel <= rtol))
-        status = fail(c, INS_ERR_NOCONV, "iteration limit reached before rtol");
+    else if (!(rel <= rtol)) {
+        char msg[192];
+        if (!mixed && S.stop)
+            snprintf(msg, sizeof msg,
+                     "the recursive residual met rtol %.3g after %d iterations, the true "
+                     "one does not (relres %.3g: round-off floor)", rtol, (int)S.iters, rel);
+        else
+            snprintf(msg, sizeof msg,
+                     "iteration limit reached before rtol %.3g (%d iterations, relres %.3g)",
+                     rtol, (int)S.iters, rel);
+        status = fail(c, INS_ERR_NOCONV, msg);
+    }
     return status;
 }
 

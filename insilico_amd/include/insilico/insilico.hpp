@@ -669,9 +669,7 @@ public:
         insilico::check(ins_get_solution(ctx_, x_.data()), ctx_);
         solved_ = true;
         if (st == INS_ERR_NOCONV)
-            throw insilico::Error(st, std::string("cgSolve: ") + ins_last_error(ctx_) +
-                                          " (relres " + std::to_string(rel) + " > rtol " +
-                                          std::to_string(rtol) + ")");
+            throw insilico::Error(st, std::string("cgSolve: ") + ins_last_error(ctx_));
         return it;
     }
     // Direct solves (Eigen3::choleskySolve / luSolve / superLUSolve,

@@ -269,6 +269,54 @@ def q2_face_nodes(fconn, elem, face, dim):
     return out
 
 
+def q1_shape(dim, xi):
+    """Q1 shape functions at the points xi (nq, dim), vertices in
+    hierarchic order (base/LagrangeShapeFun.hpp:64-86): (nq, 2^dim)"""
+    xi = np.atleast_2d(np.asarray(xi, np.float64))
+    H = _H3 if dim == 3 else _H2
+    phi = np.ones((xi.shape[0], 1 << dim))
+    for lex in range(1 << dim):
+        for d in range(dim):
+            bit = (lex >> d) & 1
+            phi[:, H[lex]] *= xi[:, d] if bit else 1.0 - xi[:, d]
+    return phi
+
+
+def element_points(xyz, conn, xi):
+    """x(xi_q) of the Q1 geometry of every element at the local points xi
+    (nq, dim): (E, nq, dim)"""
+    dim = xyz.shape[1]
+    phi = q1_shape(dim, xi)                       # (nq, npe)
+    X = xyz[conn]                                 # (E, npe, dim)
+    return np.einsum("qn,end->eqd", phi, X)
+
+
+def face_points(xyz, faces, eta):
+    """Surface points x(eta_q) and unit normals of base::SurfaceNormal
+    (base/geometry.hpp:255-346: the cross product of the surface Jacobian's
+    columns in 3-D, (J10, -J00) in 2-D) of every face at the local points
+    eta (nq, dim - 1), faces listed in the face table's order: (x, n), each
+    (n_faces, nq, dim)"""
+    dim = xyz.shape[1]
+    X = xyz[faces]                                # (F, nfn, dim)
+    eta = np.atleast_2d(np.asarray(eta, np.float64))
+    if dim == 3:
+        e0, e1 = eta[:, 0], eta[:, 1]
+        phi = np.stack([(1 - e0) * (1 - e1), e0 * (1 - e1), e0 * e1, (1 - e0) * e1], 1)
+        d0 = np.stack([-(1 - e1), 1 - e1, e1, -e1], 1)
+        d1 = np.stack([-(1 - e0), -e0, e0, 1 - e0], 1)
+        J0 = np.einsum("qn,fnd->fqd", d0, X)
+        J1 = np.einsum("qn,fnd->fqd", d1, X)
+        nrm = np.cross(J0, J1)
+    else:
+        e0 = eta[:, 0]
+        phi = np.stack([1 - e0, e0], 1)
+        t = X[:, 1, :] - X[:, 0, :]
+        nrm = np.repeat(np.stack([t[:, 1], -t[:, 0]], 1)[:, None, :], eta.shape[0], 1)
+    x = np.einsum("qn,fnd->fqd", phi, X)
+    return x, nrm / np.linalg.norm(nrm, axis=2, keepdims=True)
+
+
 def number_dofs(constrained):
     """numberDoFsConsecutively: constrained (n_nodes, dpn) bool -> dof ids."""
     c = np.asarray(constrained, bool).ravel()

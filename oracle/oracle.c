@@ -364,8 +364,10 @@ static void material(int kernel, double lam, double mu, const double F[3][3],
     }
 }
 
-static double body_force(const orc_problem* p, const double* x, int comp)
+static double body_force(const orc_problem* p, const double* x, int comp, int64_t e, int q,
+                         int nq)
 {
+    if (p->bf_kind == ORC_BF_TABLE) return p->bf_tab[(e * nq + q) * p->dpn + comp];
     if (p->bf_kind == ORC_BF_CONST) return p->bf[comp];
     if (p->bf_kind == ORC_BF_SINPROD) {
         double f = p->bf[0];
@@ -431,7 +433,8 @@ int orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe)
             detJ = contra_field(dim, X, gdphi, npe, dphi, gx);
         }
         if (p->kernel == ORC_LAPLACE && Ke) {
-            const double kappa = p->coef ? p->coef[e * p->ncoef] : 1.0;
+            /* kappa(element, xi): one value per element, or per point */
+            const double kappa = p->coef ? p->coef[e * p->ncoef + (p->ncoef == nq ? q : 0)] : 1.0;
             const double scalar = kappa * detJ * w;
             for (int M = 0; M < npe; M++)
                 for (int N = 0; N < npe; N++) {
@@ -497,7 +500,7 @@ int orc_element(const orc_problem* p, int64_t e, double* Ke, double* Fe)
                 for (int d = 0; d < dim; d++) x[d] += X[n * dim + d] * gphi[n];
             for (int n = 0; n < npe; n++)
                 for (int c = 0; c < s; c++)
-                    fb[n * s + c] += body_force(p, x, c) * phi[n] * w * detJ;
+                    fb[n * s + c] += body_force(p, x, c, e, q, nq) * phi[n] * w * detJ;
         }
     }
     if (Fe)
@@ -723,7 +726,9 @@ static void face_force(const orc_problem* p, int64_t f, double* out)
             detG = sqrt(tx * tx + ty * ty);
         }
         double t[3] = {p->traction[0], p->traction[1], p->traction[2]};
-        if (p->flux_kind != 0) {
+        if (p->tr_tab) {
+            for (int c = 0; c < s; c++) t[c] = p->tr_tab[(f * nq + q) * s + c];
+        } else if (p->flux_kind != 0) {
             /* surface point and unit normal of base::SurfaceNormal
              * (base/geometry.hpp:255-345; 2-D: (J10, -J00) from
              * base/linearAlgebra.hpp:93-99; 3-D: J.col(0) x J.col(1)) */

@@ -42,7 +42,8 @@ extern "C" {
 /* kernel ids (same numbering as the product C-ABI) */
 enum { ORC_LAPLACE = 1, ORC_STVENANT = 2, ORC_NEOHOOKE = 3 };
 /* body force kinds */
-enum { ORC_BF_NONE = 0, ORC_BF_CONST = 1, ORC_BF_SINPROD = 2, ORC_BF_COSPROD = 3 };
+enum { ORC_BF_NONE = 0, ORC_BF_CONST = 1, ORC_BF_SINPROD = 2, ORC_BF_COSPROD = 3,
+       ORC_BF_TABLE = 4 };
 
 typedef struct orc_problem {
     int dim;                 /* 2 or 3 */
@@ -97,6 +98,16 @@ typedef struct orc_problem {
      * the traction when flux_kind != 0 */
     int flux_kind;
     double flux_p[8];
+    /* Values of caller functions at the quadrature points, as the reference
+     * evaluates them there: the body force f(x(xi_q)) per element, point and
+     * component (bf_kind ORC_BF_TABLE; base/asmb/BodyForce.hpp:172-201),
+     * and the Neumann function f(x(eta_q), n(eta_q)) per face, surface point
+     * and component (in place of the traction; base/asmb/NeumannForce.hpp
+     * :132-175); NULL = not used.  A Laplace problem with ncoef equal to the
+     * number of quadrature points takes the conductivity function
+     * kappa(element, xi_q) per point (heat/Laplace.hpp:113-126). */
+    const double* bf_tab;
+    const double* tr_tab;
 } orc_problem;
 
 int  orc_gauss1d(int npts, double* w, double* x);

@@ -15,7 +15,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 LAPLACE, STVENANT, NEOHOOKE = 1, 2, 3
-BF_NONE, BF_CONST, BF_SINPROD = 0, 1, 2
+BF_NONE, BF_CONST, BF_SINPROD, BF_COSPROD, BF_TABLE = 0, 1, 2, 3, 4
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -37,6 +37,7 @@ class Problem(C.Structure):
         ("tr_kind", C.c_int), ("tr_axis", C.c_int), ("tr_x0", C.c_double),
         ("density", C.c_double), ("mass_factor", C.c_double), ("mass_hist", C.c_void_p),
         ("flux_kind", C.c_int), ("flux_p", C.c_double * 8),
+        ("bf_tab", C.c_void_p), ("tr_tab", C.c_void_p),
     ]
 
 
@@ -122,6 +123,16 @@ class OracleProblem:
         p.bf_kind = prob.bf_kind
         for i, v in enumerate(prob.bf):
             p.bf[i] = v
+        meta = getattr(prob, "meta", {})
+        if meta.get("bf_table") is not None:
+            bt = np.ascontiguousarray(meta["bf_table"], np.float64)
+            self.keep.append(bt)
+            p.bf_kind = BF_TABLE
+            p.bf_tab = _ptr(bt)
+        if meta.get("traction_table") is not None:
+            tt = np.ascontiguousarray(meta["traction_table"], np.float64)
+            self.keep.append(tt)
+            p.tr_tab = _ptr(tt)
         if prob.faces is not None and len(prob.faces):
             faces = np.ascontiguousarray(prob.faces, np.int32)
             self.keep.append(faces)
