@@ -288,6 +288,21 @@ int ins_set_csr(ins_ctx* ctx, int64_t n, const int32_t* row_ptr,
  * last iterate is still available). */
 int ins_solve(ins_ctx* ctx, int mode, int format, double rtol, int max_it,
               int* iters, double* relres);
+/* Eigen3::choleskySolve / luSolve / superLUSolve (base/solver/Eigen3.hpp:
+ * 157-230), which return the factorised solution: the FP64 system as a dense
+ * n x n matrix on the device, factorised by rocSOLVER -- kind
+ * INS_DIRECT_CHOLESKY: potrf + potrs (the lower triangle; SimplicialLDLT's
+ * symmetric positive definite case), INS_DIRECT_LU: getrf + getrs with
+ * partial pivoting (SparseLU / SuperLU).  relres = |b - A x| / |b| in FP64,
+ * as after ins_solve.  n is at most INS_DIRECT_MAX_N (the factor holds 8 n^2
+ * bytes; larger systems: INS_ERR_UNSUPPORTED, use ins_solve).
+ * INS_ERR_BREAKDOWN when Cholesky meets a non-positive pivot (the matrix is
+ * not positive definite) or LU an exactly zero pivot.  Needs an FP64
+ * assembly (or ins_set_csr) and the RHS. */
+#define INS_DIRECT_CHOLESKY 0
+#define INS_DIRECT_LU       1
+#define INS_DIRECT_MAX_N    32768
+int ins_solve_direct(ins_ctx* ctx, int kind, double* relres);
 /* Build what ins_solve(mode, format) needs (SELL form and its values,
  * solver vectors) without solving, so that a later solve or server attach
  * allocates nothing.  With INS_FMT_SELL the SELL form is kept even where its

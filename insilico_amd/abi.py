@@ -23,6 +23,7 @@ BF_NONE, BF_CONST, BF_SINPROD, BF_COSPROD, BF_TABLE = 0, 1, 2, 3, 4
 SOLVE_FP64, SOLVE_MIXED = 0, 1
 FMT_CSR, FMT_SELL = 0, 1
 DIST_SET, DIST_ADD = 0, 1
+DIRECT_CHOLESKY, DIRECT_LU, DIRECT_MAX_N = 0, 1, 32768
 
 # every symbol include/insilico_hip.h declares
 SYMBOLS = [
@@ -38,6 +39,7 @@ SYMBOLS = [
     "ins_bdf_init", "ins_bdf_step", "ins_set_traction_profile", "ins_cell_eval",
     "ins_set_neumann_flux", "ins_device_memory", "ins_solve_csr_noalloc",
     "ins_quadrature_rule", "ins_set_body_force_values", "ins_set_traction_values",
+    "ins_solve_direct",
 ]
 SLOT_FREE, SLOT_RUNNING, SLOT_FINISHED = 0, 1, 2
 REF_ZERO, REF_SINPROD, REF_KELVIN, REF_COSPROD = 0, 1, 2, 3
@@ -113,6 +115,7 @@ def load():
         "ins_rhs_norm": [vp, C.POINTER(dbl)],
         "ins_set_csr": [vp, i64, vp, vp, vp],
         "ins_solve": [vp, i32, i32, dbl, i32, C.POINTER(i32), C.POINTER(dbl)],
+        "ins_solve_direct": [vp, i32, C.POINTER(dbl)],
         "ins_get_solution": [vp, vp],
         "ins_solution_norm": [vp, C.POINTER(dbl)],
         "ins_distribute": [vp, i32],
@@ -365,6 +368,14 @@ class Context:
         if check:
             self._chk(s)
         return it.value, rr.value, s
+
+    def solve_direct(self, kind=DIRECT_CHOLESKY, check=True):
+        """dense Cholesky / LU of the FP64 system; returns (relres, status)"""
+        rr = C.c_double()
+        s = self.L.ins_solve_direct(self.h, kind, C.byref(rr))
+        if check:
+            self._chk(s)
+        return rr.value, s
 
     def prepare(self, mode=SOLVE_MIXED, fmt=FMT_SELL):
         self._chk(self.L.ins_prepare(self.h, mode, fmt))

@@ -826,6 +826,15 @@ int ins_solve(ins_ctx* h, int mode, int format, double rtol, int max_it, int* it
     return solve(c, mode, format, rtol, max_it, iters, relres);
 }
 
+int ins_solve_direct(ins_ctx* h, int kind, double* relres)
+{
+    Ctx* c = reinterpret_cast<Ctx*>(h);
+    CTX_CHECK(c);
+    if (kind != INS_DIRECT_CHOLESKY && kind != INS_DIRECT_LU)
+        return fail(c, INS_ERR_ARG, "ins_solve_direct: kind");
+    return solve_direct(c, kind, relres);
+}
+
 int ins_prepare(ins_ctx* h, int mode, int format)
 {
     Ctx* c = reinterpret_cast<Ctx*>(h);

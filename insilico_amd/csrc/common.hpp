@@ -363,6 +363,7 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
           double* relres);
 int build_sell(Ctx* c, bool force);
 int prepare(Ctx* c, int mode, int format);
+int solve_direct(Ctx* c, int kind, double* relres);
 int distribute(Ctx* c, int op);
 int bdf_step(Ctx* c, double dt, int precision, int mode, int format, double rtol, int max_it,
              int* iters, double* relres);

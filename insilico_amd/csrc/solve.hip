@@ -28,6 +28,7 @@
 #include <cstdlib>
 
 #include <hip/hip_ext.h>
+#include <rocsolver/rocsolver.h>
 
 #include "common.hpp"
 
@@ -2571,6 +2572,124 @@ int vec_norm2(Ctx* c, const double* v, int64_t n, double* out)
     INS_HIP_CHECK(c, hipGetLastError());
     INS_TRY(read_scal(c));
     *out = std::sqrt(c->h_scal->tt);
+    return INS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Direct solve: the FP64 system as a dense column-major matrix, factorised by
+// rocSOLVER (potrf/potrs or getrf/getrs) on the context's stream.
+namespace {
+
+// CSR -> dense column-major: entry (r, j) at j * n + r; the rest is zero
+__global__ void k_csr_to_dense(int64_t n, const int32_t* __restrict__ rp,
+                               const int32_t* __restrict__ col, const double* __restrict__ val,
+                               double* __restrict__ A)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t k = rp[r]; k < rp[r + 1]; k++) A[(int64_t)col[k] * n + r] = val[k];
+}
+
+struct DenseScratch {
+    double* A = nullptr;
+    rocblas_int* info = nullptr;
+    rocblas_int* ipiv = nullptr;
+    rocblas_handle h = nullptr;
+    ~DenseScratch()
+    {
+        if (h) (void)rocblas_destroy_handle(h);
+        if (A) (void)hipFree(A);
+        if (info) (void)hipFree(info);
+        if (ipiv) (void)hipFree(ipiv);
+    }
+};
+
+int rocsolver_status(Ctx* c, rocblas_status st, const char* what)
+{
+    if (st == rocblas_status_success) return INS_OK;
+    return fail(c, INS_ERR_HIP, std::string(what) + ": rocBLAS status " + std::to_string((int)st));
+}
+
+}  // namespace
+
+int solve_direct(Ctx* c, int kind, double* relres)
+{
+    if (!c->have_matrix || !c->have_rhs)
+        return fail(c, INS_ERR_STATE, "ins_solve_direct before ins_assemble (matrix and rhs)");
+    if (c->matrix_precision != INS_FP64)
+        return fail(c, INS_ERR_STATE, "ins_solve_direct needs an FP64 assembly");
+    const int64_t n = c->n_active;
+    if (n > INS_DIRECT_MAX_N)
+        return fail(c, INS_ERR_UNSUPPORTED,
+                    "ins_solve_direct: " + std::to_string(n) + " rows exceed INS_DIRECT_MAX_N (" +
+                        std::to_string(INS_DIRECT_MAX_N) + "); solve with ins_solve");
+    INS_TRY(ensure_solver_state(c, INS_SOLVE_FP64));
+    INS_TRY(solve_vectors(c, INS_FMT_CSR, INS_SOLVE_FP64, c->stream));
+    c->have_solution = false;
+    c->stats.iterations = 0;
+    c->stats.refinements = 0;
+    INS_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+    double bnorm = 0.;
+    INS_TRY(vec_norm2(c, c->sb, n, &bnorm));
+    double rel = 0.;
+    if (n > 0 && bnorm > 0.) {
+        DenseScratch d;
+        const size_t bytes = (size_t)n * (size_t)n * sizeof(double);
+        if (hipMalloc((void**)&d.A, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(c, INS_ERR_OOM, "ins_solve_direct: dense factor of " +
+                                            std::to_string(bytes) + " bytes");
+        }
+        INS_HIP_CHECK(c, hipMalloc((void**)&d.info, sizeof(rocblas_int)));
+        INS_HIP_CHECK(c, hipMemsetAsync(d.A, 0, bytes, c->stream));
+        k_csr_to_dense<<<grid_for(n, 256), 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p,
+                                                                c->val64.p, d.A);
+        INS_HIP_CHECK(c, hipGetLastError());
+        INS_HIP_CHECK(c, hipMemcpyAsync(c->sx64, c->sb, (size_t)n * sizeof(double),
+                                        hipMemcpyDeviceToDevice, c->stream));
+        INS_TRY(rocsolver_status(c, rocblas_create_handle(&d.h), "rocblas_create_handle"));
+        INS_TRY(rocsolver_status(c, rocblas_set_stream(d.h, c->stream), "rocblas_set_stream"));
+        const rocblas_int m = (rocblas_int)n;
+        if (kind == INS_DIRECT_CHOLESKY) {
+            INS_TRY(rocsolver_status(
+                c, rocsolver_dpotrf(d.h, rocblas_fill_lower, m, d.A, m, d.info), "rocsolver_dpotrf"));
+        } else {
+            INS_HIP_CHECK(c, hipMalloc((void**)&d.ipiv, (size_t)n * sizeof(rocblas_int)));
+            INS_TRY(rocsolver_status(c, rocsolver_dgetrf(d.h, m, m, d.A, m, d.ipiv, d.info),
+                                     "rocsolver_dgetrf"));
+        }
+        rocblas_int info = 0;
+        INS_HIP_CHECK(c, hipMemcpyAsync(&info, d.info, sizeof(info), hipMemcpyDeviceToHost,
+                                        c->stream));
+        INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        if (info != 0)
+            return fail(c, INS_ERR_BREAKDOWN,
+                        std::string(kind == INS_DIRECT_CHOLESKY
+                                        ? "Cholesky factorisation: non-positive pivot "
+                                        : "LU factorisation: zero pivot ") +
+                            std::to_string((int)info) + " of " + std::to_string(n));
+        if (kind == INS_DIRECT_CHOLESKY)
+            INS_TRY(rocsolver_status(
+                c, rocsolver_dpotrs(d.h, rocblas_fill_lower, m, 1, d.A, m, c->sx64, m),
+                "rocsolver_dpotrs"));
+        else
+            INS_TRY(rocsolver_status(c, rocsolver_dgetrs(d.h, rocblas_operation_none, m, 1, d.A, m,
+                                                         d.ipiv, c->sx64, m),
+                                     "rocsolver_dgetrs"));
+        INS_TRY(launch_resid(c, INS_FMT_CSR));
+        INS_TRY(read_scal(c));
+        rel = std::sqrt(c->h_scal->tt) / bnorm;
+    } else {
+        INS_HIP_CHECK(c, hipMemsetAsync(c->sx64, 0, (size_t)n * sizeof(double), c->stream));
+    }
+    INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+    INS_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    INS_HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.solve_ms = ms;
+    c->stats.relres = rel;
+    if (relres) *relres = rel;
+    c->have_solution = true;
     return INS_OK;
 }
 

@@ -33,6 +33,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -296,6 +297,92 @@ inline std::vector<std::vector<int32_t>> boundaryFieldFaces(const Mesh& m, const
     return out;
 }
 
+namespace detail_ {
+// The library's quadrature rule (ins_quadrature_rule): weights and points,
+// the first coordinate fastest
+inline void quadratureRule(int dim, int npts, std::vector<double>& w, std::vector<double>& xi)
+{
+    int nq = 1;
+    for (int d = 0; d < dim; d++) nq *= npts;
+    w.assign((std::size_t)nq, 0.);
+    xi.assign((std::size_t)nq * (dim > 0 ? dim : 1), 0.);
+    if (ins_quadrature_rule(dim, npts, w.data(), xi.data()) != nq)
+        throw Error(INS_ERR_ARG, "quadrature rule: dim " + std::to_string(dim) + ", " +
+                                     std::to_string(npts) + " points per direction");
+}
+
+// x(xi) of element e's Q1 geometry: vertices in hierarchic order, vertex v
+// at the corner (v0 ^ v1, v1, v2) of the reference cube, v_d the bits of v
+// (base/mesh/HierarchicOrder.hpp:201-272)
+inline std::array<double, 3> q1Point(const Mesh& m, std::size_t e, const double* xi)
+{
+    std::array<double, 3> x{0., 0., 0.};
+    const int npe = m.npe(), dim = m.dim;
+    for (int v = 0; v < npe; v++) {
+        const int bit[3] = {(v & 1) ^ ((v >> 1) & 1), (v >> 1) & 1, (v >> 2) & 1};
+        double phi = 1.;
+        for (int d = 0; d < dim; d++) phi *= bit[d] ? xi[d] : 1. - xi[d];
+        const std::size_t node = (std::size_t)m.conn[e * npe + v];
+        for (int d = 0; d < dim; d++) x[d] += phi * m.xyz[node * dim + d];
+    }
+    return x;
+}
+
+// Coordinates of a node of a face list: a mesh node (degree 1) or the
+// support point of a field node (degree 2, the face's vertices come first)
+inline const double* faceNodeX(const Mesh& m, const Field& f, int32_t node)
+{
+    return f.degree == 2 ? &f.support[(std::size_t)node * m.dim] : &m.xyz[(std::size_t)node * m.dim];
+}
+
+// Surface point x(eta) and unit normal of base::SurfaceNormal
+// (base/geometry.hpp:255-346) on a face given by its corner nodes in the
+// element face-table order: 3-D the cross product of the surface Jacobian's
+// columns of the bilinear face map, 2-D (t_y, -t_x) of the edge tangent
+inline void facePoint(const Mesh& m, const Field& f, const std::vector<int32_t>& face,
+                      const double* eta, std::array<double, 3>& x, std::array<double, 3>& n)
+{
+    x = {0., 0., 0.};
+    n = {0., 0., 0.};
+    if (m.dim == 3) {
+        const double e0 = eta[0], e1 = eta[1];
+        const double phi[4] = {(1 - e0) * (1 - e1), e0 * (1 - e1), e0 * e1, (1 - e0) * e1};
+        const double d0[4] = {-(1 - e1), 1 - e1, e1, -e1};
+        const double d1[4] = {-(1 - e0), -e0, e0, 1 - e0};
+        double J0[3] = {0., 0., 0.}, J1[3] = {0., 0., 0.};
+        for (int k = 0; k < 4; k++) {
+            const double* X = faceNodeX(m, f, face[k]);
+            for (int d = 0; d < 3; d++) {
+                x[d] += phi[k] * X[d];
+                J0[d] += d0[k] * X[d];
+                J1[d] += d1[k] * X[d];
+            }
+        }
+        n = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2],
+             J0[0] * J1[1] - J0[1] * J1[0]};
+    } else {
+        const double* X0 = faceNodeX(m, f, face[0]);
+        const double* X1 = faceNodeX(m, f, face[1]);
+        for (int d = 0; d < 2; d++) x[d] = (1 - eta[0]) * X0[d] + eta[0] * X1[d];
+        n = {X1[1] - X0[1], -(X1[0] - X0[0]), 0.};
+    }
+    const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int d = 0; d < 3; d++) n[d] /= len;
+}
+
+// component c of a caller's function value: an indexable vector or a scalar
+template <typename R>
+inline double component(const R& r, int c)
+{
+    if constexpr (std::is_arithmetic<R>::value) {
+        (void)c;
+        return (double)r;
+    } else {
+        return r[c];
+    }
+}
+}  // namespace detail_
+
 template <typename DIRIFUN>
 void constrainBoundary(const Mesh& m, Field& f, DIRIFUN diriFun)
 {
@@ -331,6 +418,114 @@ void constrainBoundary(const Mesh& m, Field& f, DIRIFUN diriFun)
 // Quadrature, kernels and materials (tags with their parameters)
 namespace base {
 enum Shape { QUAD = 4, HEX = 8 };
+template <Shape SHAPE>
+struct ShapeDim {
+    static constexpr unsigned value = SHAPE == HEX ? 3 : 2;
+};
+
+// Small fixed-size vector for the caller's functions (the reference's
+// base::Vector<N>::Type is an Eigen vector, base/linearAlgebra.hpp:45-60):
+// element access, the arithmetic and norms such functions use.
+template <unsigned N>
+struct VecN {
+    double v[N] = {};
+    double& operator[](std::size_t i) { return v[i]; }
+    double operator[](std::size_t i) const { return v[i]; }
+    double& operator()(std::size_t i) { return v[i]; }
+    double operator()(std::size_t i) const { return v[i]; }
+    static constexpr std::size_t size() { return N; }
+    static VecN Constant(double a)
+    {
+        VecN r;
+        for (unsigned i = 0; i < N; i++) r.v[i] = a;
+        return r;
+    }
+    static VecN Zero() { return Constant(0.); }
+    void setZero() { *this = Zero(); }
+    double dot(const VecN& o) const
+    {
+        double s = 0.;
+        for (unsigned i = 0; i < N; i++) s += v[i] * o.v[i];
+        return s;
+    }
+    double squaredNorm() const { return dot(*this); }
+    double norm() const { return std::sqrt(squaredNorm()); }
+    VecN& operator+=(const VecN& o)
+    {
+        for (unsigned i = 0; i < N; i++) v[i] += o.v[i];
+        return *this;
+    }
+    VecN& operator-=(const VecN& o)
+    {
+        for (unsigned i = 0; i < N; i++) v[i] -= o.v[i];
+        return *this;
+    }
+    VecN& operator*=(double a)
+    {
+        for (unsigned i = 0; i < N; i++) v[i] *= a;
+        return *this;
+    }
+    friend VecN operator+(VecN a, const VecN& b) { return a += b; }
+    friend VecN operator-(VecN a, const VecN& b) { return a -= b; }
+    friend VecN operator*(VecN a, double s) { return a *= s; }
+    friend VecN operator*(double s, VecN a) { return a *= s; }
+    friend VecN operator/(VecN a, double s) { return a *= 1. / s; }
+    friend VecN operator-(VecN a) { return a *= -1.; }
+};
+template <unsigned N, typename T = double>
+struct Vector {
+    typedef VecN<N> Type;
+};
+
+// base::Unstructured<SHAPE, GDEG> (base/Unstructured.hpp): the mesh with its
+// shape and dimension as compile-time constants
+template <Shape SHAPE, unsigned GDEG = 1>
+struct Unstructured : insilico::Mesh {
+    struct Node {
+        static constexpr unsigned dim = ShapeDim<SHAPE>::value;
+    };
+    struct Element {
+        static constexpr Shape shape = SHAPE;
+        struct GeomFun {
+            static constexpr unsigned degree = GDEG;
+        };
+    };
+    Unstructured() { this->dim = (int)Node::dim; }
+};
+template <typename MESH>
+struct MeshDim {  // a mesh whose dimension is only known at run time: 3
+    static constexpr unsigned value = 3;
+};
+template <Shape S, unsigned G>
+struct MeshDim<Unstructured<S, G>> {
+    static constexpr unsigned value = ShapeDim<S>::value;
+};
+
+// An element as a caller's conductivity function sees it (the reference
+// passes the GeomElement pointer of the field tuple, heat/Laplace.hpp:
+// 113-126): its mesh and id, and base::Geometry to map a local coordinate
+// to the physical point (the Q1 map, base/geometry.hpp:46-90).
+template <unsigned DIM>
+struct GeomElement {
+    const insilico::Mesh* mesh = nullptr;
+    std::size_t id = 0;
+    std::size_t getID() const { return id; }
+};
+template <typename ELEMENT>
+struct Geometry;
+template <unsigned DIM>
+struct Geometry<GeomElement<DIM>> {
+    VecN<DIM> operator()(const GeomElement<DIM>* ep, const VecN<DIM>& xi) const
+    {
+        const insilico::Mesh& m = *ep->mesh;
+        double loc[3] = {0., 0., 0.};
+        for (unsigned d = 0; d < DIM && d < 3; d++) loc[d] = xi[d];
+        const std::array<double, 3> x = insilico::detail_::q1Point(m, ep->id, loc);
+        VecN<DIM> out;
+        for (unsigned d = 0; d < DIM && d < 3; d++) out[d] = x[d];
+        return out;
+    }
+};
 
 // base::Quadrature<DEG,SHAPE>: tensor Gauss-Legendre with (DEG+2)/2 points
 // per direction (base/quad/GaussLegendre.hpp:33-36)
@@ -338,9 +533,16 @@ template <unsigned DEG, Shape SHAPE>
 struct Quadrature {
     static constexpr int numPointsPerDir = (DEG + 2) / 2;
 };
+// base::SurfaceQuadrature<DEG, SHAPE> (base/Quadrature.hpp): the same rule
+// on the faces of SHAPE
+template <unsigned DEG, Shape SHAPE>
+struct SurfaceQuadrature {
+    static constexpr int numPointsPerDir = (DEG + 2) / 2;
+};
 
 template <typename MESH, typename FIELD>
 struct FieldBinder {  // base/asmb/FieldBinder.hpp:107-188 (one field)
+    static constexpr unsigned dim = MeshDim<MESH>::value;
     MESH& mesh;
     FIELD& field;
     FieldBinder(MESH& m, FIELD& f) : mesh(m), field(f) {}
@@ -349,6 +551,32 @@ struct FieldBinder {  // base/asmb/FieldBinder.hpp:107-188 (one field)
         using Type = FieldBinder;
     };
 };
+
+namespace mesh {
+// The surface mesh of a volume mesh's boundary (base::mesh::BoundaryMeshBinder,
+// generateBoundaryMesh, base/mesh/generateBoundaryMesh.hpp): the boundary
+// faces of the field, as node lists (insilico::boundaryFieldFaces), with the
+// volume mesh they belong to.
+struct BoundaryMesh {
+    const insilico::Mesh* volume = nullptr;
+    std::vector<std::vector<int32_t>> faces;
+};
+}  // namespace mesh
+
+namespace asmb {
+// base::asmb::SurfaceFieldBinder (base/asmb/SurfaceFieldBinder.hpp): the
+// field over the faces of a boundary mesh
+template <typename BMESH, typename FIELD>
+struct SurfaceFieldBinder {
+    BMESH& boundaryMesh;
+    FIELD& field;
+    SurfaceFieldBinder(BMESH& b, FIELD& f) : boundaryMesh(b), field(f) {}
+    template <int, int>
+    struct TupleBinder {
+        using Type = SurfaceFieldBinder;
+    };
+};
+}  // namespace asmb
 }  // namespace base
 
 namespace mat {
@@ -372,15 +600,40 @@ struct NeoHookeanCompressible {
 }  // namespace mat
 
 namespace heat {
-// heat::Laplace (heat/Laplace.hpp:85-97): a constant factor, or a
-// per-element conductivity set with setConductivityFunction
+namespace detail_ {
+template <typename FTB>
+struct TupleDim {
+    static constexpr unsigned value = 3;
+};
+template <typename M, typename F>
+struct TupleDim<base::FieldBinder<M, F>> {
+    static constexpr unsigned value = base::FieldBinder<M, F>::dim;
+};
+}  // namespace detail_
+
+// heat::Laplace (heat/Laplace.hpp:85-97, 113-126): a constant factor, or a
+// conductivity function kappa(element, xi) (setConductivityFunction), which
+// the assembly evaluates on the host at every element's quadrature points
+// and uploads as per-point values; or, as an extension, one conductivity
+// per element.
 template <typename FTB = void>
 struct Laplace {
+    static constexpr unsigned dim = detail_::TupleDim<FTB>::value;
+    typedef base::GeomElement<dim> GeomElement;
+    typedef base::VecN<dim> LocalVecDim;
+    typedef std::function<double(const GeomElement*, const LocalVecDim&)> ConductivityFun;
     double factor;
     std::vector<double> perElement;
+    ConductivityFun conductivityFun;
     explicit Laplace(double f) : factor(f) {}
+    void setConductivityFunction(ConductivityFun& cf)
+    {
+        conductivityFun = cf;
+        perElement.clear();
+    }
     void setConductivityFunction(std::function<double(std::size_t)> k, std::size_t numElements)
     {
+        conductivityFun = nullptr;
         perElement.resize(numElements);
         for (std::size_t e = 0; e < numElements; e++) perElement[e] = k(e);
     }
@@ -605,6 +858,22 @@ public:
         fluxP_ = p;
         flags_ |= INS_ASM_RHS;
     }
+    // a caller's body force as its values at the quadrature points
+    // (ins_set_body_force_values layout)
+    void requestBodyForceValues(std::vector<double> f)
+    {
+        bfTable_ = std::move(f);
+        flags_ |= INS_ASM_RHS;
+    }
+    // a caller's Neumann function as its values at the surface points of the
+    // faces, with nqs points per direction (ins_set_traction_values layout)
+    void requestTractionValues(std::vector<std::vector<int32_t>> faces, std::vector<double> t,
+                               int nqs)
+    {
+        requestTraction(std::move(faces), {0., 0., 0.});
+        trTable_ = std::move(t);
+        trNq_ = nqs;
+    }
     void setPrecision(int p) { precision_ = p; }
     // forget the recorded assembly requests and the solution (a new Newton
     // iteration re-assembles into the same device context, pattern kept)
@@ -618,6 +887,9 @@ public:
         trAxis_ = -1;
         trX0_ = 0.;
         fluxKind_ = INS_REF_ZERO;
+        bfTable_.clear();
+        trTable_.clear();
+        trNq_ = 0;
         solved_ = false;
         assembled_ = false;
         hostRhs_.clear();
@@ -673,12 +945,31 @@ public:
         return it;
     }
     // Direct solves (Eigen3::choleskySolve / luSolve / superLUSolve,
-    // base/solver/Eigen3.hpp:157-230) return the solution to round-off or
-    // abort.  Here: the FP64 PCG to 1e-13 relative residual, which either
-    // meets that bound or throws INS_ERR_NOCONV (as above).
-    void choleskySolve() { cgSolve(1e-13); }
-    void luSolve() { cgSolve(1e-13); }
-    void superLUSolve() { cgSolve(1e-13); }
+    // base/solver/Eigen3.hpp:157-230) return the factorised solution or
+    // abort: a dense device factorisation (ins_solve_direct) up to
+    // INS_DIRECT_MAX_N rows, throwing insilico::Error on a failed
+    // factorisation; larger systems run the FP64 PCG to 1e-13, which meets
+    // that bound or throws INS_ERR_NOCONV (as above).
+    void choleskySolve() { directSolve(INS_DIRECT_CHOLESKY); }
+    void luSolve() { directSolve(INS_DIRECT_LU); }
+    void superLUSolve() { directSolve(INS_DIRECT_LU); }
+
+    void directSolve(int kind)
+    {
+        if (n_ > INS_DIRECT_MAX_N) {
+            cgSolve(1e-13);
+            return;
+        }
+        double rel = 0.;
+        converged_ = false;
+        insilico::check(ins_solve_direct(ctx_, kind, &rel), ctx_);
+        converged_ = true;
+        lastRelres_ = rel;
+        lastIterations_ = 0;
+        x_.resize(n_);
+        insilico::check(ins_get_solution(ctx_, x_.data()), ctx_);
+        solved_ = true;
+    }
 
     double getValue(std::size_t i) const { return x_.at(i); }  // Eigen3.hpp:293-296
     double lastRelres() const { return lastRelres_; }
@@ -749,6 +1040,7 @@ private:
         insilico::check(ins_set_dof_values(ctx_, f.value.data()), ctx_);
         insilico::check(ins_set_material(ctx_, kernel_, nq_, ncoef_, coef_.data()), ctx_);
         insilico::check(ins_set_body_force(ctx_, bfKind_, bf_.data()), ctx_);
+        if (!bfTable_.empty()) insilico::check(ins_set_body_force_values(ctx_, bfTable_.data()), ctx_);
         std::vector<int32_t> flat;
         for (auto& fc : faces_) flat.insert(flat.end(), fc.begin(), fc.end());
         const int nfn = faces_.empty() ? 0 : (int)faces_[0].size();
@@ -757,6 +1049,12 @@ private:
         insilico::check(ins_set_traction_profile(ctx_, faces_.empty() ? -1 : trAxis_, trX0_), ctx_);
         insilico::check(ins_set_neumann_flux(ctx_, faces_.empty() ? INS_REF_ZERO : fluxKind_,
                                              fluxP_.data()), ctx_);
+        if (!trTable_.empty()) {
+            if (trNq_ != nq_)
+                throw insilico::Error(INS_ERR_ARG, "the surface quadrature must have the domain "
+                                                   "quadrature's points per direction");
+            insilico::check(ins_set_traction_values(ctx_, trTable_.data()), ctx_);
+        }
         insilico::check(ins_assemble(ctx_, precision_, flags_), ctx_);
     }
 
@@ -777,6 +1075,8 @@ private:
     double trX0_ = 0.;
     int fluxKind_ = INS_REF_ZERO;
     std::array<double, 8> fluxP_{0., 0., 0., 0., 0., 0., 0., 0.};
+    std::vector<double> bfTable_, trTable_;
+    int trNq_ = 0;
     std::vector<double> x_;
     double lastRelres_ = 0.;
     int lastIterations_ = 0;
@@ -791,9 +1091,32 @@ private:
 namespace asmb {
 
 namespace detail_ {
+// the conductivity per element, or with a conductivity function its values
+// at every element's quadrature points (ncoef = npts^dim)
 template <typename FTB>
-std::vector<double> laplaceCoef(const heat::Laplace<FTB>& k, std::size_t ne)
+std::vector<double> laplaceCoef(const heat::Laplace<FTB>& k, const insilico::Mesh& m, int npts,
+                                int& ncoef)
 {
+    const std::size_t ne = m.numElements();
+    ncoef = 1;
+    if (k.conductivityFun) {
+        std::vector<double> w, xi;
+        insilico::detail_::quadratureRule(m.dim, npts, w, xi);
+        const std::size_t nq = w.size();
+        typedef typename heat::Laplace<FTB>::GeomElement GE;
+        typedef typename heat::Laplace<FTB>::LocalVecDim LV;
+        std::vector<double> c(ne * nq);
+        for (std::size_t e = 0; e < ne; e++) {
+            const GE ge{&m, e};
+            for (std::size_t q = 0; q < nq; q++) {
+                LV loc;
+                for (int d = 0; d < m.dim && d < (int)LV::size(); d++) loc[d] = xi[q * m.dim + d];
+                c[e * nq + q] = k.conductivityFun(&ge, loc);
+            }
+        }
+        ncoef = (int)nq;
+        return c;
+    }
     if (!k.perElement.empty()) return k.perElement;
     return std::vector<double>(ne, k.factor);
 }
@@ -815,8 +1138,10 @@ void stiffnessMatrixComputation(const Quadrature<DEG, SH>&, solver::Hip& solver,
                                 const heat::Laplace<LFTB>& kernel, bool incremental = true)
 {
     solver.bind(fieldBinder.mesh, fieldBinder.field);
-    solver.requestMatrix(INS_LAPLACE, Quadrature<DEG, SH>::numPointsPerDir,
-                         detail_::laplaceCoef(kernel, fieldBinder.mesh.numElements()), 1, incremental);
+    const int npts = Quadrature<DEG, SH>::numPointsPerDir;
+    int ncoef = 1;
+    std::vector<double> coef = detail_::laplaceCoef(kernel, fieldBinder.mesh, npts, ncoef);
+    solver.requestMatrix(INS_LAPLACE, npts, std::move(coef), ncoef, incremental);
 }
 
 template <typename FTB, unsigned DEG, Shape SH, typename FB, typename SFTB>
@@ -844,6 +1169,34 @@ void computeResidualForces(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& 
     solver.bind(fieldBinder.mesh, fieldBinder.field);
     solver.requestResidual(MAT::id, Quadrature<DEG, SH>::numPointsPerDir,
                            detail_::elasticCoef(kernel, fieldBinder.mesh.numElements()), 2);
+}
+
+// computeResidualForces with the heat kernels (ForceIntegrator.hpp:40-56
+// with heat/Laplace.hpp residualForce): the kernel is linear, so the
+// residual forces -K u vanish while the field is zero -- the state in which
+// heat::PoissonDriver assembles (constraints do not change the values,
+// base/dof/DegreeOfFreedom.hpp:232-242).  A nonzero field is not supported.
+namespace detail_ {
+template <typename FB>
+void heatResidualAtZero(FB& fieldBinder)
+{
+    for (double v : fieldBinder.field.value)
+        if (v != 0.)
+            throw insilico::Error(INS_ERR_UNSUPPORTED,
+                                  "computeResidualForces of a heat kernel at a nonzero field");
+}
+}  // namespace detail_
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename LFTB>
+void computeResidualForces(const Quadrature<DEG, SH>&, solver::Hip&, FB& fieldBinder,
+                           const heat::Laplace<LFTB>&)
+{
+    detail_::heatResidualAtZero(fieldBinder);
+}
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename MAT, typename SFTB>
+void computeResidualForces(const Quadrature<DEG, SH>&, solver::Hip&, FB& fieldBinder,
+                           const heat::Static<MAT, SFTB>&)
+{
+    detail_::heatResidualAtZero(fieldBinder);
 }
 
 // built-in body forces: constant vector, or A * prod_d sin(k_d x_d)
@@ -883,6 +1236,74 @@ void bodyForceComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& f
 {
     solver.bind(fieldBinder.mesh, fieldBinder.field);
     solver.requestBodyForce(INS_BF_COSPROD, {f.amplitude, f.k[0], f.k[1], f.k[2]});
+}
+
+// bodyForceComputation<FTB>(quadrature, solver, fieldBinder, forceFun)
+// (base/asmb/BodyForce.hpp:67-82, 172-201) with the caller's function
+// f(x) -> VecDoF (or a double for a scalar field): evaluated here at x(xi_q)
+// of every element's quadrature points and uploaded as a per-point table
+// (ins_set_body_force_values); the device integrates f phi w detJ.
+template <typename FTB, unsigned DEG, Shape SH, typename FB, typename FUN>
+void bodyForceComputation(const Quadrature<DEG, SH>&, solver::Hip& solver, FB& fieldBinder,
+                          const FUN& forceFun)
+{
+    constexpr unsigned D = ShapeDim<SH>::value;
+    const insilico::Mesh& m = fieldBinder.mesh;
+    const int dpn = fieldBinder.field.size;
+    if (m.dim != (int)D) throw insilico::Error(INS_ERR_ARG, "quadrature shape and mesh dimension differ");
+    std::vector<double> w, xi;
+    insilico::detail_::quadratureRule(m.dim, Quadrature<DEG, SH>::numPointsPerDir, w, xi);
+    const std::size_t nq = w.size(), ne = m.numElements();
+    std::vector<double> tab(ne * nq * dpn);
+    for (std::size_t e = 0; e < ne; e++)
+        for (std::size_t q = 0; q < nq; q++) {
+            const std::array<double, 3> p = insilico::detail_::q1Point(m, e, &xi[q * D]);
+            VecN<D> x;
+            for (unsigned d = 0; d < D; d++) x[d] = p[d];
+            const auto f = forceFun(x);
+            for (int c = 0; c < dpn; c++)
+                tab[(e * nq + q) * dpn + c] = insilico::detail_::component(f, c);
+        }
+    solver.bind(fieldBinder.mesh, fieldBinder.field);
+    solver.requestBodyForceValues(std::move(tab));
+}
+
+// neumannForceComputation<SFTB>(surfaceQuadrature, solver, surfaceFieldBinder,
+// ff) (base/asmb/NeumannForce.hpp:36-63, 132-175) with the caller's
+// function ff(x, n) -> VecDoF: evaluated here at every boundary face's
+// surface points x(eta_q) with the unit normal n(eta_q), uploaded as a
+// per-point table (ins_set_traction_values); the device integrates
+// ff phi w detG.  The surface rule must have the domain rule's points per
+// direction (the reference drivers use one kernel degree for both).
+template <typename SFTB, unsigned DEG, Shape SH, typename BMESH, typename FIELD, typename FUN>
+void neumannForceComputation(const SurfaceQuadrature<DEG, SH>&, solver::Hip& solver,
+                             SurfaceFieldBinder<BMESH, FIELD>& sfb, const FUN& ff)
+{
+    constexpr unsigned D = ShapeDim<SH>::value;
+    const insilico::Mesh& m = *sfb.boundaryMesh.volume;
+    const insilico::Field& f = sfb.field;
+    const int dpn = f.size;
+    if (m.dim != (int)D) throw insilico::Error(INS_ERR_ARG, "quadrature shape and mesh dimension differ");
+    const int npts = SurfaceQuadrature<DEG, SH>::numPointsPerDir;
+    std::vector<double> w, eta;
+    insilico::detail_::quadratureRule(m.dim - 1, npts, w, eta);
+    const std::size_t nq = w.size();
+    const auto& faces = sfb.boundaryMesh.faces;
+    std::vector<double> tab(faces.size() * nq * dpn);
+    for (std::size_t k = 0; k < faces.size(); k++)
+        for (std::size_t q = 0; q < nq; q++) {
+            std::array<double, 3> p, nrm;
+            insilico::detail_::facePoint(m, f, faces[k], &eta[q * (m.dim - 1)], p, nrm);
+            VecN<D> x, n;
+            for (unsigned d = 0; d < D; d++) {
+                x[d] = p[d];
+                n[d] = nrm[d];
+            }
+            const auto t = ff(x, n);
+            for (int c = 0; c < dpn; c++)
+                tab[(k * nq + q) * dpn + c] = insilico::detail_::component(t, c);
+        }
+    solver.requestTractionValues(faces, std::move(tab), npts);
 }
 
 // constant traction on the given boundary faces

@@ -19,10 +19,8 @@ namespace insilico {
 
 // SMF mesh files (base/io/smf/Reader.hpp:97-312): '#' comments, '!' header
 // lines, "numNodes numElements", coordinates, connectivity
-inline Mesh readSMF(const std::string& path, int dim)
+inline Mesh readSMF(std::istream& in, int dim)
 {
-    std::ifstream in(path);
-    if (!in) throw Error(INS_ERR_ARG, "cannot open " + path);
     std::string line;
     Mesh m;
     m.dim = dim;
@@ -50,6 +48,12 @@ inline Mesh readSMF(const std::string& path, int dim)
         }
     }
     return m;
+}
+inline Mesh readSMF(const std::string& path, int dim)
+{
+    std::ifstream in(path);
+    if (!in) throw Error(INS_ERR_ARG, "cannot open " + path);
+    return readSMF(in, dim);
 }
 
 // Structured grid file (base/io/sgf/Reader.hpp:155-310): '#' comment lines,

@@ -14,12 +14,12 @@ pytestmark = pytest.mark.gpu
 APPS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "insilico_amd", "apps")
 
 
-def _run(*args, timeout=120):
+def _run(*args, timeout=120, cwd=None):
     exe = os.path.join(APPS, args[0])
     if not os.path.exists(exe):
         raise RuntimeError("%s not built (run __graft_entry__.build())" % exe)
     out = subprocess.run([exe] + [str(a) for a in args[1:]], capture_output=True, text=True,
-                         timeout=timeout)
+                         timeout=timeout, cwd=cwd)
     assert out.returncode == 0, out.stderr
     return out.stdout
 
@@ -264,3 +264,95 @@ def test_diffusion_driver_app(oracle, order):
     t = steps * dt
     ref = oracle.error_norm(prob, u, 0, oracle.REF_SINPROD, [math.exp(-2 * math.pi ** 2 * t), math.pi, math.pi])
     assert abs(vals["L2 error"] - ref) <= 1e-7 * ref
+
+
+def _spsolve(rp, ci, val, b):
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.linalg import spsolve
+    n = rp.shape[0] - 1
+    return spsolve(csr_matrix((val, ci, rp), shape=(n, n)).tocsc(), b)
+
+
+def _vertex_values(prob, u, dpn):
+    """field values at the mesh vertices (the Q2 field's vertex DoF objects)"""
+    npe = prob.conn.shape[1]
+    fnode = np.zeros(prob.xyz.shape[0], np.int64)
+    fnode[prob.conn.ravel()] = prob.fconn[:, :npe].ravel()
+    return u.reshape(-1, dpn)[fnode]
+
+
+@pytest.mark.parametrize("mesh,dim", [("square_20.smf", 2), ("cube.004.smf", 3)])
+def test_poisson_driver_app(oracle, golden, read_vtk, tmp_path, mesh, dim):
+    """reference/05-mixedPoisson/mixedPoissonWithDriver.cpp's shape on the
+    drop-in layer (base::BoundaryValueProblem<Mesh, 1, 2> and
+    heat::PoissonDriver, the Dirichlet function on the DoF objects, f(x)
+    and ff(x, n) as host functions evaluated by the layer at the quadrature
+    and surface points, direct solve): the same discrete problem in the
+    oracle -- Q2 field, kernel degree 4 (3 Gauss points per direction,
+    volume and surface), its built-in cosine body force and co-normal flux
+    -- solved by a sparse direct solve: |u| over the DoF objects to 1e-10,
+    the nodal values of test.vtk to its printed digits, in 2-D and 3-D."""
+    import shutil
+    shutil.copy(os.path.join(golden, mesh), tmp_path / mesh)
+    out = _run("mixed_poisson_driver", tmp_path / mesh, dim, cwd=str(tmp_path)).splitlines()
+    ndof, unorm = int(out[0].split()[-1]), float(out[1].split()[-1])
+    xyz, conn = P.read_smf(os.path.join(golden, mesh), dim)
+    prob = P.mixed_poisson_q2(xyz, conn)
+    prob.nq = 3
+    assert ndof == prob.n_active
+    rp, ci, val, b, _ = oracle.assemble(prob)
+    x = _spsolve(rp, ci, val, b)
+    u = prob.g.copy()
+    act = prob.dof_id >= 0
+    u[act] = x[prob.dof_id[act]]
+    assert abs(unorm - np.linalg.norm(u)) <= 1e-10 * np.linalg.norm(u), (unorm, np.linalg.norm(u))
+    vtk = read_vtk(str(tmp_path / "test.vtk"))
+    got = np.asarray(vtk["point_data"]["u"], np.float64).reshape(-1)
+    want = _vertex_values(prob, u, 1).reshape(-1)
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
+
+
+def test_compressible_driver_app(oracle, golden, read_vtk, tmp_path):
+    """reference/06-elastic/compressibleWithDriver.cpp's shape on the drop-in
+    layer (base::BoundaryValueProblem<Mesh, 2, 2>, solid::CompressibleDriver,
+    the PulledSheet Dirichlet function on the DoF objects, 5 load steps of
+    u_x = 3/5 per step at x = 1): every load step converges within maxIter,
+    and the final field equals the oracle's full Newton iteration on the
+    same discrete problem (kernel degree 4: 3 Gauss points per direction,
+    incremental lifting, sparse direct solves) -- |u| to 1e-9, the nodal
+    displacements of the VTK file to its printed digits."""
+    import shutil
+    shutil.copy(os.path.join(golden, "square_20.smf"), tmp_path / "quad.020.smf")
+    (tmp_path / "input.dat").write_text(
+        "E 1000.\nnu 0.3\npull 3.0\nmaxIter 30\nloadSteps 5\ntolerance 1.e-12\n")
+    out = _run("compressible_driver", tmp_path / "quad.020.smf", tmp_path / "input.dat",
+               timeout=300).splitlines()
+    steps = [tuple(int(v) for v in ln.split()) for ln in out[:5]]
+    assert [s for s, _ in steps] == list(range(5))
+    assert all(1 <= it < 30 for _, it in steps), steps
+    unorm = float(out[5].split()[-1])
+    prob = P.pulled_sheet_q2(20, True)
+    prob.nq = 3
+    n, dim = prob.n_active, prob.dim
+    u = np.zeros(prob.n_fnodes * dim)
+    g = np.zeros_like(u)
+    con = prob.dof_id < 0
+    act = ~con
+    for step in range(5):
+        g[prob.meta["right_x"] * dim] = 0.6 * (step + 1)
+        prob.g = g
+        for it in range(30):
+            rp, ci, val, b, _ = oracle.assemble(prob, u=u, incremental=True, residual=True)
+            if np.linalg.norm(b) / n < 1e-12:
+                break
+            dx = _spsolve(rp, ci, val, b)
+            u[act] += dx[prob.dof_id[act]]
+            u[con] = g[con]
+            if np.linalg.norm(dx) / n < 1e-12:
+                break
+    ref = np.linalg.norm(u)
+    assert abs(unorm - ref) <= 1e-9 * ref, (unorm, ref)
+    vtk = read_vtk(str(tmp_path / "quad.020.vtk"))
+    got = np.asarray(vtk["point_data"]["u"], np.float64)[:, :dim]
+    want = _vertex_values(prob, u, dim)
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
