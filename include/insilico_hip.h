@@ -114,6 +114,9 @@ typedef struct ins_stats {
     int     sym_block;      /* their block size: 1 = scalar rows, 2 or 3 =
                                node blocks of a vector problem (offsets in
                                sell_sym_offsets are then node offsets)        */
+    int     assembled_sym;  /* 1: the last matrix assembly wrote the
+                               symmetric slices directly (the CSR values are
+                               produced from them on demand); 0: the CSR */
 } ins_stats;
 
 /* ---- context ------------------------------------------------------------ */
@@ -307,8 +310,13 @@ int ins_solve_direct(ins_ctx* ctx, int kind, double* relres);
  * solver vectors) without solving, so that a later solve or server attach
  * allocates nothing.  With INS_FMT_SELL the SELL form is kept even where its
  * padding exceeds 1/8 of the CSR values (ins_solve alone would fall back to
- * CSR there).  No reference counterpart (Eigen builds its solver state
- * inside cgSolve, base/solver/Eigen3.hpp:265-275). */
+ * CSR there).  May be called before the first ins_assemble (after
+ * ins_build_pattern): the storage it names, like that of the last
+ * ins_solve, is the one the next assembly writes; for a problem with
+ * symmetric slices and INS_FMT_SELL that is the slices themselves, and the
+ * CSR values are rebuilt from them when an export or a CSR solve asks for
+ * them.  No reference counterpart (Eigen builds its solver state inside
+ * cgSolve, base/solver/Eigen3.hpp:265-275). */
 int ins_prepare(ins_ctx* ctx, int mode, int format);
 /* Eigen3::getValue for all rows (base/solver/Eigen3.hpp:293-296) */
 int ins_get_solution(ins_ctx* ctx, double* x);

@@ -207,8 +207,10 @@ void setup(ins_ctx* c, const Problem& P)
         CK(c, ins_set_traction(c, 0, 0, nullptr, nullptr));
     int64_t nnz = 0;
     CK(c, ins_build_pattern(c, &nnz));
-    CK(c, ins_assemble(c, INS_FP64, INS_ASM_MATRIX | INS_ASM_RHS));
+    // the SELL form first: the assembly then writes the slices the batched
+    // solve reads (the symmetric ones where the problem has them)
     CK(c, ins_prepare(c, INS_SOLVE_MIXED, INS_FMT_SELL));
+    CK(c, ins_assemble(c, INS_FP64, INS_ASM_MATRIX | INS_ASM_RHS));
 }
 
 // ---- worker pool: generation + setup + assembly on a request's context ----------

@@ -78,6 +78,9 @@ static void release_all(Ctx* c)
     release(c, c->xp); release(c, c->bp); release(c, c->d64p); release(c, c->d32p);
     release(c, c->pinfo);
     release(c, c->bftab); release(c, c->ttab);
+    release(c, c->sell.vs64); release(c, c->sell.spw); release(c, c->sell.anode);
+    release(c, c->sell.vs); release(c, c->sell.su); release(c, c->sell.smap);
+    release(c, c->emask);
 }
 
 // A context detached from a server may still be named by the batched work
@@ -98,6 +101,9 @@ static void invalidate_pattern(Ctx* c)
     c->have_rhs = false;
     c->have_solution = false;
     c->sell.built = false;
+    c->sell.direct = false;
+    c->csr_stale = false;
+    c->have_emask = false;
 }
 
 }  // namespace ins
@@ -724,6 +730,7 @@ int ins_get_csr_values(ins_ctx* h, double* val)
     Ctx* c = reinterpret_cast<Ctx*>(h);
     CTX_CHECK(c);
     if (!c->have_matrix || !val) return fail(c, INS_ERR_STATE, "no assembled matrix");
+    INS_TRY(ensure_csr(c));
     if (c->matrix_precision == INS_FP64) return download(c, val, c->val64, (size_t)c->nnz);
     std::vector<float> tmp((size_t)c->nnz);
     INS_TRY(download(c, tmp.data(), c->val32, (size_t)c->nnz));
@@ -808,6 +815,7 @@ int ins_set_csr(ins_ctx* h, int64_t n, const int32_t* row_ptr, const int32_t* co
     c->have_matrix = true;
     c->have_solution = false;
     c->csr_from_host = true;
+    c->csr_stale = false;
     c->sell.built = false;
     c->sell.stale = true;
     INS_TRY(compute_dinv(c));

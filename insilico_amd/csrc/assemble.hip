@@ -1027,6 +1027,16 @@ struct KeElastic {
     }
 };
 
+// v + f * m with the product rounded on its own (never fused into a
+// multiply-add), so that every gather kernel adds the scaled mass terms with
+// the same bits
+template <typename TA>
+__device__ inline TA add_scaled(TA v, double f, double m)
+{
+#pragma clang fp contract(off)
+    return v + (TA)(f * m);
+}
+
 // Gather: G lanes per node, lane j handles neighbour j of the node and the
 // S x S block of CSR entries coupling the two nodes.
 template <int NPE, int S, class KE, typename TA, class ME>
@@ -1080,8 +1090,9 @@ __global__ __launch_bounds__(BLK) void k_gather(
                 for (int cr = 0; cr < S; cr++)
 #pragma unroll
                     for (int cc = 0; cc < S; cc++)
-                        v[cr][cc] += pass == 0 ? (TA)ke.get(e, la, cr, lb, cc)
-                                               : (TA)(mfac * me.get(e, la, cr, lb, cc));
+                        v[cr][cc] = pass == 0 ? v[cr][cc] + (TA)ke.get(e, la, cr, lb, cc)
+                                              : add_scaled<TA>(v[cr][cc], mfac,
+                                                               me.get(e, la, cr, lb, cc));
             }
         } else {
             for (int32_t ia = as; ia < ae; ia++) {
@@ -1098,8 +1109,9 @@ __global__ __launch_bounds__(BLK) void k_gather(
                 for (int cr = 0; cr < S; cr++)
 #pragma unroll
                     for (int cc = 0; cc < S; cc++)
-                        v[cr][cc] += pass == 0 ? (TA)ke.get(e, la, cr, lb, cc)
-                                               : (TA)(mfac * me.get(e, la, cr, lb, cc));
+                        v[cr][cc] = pass == 0 ? v[cr][cc] + (TA)ke.get(e, la, cr, lb, cc)
+                                              : add_scaled<TA>(v[cr][cc], mfac,
+                                                               me.get(e, la, cr, lb, cc));
             }
         }
         const int32_t off = nboff[nbs + j];
@@ -1138,6 +1150,8 @@ __global__ __launch_bounds__(BLK) void k_gather(
 
 // RHS per node: lifting (per element: t = -sum_c g_c K_rc, then b += t),
 // element forces, face forces; each in ascending element / face order.
+// emask (when set) names the prescribed components of every element, so the
+// lifting visits only those; the sums are the same as over all components.
 template <int NPE, int S, class KE, class ME>
 __global__ __launch_bounds__(BLK) void k_rhs(
     int64_t n_nodes, int lg_npe, const int32_t* __restrict__ adj_ptr,
@@ -1146,7 +1160,8 @@ __global__ __launch_bounds__(BLK) void k_rhs(
     const double* __restrict__ u, int incremental, int lift, KE ke,
     const double* __restrict__ fe, int lg_nfn, const int32_t* __restrict__ fadj_ptr,
     const int32_t* __restrict__ fadj, const double* __restrict__ ff, int nfn, ME me,
-    double mfac, const double* __restrict__ hist, double* __restrict__ b)
+    double mfac, const double* __restrict__ hist, const uint32_t* __restrict__ emask,
+    double* __restrict__ b)
 {
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
          n += (int64_t)gridDim.x * blockDim.x) {
@@ -1161,14 +1176,25 @@ __global__ __launch_bounds__(BLK) void k_rhs(
                     const int64_t e = adj[k] >> lg_npe;
                     const int a = adj[k] & mask;
                     double t = 0.;
-                    for (int bnode = 0; bnode < NPE; bnode++) {
-                        const int64_t mnode = conn[e * NPE + bnode];
-#pragma unroll
-                        for (int cc = 0; cc < S; cc++) {
-                            const int64_t idx = mnode * S + cc;
-                            if (dof_id[idx] >= 0) continue;
+                    if (emask) {
+                        // the set bits ascend as (bnode, cc) do below
+                        for (uint32_t m = emask[e]; m; m &= m - 1) {
+                            const int bit = __builtin_ctz(m);
+                            const int bnode = bit / S, cc = bit - bnode * S;
+                            const int64_t idx = (int64_t)conn[e * NPE + bnode] * S + cc;
                             const double gv = incremental ? g[idx] - u[idx] : g[idx];
                             t -= gv * ke.get(e, a, cr, bnode, cc);
+                        }
+                    } else {
+                        for (int bnode = 0; bnode < NPE; bnode++) {
+                            const int64_t mnode = conn[e * NPE + bnode];
+#pragma unroll
+                            for (int cc = 0; cc < S; cc++) {
+                                const int64_t idx = mnode * S + cc;
+                                if (dof_id[idx] >= 0) continue;
+                                const double gv = incremental ? g[idx] - u[idx] : g[idx];
+                                t -= gv * ke.get(e, a, cr, bnode, cc);
+                            }
                         }
                     }
                     acc += t;
@@ -1213,6 +1239,111 @@ __global__ __launch_bounds__(BLK) void k_rhs(
             }
             b[r] = acc;
         }
+    }
+}
+
+// prescribed components of every element (bit b * S + c: component c of local
+// node b), for the RHS lifting
+template <int NPE, int S>
+__global__ void k_emask(int64_t E, const int32_t* __restrict__ conn,
+                        const int32_t* __restrict__ dof_id, uint32_t* __restrict__ emask)
+{
+    static_assert(NPE * S <= 32, "one 32-bit word per element");
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int bnode = 0; bnode < NPE; bnode++) {
+            const int64_t node = conn[e * NPE + bnode];
+#pragma unroll
+            for (int cc = 0; cc < S; cc++)
+                if (dof_id[node * S + cc] < 0) m |= 1u << (bnode * S + cc);
+        }
+        emask[e] = m;
+    }
+}
+
+// Assembly straight into the symmetric slices: one lane per active node a
+// (a wavefront holds one 64-node slice), for every stored offset u_k the
+// D x D block (a, a + u_k) summed over the elements the slot's word names,
+// in node a's adjacency order (ascending element id, as k_gather sums the
+// same entries, so the values are the bits of the CSR upper entries);
+// stored k-major and lane-contiguous (a wavefront writes 64 consecutive
+// values per slot) in FP64 and FP32.  The diagonal slot gives the Jacobi
+// diagonal (1/a_ii, or 1 where a_ii = 0) of rows a * D + i.  A second pass
+// adds the scaled mass matrices of a time step after all stiffness
+// contributions, as k_gather does.
+template <int NPE, int D, class KE, typename TA, class ME>
+__global__ __launch_bounds__(BLK) void k_sym_gather(
+    int64_t N, int nu, const int32_t* __restrict__ anode, const int32_t* __restrict__ adj_ptr,
+    const int32_t* __restrict__ adj, int lg_npe, const uint32_t* __restrict__ spw, KE ke, ME me,
+    double mfac, double* __restrict__ vs64, float* __restrict__ vs32, double* __restrict__ d64,
+    float* __restrict__ d32)
+{
+    constexpr int T = D * (D + 1) / 2, D2 = D * D;
+    constexpr int BITS = PairWord<NPE>::bits, FLAG = PairWord<NPE>::flag;
+    static_assert(NPE <= 8, "32-bit words, at most 8 elements per node");
+    const int64_t nbv = T + (int64_t)(nu - 1) * D2;
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < N;
+         a += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t A = anode[a];
+    const int32_t as = adj_ptr[A];
+    const int cnt = adj_ptr[A + 1] - as;
+    const int mask = (1 << lg_npe) - 1;
+    int32_t xa[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) xa[i] = i < cnt ? adj[as + i] : 0;
+    double* __restrict__ o64 = vs64 ? vs64 + (a >> 6) * 64 * nbv + (a & 63) : nullptr;
+    float* __restrict__ o32 = vs32 + (a >> 6) * 64 * nbv + (a & 63);
+    const uint32_t* __restrict__ wp = spw + (a >> 6) * 64 * (int64_t)nu + (a & 63);
+    const int npass = me.p ? 2 : 1;
+    for (int k = 0; k < nu; k++) {
+        const uint32_t w = wp[(int64_t)k * 64];
+        TA v[D][D];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) v[i][j] = (TA)0;
+        for (int pass = 0; pass < npass; pass++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int nib = (int)((w >> (BITS * i)) & (2 * FLAG - 1));
+                if (i >= cnt || !nib) continue;
+                const int64_t e = xa[i] >> lg_npe;
+                const int la = xa[i] & mask, lb = nib & (FLAG - 1);
+#pragma unroll
+                for (int cr = 0; cr < D; cr++)
+#pragma unroll
+                    for (int cc = 0; cc < D; cc++)
+                        v[cr][cc] = pass == 0 ? v[cr][cc] + (TA)ke.get(e, la, cr, lb, cc)
+                                              : add_scaled<TA>(v[cr][cc], mfac,
+                                                               me.get(e, la, cr, lb, cc));
+            }
+        if (k == 0) {
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+#pragma unroll
+                for (int j = i; j < D; j++) {
+                    const int64_t slot = i * D - i * (i - 1) / 2 + (j - i);
+                    if (o64) o64[slot * 64] = (double)v[i][j];
+                    o32[slot * 64] = (float)v[i][j];
+                }
+                const double d = o64 ? (double)v[i][i] : (double)(float)v[i][i];
+                const double id = d != 0. ? 1. / d : 1.;
+                d64[a * D + i] = id;
+                d32[a * D + i] = (float)id;
+            }
+        } else {
+            const int64_t s0 = T + (int64_t)(k - 1) * D2;
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    if (o64) o64[(s0 + i * D + j) * 64] = (double)v[i][j];
+                    o32[(s0 + i * D + j) * 64] = (float)v[i][j];
+                }
+        }
+    }
     }
 }
 
@@ -1833,8 +1964,27 @@ int lg2(int v)
 template <int NPE, int S, class KE, class ME>
 int launch_gather_rhs(Ctx* c, KE ke, ME me, int do_matrix, int do_rhs, int lift,
                       int incremental, const double* fe, const double* ff, int precision,
-                      int* derr)
+                      int* derr, bool direct = false)
 {
+    if constexpr (NPE <= 8) {
+        if (do_matrix && direct) {
+            // straight into the symmetric slices (and the Jacobi diagonal)
+            Sell& S_ = c->sell;
+            const int64_t N = S_.snodes;
+            const unsigned g = grid_for(N, BLK);
+            double* o64 = precision == INS_FP64 ? S_.vs64.p : nullptr;
+            if (precision == INS_FP64)
+                k_sym_gather<NPE, S, KE, double, ME><<<g, BLK, 0, c->stream>>>(
+                    N, S_.nu, S_.anode.p, c->adj_ptr.p, c->adj.p, c->lg_npe, S_.spw.p, ke, me,
+                    c->mass_factor, o64, S_.vs.p, c->dinv64.p, c->dinv32.p);
+            else
+                k_sym_gather<NPE, S, KE, float, ME><<<g, BLK, 0, c->stream>>>(
+                    N, S_.nu, S_.anode.p, c->adj_ptr.p, c->adj.p, c->lg_npe, S_.spw.p, ke, me,
+                    c->mass_factor, o64, S_.vs.p, c->dinv64.p, c->dinv32.p);
+            INS_HIP_CHECK(c, hipGetLastError());
+            do_matrix = 0;
+        }
+    }
     if (do_matrix) {
         int G = 1;
         while (G < c->max_nb && G < 64) G <<= 1;
@@ -1864,7 +2014,8 @@ int launch_gather_rhs(Ctx* c, KE ke, ME me, int do_matrix, int do_rhs, int lift,
         k_rhs<NPE, S, KE, ME><<<grid_for(c->n_nodes, BLK), BLK, 0, c->stream>>>(
             c->n_nodes, c->lg_npe, c->adj_ptr.p, c->adj.p, c->conn.p, c->dof_id.p, c->g.p, c->u.p,
             incremental, lift, ke, fe, lg2(c->nfn > 0 ? c->nfn : 1), c->fadj_ptr.p, c->fadj.p, ff,
-            c->nfn, me, c->mass_factor, c->have_mhist ? c->mhist.p : nullptr, c->b.p);
+            c->nfn, me, c->mass_factor, c->have_mhist ? c->mhist.p : nullptr,
+            c->have_emask ? c->emask.p : nullptr, c->b.p);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     return INS_OK;
@@ -2013,6 +2164,34 @@ int assemble(Ctx* c, int precision, int flags)
         INS_TRY(ensure(c, c->u, (size_t)c->n_nodes * s));
     }
     double* fe = need_fe ? c->stage_fe.p : nullptr;
+    // a SELL solve of a problem with symmetric slices reads only those: the
+    // gather writes them directly, and the CSR values follow on demand
+    const Sell& SS = c->sell;
+    const bool direct = do_matrix && c->want_format == INS_FMT_SELL && SS.built && SS.ok &&
+                        SS.sym_on && SS.direct && c->deg == 1 && c->npe <= 8 &&
+                        SS.sdpn == s && c->dofs_match();
+    if (direct) {
+        const int prec_was = c->matrix_precision;
+        c->matrix_precision = precision;   // zero_sym sizes the FP64 copy by it
+        const int zs = zero_sym(c);
+        c->matrix_precision = prec_was;
+        INS_TRY(zs);
+        INS_TRY(ensure(c, c->dinv64, c->n_active));
+        INS_TRY(ensure(c, c->dinv32, c->n_active));
+    }
+    if (do_rhs && !c->have_emask && c->deg == 1 && npe * s <= 32) {
+        INS_TRY(ensure(c, c->emask, (size_t)E));
+        if (npe == 8 && s == 1)
+            k_emask<8, 1><<<grid_for(E, BLK), BLK, 0, c->stream>>>(E, c->conn.p, c->dof_id.p, c->emask.p);
+        else if (npe == 8 && s == 3)
+            k_emask<8, 3><<<grid_for(E, BLK), BLK, 0, c->stream>>>(E, c->conn.p, c->dof_id.p, c->emask.p);
+        else if (npe == 4 && s == 1)
+            k_emask<4, 1><<<grid_for(E, BLK), BLK, 0, c->stream>>>(E, c->conn.p, c->dof_id.p, c->emask.p);
+        else if (npe == 4 && s == 2)
+            k_emask<4, 2><<<grid_for(E, BLK), BLK, 0, c->stream>>>(E, c->conn.p, c->dof_id.p, c->emask.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+        c->have_emask = (npe == 8 && (s == 1 || s == 3)) || (npe == 4 && (s == 1 || s == 2));
+    }
 
     // 1. element kernels
     if (c->profiling) INS_HIP_CHECK(c, hipEventRecord(c->ev_el[0], c->stream));
@@ -2098,37 +2277,45 @@ int assemble(Ctx* c, int precision, int flags)
         INS_HIP_CHECK(c, hipGetLastError());
     }
 
-    // 2./3. gather into CSR and RHS
+    // 2./3. gather into CSR (or straight into the symmetric slices of a
+    // SELL solve) and RHS
     const int lift = do_matrix || staged;
     if (c->deg == 2) {
         INS_TRY(gather_q2(c, elastic, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr));
     } else if (!elastic) {
         if (precision == INS_FP64) {
-            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<double, 8>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<double, 4>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
             else return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
         } else {
-            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8 && s == 1) INS_TRY((launch_gather_rhs<8, 1>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else if (npe == 8 && s == 3) INS_TRY((launch_gather_rhs<8, 3>(c, KeLaplace<float, 8>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else if (npe == 4 && s == 1) INS_TRY((launch_gather_rhs<4, 1>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else if (npe == 4 && s == 2) INS_TRY((launch_gather_rhs<4, 2>(c, KeLaplace<float, 4>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
             else return fail(c, INS_ERR_UNSUPPORTED, "element/dof combination");
         }
     } else {
         if (precision == INS_FP64) {
-            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<double, 8, 3>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<double, 4, 2>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<double, 8, 3>{(const double*)c->stage_ke.p}, KeLaplace<double, 8>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<double, 4, 2>{(const double*)c->stage_ke.p}, KeLaplace<double, 4>{(const double*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
         } else {
-            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<float, 8, 3>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
-            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<float, 4, 2>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr)));
+            if (npe == 8) INS_TRY((launch_gather_rhs<8, 3>(c, KeElastic<float, 8, 3>{(const float*)c->stage_ke.p}, KeLaplace<float, 8>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
+            else INS_TRY((launch_gather_rhs<4, 2>(c, KeElastic<float, 4, 2>{(const float*)c->stage_ke.p}, KeLaplace<float, 4>{(const float*)mep}, do_matrix, do_rhs, lift, incremental, fe, ff, precision, derr, direct)));
         }
     }
     if (do_matrix) {
         c->matrix_precision = precision;
-        INS_TRY(compute_dinv(c));
-        c->sell.stale = true;
+        c->stats.assembled_sym = direct ? 1 : 0;
+        if (direct) {
+            c->csr_stale = true;    // the CSR values follow on demand
+            c->sell.stale = false;
+        } else {
+            INS_TRY(compute_dinv(c));
+            c->csr_stale = false;
+            c->sell.stale = true;
+        }
     }
     INS_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
     int herr = 0;

@@ -120,6 +120,19 @@ struct Sell {
     // scalar slices whose offsets form stencil lines (c - 1, c, c + 1): the
     // SpMV loads p once per line and shifts it across the wave
     bool slines = false;
+    // the same slices in FP64: the FP64 sweeps, the residual replacement and
+    // the final residual of SELL solves read them (the full slices are not
+    // built where the symmetric ones are)
+    DArr<double> vs64;
+    bool zeros64 = false;
+    // Assembly straight into the symmetric slices (Q1 elements of up to 8
+    // nodes, at most 8 elements per node): per stored slot (node a, offset
+    // k) a word naming the elements node a shares with node a + u_k and the
+    // local index of the latter in each (the pinfo layout, in node a's
+    // adjacency order); anode[a] = mesh node of active node a
+    bool direct = false;
+    DArr<uint32_t> spw;
+    DArr<int32_t> anode;
 };
 
 struct Ctx {
@@ -230,6 +243,17 @@ struct Ctx {
     // assembly products
     bool have_matrix = false, have_rhs = false;
     int matrix_precision = INS_FP64;
+    // storage of the last solve or ins_prepare: an assembly of a problem
+    // with symmetric slices writes them directly when it is INS_FMT_SELL
+    int want_format = INS_FMT_CSR;
+    // the CSR values predate the last assembly, which wrote only the
+    // symmetric slices (ensure_csr rebuilds them on demand)
+    bool csr_stale = false;
+    // per element: bit b * dpn + c set where component c of local node b is
+    // prescribed (elements of at most 32 such bits; the RHS lifting visits
+    // only those)
+    DArr<uint32_t> emask;
+    bool have_emask = false;
     DArr<double> val64;
     DArr<float> val32;
     DArr<double> b;
@@ -362,6 +386,9 @@ int cell_eval(Ctx* c, int quantity, const double* xi, const double* u, double* o
 int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters,
           double* relres);
 int build_sell(Ctx* c, bool force);
+int fill_sell(Ctx* c);
+int zero_sym(Ctx* c);
+int ensure_csr(Ctx* c);
 int prepare(Ctx* c, int mode, int format);
 int solve_direct(Ctx* c, int kind, double* relres);
 int distribute(Ctx* c, int op);

@@ -257,37 +257,46 @@ struct SymView {
     int dpn;            // 1: scalar rows; 2, 3: node blocks (see bsym_node)
     int64_t nn;         // node blocks: nodes per component
     int lines;          // scalar rows of stencil lines (see sym_lines_sweep)
+    const double* v64;  // the same slices in FP64 (FP64 sweeps, residual replacement)
 };
 
-template <typename TX, typename TA, int G = 8>
-__device__ inline TA sym_row(int64_t row, bool live, int64_t n, int lane, int64_t slice,
-                             const SymView& Y, const TX* __restrict__ x)
+// the symmetric slices of the value type of a sweep
+template <typename TV>
+__device__ inline const TV* sym_vals(const SymView& Y)
+{
+    if constexpr (sizeof(TV) == 8) return Y.v64;
+    else return Y.v;
+}
+
+template <typename TV, typename TX, typename TA, int G = 8>
+__device__ inline TA sym_row_v(int64_t row, bool live, int64_t n, int lane, int64_t slice,
+                               const TV* __restrict__ vals, const int32_t* __restrict__ u,
+                               int nu, const TX* __restrict__ x)
 {
     TA s = (TA)0;
     if (!live) return s;
-    const int nu = Y.nu;
     const int64_t ld = (int64_t)64 * nu;   // values per slice
-    const float* __restrict__ own = Y.v + slice * ld + lane;
+    const TV* __restrict__ own = vals + slice * ld + lane;
     const int64_t last = n - 1;
     for (int k0 = 0; k0 < nu; k0 += G) {
-        float vu[G], vl[G];
+        TV vu[G], vl[G];
         TX xu[G], xl[G];
 #pragma unroll
         for (int j = 0; j < G; j++) {
-            vu[j] = 0.f;
-            vl[j] = 0.f;
+            vu[j] = (TV)0;
+            vl[j] = (TV)0;
             xu[j] = (TX)0;
             xl[j] = (TX)0;
             const int k = k0 + j;
             if (k < nu) {
-                const int d = Y.u[k];
+                const int d = u[k];
                 int64_t iu = row + d;
                 iu = iu > last ? last : iu;   // padded entries: value 0
                 vu[j] = own[(int64_t)k * 64];
                 xu[j] = x[iu];
                 const int64_t jl = row - d;
                 if (d > 0 && jl >= 0) {
-                    vl[j] = Y.v[(jl >> 6) * ld + (int64_t)k * 64 + (jl & 63)];
+                    vl[j] = vals[(jl >> 6) * ld + (int64_t)k * 64 + (jl & 63)];
                     xl[j] = x[jl];
                 }
             }
@@ -298,6 +307,13 @@ __device__ inline TA sym_row(int64_t row, bool live, int64_t n, int lane, int64_
         for (int j = 0; j < G; j++) s += (TA)vl[j] * (TA)xl[j];
     }
     return s;
+}
+
+template <typename TX, typename TA, int G = 8>
+__device__ inline TA sym_row(int64_t row, bool live, int64_t n, int lane, int64_t slice,
+                             const SymView& Y, const TX* __restrict__ x)
+{
+    return sym_row_v<float, TX, TA, G>(row, live, n, lane, slice, Y.v, Y.u, Y.nu, x);
 }
 
 // Wave-wide shifts by one lane (the GFX9 wavefront-shift DPP controls):
@@ -445,9 +461,10 @@ __device__ inline constexpr int tri_slot(int i, int j)
     return i * D - i * (i - 1) / 2 + (j - i);
 }
 
-template <int D, typename TX, typename TA, int G = 3>
+template <int D, typename TV, typename TX, typename TA, int G = 3>
 __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, const SymView& Y,
-                                 const TX* __restrict__ x, TA (&out)[D])
+                                 const TV* __restrict__ vals, const TX* __restrict__ x,
+                                 TA (&out)[D])
 {
     constexpr int T = D * (D + 1) / 2, D2 = D * D;
     const int nu = Y.nu;
@@ -456,9 +473,9 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
 #pragma unroll
     for (int i = 0; i < D; i++) out[i] = (TA)0;
     if (!live) return;
-    const float* __restrict__ own = Y.v + slice * ld + lane;
+    const TV* __restrict__ own = vals + slice * ld + lane;
     {
-        float bv[T];
+        TV bv[T];
         TX xa[D];
 #pragma unroll
         for (int e = 0; e < T; e++) bv[e] = own[(int64_t)e * 64];
@@ -471,15 +488,15 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
                 out[i] += (TA)bv[i <= j ? tri_slot<D>(i, j) : tri_slot<D>(j, i)] * (TA)xa[j];
     }
     for (int k0 = 1; k0 < nu; k0 += G) {
-        float bu[G][D2], bl[G][D2];
+        TV bu[G][D2], bl[G][D2];
         TX xu[G][D], xl[G][D];
 #pragma unroll
         for (int g = 0; g < G; g++) {
             const int k = k0 + g;
 #pragma unroll
             for (int e = 0; e < D2; e++) {
-                bu[g][e] = 0.f;
-                bl[g][e] = 0.f;
+                bu[g][e] = (TV)0;
+                bl[g][e] = (TV)0;
             }
 #pragma unroll
             for (int j = 0; j < D; j++) {
@@ -497,7 +514,7 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
                 for (int j = 0; j < D; j++) xu[g][j] = x[j * nn + bn];
                 const int64_t m = a - d;
                 if (m >= 0) {
-                    const float* __restrict__ mp = Y.v + (m >> 6) * ld + slot * 64 + (m & 63);
+                    const TV* __restrict__ mp = vals + (m >> 6) * ld + slot * 64 + (m & 63);
 #pragma unroll
                     for (int e = 0; e < D2; e++) bl[g][e] = mp[(int64_t)e * 64];
 #pragma unroll
@@ -521,10 +538,12 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
 
 // q = A p on the node blocks: every lane writes the D rows of its node and
 // adds their p.q to the block partial
-template <int D, typename TX, typename TA>
+template <int D, typename TV, typename TX, typename TA>
 __device__ inline double bsym_sweep(Blk b, const SymView& Y, const TX* __restrict__ p,
                                     TX* __restrict__ q)
 {
+    const TV* __restrict__ vals = sym_vals<TV>(Y);
+    constexpr int G = sizeof(TV) == 8 ? 1 : 3;   // FP64 blocks: one offset at a time
     const int lane = threadIdx.x & 63;
     const int64_t nn = Y.nn;
     const int64_t nsl = (nn + 63) / 64;
@@ -534,7 +553,7 @@ __device__ inline double bsym_sweep(Blk b, const SymView& Y, const TX* __restric
         const int64_t a = slice * 64 + lane;
         const bool live = a < nn;
         TA out[D];
-        bsym_node<D, TX, TA>(live ? a : 0, live, lane, slice, Y, p, out);
+        bsym_node<D, TV, TX, TA, G>(live ? a : 0, live, lane, slice, Y, vals, p, out);
         if (live) {
 #pragma unroll
             for (int i = 0; i < D; i++) {
@@ -558,7 +577,65 @@ struct Rep {
     const double* b;
     float* r32;
     const float* dinv32;
+    int sym;   // 1: val holds the symmetric slices of the SymView's layout
 };
+
+// the rows of a true residual t = b - A x64 (FP64 arithmetic) visited by one
+// block, round-robin over the slices; f(row, t) takes each row's value.
+// RS: the storages a kernel may meet (0 full slices, 1 scalar symmetric
+// slices, 4 node-block slices, 2 any, chosen by sym and Y.dpn), so that a
+// kernel carries the registers of its own forms only.
+template <typename TVR, int RS, class F>
+__device__ inline void resid_rows(Blk b, int64_t n, const SellView& S, const TVR* __restrict__ val,
+                                  int sym, const SymView& Y, const double* __restrict__ x,
+                                  const double* __restrict__ bv, F f)
+{
+    const int lane = threadIdx.x & 63;
+    if (RS == 0) sym = 0;
+    if (RS == 1 || RS == 4) sym = 1;
+    if constexpr (RS == 4 || RS == 2) {
+      if (RS == 4 || (sym && Y.dpn > 1)) {
+        // node blocks: D rows per lane, component-major vectors
+        const int64_t nn = Y.nn;
+        const Sweep w = xcd_sweep((nn + 63) / 64, SB / 64, b);
+        for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+            const int64_t a = slice * 64 + lane;
+            const bool live = a < nn;
+            if (Y.dpn == 3) {
+                double out[3];
+                bsym_node<3, TVR, double, double, 1>(live ? a : 0, live, lane, slice, Y, val, x,
+                                                     out);
+                if (live)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) f(i * nn + a, bv[i * nn + a] - out[i]);
+            } else {
+                double out[2];
+                bsym_node<2, TVR, double, double, 1>(live ? a : 0, live, lane, slice, Y, val, x,
+                                                     out);
+                if (live)
+#pragma unroll
+                    for (int i = 0; i < 2; i++) f(i * nn + a, bv[i * nn + a] - out[i]);
+            }
+        }
+        return;
+      }
+    }
+    if constexpr (RS != 4) {
+        const Sweep w = xcd_sweep((n + 63) / 64, SB / 64, b);
+        for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+            const int64_t rho = slice * 64 + lane;
+            const bool live = rho < n;
+            const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
+            double s;
+            if (RS != 0 && sym)
+                s = sym_row_v<TVR, double, double, 2>(row, live, n, lane, slice, val, Y.u, Y.nu,
+                                                      x);
+            else
+                s = sell_row<TVR, double, double, 2>(row, live, n, lane, slice, S, val, x);
+            if (live) f(row, bv[row] - s);
+        }
+    }
+}
 
 // K1: q = A p with per-block partials of p.q; at the replacement iteration
 // (it == rep_at) the FP64 true residual instead, with partials of t.t and
@@ -568,6 +645,8 @@ struct Rep {
 // slices, 2 = any of them, chosen per call by Y (the batched serving
 // kernel).  Separate instantiations keep each row form's registers and
 // schedule out of the others' kernels (all share the replacement path).
+// FP64 sweeps (TV = double) of problems with symmetric slices read their
+// FP64 copy (SymView::v64) with the generic row forms.
 template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM = 0>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
@@ -577,33 +656,25 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
     if (it == sc->rep_at) {
-        // FP64 true residual on the full slices, round-robin placement
-        const Sweep w = xcd_sweep(nsl, SB / 64, b);
+        // FP64 true residual, round-robin placement
         double tt = 0., rz = 0.;
-        for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
-            const int64_t rho = slice * 64 + lane;
-            const bool live = rho < n;
-            const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
-            const double s = sell_row<TVR, double, double, 2>(row, live, n, lane, slice, S, R.val,
-                                                              R.x64);
-            if (live) {
-                const double ti = R.b[row] - s;
-                tt += ti * ti;
-                const float rf = (float)ti;
-                R.r32[row] = rf;
-                rz += (double)rf * (double)(R.dinv32[row] * rf);
-            }
-        }
+        resid_rows<TVR, SYM == 3 ? 1 : SYM>(b, n, S, R.val, R.sym, Y, R.x64, R.b,
+                                            [&](int64_t row, double ti) {
+            tt += ti * ti;
+            const float rf = (float)ti;
+            R.r32[row] = rf;
+            rz += (double)rf * (double)(R.dinv32[row] * rf);
+        });
         double v[2] = {tt, rz};
         const int sl[2] = {P_TT, P_RZ};
         block_partials<2>(v, part, sl, b);
         return;
     }
     if (sc->done || sc->stop) return;
-    if constexpr (sizeof(TV) == 4 && (SYM == 4 || SYM == 2)) {
+    if constexpr (SYM == 4 || (sizeof(TV) == 4 && SYM == 2)) {
         if (SYM == 4 || (Y.v && Y.dpn > 1)) {
-            double v[1] = {Y.dpn == 3 ? bsym_sweep<3, TX, TA>(b, Y, p, q)
-                                      : bsym_sweep<2, TX, TA>(b, Y, p, q)};
+            double v[1] = {Y.dpn == 3 ? bsym_sweep<3, TV, TX, TA>(b, Y, p, q)
+                                      : bsym_sweep<2, TV, TX, TA>(b, Y, p, q)};
             const int sl[1] = {P_PQ};
             block_partials<1>(v, part, sl, b);
             return;
@@ -622,15 +693,16 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     if constexpr (SYM != 3 && SYM != 4) {
         // the symmetric slices sweep per-XCD ranges: a mirror read then finds
         // the slice it needs in the same XCD's L2
-        const Sweep w = xcd_sweep(nsl, SB / 64, b, Y.v != nullptr);
+        const bool symv = SYM == 1 || (SYM == 2 && Y.v != nullptr);
+        const Sweep w = xcd_sweep(nsl, SB / 64, b, symv);
         double pq = 0.;
         for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
             const int64_t rho = slice * 64 + lane;
             const bool live = rho < n;
             const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
             TA s;
-            if constexpr (sizeof(TV) == 4 && SYM == 1) {
-                s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
+            if constexpr (SYM == 1) {
+                s = sym_row_v<TV, TX, TA>(row, live, n, lane, slice, sym_vals<TV>(Y), Y.u, Y.nu, p);
             } else if constexpr (sizeof(TV) == 4 && SYM == 2) {
                 if (Y.v) s = sym_row<TX, TA>(row, live, n, lane, slice, Y, p);
                 else s = sell_row<TV, TX, TA, SGK>(row, live, n, lane, slice, S, val, p);
@@ -989,26 +1061,15 @@ __global__ __launch_bounds__(VB) void k_fold(int64_t n, double* __restrict__ x64
     }
 }
 
-// t = b - A x in FP64; partials of t.t
+// t = b - A x in FP64; partials of t.t (on the symmetric slices when sym)
 template <typename TV>
 __global__ __launch_bounds__(SB) void k_resid_sell(int64_t n, SellView S, const TV* __restrict__ val,
-                                                   const double* __restrict__ x,
+                                                   int sym, SymView Y, const double* __restrict__ x,
                                                    const double* __restrict__ b, double* part)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t nsl = (n + 63) / 64;
     double tt = 0.;
-    const Sweep w = xcd_sweep(nsl, SB / 64);
-    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
-        const int64_t rho = slice * 64 + lane;
-        const bool live = rho < n;
-        const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
-        const double s = sell_row<TV, double, double, 4>(row, live, n, lane, slice, S, val, x);
-        if (live) {
-            const double ti = b[row] - s;
-            tt += ti * ti;
-        }
-    }
+    resid_rows<TV, 2>(grid_blk(), n, S, val, sym, Y, x, b,
+                      [&](int64_t, double ti) { tt += ti * ti; });
     double v[1] = {tt};
     const int sl[1] = {P_TT};
     block_partials<1>(v, part, sl);
@@ -1483,6 +1544,53 @@ __global__ void k_sym_map(int64_t n, const int32_t* __restrict__ rp, const int32
     }
 }
 
+// Shared-element words of the stored slots of the symmetric slices, for the
+// assembly straight into them: slot (a, k) couples active node a with active
+// node a + u_k; bits [4i, 4i + 4) of its word hold, for the i-th element of
+// node a's adjacency list (ascending element order), a presence bit (8) and
+// the local index of node a + u_k in that element, as the pair words of the
+// CSR gather do (0 for padding slots).  Words are k-major within a slice of
+// 64 nodes, like the values.
+template <int NPE>
+__global__ void k_sym_words(int64_t N, int nu, const int32_t* __restrict__ su,
+                            const int32_t* __restrict__ anode, const int32_t* __restrict__ adj_ptr,
+                            const int32_t* __restrict__ adj, const int32_t* __restrict__ conn,
+                            int lg_npe, uint32_t* __restrict__ spw)
+{
+    constexpr int BITS = PairWord<NPE>::bits, FLAG = PairWord<NPE>::flag;
+    static_assert(NPE <= 8, "32-bit words");
+    const int64_t total = (N + 63) / 64 * 64 * nu;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sl = p / (64 * (int64_t)nu);
+        const int k = (int)((p / 64) % nu);
+        const int64_t a = sl * 64 + (p & 63);
+        uint32_t w = 0;
+        if (a < N && a + su[k] < N) {
+            const int32_t A = anode[a], B = anode[a + su[k]];
+            const int32_t as = adj_ptr[A], ae = adj_ptr[A + 1];
+            for (int32_t i = 0; i < ae - as; i++) {
+                const int64_t e = adj[as + i] >> lg_npe;
+#pragma unroll
+                for (int t = 0; t < NPE; t++)
+                    if (conn[e * NPE + t] == B) w |= (uint32_t)(FLAG + t) << (BITS * i);
+            }
+        }
+        spw[p] = w;
+    }
+}
+
+// mesh node of every active node (all its D components active, ids a*D+c)
+__global__ void k_anode(int64_t n_nodes, int D, const int32_t* __restrict__ dof_id,
+                        int32_t* __restrict__ anode)
+{
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n_nodes;
+         m += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t id = dof_id[m * D];
+        if (id >= 0) anode[id / D] = (int32_t)m;
+    }
+}
+
 // slice position of every CSR entry in the node-block slices (-1: below
 // the diagonal, read at its mirror); rows interleaved, r = a * D + i
 __global__ void k_bsym_map(int64_t n, int D, const int32_t* __restrict__ rp,
@@ -1525,12 +1633,58 @@ __global__ void k_bsym_map(int64_t n, int D, const int32_t* __restrict__ rp,
 }
 
 __global__ void k_sym_scatter(int64_t nnz, const int32_t* __restrict__ smap,
-                              const float* __restrict__ v32, float* __restrict__ vs)
+                              const double* __restrict__ v64, const float* __restrict__ v32,
+                              double* __restrict__ vs64, float* __restrict__ vs)
 {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int32_t o = smap[k];
-        if (o >= 0) vs[o] = v32[k];
+        if (o < 0) continue;
+        if (vs64) vs64[o] = v64[k];
+        vs[o] = v32[k];
+    }
+}
+
+// CSR values from the symmetric slices (the assembly wrote only those): row
+// r = a * D + i of node a, column c = b * D + j; the block (a, b) is stored
+// at node a when b - a = u_k >= 0 (for b = a only its upper triangle), else
+// transposed at node b
+template <int D>
+__global__ void k_sym_to_csr(int64_t n, const int32_t* __restrict__ rp,
+                             const int32_t* __restrict__ col, const int32_t* __restrict__ su,
+                             int nu, const double* __restrict__ vs64,
+                             const float* __restrict__ vs, double* __restrict__ v64,
+                             float* __restrict__ v32)
+{
+    constexpr int T = D * (D + 1) / 2;
+    const int64_t nbv = (int64_t)T + (int64_t)(nu - 1) * D * D;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = r / D;
+        const int i = (int)(r - a * D);
+        for (int32_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int64_t bn = col[e] / D;
+            const int j = (int)(col[e] - bn * D);
+            const bool up = bn > a || (bn == a && j >= i);
+            const int64_t owner = up ? bn - a : a - bn;   // the offset u_k
+            int lo = 0, hi = nu - 1, k = 0;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (su[mid] == owner) {
+                    k = mid;
+                    break;
+                }
+                if (su[mid] < owner) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            const int64_t node = up ? a : bn;
+            const int ii = up ? i : j, jj = up ? j : i;
+            const int64_t slot = k == 0 ? (int64_t)(ii * D - ii * (ii - 1) / 2 + (jj - ii))
+                                        : (int64_t)T + (int64_t)(k - 1) * D * D + ii * D + jj;
+            const int64_t pos = (node >> 6) * 64 * nbv + slot * 64 + (node & 63);
+            if (vs64) v64[e] = vs64[pos];
+            v32[e] = vs64 ? (float)vs64[pos] : vs[pos];
+        }
     }
 }
 
@@ -1757,7 +1911,7 @@ bool use_sym(const Ctx* c, bool permuted)
 SymView sym_view(const Ctx* c)
 {
     return SymView{c->sell.vs.p, c->sell.su.p, c->sell.nu, c->sell.sdpn, c->sell.snodes,
-                   c->sell.slines ? 1 : 0};
+                   c->sell.slines ? 1 : 0, c->sell.vs64.p};
 }
 
 // Blocks of the SpMV on the full slices (sym = false) or on the symmetric
@@ -1775,17 +1929,28 @@ unsigned spmv_grid(const Ctx* c, bool sym)
 
 template <typename TV, typename TX, typename TA, typename TVR>
 int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
-                  int* g_out, int it, const TVR* rep_val)
+                  int* g_out, int it, const TVR* rep_val, int rep_sym)
 {
     const int64_t n = c->n_active;
     const Scal* sc = c->scal.p;
-    const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32};
+    const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32, rep_sym};
     if (format == INS_FMT_SELL) {
-        const bool sym = sizeof(TV) == 4 && use_sym(c, c->sperm);
+        const bool sym = use_sym(c, c->sperm);
         const unsigned g = spmv_grid(c, sym);
-        SymView Y{nullptr, nullptr, 0, 1, 0, 0};
+        SymView Y{nullptr, nullptr, 0, 1, 0, 0, nullptr};
         if (sym) Y = sym_view(c);
-        if (Y.v && Y.dpn > 1)
+        if constexpr (sizeof(TV) == 8) {
+            // FP64 sweeps: the FP64 symmetric slices, generic row forms
+            if (sym && Y.dpn > 1)
+                INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 4>, g, SB, n, sell_view(c),
+                                     sell_val, p, q, c->partial.p, sc, it, R, Y));
+            else if (sym)
+                INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 1>, g, SB, n, sell_view(c),
+                                     sell_val, p, q, c->partial.p, sc, it, R, Y));
+            else
+                INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 0>, g, SB, n, sell_view(c),
+                                     sell_val, p, q, c->partial.p, sc, it, R, Y));
+        } else if (Y.v && Y.dpn > 1)
             INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 4>, g, SB, n, sell_view(c),
                                  sell_val, p, q, c->partial.p, sc, it, R, Y));
         else if (Y.v && Y.lines)
@@ -1818,20 +1983,24 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
 }
 
 // K1 launch; the values of the residual replacement are the FP64 ones when
-// the assembly was FP64 (the FP64 solve never replaces)
+// the assembly was FP64 (the FP64 solve never replaces), in the storage of
+// the solve: CSR, the full slices, or the symmetric slices where those are
+// the ones in use
 template <typename TV, typename TX, typename TA>
 int launch_spmv(Ctx* c, int format, const TV* csr_val, const TV* sell_val, const TX* p, TX* q,
                 int* g_out, int it)
 {
     const bool sell = format == INS_FMT_SELL;
-    if constexpr (sizeof(TX) == 8)
-        return launch_spmv_r<TV, TX, TA, double>(c, format, csr_val, sell_val, p, q, g_out, it,
-                                                 sell ? c->sell.v64.p : c->val64.p);
-    if (c->matrix_precision == INS_FP64)
-        return launch_spmv_r<TV, TX, TA, double>(c, format, csr_val, sell_val, p, q, g_out, it,
-                                                 sell ? c->sell.v64.p : c->val64.p);
-    return launch_spmv_r<TV, TX, TA, float>(c, format, csr_val, sell_val, p, q, g_out, it,
-                                            sell ? c->sell.v32.p : c->val32.p);
+    const int sym = sell && use_sym(c, c->sperm) ? 1 : 0;
+    if constexpr (sizeof(TX) == 4) {
+        if (c->matrix_precision != INS_FP64)
+            return launch_spmv_r<TV, TX, TA, float>(
+                c, format, csr_val, sell_val, p, q, g_out, it,
+                sym ? c->sell.vs.p : (sell ? c->sell.v32.p : c->val32.p), sym);
+    }
+    return launch_spmv_r<TV, TX, TA, double>(
+        c, format, csr_val, sell_val, p, q, g_out, it,
+        sym ? c->sell.vs64.p : (sell ? c->sell.v64.p : c->val64.p), sym);
 }
 
 // |b - A x64| (and the residual replacement); then the finalising kernel
@@ -1844,11 +2013,15 @@ int launch_resid(Ctx* c, int format, hipStream_t stream = nullptr, Scal* scal = 
     unsigned g;
     const bool v64 = c->matrix_precision == INS_FP64;
     if (format == INS_FMT_SELL) {
-        g = sell_grid(n);
+        const int sym = use_sym(c, c->sperm) ? 1 : 0;
+        const SymView Y = sym ? sym_view(c) : SymView{nullptr, nullptr, 0, 1, 0, 0, nullptr};
+        g = sell_grid(sym && Y.dpn > 1 ? Y.nn : n);
         if (v64)
-            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v64.p, c->sx64, c->sb, c->partial.p);
+            k_resid_sell<double><<<g, SB, 0, st>>>(n, sell_view(c), sym ? c->sell.vs64.p : c->sell.v64.p,
+                                                   sym, Y, c->sx64, c->sb, c->partial.p);
         else
-            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), c->sell.v32.p, c->sx64, c->sb, c->partial.p);
+            k_resid_sell<float><<<g, SB, 0, st>>>(n, sell_view(c), sym ? c->sell.vs.p : c->sell.v32.p,
+                                                  sym, Y, c->sx64, c->sb, c->partial.p);
     } else {
         const int lpr = pick_lpr(c);
         g = csr_grid(n, lpr);
@@ -1939,6 +2112,41 @@ constexpr int SYM_MAXU = 64;
 // column-offset lists, which hold (c' - c) N + delta in the component-major
 // numbering; every CSR entry is then placed by k_bsym_map, and an entry that
 // finds no place leaves the full slices in use.
+// The words of the direct assembly (k_sym_words) for Q1 elements of up to 8
+// nodes with at most 8 elements per node whose symmetric slices hold the
+// diagonal as offset 0
+int build_sym_words(Ctx* c, int64_t N, int D)
+{
+    Sell& S = c->sell;
+    S.direct = false;
+    S.zeros = false;
+    S.zeros64 = false;
+    static const int enabled = env_int("INS_SYM_DIRECT", 1);
+    if (!enabled || c->deg != 1 || c->npe > 8 || !c->have_pinfo || c->csr_from_host ||
+        !c->dofs_match())
+        return INS_OK;
+    int32_t u0 = -1;
+    INS_HIP_CHECK(c, hipMemcpyAsync(&u0, S.su.p, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                    c->stream));
+    INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (u0 != 0) return INS_OK;
+    INS_TRY(ensure(c, S.anode, N));
+    k_anode<<<grid_for(c->n_nodes, 256), 256, 0, c->stream>>>(c->n_nodes, D, c->dof_id.p,
+                                                            S.anode.p);
+    const int64_t total = (N + 63) / 64 * 64 * S.nu;
+    INS_TRY(ensure(c, S.spw, total));
+    const unsigned g = grid_for(total, 256);
+    if (c->npe == 8)
+        k_sym_words<8><<<g, 256, 0, c->stream>>>(N, S.nu, S.su.p, S.anode.p, c->adj_ptr.p,
+                                                 c->adj.p, c->conn.p, c->lg_npe, S.spw.p);
+    else
+        k_sym_words<4><<<g, 256, 0, c->stream>>>(N, S.nu, S.su.p, S.anode.p, c->adj_ptr.p,
+                                                 c->adj.p, c->conn.p, c->lg_npe, S.spw.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    S.direct = true;
+    return INS_OK;
+}
+
 int build_bsym(Ctx* c)
 {
     Sell& S = c->sell;
@@ -2015,13 +2223,14 @@ int build_bsym(Ctx* c)
     S.sdpn = D;
     S.snodes = N;
     S.sym_on = true;
-    return INS_OK;
+    return build_sym_words(c, N, D);
 }
 
 int build_sym(Ctx* c)
 {
     Sell& S = c->sell;
     S.sym_on = false;
+    S.direct = false;
     S.zeros = false;
     S.sdpn = 1;
     S.snodes = 0;
@@ -2086,7 +2295,8 @@ int build_sym(Ctx* c)
     static const int lines_on = env_int("INS_SYM_LINES", 1);
     S.slines = S.slines && lines_on;
     S.sym_on = true;
-    return INS_OK;
+    S.snodes = c->n_active;
+    return build_sym_words(c, c->n_active, 1);
 }
 
 }  // namespace
@@ -2274,10 +2484,23 @@ int build_sell(Ctx* c, bool force)
 }
 
 // copy the current CSR values into the slices (after every assembly)
+int ensure_csr(Ctx* c);
+
 int fill_sell(Ctx* c)
 {
     Sell& S = c->sell;
     const bool f64 = c->matrix_precision == INS_FP64;
+    INS_TRY(ensure_csr(c));
+    if (S.sym_on) {
+        // the symmetric slices are the only ones the SELL solves read
+        INS_TRY(zero_sym(c));
+        k_sym_scatter<<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(
+            c->nnz, S.smap.p, f64 ? c->val64.p : nullptr, c->val32.p, f64 ? S.vs64.p : nullptr,
+            S.vs.p);
+        INS_HIP_CHECK(c, hipGetLastError());
+        S.stale = false;
+        return INS_OK;
+    }
     // a new slice structure: size the value arrays to it and zero them once
     // (the scatter writes the stored entries only; padding must read 0)
     if (f64 && !S.zero64) {
@@ -2294,17 +2517,52 @@ int fill_sell(Ctx* c)
         c->nnz, S.map.p, f64 ? c->val64.p : nullptr, c->val32.p, f64 ? S.v64.p : nullptr,
         S.v32.p);
     INS_HIP_CHECK(c, hipGetLastError());
-    if (S.sym_on) {
-        if (!S.zeros) {
-            INS_TRY(ensure(c, S.vs, S.nvs));
-            INS_HIP_CHECK(c, hipMemsetAsync(S.vs.p, 0, S.nvs * sizeof(float), c->stream));
-            S.zeros = true;
-        }
-        k_sym_scatter<<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(c->nnz, S.smap.p,
-                                                                     c->val32.p, S.vs.p);
-        INS_HIP_CHECK(c, hipGetLastError());
-    }
     S.stale = false;
+    return INS_OK;
+}
+
+// The symmetric slices sized to the current structure and zeroed once
+// (writers fill the stored slots; the padding must read 0)
+int zero_sym(Ctx* c)
+{
+    Sell& S = c->sell;
+    if (!S.zeros) {
+        INS_TRY(ensure(c, S.vs, S.nvs));
+        INS_HIP_CHECK(c, hipMemsetAsync(S.vs.p, 0, S.nvs * sizeof(float), c->stream));
+        S.zeros = true;
+    }
+    if (c->matrix_precision == INS_FP64 && !S.zeros64) {
+        INS_TRY(ensure(c, S.vs64, S.nvs));
+        INS_HIP_CHECK(c, hipMemsetAsync(S.vs64.p, 0, S.nvs * sizeof(double), c->stream));
+        S.zeros64 = true;
+    }
+    return INS_OK;
+}
+
+// CSR values current again after an assembly that wrote the symmetric
+// slices only (export, CSR solves, direct solves)
+int ensure_csr(Ctx* c)
+{
+    if (!c->csr_stale) return INS_OK;
+    Sell& S = c->sell;
+    const bool f64 = c->matrix_precision == INS_FP64;
+    if (f64) INS_TRY(ensure(c, c->val64, c->nnz));
+    INS_TRY(ensure(c, c->val32, c->nnz));
+    const int64_t n = c->n_active;
+    const unsigned g = grid_for(n, 256);
+    const double* s64 = f64 ? S.vs64.p : nullptr;
+    double* d64 = f64 ? c->val64.p : nullptr;
+    if (S.sdpn == 3)
+        k_sym_to_csr<3><<<g, 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, S.su.p, S.nu, s64,
+                                                  S.vs.p, d64, c->val32.p);
+    else if (S.sdpn == 2)
+        k_sym_to_csr<2><<<g, 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, S.su.p, S.nu, s64,
+                                                  S.vs.p, d64, c->val32.p);
+    else
+        k_sym_to_csr<1><<<g, 256, 0, c->stream>>>(n, c->row_ptr.p, c->col.p, S.su.p, S.nu, s64,
+                                                  S.vs.p, d64, c->val32.p);
+    INS_HIP_CHECK(c, hipGetLastError());
+    c->csr_stale = false;
     return INS_OK;
 }
 
@@ -2354,6 +2612,8 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
     if (format == INS_FMT_SELL && !c->sell.built) INS_TRY(build_sell(c, false));
     if (format == INS_FMT_SELL && !c->sell.ok) format = INS_FMT_CSR;
     if (format == INS_FMT_SELL && c->sell.stale) INS_TRY(fill_sell(c));
+    if (format == INS_FMT_CSR) INS_TRY(ensure_csr(c));
+    c->want_format = format;
     INS_TRY(ensure_solver_state(c, mode));
     const int64_t n = c->n_active;
     const unsigned gv = vec_grid(n);
@@ -2459,11 +2719,14 @@ int solve(Ctx* c, int mode, int format, double rtol, int max_it, int* iters, dou
 
 int prepare(Ctx* c, int mode, int format)
 {
-    if (!c->have_matrix) return fail(c, INS_ERR_STATE, "ins_prepare before ins_assemble");
+    if (!c->have_pattern) return fail(c, INS_ERR_STATE, "ins_prepare before ins_build_pattern");
     if (format == INS_FMT_SELL) {
         if (!c->sell.built || !c->sell.ok) INS_TRY(build_sell(c, true));
-        if (c->sell.ok && c->sell.stale) INS_TRY(fill_sell(c));
+        if (c->have_matrix && c->sell.ok && c->sell.stale) INS_TRY(fill_sell(c));
+    } else if (c->have_matrix) {
+        INS_TRY(ensure_csr(c));
     }
+    c->want_format = format;
     INS_TRY(ensure_solver_state(c, mode));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return INS_OK;
@@ -2623,6 +2886,7 @@ int solve_direct(Ctx* c, int kind, double* relres)
         return fail(c, INS_ERR_UNSUPPORTED,
                     "ins_solve_direct: " + std::to_string(n) + " rows exceed INS_DIRECT_MAX_N (" +
                         std::to_string(INS_DIRECT_MAX_N) + "); solve with ins_solve");
+    INS_TRY(ensure_csr(c));
     INS_TRY(ensure_solver_state(c, INS_SOLVE_FP64));
     INS_TRY(solve_vectors(c, INS_FMT_CSR, INS_SOLVE_FP64, c->stream));
     c->have_solution = false;
@@ -2737,7 +3001,8 @@ struct SlotDesc {
     float INS_GLOBAL* q;
     double INS_GLOBAL* part;
     Scal INS_GLOBAL* sc;
-    const double INS_GLOBAL* v64;  // FP64 slice values (residual replacement)
+    const double INS_GLOBAL* v64;  // FP64 slice values of the residual replacement (the
+                                   // symmetric ones when vs is set)
     double INS_GLOBAL* x64;
     const double INS_GLOBAL* b;
     const float INS_GLOBAL* vs;     // symmetric slices of the FP32 sweeps (null: none)
@@ -2786,14 +3051,14 @@ __global__ __launch_bounds__(SB) void kb_spmv(const BatchTab* __restrict__ bt, i
     const int s = find_slot(bt->pre_spmv, bt->nslot, blockIdx.x);
     const SlotDesc& d = bt->d[s];
     const Rep<double> R{(const double*)d.v64, (const double*)d.x64, (const double*)d.b,
-                        (float*)d.r, (const float*)d.dinv};
+                        (float*)d.r, (const float*)d.dinv, d.vs != nullptr ? 1 : 0};
     spmv_sell_body<float, float, float, 4, double, 2>(Blk{(int)blockIdx.x - bt->pre_spmv[s], d.g_spmv},
                                                    d.n, sell_of(d.S), (const float*)d.val,
                                                    (const float*)d.p, (float*)d.q, (double*)d.part,
                                                    (const Scal*)d.sc, (int)(it - d.join), R,
                                                    SymView{(const float*)d.vs,
                                                            (const int32_t*)d.su, d.nu, d.sdpn,
-                                                           d.snodes, d.slines});
+                                                           d.snodes, d.slines, nullptr});
 }
 
 __global__ __launch_bounds__(VB) void kb_update_xr(const BatchTab* __restrict__ bt, int64_t it)
@@ -2953,10 +3218,10 @@ int upload_table(Server* sv)
         INS_AS(d.q, c->q32.p);
         INS_AS(d.part, c->partial.p);
         INS_AS(d.sc, sv->scal + s);
-        INS_AS(d.v64, c->sell.v64.p);
+        const bool sym = use_sym(c, c->sperm);
+        INS_AS(d.v64, sym ? c->sell.vs64.p : c->sell.v64.p);
         INS_AS(d.x64, c->sx64);
         INS_AS(d.b, c->sb);
-        const bool sym = use_sym(c, c->sperm);
         INS_AS(d.vs, sym ? c->sell.vs.p : nullptr);
         INS_AS(d.su, sym ? c->sell.su.p : nullptr);
         d.nu = sym ? c->sell.nu : 0;
@@ -3067,6 +3332,7 @@ int server_attach(Server* sv, Ctx* c, double rtol, int max_it, int* slot)
     if (c->matrix_precision != INS_FP64)
         return sfail(sv, INS_ERR_UNSUPPORTED, "serving needs an FP64 assembly (FP64 residual)");
     if (c->sell.stale && (st = fill_sell(c)) != INS_OK) return ctx_fail(st);
+    c->want_format = INS_FMT_SELL;
     if ((st = ensure_solver_state(c, INS_SOLVE_MIXED)) != INS_OK) return ctx_fail(st);
     // the request's own setup (assembly, SELL) precedes its start
     SRV_HIP(sv, hipEventRecord(sv->ev_sync, c->stream));
