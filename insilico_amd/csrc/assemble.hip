@@ -217,22 +217,22 @@ __global__ __launch_bounds__(BLK) void k_elem_laplace(
                 for (int n = 0; n < NPE; n++) fb[n] += f * phi[n] * w * det;
             }
         }
-        // element-major: the NK entries of an element are contiguous, so the
-        // gather, which reads the entries of the few elements around a node,
-        // uses whole sectors
-        typedef T T2 __attribute__((ext_vector_type(2)));
-        static_assert(NK % 2 == 0, "pairs of entries");
-        T2* dst = reinterpret_cast<T2*>(ke + e * NK);
+        // 64-element slabs, entry-major inside a slab (entry k of element e at
+        // (e/64)*64*NK + k*64 + e%64): a wavefront stores 64 consecutive
+        // values per entry, and the gather, whose lanes hold consecutive
+        // nodes and so read the same entry of consecutive elements, loads
+        // contiguous runs
+        const int64_t sb = (e >> 6) * 64 * NK + (e & 63);
 #pragma unroll
-        for (int k = 0; k < NK / 2; k++) dst[k] = (T2){K[2 * k], K[2 * k + 1]};
+        for (int k = 0; k < NK; k++) ke[sb + k * 64] = K[k];
         if constexpr (WANT_M) {
-            T2* dm = reinterpret_cast<T2*>(me + e * NK);
 #pragma unroll
-            for (int k = 0; k < NK / 2; k++) dm[k] = (T2){Mm[2 * k], Mm[2 * k + 1]};
+            for (int k = 0; k < NK; k++) me[sb + k * 64] = Mm[k];
         }
         if (fe) {
+            const int64_t fb0 = (e >> 6) * 64 * NPE + (e & 63);
 #pragma unroll
-            for (int n = 0; n < NPE; n++) fe[e * NPE + n] = (double)fb[n];
+            for (int n = 0; n < NPE; n++) fe[fb0 + n * 64] = (double)fb[n];
         }
     }
 }
@@ -1005,6 +1005,9 @@ __global__ __launch_bounds__(64) void k_elem_elastic_q2(
 
 // ---------------------------------------------------------------------------
 // staged element-matrix accessors
+// heat::Laplace: the upper triangle of the symmetric element matrix; Q1
+// elements in 64-element slabs (entry k of element e at (e/64)*64*NK + k*64
+// + e%64), degree-2 elements element-major
 template <typename T, int NPE>
 struct KeLaplace {
     const T* p;
@@ -1014,7 +1017,8 @@ struct KeLaplace {
         const int i = a < b ? a : b, j = a < b ? b : a;
         const int k = i * NPE - (i * (i - 1)) / 2 + (j - i);
         constexpr int NK = NPE * (NPE + 1) / 2;
-        return (double)p[e * NK + k];
+        if constexpr (NPE <= 8) return (double)p[(e >> 6) * (64 * NK) + k * 64 + (e & 63)];
+        else return (double)p[e * NK + k];
     }
 };
 
@@ -1161,7 +1165,7 @@ __global__ __launch_bounds__(BLK) void k_rhs(
     const double* __restrict__ fe, int lg_nfn, const int32_t* __restrict__ fadj_ptr,
     const int32_t* __restrict__ fadj, const double* __restrict__ ff, int nfn, ME me,
     double mfac, const double* __restrict__ hist, const uint32_t* __restrict__ emask,
-    double* __restrict__ b)
+    int fe_slab, double* __restrict__ b)
 {
     for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
          n += (int64_t)gridDim.x * blockDim.x) {
@@ -1204,7 +1208,9 @@ __global__ __launch_bounds__(BLK) void k_rhs(
                 for (int32_t k = as; k < ae; k++) {
                     const int64_t e = adj[k] >> lg_npe;
                     const int a = adj[k] & mask;
-                    acc += fe[e * (NPE * S) + a * S + cr];
+                    // element-major, or in 64-element slabs (heat::Laplace Q1)
+                    acc += fe_slab ? fe[(e >> 6) * (64 * NPE * S) + (a * S + cr) * 64 + (e & 63)]
+                                   : fe[e * (NPE * S) + a * S + cr];
                 }
             }
             if (ff) {
@@ -2015,7 +2021,7 @@ int launch_gather_rhs(Ctx* c, KE ke, ME me, int do_matrix, int do_rhs, int lift,
             c->n_nodes, c->lg_npe, c->adj_ptr.p, c->adj.p, c->conn.p, c->dof_id.p, c->g.p, c->u.p,
             incremental, lift, ke, fe, lg2(c->nfn > 0 ? c->nfn : 1), c->fadj_ptr.p, c->fadj.p, ff,
             c->nfn, me, c->mass_factor, c->have_mhist ? c->mhist.p : nullptr,
-            c->have_emask ? c->emask.p : nullptr, c->b.p);
+            c->have_emask ? c->emask.p : nullptr, c->fe_slab ? 1 : 0, c->b.p);
         INS_HIP_CHECK(c, hipGetLastError());
     }
     return INS_OK;
@@ -2147,7 +2153,9 @@ int assemble(Ctx* c, int precision, int flags)
         ke_elems = (size_t)E * (npe * s) * (npe * s);
     if (do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
     const bool need_fe = do_rhs && (c->bf_kind != INS_BF_NONE || want_r);
-    if (need_fe && !staged) INS_TRY(ensure(c, c->stage_fe, (size_t)E * npe * s));
+    if (need_fe && !staged) INS_TRY(ensure(c, c->stage_fe, (size_t)((E + 63) / 64) * 64 * npe * s));
+    // element force vectors in 64-element slabs: those of the Q1 heat::Laplace kernel
+    if (!staged) c->fe_slab = !elastic && c->deg == 1;
     // time-step mass terms (heat::Laplace only)
     const bool mass = c->density > 0.;
     if (mass && elastic)
