@@ -68,7 +68,7 @@ static void release_all(Ctx* c)
     release(c, c->col); release(c, c->coef); release(c, c->tab64); release(c, c->tab32);
     release(c, c->faces); release(c, c->fadj_ptr); release(c, c->fadj); release(c, c->val64);
     release(c, c->val32); release(c, c->b); release(c, c->dinv64); release(c, c->dinv32);
-    release(c, c->stage_ke); release(c, c->stage_fe); release(c, c->stage_ff);
+    release(c, c->stage_ke); release(c, c->stage_fe); release(c, c->stage_ff); release(c, c->stage_pt);
     release(c, c->sell.slice_ptr); release(c, c->sell.slice_pat);
     release(c, c->sell.pat_ptr); release(c, c->sell.pat_delta); release(c, c->sell.pat_len);
     release(c, c->sell.v64); release(c, c->sell.v32); release(c, c->sell.map); release(c, c->sell.perm);

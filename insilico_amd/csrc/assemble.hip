@@ -340,217 +340,6 @@ __device__ inline void material(T lam, T mu, const T F[3][3], T S[3][3], T C[6][
 }
 
 // ---------------------------------------------------------------------------
-// solid::HyperElastic element kernel (solid/HyperElastic.hpp:110-179 tangent,
-// :210-257 residual, :282-310 effective elasticity): one wavefront per
-// element, EPB elements per block.
-template <int DIM, int NQ1>
-struct ElLds {
-    static constexpr int NPE = 1 << DIM;
-    static constexpr int NQ = IPow<NQ1, DIM>::value;
-    static constexpr int S = DIM;
-    static constexpr int NEE = NPE * S;
-    static constexpr int NA = S * S * S * S;
-    static constexpr int X = 0;                 // [NPE][3]
-    static constexpr int U = X + NPE * 3;       // [NPE][3]
-    static constexpr int GX = U + NPE * 3;      // [NQ][NPE][DIM]
-    static constexpr int DW = GX + NQ * NPE * DIM;
-    static constexpr int F = DW + NQ;           // [NQ][9]
-    static constexpr int SS = F + NQ * 9;       // [NQ][9]
-    static constexpr int C = SS + NQ * 9;       // [NQ][36]
-    static constexpr int A = C + NQ * 36;       // [NQ][NA]
-    static constexpr int FR = A + NQ * NA;      // [NQ][NEE]
-    static constexpr int FB = FR + NQ * NEE;    // [NQ][NEE]
-    static constexpr int SIZE = FB + NQ * NEE;
-};
-
-template <int DIM, int NQ1, typename T, int MAT, bool WANT_K>
-__global__ __launch_bounds__(256) void k_elem_elastic(
-    int64_t E, const int32_t* __restrict__ conn, const double* __restrict__ xyz,
-    const double* __restrict__ coef, const double* __restrict__ u, int ncoef,
-    const T* __restrict__ tab, Bf bf, int want_resid, T* __restrict__ ke,
-    double* __restrict__ fe)
-{
-    using L = ElLds<DIM, NQ1>;
-    constexpr int NPE = L::NPE, NQ = L::NQ, S = L::S, NEE = L::NEE;
-    constexpr int EPB = (NQ <= 8) ? 4 : 1;
-    using TB = Tab<DIM, NQ>;
-    __shared__ T s_tab[TB::SIZE];
-    __shared__ T s_all[EPB * L::SIZE];
-    for (int i = threadIdx.x; i < TB::SIZE; i += blockDim.x) s_tab[i] = tab[i];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    T* sm = s_all + wave * L::SIZE;
-    __syncthreads();
-    for (int64_t base = (int64_t)blockIdx.x * EPB; base < E; base += (int64_t)gridDim.x * EPB) {
-        const int64_t e = base + wave;
-        const bool valid = (wave < EPB) && (e < E);
-        if (valid && lane < NPE * 3) {
-            const int n = lane / 3, d = lane % 3;
-            const int64_t node = conn[e * NPE + n];
-            sm[L::X + lane] = d < DIM ? (T)xyz[node * DIM + d] : (T)0;
-            sm[L::U + lane] = (d < S && u) ? (T)u[node * S + d] : (T)0;
-        }
-        __syncthreads();
-        // phase 1: one quadrature point per lane
-        if (valid && lane < NQ) {
-            const int q = lane;
-            T X[NPE][DIM];
-#pragma unroll
-            for (int n = 0; n < NPE; n++)
-#pragma unroll
-                for (int d = 0; d < DIM; d++) X[n][d] = sm[L::X + n * 3 + d];
-            T gx[NPE][DIM];
-            const T det = phys_grad<DIM, NPE, T>(X, s_tab + TB::DPHI + q * NPE * DIM, gx);
-            const T w = s_tab[TB::W + q];
-            const T dw = det * w;
-#pragma unroll
-            for (int n = 0; n < NPE; n++)
-#pragma unroll
-                for (int d = 0; d < DIM; d++) sm[L::GX + (q * NPE + n) * DIM + d] = gx[n][d];
-            sm[L::DW + q] = dw;
-            // F = I + (sum_f gx_f u_f^T)^T (solid/Deformation.hpp:26-46)
-            T GU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-            for (int f = 0; f < NPE; f++)
-#pragma unroll
-                for (int J = 0; J < DIM; J++)
-#pragma unroll
-                    for (int i = 0; i < S; i++) GU[J][i] += gx[f][J] * sm[L::U + f * 3 + i];
-            T F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-#pragma unroll
-            for (int J = 0; J < DIM; J++)
-#pragma unroll
-                for (int i = 0; i < S; i++) F[i][J] += GU[J][i];
-            const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
-            T Sm[3][3], Cm[6][6];
-            material<MAT, T>(lam, mu, F, Sm, Cm, WANT_K);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    sm[L::F + q * 9 + i * 3 + j] = F[i][j];
-                    sm[L::SS + q * 9 + i * 3 + j] = Sm[i][j];
-                }
-            if (WANT_K) {
-#pragma unroll
-                for (int i = 0; i < 6; i++)
-#pragma unroll
-                    for (int j = 0; j < 6; j++) sm[L::C + q * 36 + i * 6 + j] = Cm[i][j];
-            }
-            // residual: sum_J P_iJ gx_M[J] * (detJ w), P = F S
-            if (want_resid) {
-                T P[3][3];
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int J = 0; J < 3; J++) {
-                        T t = (T)0;
-#pragma unroll
-                        for (int k = 0; k < 3; k++) t += F[i][k] * Sm[k][J];
-                        P[i][J] = t;
-                    }
-#pragma unroll
-                for (int M = 0; M < NPE; M++)
-#pragma unroll
-                    for (int i = 0; i < S; i++) {
-                        T s = (T)0;
-#pragma unroll
-                        for (int J = 0; J < S; J++) s += P[i][J] * gx[M][J];
-                        s *= dw;
-                        sm[L::FR + q * NEE + M * S + i] = s;
-                    }
-            }
-            if (bf.kind != INS_BF_NONE) {
-                const T* phi = s_tab + TB::PHI + q * NPE;
-                T x[3] = {0, 0, 0};
-#pragma unroll
-                for (int n = 0; n < NPE; n++)
-#pragma unroll
-                    for (int d = 0; d < DIM; d++) x[d] += X[n][d] * phi[n];
-#pragma unroll
-                for (int n = 0; n < NPE; n++)
-#pragma unroll
-                    for (int c = 0; c < S; c++)
-                        sm[L::FB + q * NEE + n * S + c] = body_force<T>(bf, x, DIM, c, e, q) * phi[n] * w * det;
-            }
-        }
-        __syncthreads();
-        if (WANT_K) {
-            // phase 2: A_iJkL = delta_ik S_JL + sum_AB F_iA C_{V(A,J)V(B,L)} F_kB
-            if (valid) {
-                for (int idx = lane; idx < NQ * L::NA; idx += 64) {
-                    const int q = idx / L::NA;
-                    int r = idx % L::NA;
-                    const int L_ = r % S; r /= S;
-                    const int k = r % S; r /= S;
-                    const int J = r % S; r /= S;
-                    const int i = r;
-                    const T* F = sm + L::F + q * 9;
-                    const T* Cq = sm + L::C + q * 36;
-                    T res = (i == k) ? sm[L::SS + q * 9 + J * 3 + L_] : (T)0;
-#pragma unroll
-                    for (int A = 0; A < S; A++) {
-                        const int v1 = voigt_idx(A, J);
-#pragma unroll
-                        for (int B = 0; B < S; B++)
-                            res += F[i * 3 + A] * Cq[v1 * 6 + voigt_idx(B, L_)] * F[k * 3 + B];
-                    }
-                    sm[L::A + idx] = res;
-                }
-            }
-            __syncthreads();
-            // phase 3: lane (M,N) accumulates the S x S block K_MN
-            if (valid && lane < NPE * NPE) {
-                const int M = lane / NPE, N = lane % NPE;
-                T K[S][S];
-#pragma unroll
-                for (int i = 0; i < S; i++)
-#pragma unroll
-                    for (int k = 0; k < S; k++) K[i][k] = (T)0;
-                for (int q = 0; q < NQ; q++) {
-                    T gM[S], gN[S];
-#pragma unroll
-                    for (int d = 0; d < S; d++) {
-                        gM[d] = sm[L::GX + (q * NPE + M) * DIM + d];
-                        gN[d] = sm[L::GX + (q * NPE + N) * DIM + d];
-                    }
-                    const T dw = sm[L::DW + q];
-                    const T* A = sm + L::A + q * L::NA;
-#pragma unroll
-                    for (int i = 0; i < S; i++)
-#pragma unroll
-                        for (int k = 0; k < S; k++) {
-                            T s = (T)0;
-#pragma unroll
-                            for (int J = 0; J < S; J++)
-#pragma unroll
-                                for (int L_ = 0; L_ < S; L_++)
-                                    s += gM[J] * A[((i * S + J) * S + k) * S + L_] * gN[L_];
-                            s *= dw;
-                            K[i][k] += s;
-                        }
-                }
-                T* dst = ke + e * (int64_t)(NEE * NEE) + (M * NPE + N) * S * S;
-#pragma unroll
-                for (int i = 0; i < S; i++)
-#pragma unroll
-                    for (int k = 0; k < S; k++) dst[i * S + k] = K[i][k];
-            }
-        }
-        // element force vector: body force minus internal force, each summed
-        // over the quadrature points in order
-        if (valid && fe && lane < NEE) {
-            T sb = (T)0, sr = (T)0;
-            if (bf.kind != INS_BF_NONE)
-                for (int q = 0; q < NQ; q++) sb += sm[L::FB + q * NEE + lane];
-            if (want_resid)
-                for (int q = 0; q < NQ; q++) sr += sm[L::FR + q * NEE + lane];
-            fe[e * NEE + lane] = (double)sb - (double)sr;
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Degree-2 Lagrange fields on Q1 geometry (QUAD9 / HEX27 on QUAD4 / HEX8).
 // Table layout: geometry dphi [NQ][GN][DIM], geometry phi [NQ][GN], field
 // dphi [NQ][NF][DIM], field phi [NQ][NF], weights [NQ].
@@ -794,212 +583,517 @@ __global__ __launch_bounds__(256) void k_elem_laplace_q2(
     }
 }
 
-// solid::HyperElastic on a degree-2 field (solid/HyperElastic.hpp:110-179
-// tangent, :210-257 residual, :282-310 effective elasticity), the Q1 kernel's
-// three phases with NF field functions on the Q1 geometry: lanes < NQ build
-// the geometry, F, S and C of their point; the wave forms A_iJkL; lane pairs
-// (M,N) accumulate the S x S blocks.
-template <int DIM, int NQ1>
-struct ElQ2 {
-    static constexpr int GN = 1 << DIM;
-    static constexpr int NF = IPow<3, DIM>::value;
-    static constexpr int NQ = IPow<NQ1, DIM>::value;
-    static constexpr int S = DIM;
-    static constexpr int NEE = NF * S;
-    static constexpr int NA = S * S * S * S;
-    static constexpr int X = 0;                 // [GN][3]
-    static constexpr int U = X + GN * 3;        // [NF][3]
-    static constexpr int GX = U + NF * 3;       // [NQ][NF][DIM]
-    static constexpr int DW = GX + NQ * NF * DIM;
-    static constexpr int F = DW + NQ;           // [NQ][9]
-    static constexpr int SS = F + NQ * 9;       // [NQ][9]
-    static constexpr int C = SS + NQ * 9;       // [NQ][36]
-    static constexpr int A = C + NQ * 36;       // [NQ][NA]
-    static constexpr int P = A + NQ * NA;       // [NQ][9] first Piola-Kirchhoff
-    static constexpr int FQ = P + NQ * 9;       // [NQ][3] body force at the point
-    static constexpr int DET = FQ + NQ * 3;     // [NQ]
-    static constexpr int SIZE = DET + NQ;
+// solid::HyperElastic element matrices on the matrix cores, for Q1 fields and
+// for degree-2 fields on the Q1 geometry (solid/HyperElastic.hpp:110-179
+// tangent, :210-257 residual, :282-310 effective elasticity).  With
+// beta_{M,i}[a] = sum over the index pairs (A, J) of Voigt row a of
+// F_iA dphi_{M,J} (the Voigt strain of a unit displacement of DoF (M, i)),
+//   K[(M,i)][(N,k)] = sum_q dw_q ( sum_ab beta_{M,i}[a] C_q[a][b] beta_{N,k}[b]
+//                                  + delta_ik dphi_M . S_q dphi_N ),
+// which is the reference's sum_JL dphi_{M,J} A_iJkL dphi_{N,L}, A_iJkL =
+// delta_ik S_JL + sum_AB F_iA C_{V(A,J)V(B,L)} F_kB, with the sums over
+// (A, J) and (B, L) grouped by Voigt rows.  Both terms are matrix products
+// over (point, row): Bt (dw C B) with NV rows per point and dphi^T (dw S
+// dphi) with DIM rows per point (padded to 4).
+//
+// Two kernels.  k_hyper_points (one thread per element and point) evaluates
+// the geometry, F, the material law and the residual integrand of every
+// point into a point record; k_elem_hyper (one wavefront per element) loads
+// its element's records into LDS, and for each chunk of QC points writes
+// beta and dw C beta of every DoF (and dphi, dw S dphi of every field
+// function) to LDS and advances the upper tiles of both products by 16x16x4
+// matrix-core steps; the element matrix is written as its packed upper
+// triangle, element-major.  Splitting the point work from the matrix work
+// keeps every lane of the point kernel busy (a wavefront per element would
+// leave 56 of 64 lanes idle during it) at the price of one write and one
+// read of the records.
+template <int DIM>
+struct VoigtRows;
+template <>
+struct VoigtRows<3> {
+    static constexpr int NV = 6;
+    // index pair (J, K) of row a, and its row of the 6x6 Voigt C
+    static __device__ inline int j(int a) { return a < 3 ? a : (a == 5 ? 1 : 0); }
+    static __device__ inline int k(int a) { return a < 3 ? a : (a == 3 ? 1 : 2); }
+    static __device__ inline int c(int a) { return a; }
+};
+template <>
+struct VoigtRows<2> {
+    static constexpr int NV = 3;   // xx, yy, xy: the z rows of a plane problem vanish
+    static __device__ inline int j(int a) { return a < 2 ? a : 0; }
+    static __device__ inline int k(int a) { return a < 2 ? a : 1; }
+    static __device__ inline int c(int a) { return a < 2 ? a : 3; }
 };
 
-template <int DIM, int NQ1, typename T, int MAT, bool WANT_K>
-__global__ __launch_bounds__(64) void k_elem_elastic_q2(
+// upper-triangle index of the symmetric 6x6 Voigt matrix (a <= b)
+__device__ inline int c21(int a, int b) { return a * 6 - (a * (a - 1)) / 2 + (b - a); }
+
+// Point record (elements of T): inv(J^T) [9], detJ w, F [9], S [6, Voigt
+// order], C [21, upper triangle], detJ, P = F S [9], body force [3]; stored
+// field-major over all points (field f of point p at f * n_points + p), so
+// that the point kernel's stores and the element kernel's loads of the
+// consecutive points of an element are contiguous runs
+struct PtRec {
+    static constexpr int INV = 0, DW = 9, F = 10, S = 19, C = 25, DET = 46, P = 47, FQ = 56;
+    static constexpr int K = 46;      // what the matrix needs
+    static constexpr int SIZE = 60;   // with the force integrands (16-byte multiple for FP64)
+};
+
+// The Voigt tangent C of the material law at F, upper triangle (the values
+// of material(..., wantC = true))
+template <int MAT, typename T>
+__device__ inline void store_voigt_c21(T lam, T mu, const T F[3][3], T* __restrict__ C)
+{
+    if constexpr (MAT == INS_STVENANT) {
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = i; j < 6; j++)
+                C[c21(i, j)] = (i < 3 && j < 3 ? lam : (T)0) + (i == j ? (i < 3 ? (T)2 * mu : mu) : (T)0);
+    } else {
+        T CG[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                T s = (T)0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) s += F[k][i] * F[k][j];
+                CG[i][j] = s;
+            }
+        T Ci[3][3];
+        inv3<T>(CG, Ci);
+        const T J = (F[0][0] * F[1][1] * F[2][2] + F[0][1] * F[1][2] * F[2][0] +
+                     F[0][2] * F[1][0] * F[2][1] - F[0][0] * F[1][2] * F[2][1] -
+                     F[0][1] * F[1][0] * F[2][2] - F[0][2] * F[1][1] * F[2][0]);
+        const T fac2 = mu - lam * log(J);
+#pragma unroll
+        for (int A = 0; A < 3; A++)
+#pragma unroll
+            for (int B = A; B < 3; B++) {
+                const int AB = voigt_idx(A, B);
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int D = c; D < 3; D++) {
+                        const int cD = voigt_idx(c, D);
+                        if (AB <= cD)
+                            C[c21(AB, cD)] = lam * Ci[A][B] * Ci[c][D] +
+                                             fac2 * (Ci[A][c] * Ci[B][D] + Ci[A][D] * Ci[B][c]);
+                    }
+            }
+    }
+}
+
+// tables of the element kernels: geometry dphi [NQ][GN][DIM] and phi
+// [NQ][GN], field dphi [NQ][NF][DIM] and phi [NQ][NF], weights [NQ] (for Q1
+// fields the field tables are the geometry ones)
+template <typename T>
+struct ElTab {
+    const T *gd, *gp, *fd, *fp, *w;
+};
+
+template <int DIM, int NF, int NQ1, typename T, int MAT>
+__global__ __launch_bounds__(256) void k_hyper_points(
     int64_t E, const int32_t* __restrict__ gconn, const int32_t* __restrict__ fconn,
     const double* __restrict__ xyz, const double* __restrict__ coef,
-    const double* __restrict__ u, int ncoef, const T* __restrict__ tab, Bf bf, int want_resid,
-    T* __restrict__ ke, double* __restrict__ fe)
+    const double* __restrict__ u, int ncoef, ElTab<T> tb, Bf bf, int want_k, int want_f,
+    T* __restrict__ pts)
 {
-    using L = ElQ2<DIM, NQ1>;
-    using TB = TabQ2<DIM, L::NQ>;
-    constexpr int GN = L::GN, NF = L::NF, NQ = L::NQ, S = L::S, NEE = L::NEE;
-    const T* __restrict__ s_tab = tab;   // tables read through the cache
-    __shared__ T sm[L::SIZE];
-    const int lane = threadIdx.x;
-    for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
-        if (lane < GN * 3) {
-            const int n = lane / 3, d = lane % 3;
+    constexpr int GN = 1 << DIM, NQ = IPow<NQ1, DIM>::value, S = DIM;
+    for (int64_t pt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pt < E * NQ;
+         pt += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = pt / NQ;
+        const int q = (int)(pt - e * NQ);
+        T X[GN * DIM];
+#pragma unroll
+        for (int n = 0; n < GN; n++) {
             const int64_t node = gconn[e * GN + n];
-            sm[L::X + lane] = d < DIM ? (T)xyz[node * DIM + d] : (T)0;
-        }
-        for (int i = lane; i < NF * 3; i += 64) {
-            const int n = i / 3, d = i % 3;
-            const int64_t node = fconn[e * NF + n];
-            sm[L::U + i] = (d < S && u) ? (T)u[node * S + d] : (T)0;
-        }
-        __syncthreads();
-        if (lane < NQ) {
-            const int q = lane;
-            T Xg[GN * DIM];
 #pragma unroll
-            for (int n = 0; n < GN; n++)
-#pragma unroll
-                for (int d = 0; d < DIM; d++) Xg[n * DIM + d] = sm[L::X + n * 3 + d];
-            T inv[DIM][DIM];
-            const T det = geo_inverse<DIM, T>(Xg, s_tab + TB::GD + q * GN * DIM, inv);
-            const T w = s_tab[TB::W + q];
-            const T dw = det * w;
-            sm[L::DW + q] = dw;
-            sm[L::DET + q] = det;
-            T GU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int d = 0; d < DIM; d++) X[n * DIM + d] = (T)xyz[node * DIM + d];
+        }
+        T inv[DIM][DIM];
+        const T det = geo_inverse<DIM, T>(X, tb.gd + q * GN * DIM, inv);
+        const T dw = det * tb.w[q];
+        // F = I + (sum_f dphi_f u_f^T)^T (solid/Deformation.hpp:26-46)
+        T GU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (u) {
+#pragma unroll 1
             for (int f = 0; f < NF; f++) {
-                const T* fd = s_tab + TB::FD + (q * NF + f) * DIM;
-                T g[DIM];
+                const T* fd = tb.fd + (q * NF + f) * DIM;
+                const int64_t node = fconn[e * NF + f];
+                T uf[S];
 #pragma unroll
-                for (int i = 0; i < DIM; i++) {
-                    T s = (T)0;
+                for (int i = 0; i < S; i++) uf[i] = (T)u[node * S + i];
 #pragma unroll
-                    for (int a = 0; a < DIM; a++) s += inv[i][a] * fd[a];
-                    g[i] = s;
-                    sm[L::GX + (q * NF + f) * DIM + i] = s;
+                for (int J = 0; J < DIM; J++) {
+                    T g = (T)0;
+#pragma unroll
+                    for (int a = 0; a < DIM; a++) g += inv[J][a] * fd[a];
+#pragma unroll
+                    for (int i = 0; i < S; i++) GU[J][i] += g * uf[i];
                 }
-#pragma unroll
-                for (int J = 0; J < DIM; J++)
-#pragma unroll
-                    for (int i = 0; i < S; i++) GU[J][i] += g[J] * sm[L::U + f * 3 + i];
             }
-            T F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        }
+        T Fm[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 #pragma unroll
-            for (int J = 0; J < DIM; J++)
+        for (int J = 0; J < DIM; J++)
 #pragma unroll
-                for (int i = 0; i < S; i++) F[i][J] += GU[J][i];
-            const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
-            T Sm[3][3], Cm[6][6];
-            material<MAT, T>(lam, mu, F, Sm, Cm, WANT_K);
+            for (int i = 0; i < S; i++) Fm[i][J] += GU[J][i];
+        const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
+        T Sm[3][3], Cm[6][6];
+        material<MAT, T>(lam, mu, Fm, Sm, Cm, false);
+        const int64_t np = E * NQ;
+        T* rec = pts + pt;   // field f at rec[f * np]
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) rec[(int64_t)(PtRec::INV + i * 3 + a) * np] = (i < DIM && a < DIM) ? inv[i][a] : (T)0;
+        rec[(int64_t)(PtRec::DW) * np] = dw;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) rec[(int64_t)(PtRec::F + i * 3 + j) * np] = Fm[i][j];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = i; j < 3; j++) rec[(int64_t)(PtRec::S + voigt_idx(i, j)) * np] = Sm[i][j];
+        if (want_k) {
+            T c[21];
+            store_voigt_c21<MAT, T>(lam, mu, Fm, c);
+#pragma unroll
+            for (int k = 0; k < 21; k++) rec[(int64_t)(PtRec::C + k) * np] = c[k];
+        }
+        if (want_f) {
+            rec[(int64_t)(PtRec::DET) * np] = det;
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    sm[L::F + q * 9 + i * 3 + j] = F[i][j];
-                    sm[L::SS + q * 9 + i * 3 + j] = Sm[i][j];
+                for (int J = 0; J < 3; J++) {
+                    T t = (T)0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) t += Fm[i][k] * Sm[k][J];
+                    rec[(int64_t)(PtRec::P + i * 3 + J) * np] = t;
                 }
-            if (WANT_K) {
-#pragma unroll
-                for (int i = 0; i < 6; i++)
-#pragma unroll
-                    for (int j = 0; j < 6; j++) sm[L::C + q * 36 + i * 6 + j] = Cm[i][j];
-            }
-            if (want_resid) {
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int J = 0; J < 3; J++) {
-                        T t = (T)0;
-#pragma unroll
-                        for (int k = 0; k < 3; k++) t += F[i][k] * Sm[k][J];
-                        sm[L::P + q * 9 + i * 3 + J] = t;
-                    }
-            }
+            T x[3] = {0, 0, 0};
             if (bf.kind != INS_BF_NONE) {
-                T x[3] = {0, 0, 0};
 #pragma unroll
                 for (int n = 0; n < GN; n++)
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) x[d] += Xg[n * DIM + d] * s_tab[TB::GP + q * GN + n];
+                    for (int d = 0; d < DIM; d++) x[d] += X[n * DIM + d] * tb.gp[q * GN + n];
+            }
 #pragma unroll
-                for (int c = 0; c < S; c++) sm[L::FQ + q * 3 + c] = body_force<T>(bf, x, DIM, c, e, q);
+            for (int c = 0; c < 3; c++)
+                rec[(int64_t)(PtRec::FQ + c) * np] = (bf.kind != INS_BF_NONE && c < S) ? body_force<T>(bf, x, DIM, c, e, q) : (T)0;
+        }
+    }
+}
+
+// Ordering of LDS accesses among the lanes of one wavefront (a block of 64
+// threads): a wavefront's LDS operations are performed in issue order, so a
+// compiler barrier for memory operations is all that is needed (wavefront-
+// scope fences emit no waits).  Unlike __syncthreads() it does not wait for
+// the wavefront's outstanding global loads and stores.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int DIM, int NF, int NQ1, int QC>
+struct Hyp {
+    static constexpr int NQ = IPow<NQ1, DIM>::value;
+    static constexpr int S = DIM;
+    static constexpr int NEE = NF * S;
+    static constexpr int NV = VoigtRows<DIM>::NV;
+    static constexpr int NT = (NEE + 15) / 16;   // 16-wide tiles of the element matrix
+    static constexpr int NTG = (NF + 15) / 16;   // ... of the geometric term
+    static constexpr int KC = QC * NV;           // Voigt rows of a chunk
+    static constexpr int KCP = (KC + 3) / 4 * 4;
+    static constexpr int KG = QC * 4;            // gradient rows of a chunk (DIM used)
+    static constexpr int NK = NEE * (NEE + 1) / 2;
+    // LDS layout (elements of T)
+    static constexpr int R = 0;                  // [NQ][PtRec::SIZE] the element's point records
+    static constexpr int BV = R + NQ * PtRec::SIZE;   // [KCP][NEE] beta
+    static constexpr int CB = BV + KCP * NEE;    // [KCP][NEE] dw C beta
+    static constexpr int GG = CB + KCP * NEE;    // [KG][NF] dphi
+    static constexpr int SG = GG + KG * NF;      // [KG][NF] dw S dphi
+    static constexpr int KGO = SG + KG * NF;     // [NF][NF] geometric term
+    // the reference gradients of the field [NQ][NF][DIM] in LDS when small
+    static constexpr bool FDL = NQ * NF * DIM <= 512;
+    static constexpr int FD = KGO + NF * NF;
+    static constexpr int SIZE = FD + (FDL ? NQ * NF * DIM : 0);
+    // the packed upper triangle is assembled in the product buffers when it
+    // fits there and then written out contiguously; otherwise every lane
+    // writes its tile entries straight to memory
+    static constexpr bool PACK = NK <= 2 * KCP * NEE;
+    static constexpr int PK = BV;
+    // upper tiles of the element matrix, taken in passes of at most TPP
+    // (the accumulators of one pass stay in registers; every pass sweeps the
+    // points again)
+    static constexpr int NTT = NT * (NT + 1) / 2;
+    static constexpr int TPP = 12;
+    static constexpr int NPASS = (NTT + TPP - 1) / TPP;
+    static constexpr int TP = NTT < TPP ? NTT : TPP;
+};
+
+template <int DIM, int NF, int NQ1, int QC, typename T, bool WANT_K>
+__global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restrict__ pts,
+                                                   ElTab<T> tb, int want_f, int want_resid,
+                                                   int has_bf, T* __restrict__ ke,
+                                                   double* __restrict__ fe)
+{
+    using L = Hyp<DIM, NF, NQ1, QC>;
+    using V = VoigtRows<DIM>;
+    using MF = Mfma<T>;
+    constexpr int NQ = L::NQ, S = L::S, NEE = L::NEE, NV = L::NV;
+    constexpr int NT = L::NT, NTG = L::NTG, RS = PtRec::SIZE;
+    __shared__ T sm[L::SIZE];
+    const int lane = threadIdx.x;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    // rows of the chunk buffers that no point fills stay zero
+    for (int i = L::KC * NEE + lane; i < L::KCP * NEE; i += 64) {
+        sm[L::BV + i] = (T)0;
+        sm[L::CB + i] = (T)0;
+    }
+    for (int i = lane; i < L::KG * NF; i += 64) {
+        sm[L::GG + i] = (T)0;
+        sm[L::SG + i] = (T)0;
+    }
+    if (L::FDL)
+        for (int i = lane; i < NQ * NF * DIM; i += 64) sm[L::FD + i] = tb.fd[i];
+    const T* __restrict__ fdt = L::FDL ? sm + L::FD : tb.fd;
+    // the point records of an element (field f of its point q at f * np +
+    // e * NQ + q) go to LDS as [q][f]; those of the next element are loaded
+    // into registers while this one is processed (when they take at most 8
+    // values per lane)
+    constexpr int NR = NQ * RS;
+    constexpr int RPL = (NR + 63) / 64;
+    constexpr bool PRE = RPL <= 8;
+    const int64_t np = E * NQ;
+    T pre[PRE ? RPL : 1];
+    auto fetch = [&](int64_t ee) {
+        if constexpr (PRE) {
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                const int i = lane + 64 * r, f = i / NQ, q = i - f * NQ;
+                pre[r] = (ee < E && i < NR) ? pts[f * np + ee * NQ + q] : (T)0;
             }
         }
-        __syncthreads();
-        if (WANT_K) {
-            for (int idx = lane; idx < NQ * L::NA; idx += 64) {
-                const int q = idx / L::NA;
-                int r = idx % L::NA;
-                const int L_ = r % S; r /= S;
-                const int k = r % S; r /= S;
-                const int J = r % S; r /= S;
-                const int i = r;
-                const T* F = sm + L::F + q * 9;
-                const T* Cq = sm + L::C + q * 36;
-                T res = (i == k) ? sm[L::SS + q * 9 + J * 3 + L_] : (T)0;
+    };
+    // every XCD (blocks are placed round-robin) takes a contiguous eighth of
+    // the elements, so that the 64 elements of one slab of the output are
+    // written through one L2
+    const int xcd = blockIdx.x % 8, nloc = gridDim.x / 8;
+    const int64_t per = ((E + 7) / 8 + 63) / 64 * 64;
+    const int64_t lo = xcd * per, hi = lo + per < E ? lo + per : E;
+    fetch(lo + blockIdx.x / 8 < hi ? lo + blockIdx.x / 8 : E);
+    for (int64_t e = lo + blockIdx.x / 8; e < hi; e += nloc) {
+        if constexpr (PRE) {
 #pragma unroll
-                for (int A = 0; A < S; A++) {
-                    const int v1 = voigt_idx(A, J);
-#pragma unroll
-                    for (int B = 0; B < S; B++)
-                        res += F[i * 3 + A] * Cq[v1 * 6 + voigt_idx(B, L_)] * F[k * 3 + B];
-                }
-                sm[L::A + idx] = res;
+            for (int r = 0; r < RPL; r++) {
+                const int i = lane + 64 * r, f = i / NQ, q = i - f * NQ;
+                if (i < NR) sm[L::R + q * RS + f] = pre[r];
             }
-            __syncthreads();
-            for (int pr = lane; pr < NF * NF; pr += 64) {
-                const int M = pr / NF, N = pr % NF;
-                T K[S][S];
+            fetch(e + nloc < hi ? e + nloc : E);
+        } else {
+            for (int i = lane; i < NR; i += 64) {
+                const int f = i / NQ, q = i - f * NQ;
+                sm[L::R + q * RS + f] = pts[f * np + e * NQ + q];
+            }
+        }
+        wave_lds_sync();
+        if (WANT_K) {
+            constexpr int TP = L::TP;
 #pragma unroll
-                for (int i = 0; i < S; i++)
+            for (int pass = 0; pass < L::NPASS; pass++) {
+                typename MF::v4 acc[TP], accg[NTG * (NTG + 1) / 2];
 #pragma unroll
-                    for (int k = 0; k < S; k++) K[i][k] = (T)0;
-                for (int q = 0; q < NQ; q++) {
-                    T gM[S], gN[S];
+                for (int t = 0; t < TP; t++) acc[t] = (typename MF::v4){0, 0, 0, 0};
 #pragma unroll
-                    for (int d = 0; d < S; d++) {
-                        gM[d] = sm[L::GX + (q * NF + M) * DIM + d];
-                        gN[d] = sm[L::GX + (q * NF + N) * DIM + d];
-                    }
-                    const T dw = sm[L::DW + q];
-                    const T* A = sm + L::A + q * L::NA;
+                for (int t = 0; t < NTG * (NTG + 1) / 2; t++) accg[t] = (typename MF::v4){0, 0, 0, 0};
+#pragma unroll 1
+                for (int q0 = 0; q0 < NQ; q0 += QC) {
+                    // beta and dw C beta of every DoF, dphi and dw S dphi of
+                    // every field function, at the chunk's points
+                    for (int it = lane; it < QC * NF; it += 64) {
+                        const int qc = it / NF, M = it - qc * NF, q = q0 + qc;
+                        const bool qin = q < NQ;
+                        const T* rc = sm + L::R + (qin ? q : 0) * RS;
+                        T g[3] = {0, 0, 0};
+                        if (qin) {
+                            const T* fd = fdt + (q * NF + M) * DIM;
 #pragma unroll
-                    for (int i = 0; i < S; i++)
+                            for (int J = 0; J < DIM; J++) {
+                                T s = (T)0;
 #pragma unroll
-                        for (int k = 0; k < S; k++) {
-                            T s = (T)0;
-#pragma unroll
-                            for (int J = 0; J < S; J++)
-#pragma unroll
-                                for (int L_ = 0; L_ < S; L_++)
-                                    s += gM[J] * A[((i * S + J) * S + k) * S + L_] * gN[L_];
-                            s *= dw;
-                            K[i][k] += s;
+                                for (int a = 0; a < DIM; a++) s += rc[PtRec::INV + J * 3 + a] * fd[a];
+                                g[J] = s;
+                            }
                         }
+                        const T dw = qin ? rc[PtRec::DW] : (T)0;
+#pragma unroll 1
+                        for (int i = 0; i < S; i++) {
+                            T be[NV];
+#pragma unroll
+                            for (int a = 0; a < NV; a++) {
+                                const int J = V::j(a), K = V::k(a);
+                                T b = rc[PtRec::F + i * 3 + J] * g[K];
+                                if (J != K) b += rc[PtRec::F + i * 3 + K] * g[J];
+                                be[a] = b;
+                            }
+#pragma unroll
+                            for (int a = 0; a < NV; a++) {
+                                T cb = (T)0;
+#pragma unroll
+                                for (int b = 0; b < NV; b++) {
+                                    const int ca = V::c(a), cbb = V::c(b);
+                                    cb += rc[PtRec::C + (ca <= cbb ? c21(ca, cbb) : c21(cbb, ca))] * be[b];
+                                }
+                                sm[L::BV + (qc * NV + a) * NEE + M * S + i] = be[a];
+                                sm[L::CB + (qc * NV + a) * NEE + M * S + i] = dw * cb;
+                            }
+                        }
+                        if (pass == 0) {
+#pragma unroll
+                            for (int J = 0; J < DIM; J++) {
+                                T sg = (T)0;
+#pragma unroll
+                                for (int Lx = 0; Lx < DIM; Lx++) sg += rc[PtRec::S + voigt_idx(J, Lx)] * g[Lx];
+                                sm[L::GG + (qc * 4 + J) * NF + M] = g[J];
+                                sm[L::SG + (qc * 4 + J) * NF + M] = dw * sg;
+                            }
+                        }
+                    }
+                    wave_lds_sync();
+                    // this pass's upper tiles of Bt (dw C B) over the chunk's
+                    // Voigt rows
+#pragma unroll 2
+                    for (int s0 = 0; s0 < L::KCP; s0 += 4) {
+                        const T* br = sm + L::BV + (s0 + k4) * NEE;
+                        const T* cr = sm + L::CB + (s0 + k4) * NEE;
+                        T av[NT], bv[NT];
+#pragma unroll
+                        for (int t = 0; t < NT; t++) {
+                            const int p = t * 16 + r16;
+                            av[t] = p < NEE ? br[p] : (T)0;
+                            bv[t] = p < NEE ? cr[p] : (T)0;
+                        }
+                        int t = 0;
+#pragma unroll
+                        for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                            for (int tj = ti; tj < NT; tj++) {
+                                if (t >= pass * TP && t < pass * TP + TP)
+                                    acc[t - pass * TP] = MF::op(av[ti], bv[tj], acc[t - pass * TP]);
+                                t++;
+                            }
+                    }
+                    // ... and of dphi^T (dw S dphi), one step per point
+                    if (pass == 0) {
+#pragma unroll 1
+                        for (int s0 = 0; s0 < L::KG; s0 += 4) {
+                            const T* gr = sm + L::GG + (s0 + k4) * NF;
+                            const T* sr = sm + L::SG + (s0 + k4) * NF;
+                            T av[NTG], bv[NTG];
+#pragma unroll
+                            for (int t = 0; t < NTG; t++) {
+                                const int M = t * 16 + r16;
+                                av[t] = M < NF ? gr[M] : (T)0;
+                                bv[t] = M < NF ? sr[M] : (T)0;
+                            }
+                            int t = 0;
+#pragma unroll
+                            for (int ti = 0; ti < NTG; ti++)
+#pragma unroll
+                                for (int tj = ti; tj < NTG; tj++) {
+                                    accg[t] = MF::op(av[ti], bv[tj], accg[t]);
+                                    t++;
+                                }
+                        }
+                    }
+                    wave_lds_sync();
                 }
-                T* dst = ke + e * (int64_t)(NEE * NEE) + (M * NF + N) * S * S;
+                if (pass == 0) {
+                    // the geometric term to LDS
+                    int t = 0;
 #pragma unroll
-                for (int i = 0; i < S; i++)
+                    for (int ti = 0; ti < NTG; ti++)
 #pragma unroll
-                    for (int k = 0; k < S; k++) dst[i * S + k] = K[i][k];
+                        for (int tj = ti; tj < NTG; tj++) {
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int M = ti * 16 + MF::row(lane, r), N = tj * 16 + r16;
+                                if (M < NF && N < NF) sm[L::KGO + M * NF + N] = accg[t][r];
+                            }
+                            t++;
+                        }
+                    wave_lds_sync();
+                }
+                // the packed upper triangle: material part plus the geometric
+                // part on the diagonal of every node block; in memory entry k
+                // of element e is at (e/64)*64*NK + k*64 + e%64
+                constexpr int OS = L::PACK ? 1 : 64;
+                T* dst = L::PACK ? sm + L::PK : ke + (e >> 6) * (64 * (int64_t)L::NK) + (e & 63);
+                int t = 0;
+#pragma unroll
+                for (int ti = 0; ti < NT; ti++)
+#pragma unroll
+                    for (int tj = ti; tj < NT; tj++) {
+                        if (t >= pass * TP && t < pass * TP + TP) {
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int p = ti * 16 + MF::row(lane, r), qq = tj * 16 + r16;
+                                if (p < NEE && qq < NEE && p <= qq) {
+                                    const int M = p / S, i = p - M * S, N = qq / S, k = qq - N * S;
+                                    T v = acc[t - pass * TP][r];
+                                    if (i == k) v += sm[L::KGO + M * NF + N];
+                                    dst[(p * NEE - (p * (p - 1)) / 2 + (qq - p)) * OS] = v;
+                                }
+                            }
+                        }
+                        t++;
+                    }
+            }
+            if (L::PACK) {
+                wave_lds_sync();
+                T* dst = ke + (e >> 6) * (64 * (int64_t)L::NK) + (e & 63);
+                for (int k = lane; k < L::NK; k += 64) dst[k * 64] = sm[L::PK + k];
             }
         }
         // element force vector, one DoF per lane, each sum over the points in
         // order: body force f phi w detJ, internal force (P grad phi) detJ w
-        if (fe) {
+        if (want_f) {
             for (int j = lane; j < NEE; j += 64) {
-                const int M = j / S, i = j - (j / S) * S;
+                const int M = j / S, i = j - M * S;
                 T sb = (T)0, sr = (T)0;
-                if (bf.kind != INS_BF_NONE)
-                    for (int q = 0; q < NQ; q++)
-                        sb += sm[L::FQ + q * 3 + i] * s_tab[TB::FP + q * NF + M] * s_tab[TB::W + q] *
-                              sm[L::DET + q];
-                if (want_resid)
+                if (has_bf)
+#pragma unroll 1
                     for (int q = 0; q < NQ; q++) {
+                        const T* rc = sm + L::R + q * RS;
+                        sb += rc[PtRec::FQ + i] * tb.fp[q * NF + M] * tb.w[q] * rc[PtRec::DET];
+                    }
+                if (want_resid)
+#pragma unroll 1
+                    for (int q = 0; q < NQ; q++) {
+                        const T* rc = sm + L::R + q * RS;
+                        const T* fd = tb.fd + (q * NF + M) * DIM;
                         T t = (T)0;
 #pragma unroll
-                        for (int J = 0; J < S; J++)
-                            t += sm[L::P + q * 9 + i * 3 + J] * sm[L::GX + (q * NF + M) * DIM + J];
-                        t *= sm[L::DW + q];
+                        for (int J = 0; J < DIM; J++) {
+                            T g = (T)0;
+#pragma unroll
+                            for (int a = 0; a < DIM; a++) g += rc[PtRec::INV + J * 3 + a] * fd[a];
+                            t += rc[PtRec::P + i * 3 + J] * g;
+                        }
+                        t *= rc[PtRec::DW];
                         sr += t;
                     }
                 fe[e * NEE + j] = (double)sb - (double)sr;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 
@@ -1022,12 +1116,19 @@ struct KeLaplace {
     }
 };
 
+// solid::HyperElastic: the packed upper triangle of the symmetric element
+// matrix over DoFs (node, component) in 64-element slabs (entry k of element
+// e at (e/64)*64*NK + k*64 + e%64)
 template <typename T, int NPE, int S>
 struct KeElastic {
     const T* p;
     __device__ inline double get(int64_t e, int a, int cr, int b, int cc) const
     {
-        return (double)p[e * (int64_t)(NPE * S * NPE * S) + ((a * NPE + b) * S + cr) * S + cc];
+        constexpr int NEE = NPE * S, NK = NEE * (NEE + 1) / 2;
+        const int r = a * S + cr, c = b * S + cc;
+        const int i = r < c ? r : c, j = r < c ? c : r;
+        const int k = i * NEE - (i * (i - 1)) / 2 + (j - i);
+        return (double)p[(e >> 6) * (64 * (int64_t)NK) + k * 64 + (e & 63)];
     }
 };
 
@@ -1893,56 +1994,51 @@ int launch_laplace_q2(Ctx* c, const T* tab, Bf bf, T* ke, double* fe, T* me)
     return INS_OK;
 }
 
-template <int DIM, int NQ1, typename T, int MAT>
-int launch_elastic_q2_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
-{
-    const unsigned g = grid_for(c->n_elem, 1, 1u << 22);
-    if (want_k)
-        k_elem_elastic_q2<DIM, NQ1, T, MAT, true><<<g, 64, 0, c->stream>>>(
-            c->n_elem, c->gconn.p, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf,
-            want_r, ke, fe);
-    else
-        k_elem_elastic_q2<DIM, NQ1, T, MAT, false><<<g, 64, 0, c->stream>>>(
-            c->n_elem, c->gconn.p, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf,
-            want_r, ke, fe);
-    INS_HIP_CHECK(c, hipGetLastError());
-    return INS_OK;
-}
-
-template <int DIM, typename T, int MAT>
-int launch_elastic_q2(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
-{
-    switch (c->nq) {
-    case 1: return launch_elastic_q2_nq<DIM, 1, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
-    case 2: return launch_elastic_q2_nq<DIM, 2, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
-    case 3: return launch_elastic_q2_nq<DIM, 3, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
-    default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
-    }
-}
-
-template <int DIM, int NQ1, typename T, int MAT>
-int launch_elastic_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+// the hyperelastic element kernels for a field of NF functions per element:
+// the point records, then the element matrices with chunks of points that
+// fill the 64 lanes (HEX8: 4 points of 8 functions, HEX27: 2 points of 27)
+template <int DIM, int NF, int NQ1, typename T, int MAT>
+int launch_hyper_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
 {
     constexpr int NQ = IPow<NQ1, DIM>::value;
-    constexpr int EPB = (NQ <= 8) ? 4 : 1;
-    const unsigned g = grid_for((c->n_elem + EPB - 1) / EPB, 1, 1u << 22);
+    constexpr int QC0 = 64 / NF < 4 ? 64 / NF : 4;
+    constexpr int QC = QC0 < NQ ? QC0 : NQ;
+    ElTab<T> tb;
+    if constexpr (NF == (1 << DIM)) {
+        using TB = Tab<DIM, NQ>;
+        tb = ElTab<T>{tab + TB::DPHI, tab + TB::PHI, tab + TB::DPHI, tab + TB::PHI, tab + TB::W};
+    } else {
+        using TB = TabQ2<DIM, NQ>;
+        tb = ElTab<T>{tab + TB::GD, tab + TB::GP, tab + TB::FD, tab + TB::FP, tab + TB::W};
+    }
+    const int want_f = fe != nullptr;
+    const int64_t E = c->n_elem;
+    INS_TRY(ensure(c, c->stage_pt, (size_t)E * NQ * PtRec::SIZE * sizeof(T)));
+    T* pts = (T*)c->stage_pt.p;
+    k_hyper_points<DIM, NF, NQ1, T, MAT><<<grid_for(E * NQ, 256), 256, 0, c->stream>>>(
+        E, c->geo_conn(), c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tb, bf, want_k,
+        want_f, pts);
+    INS_HIP_CHECK(c, hipGetLastError());
+    // a whole number of blocks per XCD (the kernel gives each XCD an eighth)
+    const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((E + 7) / 8 * 8, 1 << 22));
+    const int has_bf = bf.kind != INS_BF_NONE;
     if (want_k)
-        k_elem_elastic<DIM, NQ1, T, MAT, true><<<g, 64 * EPB, 0, c->stream>>>(
-            c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf, want_r, ke, fe);
+        k_elem_hyper<DIM, NF, NQ1, QC, T, true><<<g, 64, 0, c->stream>>>(
+            E, pts, tb, want_f, want_r, has_bf, ke, fe);
     else
-        k_elem_elastic<DIM, NQ1, T, MAT, false><<<g, 64 * EPB, 0, c->stream>>>(
-            c->n_elem, c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tab, bf, want_r, ke, fe);
+        k_elem_hyper<DIM, NF, NQ1, QC, T, false><<<g, 64, 0, c->stream>>>(
+            E, pts, tb, want_f, want_r, has_bf, ke, fe);
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }
 
-template <int DIM, typename T, int MAT>
-int launch_elastic(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
+template <int DIM, int NF, typename T, int MAT>
+int launch_hyper(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
 {
     switch (c->nq) {
-    case 1: return launch_elastic_nq<DIM, 1, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
-    case 2: return launch_elastic_nq<DIM, 2, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
-    case 3: return launch_elastic_nq<DIM, 3, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 1: return launch_hyper_nq<DIM, NF, 1, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 2: return launch_hyper_nq<DIM, NF, 2, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
+    case 3: return launch_hyper_nq<DIM, NF, 3, T, MAT>(c, tab, bf, want_k, want_r, ke, fe);
     default: return fail(c, INS_ERR_UNSUPPORTED, "quadrature with more than 3 points per direction");
     }
 }
@@ -1950,14 +2046,15 @@ int launch_elastic(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, d
 template <typename T, int DIM>
 int launch_elastic_mat(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, double* fe)
 {
+    constexpr int NF1 = 1 << DIM, NF2 = DIM == 3 ? 27 : 9;
     if (c->deg == 2) {
         if (c->kernel == INS_STVENANT)
-            return launch_elastic_q2<DIM, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
-        return launch_elastic_q2<DIM, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
+            return launch_hyper<DIM, NF2, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
+        return launch_hyper<DIM, NF2, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
     }
     if (c->kernel == INS_STVENANT)
-        return launch_elastic<DIM, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
-    return launch_elastic<DIM, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
+        return launch_hyper<DIM, NF1, T, INS_STVENANT>(c, tab, bf, want_k, want_r, ke, fe);
+    return launch_hyper<DIM, NF1, T, INS_NEOHOOKE>(c, tab, bf, want_k, want_r, ke, fe);
 }
 
 int lg2(int v)
@@ -2150,7 +2247,7 @@ int assemble(Ctx* c, int precision, int flags)
     if (!elastic)
         ke_elems = (size_t)((E + 63) / 64) * 64 * (npe * (npe + 1) / 2);
     else
-        ke_elems = (size_t)E * (npe * s) * (npe * s);
+        ke_elems = (size_t)((E + 63) / 64) * 64 * (npe * s) * (npe * s + 1) / 2;
     if (do_matrix) INS_TRY(ensure(c, c->stage_ke, ke_elems * esize));
     const bool need_fe = do_rhs && (c->bf_kind != INS_BF_NONE || want_r);
     if (need_fe && !staged) INS_TRY(ensure(c, c->stage_fe, (size_t)((E + 63) / 64) * 64 * npe * s));

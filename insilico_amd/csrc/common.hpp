@@ -261,6 +261,7 @@ struct Ctx {
     DArr<float> dinv32;
     DArr<uint8_t> stage_ke;  // element matrices (type depends on precision)
     DArr<uint8_t> stage_me;  // element mass matrices (time steps)
+    DArr<uint8_t> stage_pt;  // point records of the hyperelastic kernels
     DArr<double> stage_fe;   // element force vectors (FP64)
     bool fe_slab = false;    // ... in 64-element slabs (heat::Laplace Q1) or element-major
     DArr<double> stage_ff;   // face force vectors
