@@ -1720,10 +1720,17 @@ unsigned round8(int64_t g)
     return (unsigned)((g + 7) / 8 * 8);
 }
 
+int env_int(const char* name, int dflt);
+
+// blocks of the vector kernels: at most INS_VEC_BLOCKS (256; 8..2048).  At
+// 128^3 the mixed step takes 35.3 ms with 192-256 blocks, 35.8 with 512,
+// 36.7 with 128 and 42.7 with 2048 (every block sums the partials of the
+// kernel before)
 unsigned vec_grid(int64_t n)
 {
+    static const int cap = std::max(8, std::min(MAXG, env_int("INS_VEC_BLOCKS", 256)));
     int64_t g = (n + VB - 1) / VB;
-    return round8(std::max<int64_t>(1, std::min<int64_t>(g, 512)));
+    return round8(std::max<int64_t>(1, std::min<int64_t>(g, cap)));
 }
 
 unsigned sell_grid(int64_t n)
