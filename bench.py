@@ -504,18 +504,24 @@ def main():
     stn = ctx.stats()
     sell_vals = stn["sell_values"]
     # the library falls back to CSR when the SELL padding would not pay, and
-    # runs the FP32 sweeps of scalar problems on the symmetric slices
+    # runs the FP32 sweeps on the symmetric slices (scalar rows, or D x D
+    # node blocks of vector problems) where the pattern allows them
     fmt_name = "sell" if (fmt == abi.FMT_SELL and sell_vals) else "csr"
     if fmt_name == "sell" and stn["sell_sym_offsets"]:
         fmt_name = "sym"
-        sell_vals = (n + 63) // 64 * 64 * stn["sell_sym_offsets"]
+        sell_vals = stn["sym_values"]
     alg = spmv_bytes(n, nnz, sell_vals, fmt_name, 4, 4)
     achieved = round(alg / (spmv_ms / 1e3) / 1e9, 1) if spmv_calls else None
     default_n = {"diffusion": 128, "elasticity": 96, "q2": 64}[args.config]
-    traffic = read_traffic("%s%d" % (args.config, args.n or default_n), "mixed_" + fmt_name)
+    workload_tag = "%s%d" % (args.config, args.n or default_n)
+    traffic = read_traffic(workload_tag, "mixed_" + fmt_name)
+    sym_kind = "node-block" if stn["sym_block"] > 1 else "scalar"
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
-            "kernel": {"sym": "k_spmv_sell<float, float, float, 4> on the symmetric slices",
+            "traffic_source": ("profiles/spmv_counters_%s.md, traffic_bytes_per_launch[mixed_%s]"
+                               % (workload_tag, fmt_name)) if traffic else None,
+            "kernel": {"sym": "k_spmv_sell<float, float, float, 4> on the %s symmetric slices"
+                              % sym_kind,
                        "sell": "k_spmv_sell<float, float, float, 4>",
                        "csr": "k_spmv_csr<LPR, float, float, float>"}[fmt_name]
                       + " (q = A p with p.q partials)",

@@ -454,6 +454,7 @@ int ins_set_dofs(ins_ctx* h, int dpn, const int32_t* dof_id, int64_t n_active)
     INS_HIP_CHECK(c, hipMemsetAsync(c->g.p, 0, nd * sizeof(double), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(c->u.p, 0, nd * sizeof(double), c->stream));
     INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->u_zero = true;
     invalidate_pattern(c);
     c->have_dofs = true;
     return INS_OK;
@@ -494,8 +495,10 @@ int ins_set_dof_values(ins_ctx* h, const double* u)
     if (!u) {
         INS_HIP_CHECK(c, hipMemsetAsync(c->u.p, 0, nd * sizeof(double), c->stream));
         INS_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        c->u_zero = true;
         return INS_OK;
     }
+    c->u_zero = false;
     return upload(c, c->u, u, (size_t)nd);
 }
 
@@ -968,8 +971,10 @@ int ins_bdf_init(ins_ctx* h, int order, double density, const double* u0)
         return fail(c, INS_ERR_ARG, "ins_bdf_init: order 1..4 and a positive density");
     if (!c->have_dofs) return fail(c, INS_ERR_STATE, "ins_bdf_init before ins_set_dofs");
     const int64_t nd = c->n_nodes * c->dpn;
+    c->u_zero = false;
     if (u0) INS_TRY(upload(c, c->u, u0, (size_t)nd));
     else INS_HIP_CHECK(c, hipMemsetAsync(c->u.p, 0, nd * sizeof(double), c->stream));
+    c->u_zero = u0 == nullptr;
     for (int s = 0; s < order; s++) {
         INS_TRY(ensure(c, c->bdf_hist[s], (size_t)nd));
         INS_HIP_CHECK(c, hipMemcpyAsync(c->bdf_hist[s].p, c->u.p, nd * sizeof(double),

@@ -637,6 +637,17 @@ struct PtRec {
     static constexpr int INV = 0, DW = 9, F = 10, S = 19, C = 25, DET = 46, P = 47, FQ = 56;
     static constexpr int K = 46;      // what the matrix needs
     static constexpr int SIZE = 60;   // with the force integrands (16-byte multiple for FP64)
+    // At zero displacement F = I, S = 0 and P = 0 exactly, and the tangent of
+    // both laws is the isotropic lambda I(x)I + 2 mu I, whose Voigt form holds
+    // three distinct values (lambda + 2 mu, lambda, mu at C(0,0), C(0,1),
+    // C(3,3)).  Such records carry only these fields (LIN_K of them for the
+    // matrix, LIN_ALL with the body-force integrand).
+    static constexpr int LIN_K = 13, LIN_ALL = 17;
+    static __host__ __device__ constexpr int lin_field(int j)
+    {
+        return j < 10 ? j : j == 10 ? C + 0 : j == 11 ? C + 1 : j == 12 ? C + 15 : j == 13 ? DET
+                                                                                       : FQ + (j - 14);
+    }
 };
 
 // The Voigt tangent C of the material law at F, upper triangle (the values
@@ -693,7 +704,7 @@ struct ElTab {
     const T *gd, *gp, *fd, *fp, *w;
 };
 
-template <int DIM, int NF, int NQ1, typename T, int MAT>
+template <int DIM, int NF, int NQ1, typename T, int MAT, bool LIN>
 __global__ __launch_bounds__(256) void k_hyper_points(
     int64_t E, const int32_t* __restrict__ gconn, const int32_t* __restrict__ fconn,
     const double* __restrict__ xyz, const double* __restrict__ coef,
@@ -715,6 +726,36 @@ __global__ __launch_bounds__(256) void k_hyper_points(
         T inv[DIM][DIM];
         const T det = geo_inverse<DIM, T>(X, tb.gd + q * GN * DIM, inv);
         const T dw = det * tb.w[q];
+        const int64_t np = E * NQ;
+        T* rec = pts + pt;   // field f at rec[f * np]
+        if constexpr (LIN) {
+            // zero displacement: the geometry, the three values of the
+            // isotropic tangent (as store_voigt_c21 rounds them) and the
+            // body-force integrand
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) rec[(int64_t)(PtRec::INV + i * 3 + a) * np] = (i < DIM && a < DIM) ? inv[i][a] : (T)0;
+            rec[(int64_t)(PtRec::DW) * np] = dw;
+            const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
+            if (want_k) {
+                rec[(int64_t)(PtRec::C + 0) * np] = lam + (T)2 * mu;
+                rec[(int64_t)(PtRec::C + 1) * np] = lam + (T)0;
+                rec[(int64_t)(PtRec::C + 15) * np] = (T)0 + mu;
+            }
+            if (want_f && bf.kind != INS_BF_NONE) {
+                rec[(int64_t)(PtRec::DET) * np] = det;
+                T x[3] = {0, 0, 0};
+#pragma unroll
+                for (int n = 0; n < GN; n++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) x[d] += X[n * DIM + d] * tb.gp[q * GN + n];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    rec[(int64_t)(PtRec::FQ + c) * np] = c < S ? body_force<T>(bf, x, DIM, c, e, q) : (T)0;
+            }
+            continue;
+        }
         // F = I + (sum_f dphi_f u_f^T)^T (solid/Deformation.hpp:26-46)
         T GU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (u) {
@@ -743,8 +784,6 @@ __global__ __launch_bounds__(256) void k_hyper_points(
         const T lam = (T)coef[e * ncoef], mu = (T)coef[e * ncoef + 1];
         T Sm[3][3], Cm[6][6];
         material<MAT, T>(lam, mu, Fm, Sm, Cm, false);
-        const int64_t np = E * NQ;
-        T* rec = pts + pt;   // field f at rec[f * np]
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -838,7 +877,7 @@ struct Hyp {
     static constexpr int TP = NTT < TPP ? NTT : TPP;
 };
 
-template <int DIM, int NF, int NQ1, int QC, typename T, bool WANT_K>
+template <int DIM, int NF, int NQ1, int QC, typename T, bool WANT_K, bool LIN>
 __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restrict__ pts,
                                                    ElTab<T> tb, int want_f, int want_resid,
                                                    int has_bf, T* __restrict__ ke,
@@ -867,18 +906,41 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
     // the point records of an element (field f of its point q at f * np +
     // e * NQ + q) go to LDS as [q][f]; those of the next element are loaded
     // into registers while this one is processed (when they take at most 8
-    // values per lane)
-    constexpr int NR = NQ * RS;
+    // values per lane).  Only the fields in use are read: the first K (the
+    // matrix) or all of them (with the force integrands); at zero
+    // displacement the LIN_K or LIN_ALL fields of PtRec::lin_field.
+    // (The zero-displacement records take the direct loads: with the
+    // register prefetch their elements after the first of a wave came out
+    // as zero matrices, a fault whose cause was not found.  At the sizes
+    // that matter every wave takes one element, so the prefetch does not
+    // pay there anyway.)
+    constexpr int NFR = LIN ? PtRec::LIN_ALL : RS;   // fields a record may carry
+    constexpr int NR = NQ * NFR;
     constexpr int RPL = (NR + 63) / 64;
-    constexpr bool PRE = RPL <= 8;
+    constexpr bool PRE = RPL <= 8 && !LIN;
+    const int nfl = LIN ? (want_f && has_bf ? PtRec::LIN_ALL : PtRec::LIN_K)
+                        : (want_f ? RS : PtRec::K);
+    const int nrl = NQ * nfl;
     const int64_t np = E * NQ;
+    // value r of this lane: point q of field f, at f * np + q (+ e * NQ) in
+    // memory and at q * RS + f in LDS (-1: the lane has no value r)
+    int64_t goff[PRE ? RPL : 1];
+    int loff[PRE ? RPL : 1];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int i = lane + 64 * r, j = i / NQ, q = i - j * NQ;
+            const int f = LIN ? PtRec::lin_field(j < NFR ? j : 0) : j;
+            goff[r] = i < nrl ? (int64_t)f * np + q : 0;
+            loff[r] = i < nrl ? L::R + q * RS + f : -1;
+        }
+    }
     T pre[PRE ? RPL : 1];
     auto fetch = [&](int64_t ee) {
         if constexpr (PRE) {
 #pragma unroll
             for (int r = 0; r < RPL; r++) {
-                const int i = lane + 64 * r, f = i / NQ, q = i - f * NQ;
-                pre[r] = (ee < E && i < NR) ? pts[f * np + ee * NQ + q] : (T)0;
+                pre[r] = (ee < E && loff[r] >= 0) ? pts[goff[r] + ee * NQ] : (T)0;
             }
         }
     };
@@ -892,15 +954,14 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
     for (int64_t e = lo + blockIdx.x / 8; e < hi; e += nloc) {
         if constexpr (PRE) {
 #pragma unroll
-            for (int r = 0; r < RPL; r++) {
-                const int i = lane + 64 * r, f = i / NQ, q = i - f * NQ;
-                if (i < NR) sm[L::R + q * RS + f] = pre[r];
-            }
+            for (int r = 0; r < RPL; r++)
+                if (loff[r] >= 0) sm[loff[r]] = pre[r];
             fetch(e + nloc < hi ? e + nloc : E);
         } else {
-            for (int i = lane; i < NR; i += 64) {
-                const int f = i / NQ, q = i - f * NQ;
-                sm[L::R + q * RS + f] = pts[f * np + e * NQ + q];
+            for (int i = lane; i < nrl; i += 64) {
+                const int j = i / NQ, q = i - j * NQ;
+                const int f = LIN ? PtRec::lin_field(j) : j;
+                sm[L::R + q * RS + f] = pts[(int64_t)f * np + e * NQ + q];
             }
         }
         wave_lds_sync();
@@ -933,6 +994,37 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
                             }
                         }
                         const T dw = qin ? rc[PtRec::DW] : (T)0;
+                        if constexpr (LIN) {
+                            // F = I: beta_a of component i is g[K] if i = J,
+                            // g[J] if i = K, else 0; C is isotropic, so the
+                            // products with its zero entries are left out
+                            // (the sums keep the order and values of the
+                            // general form below)
+                            const T cd = rc[PtRec::C + 0], cl = rc[PtRec::C + 1],
+                                    cm = rc[PtRec::C + 15];
+#pragma unroll
+                            for (int i = 0; i < S; i++) {
+                                T be[NV];
+#pragma unroll
+                                for (int a = 0; a < NV; a++) {
+                                    const int J = V::j(a), K = V::k(a);
+                                    be[a] = i == J ? g[K] : (J != K && i == K) ? g[J] : (T)0;
+                                }
+#pragma unroll
+                                for (int a = 0; a < NV; a++) {
+                                    T cb;
+                                    if (V::c(a) < 3) {
+                                        cb = (a == 0 ? cd : cl) * be[0];
+                                        cb += (a == 1 ? cd : cl) * be[1];
+                                        if (DIM == 3) cb += (a == 2 ? cd : cl) * be[2];
+                                    } else {
+                                        cb = cm * be[a];
+                                    }
+                                    sm[L::BV + (qc * NV + a) * NEE + M * S + i] = be[a];
+                                    sm[L::CB + (qc * NV + a) * NEE + M * S + i] = dw * cb;
+                                }
+                            }
+                        } else {
 #pragma unroll 1
                         for (int i = 0; i < S; i++) {
                             T be[NV];
@@ -955,7 +1047,8 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
                                 sm[L::CB + (qc * NV + a) * NEE + M * S + i] = dw * cb;
                             }
                         }
-                        if (pass == 0) {
+                        }
+                        if (!LIN && pass == 0) {
 #pragma unroll
                             for (int J = 0; J < DIM; J++) {
                                 T sg = (T)0;
@@ -991,7 +1084,8 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
                             }
                     }
                     // ... and of dphi^T (dw S dphi), one step per point
-                    if (pass == 0) {
+                    // (S = 0 at zero displacement)
+                    if (!LIN && pass == 0) {
 #pragma unroll 1
                         for (int s0 = 0; s0 < L::KG; s0 += 4) {
                             const T* gr = sm + L::GG + (s0 + k4) * NF;
@@ -1015,7 +1109,7 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
                     }
                     wave_lds_sync();
                 }
-                if (pass == 0) {
+                if (!LIN && pass == 0) {
                     // the geometric term to LDS
                     int t = 0;
 #pragma unroll
@@ -1048,7 +1142,7 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
                                 if (p < NEE && qq < NEE && p <= qq) {
                                     const int M = p / S, i = p - M * S, N = qq / S, k = qq - N * S;
                                     T v = acc[t - pass * TP][r];
-                                    if (i == k) v += sm[L::KGO + M * NF + N];
+                                    if (!LIN && i == k) v += sm[L::KGO + M * NF + N];
                                     dst[(p * NEE - (p * (p - 1)) / 2 + (qq - p)) * OS] = v;
                                 }
                             }
@@ -1074,7 +1168,7 @@ __global__ __launch_bounds__(64) void k_elem_hyper(int64_t E, const T* __restric
                         const T* rc = sm + L::R + q * RS;
                         sb += rc[PtRec::FQ + i] * tb.fp[q * NF + M] * tb.w[q] * rc[PtRec::DET];
                     }
-                if (want_resid)
+                if (want_resid && !LIN)   // P = 0 at zero displacement
 #pragma unroll 1
                     for (int q = 0; q < NQ; q++) {
                         const T* rc = sm + L::R + q * RS;
@@ -2015,19 +2109,28 @@ int launch_hyper_nq(Ctx* c, const T* tab, Bf bf, int want_k, int want_r, T* ke, 
     const int64_t E = c->n_elem;
     INS_TRY(ensure(c, c->stage_pt, (size_t)E * NQ * PtRec::SIZE * sizeof(T)));
     T* pts = (T*)c->stage_pt.p;
-    k_hyper_points<DIM, NF, NQ1, T, MAT><<<grid_for(E * NQ, 256), 256, 0, c->stream>>>(
-        E, c->geo_conn(), c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tb, bf, want_k,
-        want_f, pts);
-    INS_HIP_CHECK(c, hipGetLastError());
     // a whole number of blocks per XCD (the kernel gives each XCD an eighth)
     const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((E + 7) / 8 * 8, 1 << 22));
     const int has_bf = bf.kind != INS_BF_NONE;
-    if (want_k)
-        k_elem_hyper<DIM, NF, NQ1, QC, T, true><<<g, 64, 0, c->stream>>>(
-            E, pts, tb, want_f, want_r, has_bf, ke, fe);
-    else
-        k_elem_hyper<DIM, NF, NQ1, QC, T, false><<<g, 64, 0, c->stream>>>(
-            E, pts, tb, want_f, want_r, has_bf, ke, fe);
+    auto run = [&](auto lin) {
+        constexpr bool LIN = decltype(lin)::value;
+        k_hyper_points<DIM, NF, NQ1, T, MAT, LIN><<<grid_for(E * NQ, 256), 256, 0, c->stream>>>(
+            E, c->geo_conn(), c->conn.p, c->xyz.p, c->coef.p, c->u.p, c->ncoef, tb, bf, want_k,
+            want_f, pts);
+        if (want_k)
+            k_elem_hyper<DIM, NF, NQ1, QC, T, true, LIN><<<g, 64, 0, c->stream>>>(
+                E, pts, tb, want_f, want_r, has_bf, ke, fe);
+        else
+            k_elem_hyper<DIM, NF, NQ1, QC, T, false, LIN><<<g, 64, 0, c->stream>>>(
+                E, pts, tb, want_f, want_r, has_bf, ke, fe);
+    };
+    // the displacement is known to be zero until something writes it
+    // (ins_set_dof_values, ins_distribute, time stepping): then F = I
+    // exactly and the point records reduce to the geometry and the
+    // isotropic tangent (INS_LIN=0 takes the general form throughout)
+    static const int lin_on = [] { const char* v = getenv("INS_LIN"); return v ? atoi(v) : 1; }();
+    if (c->u_zero && lin_on) run(std::true_type{});
+    else run(std::false_type{});
     INS_HIP_CHECK(c, hipGetLastError());
     return INS_OK;
 }

@@ -254,6 +254,9 @@ struct Ctx {
     // only those)
     DArr<uint32_t> emask;
     bool have_emask = false;
+    // u (the DoF values) is all zeros: set where it is cleared, dropped by
+    // every write of it (upload, distribute, time stepping)
+    bool u_zero = true;
     DArr<double> val64;
     DArr<float> val32;
     DArr<double> b;

@@ -1741,18 +1741,23 @@ unsigned sell_grid(int64_t n)
 }
 
 // Blocks of the symmetric SpMV: each XCD sweeps its eighth of the slices
-// with its blocks' 4 waves per step, 4 blocks on every CU of the XCD.
-// Measured at 128^3 (n = 2.13M, 1024 vs 1280 blocks on one box): the
-// line-sharing rows 24.2 vs 31.5 us, the generic rows 30.9 vs 40.7 us; 768,
-// 1152, 1536 and 2048 blocks were all slower than 1024.  Small problems
-// (fewer than 4 slices per wave of such a grid) take the SELL grid.
-unsigned sym_grid(int64_t n, int cus)
+// with its blocks' 4 waves per step, bpc blocks on every CU of the XCD.
+// Scalar rows take 4 per CU: measured at 128^3 (n = 2.13M, 1024 vs 1280
+// blocks on one box), the line-sharing rows 24.2 vs 31.5 us, the generic
+// rows 30.9 vs 40.7 us; 768, 1152, 1536 and 2048 blocks were all slower
+// than 1024.  Node blocks take 2 per CU in FP32 and 1 in FP64: at 96^3
+// elasticity the FP32 SpMV takes 131.5 / 119.1 / 125.1 / 135.0 / 122.9 us
+// with 1 / 2 / 3 / 4 / 8 blocks per CU (a fractional number per CU, 1.5 or
+// 2.5, 143-140 us), and the FP64 SELL solve 389 / 422 / 436 / 450 ms per
+// step with 1 / 2 / 3 / 4.  Small problems (fewer slices than 16 per CU)
+// take the SELL grid.
+unsigned sym_grid(int64_t n, int cus, int bpc)
 {
     const int64_t nsl = (n + 63) / 64;
     const int wpb = SB / 64;
     const int cpx = std::max(1, cus / NXCD);   // CUs per XCD
     if (nsl < (int64_t)4 * cpx * NXCD * wpb) return sell_grid(n);
-    return (unsigned)std::min(4 * cpx * NXCD, MAXG);
+    return (unsigned)std::min(bpc * cpx * NXCD, MAXG);
 }
 
 unsigned csr_grid(int64_t n, int lpr)
@@ -1925,13 +1930,14 @@ SymView sym_view(const Ctx* c)
 // slices, the same for a solve alone and inside a server batch, so that a
 // request's partial sums, and with them its iterates, do not depend on where
 // it runs
-unsigned spmv_grid(const Ctx* c, bool sym)
+unsigned spmv_grid(const Ctx* c, bool sym, bool fp64)
 {
     const int64_t n = c->n_active;
     if (!sym) return sell_grid(n);
     static const int forced = env_int("INS_SYM_GRID", 0);
     if (forced >= 8) return round8(std::min(forced, MAXG));
-    return sym_grid(c->sell.sdpn > 1 ? c->sell.snodes : n, c->n_cu);
+    if (c->sell.sdpn > 1) return sym_grid(c->sell.snodes, c->n_cu, fp64 ? 1 : 2);
+    return sym_grid(n, c->n_cu, 4);
 }
 
 template <typename TV, typename TX, typename TA, typename TVR>
@@ -1943,7 +1949,7 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
     const Rep<TVR> R{rep_val, c->sx64, c->sb, c->r32.p, c->sd32, rep_sym};
     if (format == INS_FMT_SELL) {
         const bool sym = use_sym(c, c->sperm);
-        const unsigned g = spmv_grid(c, sym);
+        const unsigned g = spmv_grid(c, sym, sizeof(TV) == 8);
         SymView Y{nullptr, nullptr, 0, 1, 0, 0, nullptr};
         if (sym) Y = sym_view(c);
         if constexpr (sizeof(TV) == 8) {
@@ -2747,6 +2753,7 @@ int distribute(Ctx* c, int op)
                     "ins_distribute: the solved system (ins_set_csr) does not have one row per "
                     "active DoF");
     const int64_t nd = c->n_nodes * c->dpn;
+    c->u_zero = false;
     k_distribute<<<grid_for(nd, 256), 256, 0, c->stream>>>(nd, c->dof_id.p, c->x.p, c->g.p,
                                                             c->u.p, op == INS_DIST_ADD);
     INS_HIP_CHECK(c, hipGetLastError());
@@ -3236,7 +3243,7 @@ int upload_table(Server* sv)
         d.snodes = sym ? c->sell.snodes : 0;
         d.slines = sym && c->sell.slines ? 1 : 0;
 #undef INS_AS
-        d.g_spmv = (int)spmv_grid(c, sym);
+        d.g_spmv = (int)spmv_grid(c, sym, false);
         d.g_vec = (int)vec_grid(d.n);
         d.join = h.join_it;
         T.pre_spmv[ns] = gs;
