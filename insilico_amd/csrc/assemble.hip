@@ -1481,10 +1481,9 @@ __global__ __launch_bounds__(BLK) void k_sym_gather(
     double mfac, double* __restrict__ vs64, float* __restrict__ vs32, double* __restrict__ d64,
     float* __restrict__ d32)
 {
-    constexpr int T = D * (D + 1) / 2, D2 = D * D;
     constexpr int BITS = PairWord<NPE>::bits, FLAG = PairWord<NPE>::flag;
     static_assert(NPE <= 8, "32-bit words, at most 8 elements per node");
-    const int64_t nbv = T + (int64_t)(nu - 1) * D2;
+    const SymLayout L{D, nu};
     for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < N;
          a += (int64_t)gridDim.x * blockDim.x) {
     const int32_t A = anode[a];
@@ -1494,8 +1493,7 @@ __global__ __launch_bounds__(BLK) void k_sym_gather(
     int32_t xa[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) xa[i] = i < cnt ? adj[as + i] : 0;
-    double* __restrict__ o64 = vs64 ? vs64 + (a >> 6) * 64 * nbv + (a & 63) : nullptr;
-    float* __restrict__ o32 = vs32 + (a >> 6) * 64 * nbv + (a & 63);
+    const int64_t base = L.pos(a, 0);   // slot s at base + L.pos(0, s)
     const uint32_t* __restrict__ wp = spw + (a >> 6) * 64 * (int64_t)nu + (a & 63);
     const int npass = me.p ? 2 : 1;
     for (int k = 0; k < nu; k++) {
@@ -1525,23 +1523,23 @@ __global__ __launch_bounds__(BLK) void k_sym_gather(
             for (int i = 0; i < D; i++) {
 #pragma unroll
                 for (int j = i; j < D; j++) {
-                    const int64_t slot = i * D - i * (i - 1) / 2 + (j - i);
-                    if (o64) o64[slot * 64] = (double)v[i][j];
-                    o32[slot * 64] = (float)v[i][j];
+                    const int64_t o = base + L.pos(0, L.slot(0, i, j));
+                    if (vs64) vs64[o] = (double)v[i][j];
+                    vs32[o] = (float)v[i][j];
                 }
-                const double d = o64 ? (double)v[i][i] : (double)(float)v[i][i];
+                const double d = vs64 ? (double)v[i][i] : (double)(float)v[i][i];
                 const double id = d != 0. ? 1. / d : 1.;
                 d64[a * D + i] = id;
                 d32[a * D + i] = (float)id;
             }
         } else {
-            const int64_t s0 = T + (int64_t)(k - 1) * D2;
 #pragma unroll
             for (int i = 0; i < D; i++)
 #pragma unroll
                 for (int j = 0; j < D; j++) {
-                    if (o64) o64[(s0 + i * D + j) * 64] = (double)v[i][j];
-                    o32[(s0 + i * D + j) * 64] = (float)v[i][j];
+                    const int64_t o = base + L.pos(0, L.slot(k, i, j));
+                    if (vs64) vs64[o] = (double)v[i][j];
+                    vs32[o] = (float)v[i][j];
                 }
         }
     }

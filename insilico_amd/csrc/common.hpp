@@ -345,6 +345,29 @@ void release(Ctx* c, DArr<T>& a)
 }
 
 // grid helper
+// Positions in the symmetric slices (64 rows or nodes per slice), slot-major
+// and lane contiguous: slot s of row or node a at
+// (a/64)*64*width + s*64 + a%64.  Scalar rows (D = 1): slot k is the entry
+// of offset u_k.  Node blocks (D = 2, 3): slots 0..T-1 the upper triangle
+// of the diagonal block (row-major, i <= j), slot T + (k-1)D^2 + iD + j the
+// entry (i, j) of block k >= 1, T = D(D+1)/2.
+struct SymLayout {
+    int D, nu;
+    __host__ __device__ int64_t width() const
+    {
+        return (int64_t)D * (D + 1) / 2 + (int64_t)(nu - 1) * D * D;
+    }
+    __host__ __device__ int slot(int k, int i, int j) const
+    {
+        if (k == 0) return i * D - i * (i - 1) / 2 + (j - i);
+        return D * (D + 1) / 2 + (k - 1) * D * D + i * D + j;
+    }
+    __host__ __device__ int64_t pos(int64_t a, int s) const
+    {
+        return (a >> 6) * 64 * width() + (int64_t)s * 64 + (a & 63);
+    }
+};
+
 inline unsigned grid_for(int64_t work, int block, unsigned cap = 65535u * 8)
 {
     int64_t g = (work + block - 1) / block;

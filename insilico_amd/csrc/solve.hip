@@ -135,6 +135,7 @@ struct Sweep {
 // items are visited as first + j*stride (+ per-block offsets < step)
 __device__ int g_xcd_mode = 0;  // 1: per-XCD ranges for every kernel (INS_XCD)
 __device__ int g_nt = 0;        // 1: streaming (non-temporal) vector stores and value loads
+__device__ int g_bsym_bands = 4;   // banded slice order of the node-block SpMV (INS_BSYM_BANDS)
 // mixed: a replacement restarts the direction when |b - A x|^2 exceeds this
 // multiple of the recursive |r|^2 (INS_RESTART_RATIO)
 __device__ double g_restart_ratio = 64.;
@@ -448,17 +449,29 @@ __device__ inline TA csr_row(int64_t row, int sub, const int32_t* __restrict__ r
 
 // Symmetric node-block slices (vector problems, one lane per node of a
 // 64-node slice): out[i] = sum over the node's D x D blocks of the rows of
-// its D components.  Blocks (a, a + u_k), u_k >= 0, are stored at node a,
-// k-major and lane contiguous: slot 0 the upper triangle of the diagonal
-// block (i <= j, row-major), slot T + (k - 1) D^2 + i D + j the entry (i, j)
-// of block k >= 1.  Block (a, a - u_k) is block (a - u_k, a) transposed, read
-// at node a - u_k.  Vectors are component-major: component c of node b at
-// c * nn + b.  Offsets are taken G at a time so that G blocks of loads are
-// in flight per wave.
+// its D components.  Blocks (a, a + u_k), u_k >= 0, are stored at node a in
+// the slots of SymLayout, slot-major and lane contiguous.  Block
+// (a, a - u_k) is block (a - u_k, a) transposed, read at node a - u_k.
+// Vectors are component-major: component c of node b at c * nn + b.
+// Offsets are taken G at a time so that G blocks of loads are in flight per
+// wave.  (Slots grouped four by four for 16-byte loads, one access per
+// group and lane, were measured slower: 126.9 against 117.0 us per FP32
+// SpMV at 96^3 elasticity.)
 template <int D>
 __device__ inline constexpr int tri_slot(int i, int j)
 {
     return i * D - i * (i - 1) / 2 + (j - i);
+}
+
+// the D*D entries (row-major) of block k >= 1 of the node whose slot 0 is
+// at p (its lane included)
+template <int D, typename TV>
+__device__ inline void bsym_block(const TV* __restrict__ p, int k, TV (&b)[D * D])
+{
+    constexpr int T = D * (D + 1) / 2, D2 = D * D;
+    const int64_t s0 = T + (int64_t)(k - 1) * D2;
+#pragma unroll
+    for (int e = 0; e < D2; e++) b[e] = p[(s0 + e) * 64];
 }
 
 template <int D, typename TV, typename TX, typename TA, int G = 3>
@@ -469,16 +482,16 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
     constexpr int T = D * (D + 1) / 2, D2 = D * D;
     const int nu = Y.nu;
     const int64_t nn = Y.nn, last = nn - 1;
-    const int64_t ld = (int64_t)64 * (T + (nu - 1) * D2);
+    const int64_t ld = 64 * SymLayout{D, nu}.width();
 #pragma unroll
     for (int i = 0; i < D; i++) out[i] = (TA)0;
     if (!live) return;
     const TV* __restrict__ own = vals + slice * ld + lane;
     {
         TV bv[T];
-        TX xa[D];
 #pragma unroll
         for (int e = 0; e < T; e++) bv[e] = own[(int64_t)e * 64];
+        TX xa[D];
 #pragma unroll
         for (int j = 0; j < D; j++) xa[j] = x[j * nn + a];
 #pragma unroll
@@ -505,18 +518,15 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
             }
             if (k < nu) {
                 const int d = Y.u[k];
-                const int64_t slot = T + (int64_t)(k - 1) * D2;
                 int64_t bn = a + d;
                 bn = bn > last ? last : bn;   // padded blocks hold zeros
-#pragma unroll
-                for (int e = 0; e < D2; e++) bu[g][e] = own[(slot + e) * 64];
+                bsym_block<D, TV>(own, k, bu[g]);
 #pragma unroll
                 for (int j = 0; j < D; j++) xu[g][j] = x[j * nn + bn];
                 const int64_t m = a - d;
                 if (m >= 0) {
-                    const TV* __restrict__ mp = vals + (m >> 6) * ld + slot * 64 + (m & 63);
-#pragma unroll
-                    for (int e = 0; e < D2; e++) bl[g][e] = mp[(int64_t)e * 64];
+                    const TV* __restrict__ mp = vals + (m >> 6) * ld + (m & 63);
+                    bsym_block<D, TV>(mp, k, bl[g]);
 #pragma unroll
                     for (int j = 0; j < D; j++) xl[g][j] = x[j * nn + m];
                 }
@@ -536,6 +546,92 @@ __device__ inline void bsym_node(int64_t a, bool live, int lane, int64_t slice, 
     }
 }
 
+// Node blocks whose node offsets are stencil lines (u = {0, 1, c1 - 1, c1,
+// c1 + 1, ...}, NU = 14 in 3-D, 5 in 2-D), FP32: as sym_lines_sweep does for
+// scalar rows, a wave loads each line of p of each component once (64
+// consecutive nodes) and takes the neighbouring nodes by wave shifts, 3D
+// instead of D * (2 NU - 1) vector loads of p per node.  The blocks of a
+// line (u = c - 1, c, c + 1) are taken together.  Only for slices whose
+// lines stay inside the nodes (the caller checks).
+template <int D, int NU>
+__device__ inline void bsym_lines_node(int64_t a, int lane, int64_t slice, const SymView& Y,
+                                       const float* __restrict__ vals,
+                                       const float* __restrict__ p, const int (&du)[NU],
+                                       float (&out)[D])
+{
+    constexpr int T = D * (D + 1) / 2, D2 = D * D;
+    const int64_t nn = Y.nn;
+    const int64_t ld = 64 * SymLayout{D, NU}.width();
+    const float* __restrict__ own = vals + slice * ld + lane;
+#pragma unroll
+    for (int i = 0; i < D; i++) out[i] = 0.f;
+    // line 0: the diagonal block and the block of u = 1 (and its mirror)
+    {
+        float bv[T], bu[D2], bl[D2], x0[D], xu[D], xl[D];
+#pragma unroll
+        for (int e = 0; e < T; e++) bv[e] = own[(int64_t)e * 64];
+        bsym_block<D, float>(own, 1, bu);
+        const int64_t m = a - 1;
+        bsym_block<D, float>(vals + (m >> 6) * ld + (m & 63), 1, bl);
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            const float* __restrict__ pb = p + j * nn + slice * 64;
+            const float c0 = pb[lane];
+            x0[j] = c0;
+            xu[j] = wave_shl1(c0, pb[64]);
+            xl[j] = wave_shr1(c0, pb[-1]);
+        }
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) out[i] += bv[i <= j ? tri_slot<D>(i, j) : tri_slot<D>(j, i)] * x0[j];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) out[i] += bu[i * D + j] * xu[j];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) out[i] += bl[j * D + i] * xl[j];
+    }
+    // lines m: blocks k = 2 + 3m, 3 + 3m, 4 + 3m (u = c - 1, c, c + 1)
+#pragma unroll 1
+    for (int mline = 0; mline < (NU - 2) / 3; mline++) {
+        const int k0 = 2 + 3 * mline;
+        const int c = du[k0 + 1];
+        float bu[3][D2], bl[3][D2], xu[3][D], xl[3][D];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            bsym_block<D, float>(own, k0 + t, bu[t]);
+            const int64_t m = a - du[k0 + t];
+            bsym_block<D, float>(vals + (m >> 6) * ld + (m & 63), k0 + t, bl[t]);
+        }
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            const float* __restrict__ pb = p + j * nn + slice * 64;
+            const float au = pb[c + lane];
+            xu[0][j] = wave_shr1(au, pb[c - 1]);
+            xu[1][j] = au;
+            xu[2][j] = wave_shl1(au, pb[c + 64]);
+            const float al = pb[lane - c];
+            xl[0][j] = wave_shl1(al, pb[64 - c]);
+            xl[1][j] = al;
+            xl[2][j] = wave_shr1(al, pb[-c - 1]);
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) out[i] += bu[t][i * D + j] * xu[t][j];
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) out[i] += bl[t][j * D + i] * xl[t][j];
+        }
+    }
+}
+
 // q = A p on the node blocks: every lane writes the D rows of its node and
 // adds their p.q to the block partial
 template <int D, typename TV, typename TX, typename TA>
@@ -547,12 +643,38 @@ __device__ inline double bsym_sweep(Blk b, const SymView& Y, const TX* __restric
     const int lane = threadIdx.x & 63;
     const int64_t nn = Y.nn;
     const int64_t nsl = (nn + 63) / 64;
-    const Sweep w = xcd_sweep(nsl, SB / 64, b, true);
     double pq = 0.;
-    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
+    // stencil lines (FP32): the line form for the slices whose lines stay
+    // inside the nodes
+    constexpr int NUL = D == 3 ? 14 : 5;
+    const bool lines = sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TA) == 4 && Y.lines &&
+                       Y.nu == NUL;
+    int du[NUL];
+    int64_t s_lo = 1, s_hi = -1;
+    if (lines) {
+#pragma unroll
+        for (int k = 0; k < NUL; k++) du[k] = __builtin_amdgcn_readfirstlane(Y.u[k]);
+        s_lo = 1 + (du[NUL - 1] >> 6);
+        s_hi = (nn - 1 - du[NUL - 1] - 64) / 64;
+    }
+    auto visit = [&](int64_t slice) {
         const int64_t a = slice * 64 + lane;
         const bool live = a < nn;
         TA out[D];
+        if constexpr (sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TA) == 4) {
+            if (lines && slice >= s_lo && slice <= s_hi) {
+                float o[D];
+                bsym_lines_node<D, NUL>(a, lane, slice, Y, (const float*)vals, (const float*)p,
+                                        du, o);
+#pragma unroll
+                for (int i = 0; i < D; i++) {
+                    const float qs = o[i];
+                    st(&q[i * nn + a], qs, g_nt);
+                    pq += (double)p[i * nn + a] * (double)qs;
+                }
+                return;
+            }
+        }
         bsym_node<D, TV, TX, TA, G>(live ? a : 0, live, lane, slice, Y, vals, p, out);
         if (live) {
 #pragma unroll
@@ -562,7 +684,32 @@ __device__ inline double bsym_sweep(Blk b, const SymView& Y, const TX* __restric
                 pq += (double)p[i * nn + a] * (double)qs;
             }
         }
+    };
+    const int bands = g_bsym_bands;
+    if (bands > 0 && b.n % NXCD == 0) {
+        // banded order: the XCD's slices as rows of P (the largest node
+        // offset in slices, about one node plane), taken band by band of
+        // P / bands columns, row after row, so that a slice and the slices
+        // one plane back holding its mirrors are visited about P / bands
+        // slices apart instead of P.  At 96^3 elasticity with 0 / 2 / 4 / 8
+        // / 16 bands: FP32 SpMV 118.9 / 117.8 / 117.1 / 120.2 / 117.3 us,
+        // FP64 SELL solve 397 / 379 / 372 / 375 / 368 ms per step
+        const int step = SB / 64, xcd = b.id % NXCD, local = b.id / NXCD, nloc = b.n / NXCD;
+        int per = (int)((nsl + NXCD - 1) / NXCD);
+        per = (per + step - 1) / step * step;
+        const int lo = xcd * per, hi = lo + per < (int)nsl ? lo + per : (int)nsl;
+        const int P = Y.u[Y.nu - 1] / 64 > 1 ? Y.u[Y.nu - 1] / 64 : 1;
+        const int R = hi > lo ? (hi - lo + P - 1) / P : 0;
+        const int Wb = (P + bands - 1) / bands, RW = R * Wb;
+        for (int v = local * step + wave_in_block(); v < bands * RW; v += nloc * step) {
+            const int h = v / RW, t = v - h * RW, r = t / Wb, c = h * Wb + (t - r * Wb);
+            const int slice = lo + r * P + c;
+            if (c < P && slice < hi) visit(slice);
+        }
+        return pq;
     }
+    const Sweep w = xcd_sweep(nsl, SB / 64, b, true);
+    for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) visit(slice);
     return pq;
 }
 
@@ -1595,9 +1742,9 @@ __global__ void k_anode(int64_t n_nodes, int D, const int32_t* __restrict__ dof_
 // the diagonal, read at its mirror); rows interleaved, r = a * D + i
 __global__ void k_bsym_map(int64_t n, int D, const int32_t* __restrict__ rp,
                            const int32_t* __restrict__ col, const int32_t* __restrict__ su, int nu,
-                           int64_t nbv, int32_t* __restrict__ smap, int* err)
+                           int32_t* __restrict__ smap, int* err)
 {
-    const int T = D * (D + 1) / 2;
+    const SymLayout L{D, nu};
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += (int64_t)gridDim.x * blockDim.x) {
         const int64_t a = r / D;
@@ -1625,9 +1772,7 @@ __global__ void k_bsym_map(int64_t n, int D, const int32_t* __restrict__ rp,
                 atomicOr(err, 1);
                 continue;
             }
-            const int64_t slot = k == 0 ? (int64_t)(i * D - i * (i - 1) / 2 + (j - i))
-                                        : (int64_t)T + (int64_t)(k - 1) * D * D + i * D + j;
-            smap[e] = (int32_t)((a >> 6) * 64 * nbv + slot * 64 + (a & 63));
+            smap[e] = (int32_t)L.pos(a, L.slot(k, i, j));
         }
     }
 }
@@ -1656,8 +1801,7 @@ __global__ void k_sym_to_csr(int64_t n, const int32_t* __restrict__ rp,
                              const float* __restrict__ vs, double* __restrict__ v64,
                              float* __restrict__ v32)
 {
-    constexpr int T = D * (D + 1) / 2;
-    const int64_t nbv = (int64_t)T + (int64_t)(nu - 1) * D * D;
+    const SymLayout L{D, nu};
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += (int64_t)gridDim.x * blockDim.x) {
         const int64_t a = r / D;
@@ -1679,9 +1823,7 @@ __global__ void k_sym_to_csr(int64_t n, const int32_t* __restrict__ rp,
             }
             const int64_t node = up ? a : bn;
             const int ii = up ? i : j, jj = up ? j : i;
-            const int64_t slot = k == 0 ? (int64_t)(ii * D - ii * (ii - 1) / 2 + (jj - ii))
-                                        : (int64_t)T + (int64_t)(k - 1) * D * D + ii * D + jj;
-            const int64_t pos = (node >> 6) * 64 * nbv + slot * 64 + (node & 63);
+            const int64_t pos = L.pos(node, L.slot(k, ii, jj));
             if (vs64) v64[e] = vs64[pos];
             v32[e] = vs64 ? (float)vs64[pos] : vs[pos];
         }
@@ -1811,6 +1953,8 @@ int ensure_solver_state(Ctx* c, int mode)
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_mode), &mode, sizeof(int)));
         const int nt = env_int("INS_NT", 0);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_nt), &nt, sizeof(int)));
+        const int bands = env_int("INS_BSYM_BANDS", 4);
+        INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_bsym_bands), &bands, sizeof(int)));
         const double ratio = env_double("INS_RESTART_RATIO", 64.);
         INS_HIP_CHECK(c, hipMemcpyToSymbol(HIP_SYMBOL(g_restart_ratio), &ratio, sizeof(double)));
     }
@@ -2212,8 +2356,7 @@ int build_bsym(Ctx* c)
         if (!std::binary_search(up.begin(), up.end(), d)) return INS_OK;
     const int nu = (int)up.size();
     if (nu == 0 || nu > SYM_MAXU || up[0] != 0) return INS_OK;
-    const int64_t nbv = (int64_t)D * (D + 1) / 2 + (int64_t)(nu - 1) * D * D;
-    const int64_t nvs = (N + 63) / 64 * 64 * nbv;
+    const int64_t nvs = (N + 63) / 64 * 64 * SymLayout{D, nu}.width();
     const int64_t upper = (c->nnz + c->n_active) / 2;
     if (nvs > upper + upper / 8 + 64 * 64 || nvs >= INT32_MAX) return INS_OK;
     INS_TRY(ensure(c, S.su, nu));
@@ -2224,7 +2367,7 @@ int build_bsym(Ctx* c)
     INS_HIP_CHECK(c, hipMallocAsync((void**)&err, sizeof(int), c->stream));
     INS_HIP_CHECK(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
     k_bsym_map<<<grid_for(n, 256), 256, 0, c->stream>>>(n, D, c->row_ptr.p, c->col.p, S.su.p, nu,
-                                                       nbv, S.smap.p, err);
+                                                       S.smap.p, err);
     INS_HIP_CHECK(c, hipGetLastError());
     int herr = 0;
     INS_HIP_CHECK(c, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -2235,6 +2378,15 @@ int build_bsym(Ctx* c)
     S.nvs = nvs;
     S.sdpn = D;
     S.snodes = N;
+    // node offsets on stencil lines (as for scalar rows): {0, 1, c1 - 1, c1,
+    // c1 + 1, ...}, 14 of them in 3-D and 5 in 2-D
+    S.slines = nu == (D == 3 ? 14 : 5) && up[0] == 0 && up[1] == 1;
+    for (int m = 0; S.slines && m < (nu - 2) / 3; m++) {
+        const int32_t c0 = up[3 + 3 * m];
+        S.slines = up[2 + 3 * m] == c0 - 1 && up[4 + 3 * m] == c0 + 1;
+    }
+    static const int lines_on = env_int("INS_SYM_LINES", 1);
+    S.slines = S.slines && lines_on;
     S.sym_on = true;
     return build_sym_words(c, N, D);
 }

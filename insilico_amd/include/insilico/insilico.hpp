@@ -1037,7 +1037,16 @@ private:
             meshSent_ = true;
         }
         insilico::check(ins_set_prescribed(ctx_, f.prescribed.data()), ctx_);
-        insilico::check(ins_set_dof_values(ctx_, f.value.data()), ctx_);
+        // all-zero DoF values go over as NULL: the library then knows the
+        // displacement is zero and assembles the hyperelastic terms in their
+        // zero-displacement form (the same element matrices)
+        bool zero = true;
+        for (double v : f.value)
+            if (v != 0.) {
+                zero = false;
+                break;
+            }
+        insilico::check(ins_set_dof_values(ctx_, zero ? nullptr : f.value.data()), ctx_);
         insilico::check(ins_set_material(ctx_, kernel_, nq_, ncoef_, coef_.data()), ctx_);
         insilico::check(ins_set_body_force(ctx_, bfKind_, bf_.data()), ctx_);
         if (!bfTable_.empty()) insilico::check(ins_set_body_force_values(ctx_, bfTable_.data()), ctx_);
