@@ -113,6 +113,19 @@ __device__ inline void totals(const double* part, int cnt, const int (&slot)[NV]
 
 __device__ inline bool leader(Blk b = grid_blk()) { return b.id == 0 && threadIdx.x == 0; }
 
+// this thread's share of totals<NV> (the loads only; block_allsum finishes
+// it), so that a kernel can issue them together with its other loads
+template <int NV>
+__device__ inline void totals_share(const double* part, int cnt, const int (&slot)[NV],
+                                    double (&t)[NV])
+{
+#pragma unroll
+    for (int k = 0; k < NV; k++) t[k] = 0.;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++) t[k] += part[slot[k] * MAXG + i];
+}
+
 // index of the calling wavefront within its block, as a wave-uniform value:
 // slice indices derived from it stay in scalar registers, so the per-slice
 // offset, pattern and column-offset reads go through the scalar cache rather
@@ -768,14 +781,16 @@ __device__ inline void resid_rows(Blk b, int64_t n, const SellView& S, const TVR
       }
     }
     if constexpr (RS != 4) {
-        const Sweep w = xcd_sweep((n + 63) / 64, SB / 64, b);
+        // the symmetric slices sweep per-XCD ranges, as their SpMV does (a
+        // mirror read then finds its slice in the same XCD's L2)
+        const Sweep w = xcd_sweep((n + 63) / 64, SB / 64, b, RS != 0 && sym);
         for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
             const int64_t rho = slice * 64 + lane;
             const bool live = rho < n;
             const int64_t row = live ? (S.perm ? (int64_t)S.perm[rho] : rho) : 0;
             double s;
             if (RS != 0 && sym)
-                s = sym_row_v<TVR, double, double, 2>(row, live, n, lane, slice, val, Y.u, Y.nu,
+                s = sym_row_v<TVR, double, double, 8>(row, live, n, lane, slice, val, Y.u, Y.nu,
                                                       x);
             else
                 s = sell_row<TVR, double, double, 2>(row, live, n, lane, slice, S, val, x);
@@ -940,7 +955,17 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
                                       const TD* __restrict__ dinv, double* part, int g_pq,
                                       Scal* sc, int it)
 {
-    if (it == sc->rep_at) {
+    // everything this kernel waits on before its sweep -- the scalars of
+    // the last iterations and this thread's share of K1's p.q partials --
+    // is loaded at once (one memory latency instead of three in a row)
+    const int rep_at = sc->rep_at, done = sc->done, stop = sc->stop;
+    const double rho_it = sc->rho[it & 1];
+    double tpq[1];
+    {
+        const int slq[1] = {P_PQ};
+        totals_share<1>(part, g_pq, slq, tpq);
+    }
+    if (it == rep_at) {
         // replacement iteration: accept or continue from the true residual
         // of K1 (one block; the others have nothing to do)
         if (b.id != 0) return;
@@ -969,17 +994,14 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
         }
         return;
     }
-    if (sc->done) return;
-    if (sc->stop) {
+    if (done) return;
+    if (stop) {
         // converged in the last K3: done from here on
         if (leader(b)) sc->done = 1;
         return;
     }
-    const int par = it & 1;
-    double t[1];
-    const int slq[1] = {P_PQ};
-    totals<1>(part, g_pq, slq, t);
-    const double pq = t[0];
+    block_allsum<1>(tpq);
+    const double pq = tpq[0];
     if (!(pq > 0.) || !isfinite(pq)) {
         if (leader(b)) {
             sc->pq = pq;
@@ -988,7 +1010,7 @@ __device__ inline void update_xr_body(Blk b, int64_t n, TX* __restrict__ x, TX* 
         }
         return;
     }
-    const double ad = sc->rho[par] / pq;
+    const double ad = rho_it / pq;
     if (leader(b)) {
         sc->pq = pq;
         sc->alpha = ad;
@@ -1059,23 +1081,29 @@ __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
     VT* __restrict__ pv = reinterpret_cast<VT*>(p);
     const VT* __restrict__ rv = reinterpret_cast<const VT*>(r);
     const VT* __restrict__ dv = reinterpret_cast<const VT*>(dinv);
-    if (it == sc->rep_at) {
-        if (!sc->restart || sc->done) return;
+    // the scalars and this thread's share of K2's partials, loaded at once
+    const int rep_at = sc->rep_at, restart = sc->restart, done = sc->done;
+    const int par = it & 1;
+    const double rho = sc->rho[par], thr = sc->thr, delta2 = sc->delta2, rref = sc->rref;
+    const double alpha_d = sc->alpha;
+    double t[2];
+    {
+        const int sl[2] = {P_RR, P_RZ};
+        totals_share<2>(part, g, sl, t);
+    }
+    if (it == rep_at) {
+        if (!restart || done) return;
         for (int64_t i = w.first + threadIdx.x; i < w.end; i += w.stride) pv[i] = dv[i] * rv[i];
         for (int64_t i = nv * W + (int64_t)b.id * VB + threadIdx.x; i < n; i += (int64_t)b.n * VB)
             p[i] = (TX)dinv[i] * r[i];
         return;
     }
-    if (sc->done) return;
-    const int par = it & 1;
-    double t[2];
-    const int sl[2] = {P_RR, P_RZ};
-    totals<2>(part, g, sl, t);
+    if (done) return;
+    block_allsum<2>(t);
     const double rr = t[0], rz = t[1];
     const bool bad = !isfinite(rr);
-    const bool conv = rr < sc->thr;
-    const bool pause = mixed && !bad && !conv && rr < sc->delta2 * sc->rref;
-    const double rho = sc->rho[par];
+    const bool conv = rr < thr;
+    const bool pause = mixed && !bad && !conv && rr < delta2 * rref;
     if (leader(b)) {
         sc->rr = rr;
         if (bad) {
@@ -1096,7 +1124,7 @@ __device__ inline void update_p_body(Blk b, int64_t n, TX* __restrict__ p,
     // x += alpha p with the direction of this iteration's SpMV (alpha from
     // K2), done here where p is read anyway; then fold x32 into x64 before
     // a replacement, and p = D^-1 r + beta p
-    const TX alpha = (TX)sc->alpha;
+    const TX alpha = (TX)alpha_d;
     if (mixed && (conv || pause)) {
         // every item is read and written by one thread in one visit (the p
         // update of another thread's item must not precede its x update)
