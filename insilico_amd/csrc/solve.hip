@@ -346,20 +346,111 @@ __device__ inline float wave_shl1(float v, float edge)
                                                                  0xf, 0xf, false));
 }
 
+__device__ inline double wave_shr1(double v, double edge)
+{
+    const uint64_t a = __builtin_bit_cast(uint64_t, v), e = __builtin_bit_cast(uint64_t, edge);
+    const uint32_t lo = __builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x138,
+                                                    0xf, 0xf, false);
+    const uint32_t hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32),
+                                                    (int)(uint32_t)(a >> 32), 0x138, 0xf, 0xf,
+                                                    false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ inline double wave_shl1(double v, double edge)
+{
+    const uint64_t a = __builtin_bit_cast(uint64_t, v), e = __builtin_bit_cast(uint64_t, edge);
+    const uint32_t lo = __builtin_amdgcn_update_dpp((int)(uint32_t)e, (int)(uint32_t)a, 0x130,
+                                                    0xf, 0xf, false);
+    const uint32_t hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(e >> 32),
+                                                    (int)(uint32_t)(a >> 32), 0x130, 0xf, 0xf,
+                                                    false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Symmetric slices whose offsets are stencil lines: u = {0, 1, c1 - 1, c1,
 // c1 + 1, c2 - 1, ...} (NU = 14: the 27-point rows of a 3-D grid; NU = 5:
-// the 9-point rows of a 2-D grid), so that every row reads p on the lines
-// c - 1, c, c + 1 around the centres 0, +-c_m.  A wave loads each line of p
+// the 9-point rows of a 2-D grid), so that every row reads x on the lines
+// c - 1, c, c + 1 around the centres 0, +-c_m.  A wave loads each line of x
 // once (64 consecutive values, coalesced) and takes its two neighbours by
 // wave shifts, with the two values past the wave's ends as edge values: 9
-// p loads per row instead of 27.  Every address is a wave-uniform base plus
+// x loads per row instead of 27.  Every address is a wave-uniform base plus
 // a lane offset fixed for the whole sweep (the mirror offsets are computed
-// once).  The slices whose lines reach past row 0 or row n - 1 take
-// sym_row; the sums run in sym_row's order, so the results are the same
-// bits.
-template <int NU>
-__device__ inline double sym_lines_sweep(Blk b, int64_t n, const SymView& Y,
-                                         const float* __restrict__ p, float* __restrict__ q)
+// once).
+//
+// One row of an interior slice (own: the slice's values, mb: the values of
+// the slice `back` slices earlier, pb: x at the slice's first row).  The
+// offsets are loaded LG at a time (LG a multiple of 8: 16 keeps all FP32
+// loads of a row in flight, 8 holds FP64 rows to the registers of the
+// generic row) and summed eight at a time in sym_row_v's order, so the row
+// sums are the same bits.
+template <int NU, int LG, typename TV, typename TX, typename TA>
+__device__ inline TA sym_lines_row(int lane, const TV* __restrict__ own, const TV* __restrict__ mb,
+                                   const uint32_t (&mo)[NU], const int (&du)[NU],
+                                   const TX* __restrict__ pb)
+{
+    static_assert(LG % 8 == 0, "loads in whole groups of the sums");
+    // every later group starts a line (k0 = LG = 2 + 3m), so a line's
+    // offsets lie in one group
+    static_assert(LG >= NU || (LG % 3 == 2 && NU <= 2 * LG), "a line within one load group");
+    TA s = (TA)0;
+#pragma unroll
+    for (int k0 = 0; k0 < NU; k0 += LG) {
+        TV vu[LG], vl[LG];
+        TX xu[LG], xl[LG];
+#pragma unroll
+        for (int j = 0; j < LG; j++) {
+            const int k = k0 + j;
+            vu[j] = (TV)0;
+            vl[j] = (TV)0;
+            xu[j] = (TX)0;
+            xl[j] = (TX)0;
+            if (k < NU) vu[j] = own[k * 64 + lane];
+            if (k > 0 && k < NU) vl[j] = mb[mo[k]];
+        }
+        // x line by line: line 0 (offsets 0, 1), line m (c - 1, c, c + 1
+        // at k = 2 + 3m, 3 + 3m, 4 + 3m)
+#pragma unroll
+        for (int j = 0; j < LG; j++) {
+            const int k = k0 + j;
+            if (k == 0 && j + 1 < LG) {
+                const TX a = pb[lane];
+                xu[j] = a;
+                xu[j + 1] = wave_shl1(a, pb[64]);
+                xl[j + 1] = wave_shr1(a, pb[-1]);
+            } else if (k >= 2 && k < NU && (k - 2) % 3 == 0 && j + 2 < LG) {
+                const int c = du[k + 1];
+                const TX au = pb[c + lane];
+                xu[j] = wave_shr1(au, pb[c - 1]);
+                xu[j + 1] = au;
+                xu[j + 2] = wave_shl1(au, pb[c + 64]);
+                const TX al = pb[lane - c];
+                xl[j] = wave_shl1(al, pb[64 - c]);
+                xl[j + 1] = al;
+                xl[j + 2] = wave_shr1(al, pb[-c - 1]);
+            }
+        }
+#pragma unroll
+        for (int g0 = 0; g0 < LG; g0 += 8) {
+#pragma unroll
+            for (int j = g0; j < g0 + 8; j++)
+                if (k0 + j < NU) s += (TA)vu[j] * (TA)xu[j];
+#pragma unroll
+            for (int j = g0; j < g0 + 8; j++)
+                if (k0 + j < NU) s += (TA)vl[j] * (TA)xl[j];
+        }
+        // the next group's loads start after these sums (registers of one
+        // group live at a time)
+        if (k0 + LG < NU) __builtin_amdgcn_sched_barrier(0);
+    }
+    return s;
+}
+
+// Every row of the slices of one block (per-XCD ranges) with its sum
+// (A x)_row: f(row, s) for the rows below n.  The slices whose lines reach
+// past row 0 or row n - 1 take sym_row_v.
+template <int NU, int LG, typename TV, typename TX, typename TA, class F>
+__device__ inline void sym_lines_visit(Blk b, int64_t n, const SymView& Y,
+                                       const TV* __restrict__ vals, const TX* __restrict__ x, F f)
 {
     const int lane = threadIdx.x & 63;
     const int64_t nsl = (n + 63) / 64;
@@ -377,58 +468,33 @@ __device__ inline double sym_lines_sweep(Blk b, int64_t n, const SymView& Y,
         mo[k] = (uint32_t)((int64_t)(back + (t >> 6)) * ld + k * 64 + (t & 63));
     }
     const int64_t s_lo = back, s_hi = (n - 1 - dmax - 64) / 64;
-    double pq = 0.;
     for (int64_t slice = w.first + wave_in_block(); slice < w.end; slice += w.stride) {
         const int64_t row = slice * 64 + lane;
         const bool live = row < n;
-        float s;
-        if (slice < s_lo || slice > s_hi) {
-            s = sym_row<float, float, 8>(live ? row : 0, live, n, lane, slice, Y, p);
-        } else {
-            const float* __restrict__ own = Y.v + slice * ld;
-            const float* __restrict__ mb = Y.v + (slice - back) * ld;
-            const float* __restrict__ pb = p + slice * 64;
-            float vu[NU], vl[NU], xu[NU], xl[NU];
-#pragma unroll
-            for (int k = 0; k < NU; k++) {
-                vu[k] = own[k * 64 + lane];
-                vl[k] = k > 0 ? mb[mo[k]] : 0.f;
-            }
-            {
-                const float a = pb[lane];
-                xu[0] = a;
-                xu[1] = wave_shl1(a, pb[64]);
-                xl[0] = 0.f;
-                xl[1] = wave_shr1(a, pb[-1]);
-            }
-#pragma unroll
-            for (int m = 0; m < (NU - 2) / 3; m++) {
-                const int c = du[3 + 3 * m];
-                const float au = pb[c + lane];
-                xu[2 + 3 * m] = wave_shr1(au, pb[c - 1]);
-                xu[3 + 3 * m] = au;
-                xu[4 + 3 * m] = wave_shl1(au, pb[c + 64]);
-                const float al = pb[lane - c];
-                xl[2 + 3 * m] = wave_shl1(al, pb[64 - c]);
-                xl[3 + 3 * m] = al;
-                xl[4 + 3 * m] = wave_shr1(al, pb[-c - 1]);
-            }
-            s = 0.f;
-#pragma unroll
-            for (int k0 = 0; k0 < NU; k0 += 8) {
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (k0 + j < NU) s += vu[k0 + j] * xu[k0 + j];
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    if (k0 + j < NU) s += vl[k0 + j] * xl[k0 + j];
-            }
-        }
-        if (live) {
-            st(&q[row], s, g_nt);
-            pq += (double)p[row] * (double)s;
-        }
+        TA s;
+        if (slice < s_lo || slice > s_hi)
+            s = sym_row_v<TV, TX, TA, 8>(live ? row : 0, live, n, lane, slice, vals, Y.u, Y.nu, x);
+        else
+            s = sym_lines_row<NU, LG, TV, TX, TA>(lane, vals + slice * ld,
+                                                  vals + (slice - back) * ld, mo, du,
+                                                  x + slice * 64);
+        if (live) f(row, s);
     }
+}
+
+// q = A p over the stencil-line slices; returns this thread's share of p.q
+template <int NU, typename TV, typename TX, typename TA>
+__device__ inline double sym_lines_sweep(Blk b, int64_t n, const SymView& Y,
+                                         const TV* __restrict__ vals, const TX* __restrict__ p,
+                                         TX* __restrict__ q)
+{
+    double pq = 0.;
+    sym_lines_visit<NU, sizeof(TV) == 4 ? 16 : 8, TV, TX, TA>(
+        b, n, Y, vals, p, [&](int64_t row, TA s) {
+            const TX qs = (TX)s;
+            st(&q[row], qs, g_nt);
+            pq += (double)p[row] * (double)qs;
+        });
     return pq;
 }
 
@@ -808,7 +874,7 @@ __device__ inline void resid_rows(Blk b, int64_t n, const SellView& S, const TVR
 // kernel).  Separate instantiations keep each row form's registers and
 // schedule out of the others' kernels (all share the replacement path).
 // FP64 sweeps (TV = double) of problems with symmetric slices read their
-// FP64 copy (SymView::v64) with the generic row forms.
+// FP64 copy (SymView::v64), in the same row forms.
 template <typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM = 0>
 __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
                                       const TV* __restrict__ val, const TX* __restrict__ p,
@@ -820,6 +886,8 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
     if (it == sc->rep_at) {
         // FP64 true residual, round-robin placement
         double tt = 0., rz = 0.;
+        // (the line form in FP64 holds about 150 registers, which would cost
+        // the FP32 sweep of this kernel its occupancy: generic rows here)
         resid_rows<TVR, SYM == 3 ? 1 : SYM>(b, n, S, R.val, R.sym, Y, R.x64, R.b,
                                             [&](int64_t row, double ti) {
             tt += ti * ti;
@@ -842,11 +910,12 @@ __device__ inline void spmv_sell_body(Blk b, int64_t n, const SellView& S,
             return;
         }
     }
-    if constexpr (sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TA) == 4 &&
-                  (SYM == 3 || SYM == 2)) {
+    if constexpr (SYM == 3 ||
+                  (SYM == 2 && sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TA) == 4)) {
         if (SYM == 3 || (Y.v && Y.lines)) {
-            double v[1] = {Y.nu == 14 ? sym_lines_sweep<14>(b, n, Y, p, q)
-                                      : sym_lines_sweep<5>(b, n, Y, p, q)};
+            const TV* __restrict__ sv = sym_vals<TV>(Y);
+            double v[1] = {Y.nu == 14 ? sym_lines_sweep<14, TV, TX, TA>(b, n, Y, sv, p, q)
+                                      : sym_lines_sweep<5, TV, TX, TA>(b, n, Y, sv, p, q)};
             const int sl[1] = {P_PQ};
             block_partials<1>(v, part, sl, b);
             return;
@@ -2109,7 +2178,8 @@ unsigned spmv_grid(const Ctx* c, bool sym, bool fp64)
     static const int forced = env_int("INS_SYM_GRID", 0);
     if (forced >= 8) return round8(std::min(forced, MAXG));
     if (c->sell.sdpn > 1) return sym_grid(c->sell.snodes, c->n_cu, fp64 ? 1 : 2);
-    return sym_grid(n, c->n_cu, 4);
+    // the FP64 stencil-line rows hold about 150 registers: 3 waves per SIMD
+    return sym_grid(n, c->n_cu, fp64 && c->sell.slines ? 3 : 4);
 }
 
 template <typename TV, typename TX, typename TA, typename TVR>
@@ -2125,9 +2195,13 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
         SymView Y{nullptr, nullptr, 0, 1, 0, 0, nullptr};
         if (sym) Y = sym_view(c);
         if constexpr (sizeof(TV) == 8) {
-            // FP64 sweeps: the FP64 symmetric slices, generic row forms
+            // FP64 sweeps: the FP64 symmetric slices (stencil lines, generic
+            // scalar rows or node blocks)
             if (sym && Y.dpn > 1)
                 INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 4>, g, SB, n, sell_view(c),
+                                     sell_val, p, q, c->partial.p, sc, it, R, Y));
+            else if (sym && Y.lines)
+                INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 3>, g, SB, n, sell_view(c),
                                      sell_val, p, q, c->partial.p, sc, it, R, Y));
             else if (sym)
                 INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 1>, g, SB, n, sell_view(c),
