@@ -516,12 +516,15 @@ def main():
     workload_tag = "%s%d" % (args.config, args.n or default_n)
     traffic = read_traffic(workload_tag, "mixed_" + fmt_name)
     sym_kind = "node-block" if stn["sym_block"] > 1 else "scalar"
+    sym_kernel = ("k_spmv_cap<5, float, float, float, 4, double, 3> (stencil-line rows)"
+                  if stn["sym_block"] == 1 and stn["sym_lines"] else
+                  "k_spmv_sell<float, float, float, 4, double, %d>"
+                  % (4 if stn["sym_block"] > 1 else 1))
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
             "traffic_source": ("profiles/spmv_counters_%s.md, traffic_bytes_per_launch[mixed_%s]"
                                % (workload_tag, fmt_name)) if traffic else None,
-            "kernel": {"sym": "k_spmv_sell<float, float, float, 4> on the %s symmetric slices"
-                              % sym_kind,
+            "kernel": {"sym": "%s on the %s symmetric slices" % (sym_kernel, sym_kind),
                        "sell": "k_spmv_sell<float, float, float, 4>",
                        "csr": "k_spmv_csr<LPR, float, float, float>"}[fmt_name]
                       + " (q = A p with p.q partials)",

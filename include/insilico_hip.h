@@ -117,6 +117,9 @@ typedef struct ins_stats {
     int     assembled_sym;  /* 1: the last matrix assembly wrote the
                                symmetric slices directly (the CSR values are
                                produced from them on demand); 0: the CSR */
+    int     sym_lines;      /* 1: the symmetric rows are stencil lines (a
+                               structured grid): the sweeps load each line of
+                               the vector once per wavefront               */
 } ins_stats;
 
 /* ---- context ------------------------------------------------------------ */

@@ -56,7 +56,8 @@ class Stats(C.Structure):
                 ("sell_patterns", C.c_int), ("sell_sym_offsets", C.c_int),
                 ("rep_ms", C.c_double), ("rep_calls", C.c_int64),
                 ("element_ms", C.c_double), ("sym_values", C.c_int64),
-                ("sym_block", C.c_int), ("assembled_sym", C.c_int)]
+                ("sym_block", C.c_int), ("assembled_sym", C.c_int),
+                ("sym_lines", C.c_int)]
 
 
 class InsError(RuntimeError):

@@ -230,6 +230,7 @@ int ins_get_stats(const ins_ctx* h, ins_stats* out)
     out->sell_sym_offsets = sym ? c->sell.nu : 0;
     out->sym_values = sym ? c->sell.nvs : 0;
     out->sym_block = sym ? c->sell.sdpn : 0;
+    out->sym_lines = sym && c->sell.slines ? 1 : 0;
     return INS_OK;
 }
 

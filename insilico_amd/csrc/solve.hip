@@ -961,6 +961,19 @@ __global__ __launch_bounds__(SB) void k_spmv_sell(int64_t n, SellView S, const T
     spmv_sell_body<TV, TX, TA, SGK, TVR, SYM>(grid_blk(), n, S, val, p, q, part, sc, it, R, Y);
 }
 
+// The FP32 stencil-line SpMV held to the registers of WPE waves per SIMD.
+// Uncapped it takes 97 VGPRs, allocated in granules of 8 as 104 (4 waves
+// per SIMD); at 96 (WPE = 5) the 128^3 sweep takes 26.7 instead of 28.5 us
+// on the same grid of 4 blocks per CU.  (The FP64 line rows held to 128
+// registers for 4 waves spill, 92 bytes per lane, and run slower.)
+template <int WPE, typename TV, typename TX, typename TA, int SGK, typename TVR, int SYM>
+__global__ __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_spmv_cap(
+    int64_t n, SellView S, const TV* __restrict__ val, const TX* __restrict__ p,
+    TX* __restrict__ q, double* part, const Scal* sc, int it, Rep<TVR> R, SymView Y)
+{
+    spmv_sell_body<TV, TX, TA, SGK, TVR, SYM>(grid_blk(), n, S, val, p, q, part, sc, it, R, Y);
+}
+
 template <int LPR, typename TV, typename TX, typename TA, typename TVR>
 __global__ __launch_bounds__(SB) void k_spmv_csr(int64_t n, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ col,
@@ -2171,6 +2184,9 @@ SymView sym_view(const Ctx* c)
 // slices, the same for a solve alone and inside a server batch, so that a
 // request's partial sums, and with them its iterates, do not depend on where
 // it runs
+// the FP32 stencil-line SpMV in its register-capped instantiation (INS_SYM_CAP)
+static const int g_cap = env_int("INS_SYM_CAP", 1);
+
 unsigned spmv_grid(const Ctx* c, bool sym, bool fp64)
 {
     const int64_t n = c->n_active;
@@ -2212,6 +2228,9 @@ int launch_spmv_r(Ctx* c, int format, const TV* csr_val, const TV* sell_val, con
         } else if (Y.v && Y.dpn > 1)
             INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 4>, g, SB, n, sell_view(c),
                                  sell_val, p, q, c->partial.p, sc, it, R, Y));
+        else if (Y.v && Y.lines && g_cap)
+            INS_TRY(launch_timed(c, k_spmv_cap<5, TV, TX, TA, 4, TVR, 3>, g, SB, n,
+                                 sell_view(c), sell_val, p, q, c->partial.p, sc, it, R, Y));
         else if (Y.v && Y.lines)
             INS_TRY(launch_timed(c, k_spmv_sell<TV, TX, TA, 4, TVR, 3>, g, SB, n, sell_view(c),
                                  sell_val, p, q, c->partial.p, sc, it, R, Y));
